@@ -1,0 +1,1192 @@
+// er_capi.hip -- host side of the C ABI (include/er_capi.h): context, device
+// buffers, per-frame orchestration of the kernels in er_kernels.h.
+#include "er_capi.h"
+#include "er_kernels.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace {
+
+constexpr int NI_B = 16;  // insertion interleave of the batched path
+
+template <typename T> struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    hipError_t ensure(size_t count)
+    {
+        if (count <= n && p) return hipSuccess;
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+        hipError_t e = hipMalloc((void **)&p, std::max<size_t>(count, 1) * sizeof(T));
+        if (e == hipSuccess) n = count;
+        return e;
+    }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
+}  // namespace
+
+struct er_ctx {
+    // sizes and parameters
+    int numatm = 0, nummol = 0, ntype = 0, numslv = 0;
+    er_params prm{};
+    int K1 = 0, K2 = 0, K3 = 0, H = 0, ord = 0;
+    size_t G = 0, Gc = 0;
+    PairParams pp{};
+    std::vector<int> molbeg;       // 0-based [nummol+1]
+    std::vector<int> uvspec;       // [nummol]
+    std::vector<float> molq;       // [nummol]
+    std::vector<int> sp_begin, sp_end;
+    float qtot = 0.f;              // sum of physical molecule charges
+
+    // device topology
+    DBuf<float> d_charge, d_ljene, d_ljlensq, d_molq;
+    DBuf<int> d_ljtype, d_molbeg, d_spb, d_spe;
+
+    // solute
+    int nslt = 0;                  // sites per insertion / per solute molecule
+    float qslt = 0.f;              // refs test-particle charge
+    DBuf<float> d_sltq;
+    DBuf<int> d_sltlj;
+    int nsltmol = 0;               // soln: number of solute molecules
+    std::vector<int> tagslt;       // 0-based
+    DBuf<int> d_tag;
+
+    // mesh
+    int ermax = 0, esmax = 0;
+    std::vector<int> uvmax, uvsoft, idmin;
+    std::vector<double> uvcrd, escrd;
+    DBuf<double> d_uvcrd, d_escrd;
+    DBuf<int> d_uvmax, d_uvsoft, d_idmin;
+    bool have_mesh = false;
+
+    // frame
+    bool have_frame = false, frame_dirty = false, cell_changed = true;
+    float cell[9] = {0}, prev_cell[9] = {0};
+    bool have_prev_cell = false;
+    FrameConst fc{};
+    double invcl_d[9] = {0}, volume_d = 0.0;
+    float w_sys = 1.f;
+    DBuf<float> d_sitepos_own;
+    const float *d_sitepos = nullptr;
+    DBuf<float4> d_posn, d_molref;
+    DBuf<float> d_splw;
+    DBuf<int4> d_splg;
+    std::vector<float> h_pin;      // host staging (pageable)
+
+    // insertions
+    int nins = 0;
+    bool have_ins = false;
+    DBuf<float> d_sltpos_own, d_wslt;
+    const float *d_sltpos = nullptr;
+    bool wslt_given = false;
+    DBuf<float4> d_insposn, d_insref;
+    DBuf<float> d_insw, d_selfreal;
+    DBuf<int4> d_insg;
+
+    // PME
+    std::vector<double> splfc[3];
+    DBuf<double> d_splfc1, d_splfc2, d_splfc3;
+    DBuf<float> d_engfac;
+    DBuf<float2> d_tw1, d_tw2, d_tw3, d_twm;
+    DBuf<double> d_selfpart;
+    DBuf<float2> d_T;
+    DBuf<float> d_phi;
+    int chunk = 128;
+
+    // spatial order / cell list (per frame)
+    int nc = 1, ncell = 1;
+    float cell_w = 1.f;            // smallest cell width (perpendicular), Angstrom
+    DBuf<int> d_perm;
+    DBuf<unsigned int> d_keyrank, d_ccnt, d_cstart;
+    DBuf<float> d_rmolmax;
+
+    // MFMA accumulator layout (lane, register) -> (row, col), measured once
+    bool use_mfma = false;
+    DBuf<int> d_dmap;
+
+    // energies / histograms
+    DBuf<float> d_U, d_u0, d_meshf, d_resself;
+    DBuf<double> d_flpart;
+    int nslice = 1;
+    DBuf<uint32_t> d_C;
+    DBuf<double> d_w, d_flceng;
+    int last_NI = NI_B;
+    int last_nins = 0;
+
+    // accumulators
+    DBuf<double> d_edens, d_ecorr, d_eself, d_state, d_slnuv;
+    DBuf<int> d_flags, d_warn;
+    DBuf<float> d_fstate;
+    DBuf<unsigned int> d_minmax;
+    DBuf<unsigned long long> d_errkey;
+
+    // diagnostics: event pairs recorded around launches while profiling
+    bool prof = false;
+    struct ProfRec { int kind; hipEvent_t a, b; };
+    std::vector<ProfRec> prof_recs;
+    std::vector<hipEvent_t> prof_pool;
+    double prof_ms[ER_K_COUNT] = {0};
+    int64_t prof_n[ER_K_COUNT] = {0};
+
+    hipStream_t stream = nullptr;
+    std::string errmsg;
+    char halt_code[8] = {0};
+    int halt_species = 0;
+    float halt_energy = 0.f;
+};
+
+static int fail(er_ctx *c, int code, const char *fmt, ...)
+{
+    if (c) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        c->errmsg = buf;
+    }
+    return code;
+}
+
+static hipEvent_t prof_event(er_ctx *c)
+{
+    if (!c->prof_pool.empty()) {
+        hipEvent_t e = c->prof_pool.back();
+        c->prof_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// RAII bracket: records an event pair around the launches in its scope
+struct ProfScope {
+    er_ctx *c;
+    int kind;
+    hipEvent_t a = nullptr;
+    ProfScope(er_ctx *c_, int k) : c(c_), kind(k)
+    {
+        if (c->prof && (a = prof_event(c))) (void)hipEventRecord(a, c->stream);
+    }
+    ~ProfScope()
+    {
+        if (!a) return;
+        hipEvent_t b = prof_event(c);
+        if (!b) return;
+        (void)hipEventRecord(b, c->stream);
+        c->prof_recs.push_back({kind, a, b});
+    }
+};
+
+static void prof_collect(er_ctx *c)
+{
+    if (c->prof_recs.empty()) return;
+    (void)hipStreamSynchronize(c->stream);
+    for (auto &r : c->prof_recs) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+            c->prof_ms[r.kind] += ms;
+            c->prof_n[r.kind] += 1;
+        }
+        c->prof_pool.push_back(r.a);
+        c->prof_pool.push_back(r.b);
+    }
+    c->prof_recs.clear();
+}
+
+#define HIPCHK(c, x)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            return fail((c), ER_E_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                                 \
+    } while (0)
+
+static bool fft_len_ok(int n)
+{
+    static const int ok[] = {32, 36, 40, 48, 64, 72, 80};
+    for (int v : ok)
+        if (v == n) return true;
+    return false;
+}
+
+// cardinal B-spline M_n (spline.F90:46-58) in double for the |b|^2 factors
+static double mn_spline(int n, double x)
+{
+    if (!(x > 0.0 && x < (double)n)) return 0.0;
+    if (n == 2) return 1.0 - std::fabs(x - 1.0);
+    return (x * mn_spline(n - 1, x) + ((double)n - x) * mn_spline(n - 1, x - 1.0)) / (double)(n - 1);
+}
+
+// init_spline_axis (recpcal.F90:97-119)
+static std::vector<double> spline_axis(int order, int K)
+{
+    std::vector<double> s(K);
+    for (int rci = 0; rci < K; ++rci) {
+        double re = 0.0, im = 0.0;
+        for (int spi = 0; spi <= order - 2; ++spi) {
+            const double chr = mn_spline(order, (double)(spi + 1));
+            const double a = 2.0 * M_PI * (double)(spi * rci) / (double)K;
+            re += chr * std::cos(a);
+            im += chr * std::sin(a);
+        }
+        s[rci] = re * re + im * im;
+    }
+    return s;
+}
+
+static hipError_t upload_tw(DBuf<float2> &b, int N, hipStream_t s)
+{
+    std::vector<float2> t(N);
+    for (int k = 0; k < N; ++k) {
+        const double a = 2.0 * M_PI * (double)k / (double)N;
+        t[k] = make_float2((float)std::cos(a), (float)std::sin(a));
+    }
+    hipError_t e = b.ensure(N);
+    if (e != hipSuccess) return e;
+    (void)s;
+    return hipMemcpy(b.p, t.data(), N * sizeof(float2), hipMemcpyHostToDevice);
+}
+
+// GROMACS force-switch constants (realcal.F90:899-915)
+static void gmx_params(int pw, float lw, float up, float &A, float &B, float &C)
+{
+    const float df = up - lw;
+    A = -(float)pw * ((float)(pw + 4) * up - (float)(pw + 1) * lw) / (std::pow(up, (float)(pw + 2)) * df * df) / 3.0f;
+    B = (float)pw * ((float)(pw + 3) * up - (float)(pw + 1) * lw) / (std::pow(up, (float)(pw + 2)) * df * df * df) / 4.0f;
+    C = 1.0f / std::pow(up, (float)pw) - A * df * df * df - B * df * df * df * df;
+}
+
+extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx **out)
+{
+    if (!topo || !prm || !out) return ER_E_ARG;
+    *out = nullptr;
+    er_ctx *c = new er_ctx();
+    auto bad = [&](int code) { *out = c; return code; };
+    if (topo->numatm <= 0 || topo->nummol <= 0 || topo->ljtype_max <= 0 || topo->numslv <= 0 ||
+        !topo->mol_begin_index || !topo->ljtype || !topo->uvspec || !topo->charge || !topo->ljene_mat ||
+        !topo->ljlensq_mat)
+        return bad(fail(c, ER_E_ARG, "er_create: incomplete topology"));
+    if (prm->cltype != 2)
+        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: only cltype=2 (PME) is implemented"));
+    if (prm->boxshp != 1)
+        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: only boxshp=1 (periodic) is implemented"));
+    if (prm->slttype < 1 || prm->slttype > 3) return bad(fail(c, ER_E_ARG, "er_create: slttype"));
+    if (prm->ljswitch < 0 || prm->ljswitch > 3) return bad(fail(c, ER_E_ARG, "er_create: ljswitch"));
+    if (!(prm->splodr == 4 || prm->splodr == 6 || prm->splodr == 8))
+        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: splodr must be 4, 6 or 8"));
+    if (!fft_len_ok(prm->ms[0]) || !fft_len_ok(prm->ms[1]) || prm->ms[1] != prm->ms[2])
+        return bad(fail(c, ER_E_UNSUPPORTED,
+                        "er_create: PME grid %dx%dx%d not supported (each of ms1max and ms2max=ms3max "
+                        "must be one of 32 36 40 48 64 72 80)",
+                        prm->ms[0], prm->ms[1], prm->ms[2]));
+    c->numatm = topo->numatm;
+    c->nummol = topo->nummol;
+    c->ntype = topo->ljtype_max;
+    c->numslv = topo->numslv;
+    c->prm = *prm;
+    c->K1 = prm->ms[0];
+    c->K2 = prm->ms[1];
+    c->K3 = prm->ms[2];
+    c->H = c->K1 / 2 + 1;
+    c->ord = prm->splodr;
+    c->G = (size_t)c->K1 * c->K2 * c->K3;
+    c->Gc = (size_t)c->H * c->K2 * c->K3;
+    if (const char *e = getenv("ER_CHUNK")) c->chunk = std::max(NI_B, atoi(e) / NI_B * NI_B);
+
+    c->molbeg.resize(c->nummol + 1);
+    for (int j = 0; j <= c->nummol; ++j) c->molbeg[j] = topo->mol_begin_index[j] - 1;
+    if (c->molbeg[0] != 0 || c->molbeg[c->nummol] != c->numatm)
+        return bad(fail(c, ER_E_ARG, "er_create: mol_begin_index inconsistent with numatm"));
+    for (int j = 0; j < c->nummol; ++j)
+        if (c->molbeg[j + 1] <= c->molbeg[j]) return bad(fail(c, ER_E_ARG, "er_create: empty molecule"));
+    std::vector<int> lj(c->numatm);
+    for (int a = 0; a < c->numatm; ++a) {
+        lj[a] = topo->ljtype[a] - 1;
+        if (lj[a] < 0 || lj[a] >= c->ntype) return bad(fail(c, ER_E_ARG, "er_create: ljtype out of range"));
+    }
+    c->uvspec.assign(topo->uvspec, topo->uvspec + c->nummol);
+    c->molq.resize(c->nummol);
+    c->qtot = 0.f;
+    for (int j = 0; j < c->nummol; ++j) {
+        float q = 0.f;  // mol_charge, setconf.F90:838-840
+        for (int a = c->molbeg[j]; a < c->molbeg[j + 1]; ++a) q += topo->charge[a];
+        c->molq[j] = q;
+        c->qtot += q;   // volcorrect total_charge, engproc.F90:1032-1033
+    }
+    // species ranges: setparam numbers molecules species by species
+    c->sp_begin.assign(c->numslv, 0);
+    c->sp_end.assign(c->numslv, 0);
+    std::vector<int> seen(c->numslv + 1, 0);
+    for (int j = 0; j < c->nummol; ++j) {
+        const int s = c->uvspec[j];
+        if (s < 0 || s > c->numslv) return bad(fail(c, ER_E_ARG, "er_create: uvspec out of range"));
+        if (s == 0) continue;
+        if (!seen[s]) {
+            seen[s] = 1;
+            c->sp_begin[s - 1] = j;
+        } else if (c->sp_end[s - 1] != j) {
+            return bad(fail(c, ER_E_UNSUPPORTED, "er_create: molecules of species %d are not contiguous", s));
+        }
+        c->sp_end[s - 1] = j + 1;
+    }
+
+    // pair parameters
+    PairParams &pp = c->pp;
+    pp.elecut = prm->elecut;
+    pp.lwljcut = prm->lwljcut;
+    pp.upljcut = prm->upljcut;
+    pp.screen = prm->screen;
+    pp.ljswitch = prm->ljswitch;
+    pp.lw3 = prm->lwljcut * prm->lwljcut * prm->lwljcut;
+    pp.up3 = prm->upljcut * prm->upljcut * prm->upljcut;
+    pp.lw6 = pp.lw3 * pp.lw3;
+    pp.up6 = pp.up3 * pp.up3;
+    gmx_params(12, prm->lwljcut, prm->upljcut, pp.repA, pp.repB, pp.repC);
+    gmx_params(6, prm->lwljcut, prm->upljcut, pp.attA, pp.attB, pp.attC);
+
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipStream_t s = c->stream;
+    HIPCHK(c, c->d_charge.ensure(c->numatm));
+    HIPCHK(c, hipMemcpyAsync(c->d_charge.p, topo->charge, c->numatm * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_ljtype.ensure(c->numatm));
+    HIPCHK(c, hipMemcpyAsync(c->d_ljtype.p, lj.data(), c->numatm * sizeof(int), hipMemcpyHostToDevice, s));
+    const size_t nt2 = (size_t)c->ntype * c->ntype;
+    HIPCHK(c, c->d_ljene.ensure(nt2));
+    HIPCHK(c, hipMemcpyAsync(c->d_ljene.p, topo->ljene_mat, nt2 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_ljlensq.ensure(nt2));
+    HIPCHK(c, hipMemcpyAsync(c->d_ljlensq.p, topo->ljlensq_mat, nt2 * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_molbeg.ensure(c->nummol + 1));
+    HIPCHK(c, hipMemcpyAsync(c->d_molbeg.p, c->molbeg.data(), (c->nummol + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_molq.ensure(c->nummol));
+    HIPCHK(c, hipMemcpyAsync(c->d_molq.p, c->molq.data(), c->nummol * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_spb.ensure(c->numslv));
+    HIPCHK(c, hipMemcpyAsync(c->d_spb.p, c->sp_begin.data(), c->numslv * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_spe.ensure(c->numslv));
+    HIPCHK(c, hipMemcpyAsync(c->d_spe.p, c->sp_end.data(), c->numslv * sizeof(int), hipMemcpyHostToDevice, s));
+
+    // PME constants
+    for (int d = 0; d < 3; ++d) c->splfc[d] = spline_axis(c->ord, prm->ms[d]);
+    HIPCHK(c, c->d_splfc1.ensure(c->K1));
+    HIPCHK(c, hipMemcpyAsync(c->d_splfc1.p, c->splfc[0].data(), c->K1 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_splfc2.ensure(c->K2));
+    HIPCHK(c, hipMemcpyAsync(c->d_splfc2.p, c->splfc[1].data(), c->K2 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_splfc3.ensure(c->K3));
+    HIPCHK(c, hipMemcpyAsync(c->d_splfc3.p, c->splfc[2].data(), c->K3 * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, upload_tw(c->d_tw1, c->K1, s));
+    HIPCHK(c, upload_tw(c->d_tw2, c->K2, s));
+    HIPCHK(c, upload_tw(c->d_tw3, c->K3, s));
+    HIPCHK(c, upload_tw(c->d_twm, c->K1 / 2, s));
+    HIPCHK(c, c->d_engfac.ensure(c->Gc));
+
+    // per-atom frame buffers
+    HIPCHK(c, c->d_posn.ensure(c->numatm));
+    HIPCHK(c, c->d_splw.ensure((size_t)c->numatm * 3 * c->ord));
+    HIPCHK(c, c->d_splg.ensure(c->numatm));
+    HIPCHK(c, c->d_molref.ensure(c->nummol));
+    HIPCHK(c, c->d_sitepos_own.ensure((size_t)3 * c->numatm));
+
+    // accumulators (state survives engclear: voffset, minuv/maxuv)
+    HIPCHK(c, c->d_state.ensure(4));
+    HIPCHK(c, c->d_flags.ensure(4));
+    HIPCHK(c, c->d_fstate.ensure(4));
+    HIPCHK(c, c->d_warn.ensure(1));
+    HIPCHK(c, c->d_errkey.ensure(1));
+    HIPCHK(c, c->d_minmax.ensure(2 * (size_t)(c->numslv + 1)));
+    HIPCHK(c, c->d_slnuv.ensure(c->numslv));
+    HIPCHK(c, hipMemsetAsync(c->d_state.p, 0, 4 * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(c->d_flags.p, 0, 4 * sizeof(int), s));
+    HIPCHK(c, hipMemsetAsync(c->d_fstate.p, 0, 4 * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(c->d_warn.p, 0, sizeof(int), s));
+    HIPCHK(c, hipMemsetAsync(c->d_errkey.p, 0xff, sizeof(unsigned long long), s));
+    HIPCHK(c, hipMemsetAsync(c->d_slnuv.p, 0, c->numslv * sizeof(double), s));
+    {
+        std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
+        auto ord = [](float f) {
+            unsigned int u;
+            std::memcpy(&u, &f, 4);
+            return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+        };
+        for (int k = 0; k <= c->numslv; ++k) {
+            mm[2 * k] = ord(3.402823466e38f);       // minuv = huge (engproc.F90:299-302)
+            mm[2 * k + 1] = ord(-3.402823466e38f);  // maxuv = -huge
+        }
+        HIPCHK(c, hipMemcpy(c->d_minmax.p, mm.data(), mm.size() * sizeof(unsigned int), hipMemcpyHostToDevice));
+    }
+    // Accumulator layout of the FP64 MFMA: each output register of a known
+    // product names its own (row, col); use the MFMA path only if the
+    // measured map is a bijection onto the 16x16 tile.
+    {
+        DBuf<double> probe;
+        HIPCHK(c, probe.ensure(256));
+        hipLaunchKernelGGL(k_mfma_layout, dim3(1), dim3(64), 0, s, probe.p);
+        HIPCHK(c, hipGetLastError());
+        std::vector<double> d(256);
+        HIPCHK(c, hipMemcpyAsync(d.data(), probe.p, 256 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        probe.release();
+        std::vector<int> map(256), seen(256, 0);
+        bool ok = true;
+        for (int k = 0; k < 256 && ok; ++k) {
+            const double v = d[k];
+            const int iv = (int)v;
+            if (v != (double)iv || iv < 0 || iv > 255 || seen[iv]) ok = false;
+            else {
+                seen[iv] = 1;
+                map[k] = iv;  // = row + 16*col
+            }
+        }
+        c->use_mfma = ok;
+        if (ok) {
+            HIPCHK(c, c->d_dmap.ensure(256));
+            HIPCHK(c, hipMemcpy(c->d_dmap.p, map.data(), 256 * sizeof(int), hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(c, c->d_rmolmax.ensure(1));
+    HIPCHK(c, hipStreamSynchronize(s));
+    *out = c;
+    return ER_OK;
+}
+
+extern "C" int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsoft, const double *uvcrd,
+                           int32_t esmax, const double *escrd)
+{
+    if (!c) return ER_E_ARG;
+    if (!uvmax || !uvsoft || !uvcrd) return fail(c, ER_E_ARG, "er_set_mesh: null argument");
+    c->uvmax.assign(uvmax, uvmax + c->numslv);
+    c->uvsoft.assign(uvsoft, uvsoft + c->numslv);
+    c->idmin.assign(c->numslv, 0);
+    int tot = 0;
+    for (int k = 0; k < c->numslv; ++k) {
+        if (c->uvmax[k] <= 0) return fail(c, ER_E_ARG, "er_set_mesh: uvmax must be positive");
+        c->idmin[k] = tot;
+        tot += c->uvmax[k];
+    }
+    c->ermax = tot;
+    c->uvcrd.assign(uvcrd, uvcrd + tot);
+    for (int k = 0; k < c->numslv; ++k)
+        for (int b = 1; b < c->uvmax[k]; ++b)
+            if (!(c->uvcrd[c->idmin[k] + b - 1] < c->uvcrd[c->idmin[k] + b]))
+                return fail(c, ER_E_ARG, "er_set_mesh: mesh of species %d not strictly increasing", k + 1);
+    c->esmax = (c->prm.selfcal == 1) ? esmax : 0;
+    if (c->prm.selfcal == 1) {
+        if (!escrd || esmax <= 0) return fail(c, ER_E_ARG, "er_set_mesh: selfcal needs escrd");
+        c->escrd.assign(escrd, escrd + esmax);
+    }
+    hipStream_t s = c->stream;
+    HIPCHK(c, c->d_uvcrd.ensure(tot));
+    HIPCHK(c, hipMemcpyAsync(c->d_uvcrd.p, c->uvcrd.data(), tot * sizeof(double), hipMemcpyHostToDevice, s));
+    {
+        // for FP32 v:  v > c (compared in FP64)  <=>  v > RD32(c)
+        std::vector<float> mf(tot);
+        for (int b = 0; b < tot; ++b) {
+            float f = (float)c->uvcrd[b];
+            if ((double)f > c->uvcrd[b]) f = std::nextafter(f, -INFINITY);
+            mf[b] = f;
+        }
+        HIPCHK(c, c->d_meshf.ensure(tot));
+        HIPCHK(c, hipMemcpy(c->d_meshf.p, mf.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+    }
+    HIPCHK(c, c->d_uvmax.ensure(c->numslv));
+    HIPCHK(c, hipMemcpyAsync(c->d_uvmax.p, c->uvmax.data(), c->numslv * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_uvsoft.ensure(c->numslv));
+    HIPCHK(c, hipMemcpyAsync(c->d_uvsoft.p, c->uvsoft.data(), c->numslv * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_idmin.ensure(c->numslv));
+    HIPCHK(c, hipMemcpyAsync(c->d_idmin.p, c->idmin.data(), c->numslv * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_escrd.ensure(std::max(1, c->esmax)));
+    if (c->esmax > 0)
+        HIPCHK(c, hipMemcpyAsync(c->d_escrd.p, c->escrd.data(), c->esmax * sizeof(double), hipMemcpyHostToDevice, s));
+    HIPCHK(c, c->d_edens.ensure(tot));
+    HIPCHK(c, c->d_ecorr.ensure((size_t)tot * tot));
+    HIPCHK(c, c->d_eself.ensure(std::max(1, c->esmax)));
+    HIPCHK(c, hipStreamSynchronize(s));
+    c->have_mesh = true;
+    return er_clear(c);
+}
+
+extern "C" int er_set_solute(er_ctx *c, int32_t nsite, const int32_t *ljtype, const float *charge)
+{
+    if (!c) return ER_E_ARG;
+    if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "er_set_solute: soln runs use er_set_solute_molecules");
+    if (nsite <= 0 || !ljtype || !charge) return fail(c, ER_E_ARG, "er_set_solute: bad arguments");
+    std::vector<int> lj(nsite);
+    c->qslt = 0.f;
+    for (int s = 0; s < nsite; ++s) {
+        lj[s] = ljtype[s] - 1;
+        if (lj[s] < 0 || lj[s] >= c->ntype) return fail(c, ER_E_ARG, "er_set_solute: ljtype out of range");
+        c->qslt += charge[s];
+    }
+    c->nslt = nsite;
+    HIPCHK(c, c->d_sltq.ensure(nsite));
+    HIPCHK(c, hipMemcpy(c->d_sltq.p, charge, nsite * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, c->d_sltlj.ensure(nsite));
+    HIPCHK(c, hipMemcpy(c->d_sltlj.p, lj.data(), nsite * sizeof(int), hipMemcpyHostToDevice));
+    return ER_OK;
+}
+
+extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *tagslt)
+{
+    if (!c) return ER_E_ARG;
+    if (c->prm.slttype != 1) return fail(c, ER_E_STATE, "er_set_solute_molecules: only for soln runs");
+    if (nslt <= 0 || !tagslt) return fail(c, ER_E_ARG, "er_set_solute_molecules: bad arguments");
+    c->tagslt.resize(nslt);
+    int ns = -1;
+    for (int k = 0; k < nslt; ++k) {
+        const int m = tagslt[k] - 1;
+        if (m < 0 || m >= c->nummol) return fail(c, ER_E_ARG, "er_set_solute_molecules: index out of range");
+        const int sz = c->molbeg[m + 1] - c->molbeg[m];
+        if (ns >= 0 && sz != ns) return fail(c, ER_E_ARG, "set_slt: solute molecules differ in size");
+        ns = sz;
+        c->tagslt[k] = m;
+    }
+    c->nsltmol = nslt;
+    c->nslt = ns;
+    HIPCHK(c, c->d_tag.ensure(nslt));
+    HIPCHK(c, hipMemcpy(c->d_tag.p, c->tagslt.data(), nslt * sizeof(int), hipMemcpyHostToDevice));
+    return ER_OK;
+}
+
+// realcal_prepare's cell handling (realcal.F90:776-896) + get_inverted_cell
+static int frame_cell(er_ctx *c, const float *cell)
+{
+    // column-major: cell(i,j) = cell[i + 3j]; sub-diagonal (2,1),(3,1),(3,2)
+    if (cell[1] != 0.f || cell[2] != 0.f || cell[5] != 0.f)
+        return fail(c, ER_E_UNSUPPORTED, "cell is not upper triangular (make_cell_uppertriangular path)");
+    float cn[9];
+    std::memcpy(cn, cell, sizeof cn);
+    // normalize_cell_vector (realcal.F90:826-830): only column 1 can be flipped
+    for (int i = 0; i < 3; ++i)
+        if (cn[0] < 0.f)
+            for (int r = 0; r < 3; ++r) cn[r + 3 * i] = -cn[r + 3 * i];
+    const float cutoff = std::min(c->prm.elecut, c->prm.upljcut);
+    for (int i = 0; i < 3; ++i)
+        if (cutoff > cn[i + 3 * i] * 0.5f)
+            return fail(c, ER_E_ARG,
+                        "One of axis in periodic cell is too small. This is either the box is too small "
+                        "compared to the cell, or the periodic cell is too skewed.");
+    FrameConst &fc = c->fc;
+    std::memcpy(fc.cellN, cn, sizeof cn);
+    for (int i = 0; i < 3; ++i) {
+        fc.lenN[i] = std::fabs(cn[i + 3 * i]);
+        fc.invN[i] = 1.0f / fc.lenN[i];
+    }
+    fc.is_cuboid = !(std::fabs(cell[3]) > 1e-8f || std::fabs(cell[6]) > 1e-8f || std::fabs(cell[7]) > 1e-8f);
+    // inverse cell in double (engproc.F90:1049-1070)
+    double m[9];
+    for (int k = 0; k < 9; ++k) m[k] = cell[k];
+#define M(i, j) m[(i) + 3 * (j)]
+    const double v = M(0, 0) * M(1, 1) * M(2, 2) + M(0, 1) * M(1, 2) * M(2, 0) + M(0, 2) * M(1, 0) * M(2, 1) -
+                     M(0, 2) * M(1, 1) * M(2, 0) - M(0, 1) * M(1, 0) * M(2, 2) - M(0, 0) * M(1, 2) * M(2, 1);
+    double *iv = c->invcl_d;
+    iv[0 + 3 * 0] = M(1, 1) * M(2, 2) - M(1, 2) * M(2, 1);
+    iv[0 + 3 * 1] = M(0, 2) * M(2, 1) - M(0, 1) * M(2, 2);
+    iv[0 + 3 * 2] = M(0, 1) * M(1, 2) - M(0, 2) * M(1, 1);
+    iv[1 + 3 * 0] = M(1, 2) * M(2, 0) - M(1, 0) * M(2, 2);
+    iv[1 + 3 * 1] = M(0, 0) * M(2, 2) - M(0, 2) * M(2, 0);
+    iv[1 + 3 * 2] = M(0, 2) * M(1, 0) - M(0, 0) * M(1, 2);
+    iv[2 + 3 * 0] = M(1, 0) * M(2, 1) - M(1, 1) * M(2, 0);
+    iv[2 + 3 * 1] = M(0, 1) * M(2, 0) - M(0, 0) * M(2, 1);
+    iv[2 + 3 * 2] = M(0, 0) * M(1, 1) - M(0, 1) * M(1, 0);
+#undef M
+    for (int k = 0; k < 9; ++k) {
+        iv[k] /= v;
+        fc.invcl[k] = (float)iv[k];
+    }
+    c->volume_d = v;
+    fc.volume = (float)v;
+    // cell-list resolution: nc cells per axis of >= 3.5 A perpendicular width
+    {
+        double wmin = 1e300;
+        for (int k = 0; k < 3; ++k) {
+            const int i1 = (k + 1) % 3, i2 = (k + 2) % 3;
+            const double *a = m + 3 * i1, *b = m + 3 * i2;
+            const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+            wmin = std::min(wmin, std::fabs(v) / std::sqrt(cx * cx + cy * cy + cz * cz));
+        }
+        c->nc = std::max(1, std::min(32, (int)(wmin / 3.5)));
+        c->ncell = 1 << (3 * 5);  // Morton keys of 5 bits per axis
+        c->cell_w = (float)(wmin / c->nc);
+    }
+    c->cell_changed = !c->have_prev_cell || std::memcmp(c->prev_cell, cell, sizeof c->prev_cell) != 0;
+    std::memcpy(c->cell, cell, sizeof c->cell);
+    return ER_OK;
+}
+
+extern "C" int er_set_frame(er_ctx *c, const float *sitepos, const float *cell, float stat_weight_system)
+{
+    if (!c) return ER_E_ARG;
+    if (!sitepos || !cell) return fail(c, ER_E_ARG, "er_set_frame: null argument");
+    int rc = frame_cell(c, cell);
+    if (rc) return rc;
+    // the previous run may still read the buffer
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_sitepos_own.p, sitepos, (size_t)3 * c->numatm * sizeof(float), hipMemcpyHostToDevice));
+    c->d_sitepos = c->d_sitepos_own.p;
+    c->w_sys = stat_weight_system;
+    c->have_frame = true;
+    c->frame_dirty = true;
+    return ER_OK;
+}
+
+extern "C" int er_set_frame_device(er_ctx *c, const float *sitepos_dev, const float *cell, float stat_weight_system)
+{
+    if (!c) return ER_E_ARG;
+    if (!sitepos_dev || !cell) return fail(c, ER_E_ARG, "er_set_frame_device: null argument");
+    int rc = frame_cell(c, cell);
+    if (rc) return rc;
+    c->d_sitepos = sitepos_dev;
+    c->w_sys = stat_weight_system;
+    c->have_frame = true;
+    c->frame_dirty = true;
+    return ER_OK;
+}
+
+static int ins_common(er_ctx *c, int32_t nins, const float *wslt, bool copies)
+{
+    if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "insertions are for refs runs");
+    if (c->nslt <= 0) return fail(c, ER_E_STATE, "er_set_solute must precede insertions");
+    if (nins <= 0) return fail(c, ER_E_ARG, "nins must be positive");
+    // a run still in flight may read the buffers about to be overwritten
+    if (copies || wslt) HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->nins = nins;
+    HIPCHK(c, c->d_wslt.ensure(nins));
+    c->wslt_given = wslt != nullptr;
+    if (wslt) HIPCHK(c, hipMemcpy(c->d_wslt.p, wslt, nins * sizeof(float), hipMemcpyHostToDevice));
+    c->have_ins = true;
+    return ER_OK;
+}
+
+extern "C" int er_set_insertions(er_ctx *c, int32_t nins, const float *sltpos, const float *stat_weight_solute)
+{
+    if (!c) return ER_E_ARG;
+    if (!sltpos) return fail(c, ER_E_ARG, "er_set_insertions: null sltpos");
+    int rc = ins_common(c, nins, stat_weight_solute, true);
+    if (rc) return rc;
+    const size_t n = (size_t)3 * c->nslt * nins;
+    HIPCHK(c, c->d_sltpos_own.ensure(n));
+    HIPCHK(c, hipMemcpy(c->d_sltpos_own.p, sltpos, n * sizeof(float), hipMemcpyHostToDevice));
+    c->d_sltpos = c->d_sltpos_own.p;
+    return ER_OK;
+}
+
+extern "C" int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sltpos_dev,
+                                        const float *stat_weight_solute)
+{
+    if (!c) return ER_E_ARG;
+    if (!sltpos_dev) return fail(c, ER_E_ARG, "er_set_insertions_device: null sltpos");
+    int rc = ins_common(c, nins, stat_weight_solute, false);
+    if (rc) return rc;
+    c->d_sltpos = sltpos_dev;
+    return ER_OK;
+}
+
+// ---- kernel dispatch on compile-time grid sizes and spline order ----
+
+template <int K> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch)
+{
+    const int NT = ((K + 63) / 64) * 64;
+    const size_t lds = (size_t)K * (K + 1) * sizeof(float2) + (size_t)ER_SCH * 2 * K * sizeof(float2) +
+                       ER_SCH * sizeof(float2) + (NT / 64) * sizeof(double);
+    static bool attr = false;
+    if (!attr && lds > 65536) {
+        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K, K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_pme_spec2d<K, K>), dim3(c->H, nch), dim3(NT), lds, c->stream, c->nslt, c->ord, c->K1,
+                       n_off, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_engfac.p, c->d_tw1.p, c->d_tw2.p,
+                       c->d_tw3.p, c->d_T.p, c->d_selfpart.p);
+    return hipGetLastError();
+}
+
+static hipError_t launch_spec2d(er_ctx *c, int n_off, int nch)
+{
+    switch (c->K2) {
+    case 32: return launch_spec2d_k<32>(c, n_off, nch);
+    case 36: return launch_spec2d_k<36>(c, n_off, nch);
+    case 40: return launch_spec2d_k<40>(c, n_off, nch);
+    case 48: return launch_spec2d_k<48>(c, n_off, nch);
+    case 64: return launch_spec2d_k<64>(c, n_off, nch);
+    case 72: return launch_spec2d_k<72>(c, n_off, nch);
+    case 80: return launch_spec2d_k<80>(c, n_off, nch);
+    }
+    return hipErrorInvalidValue;
+}
+
+static int divisor_le(int K, int cap)
+{
+    for (int r = std::min(K, cap); r >= 1; --r)
+        if (K % r == 0) return r;
+    return 1;
+}
+
+template <int K1, int NI> static hipError_t launch_axis1_k(er_ctx *c, int nch)
+{
+    const int R = (NI == 1) ? c->K3 : divisor_le(c->K3, 256 / NI);
+    const int H = K1 / 2 + 1;
+    const int LS = (H % 2 == 1) ? H : H + 1;
+    const size_t lds = std::max((size_t)R * NI * LS * sizeof(float2), (size_t)R * K1 * NI * sizeof(float));
+    const int nt = ((R * NI + 63) / 64) * 64;
+    static bool attr = false;
+    if (!attr && lds > 65536) {
+        (void)hipFuncSetAttribute((const void *)k_pme_axis1<K1, NI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        attr = true;
+    }
+    const int ntile = (nch + NI - 1) / NI;
+    hipLaunchKernelGGL((k_pme_axis1<K1, NI>), dim3(c->K3 / R, c->K2, ntile), dim3(nt), lds, c->stream, nch, c->K2,
+                       c->K3, R, c->d_T.p, c->d_phi.p, c->d_tw1.p, c->d_twm.p);
+    return hipGetLastError();
+}
+
+template <int NI> static hipError_t launch_axis1(er_ctx *c, int nch)
+{
+    switch (c->K1) {
+    case 32: return launch_axis1_k<32, NI>(c, nch);
+    case 36: return launch_axis1_k<36, NI>(c, nch);
+    case 40: return launch_axis1_k<40, NI>(c, nch);
+    case 48: return launch_axis1_k<48, NI>(c, nch);
+    case 64: return launch_axis1_k<64, NI>(c, nch);
+    case 72: return launch_axis1_k<72, NI>(c, nch);
+    case 80: return launch_axis1_k<80, NI>(c, nch);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int ORD, int NI, int LPM> static hipError_t launch_gather_k(er_ctx *c, int n_off, int nch)
+{
+    constexpr int MPW = 64 / LPM;
+    constexpr int NXCD = 8;  // MI355X: 8 XCDs, workgroups dispatched round-robin
+    const int mols_per_block = 4 * MPW;
+    int ngx = (c->nummol + mols_per_block - 1) / mols_per_block;
+    ngx = (ngx + NXCD - 1) / NXCD * NXCD;
+    const int ngy = (nch + LPM - 1) / LPM;
+    const bool soln = c->prm.slttype == 1;
+    hipLaunchKernelGGL((k_gather<ORD, NI, LPM>), dim3(ngx * ngy), dim3(256), 0, c->stream, c->nummol, nch, n_off,
+                       c->K1, c->K2, c->K3, c->d_perm.p, ngx, NXCD, c->d_molbeg.p, c->d_posn.p, c->d_splw.p,
+                       c->d_splg.p, c->d_molq.p, c->d_phi.p, soln ? c->d_tag.p : (const int *)nullptr, c->qslt,
+                       c->prm.screen, c->fc.volume, c->d_U.p);
+    return hipGetLastError();
+}
+
+template <int NI, int LPM> static hipError_t launch_gather(er_ctx *c, int n_off, int nch)
+{
+    switch (c->ord) {
+    case 4: return launch_gather_k<4, NI, LPM>(c, n_off, nch);
+    case 6: return launch_gather_k<6, NI, LPM>(c, n_off, nch);
+    case 8: return launch_gather_k<8, NI, LPM>(c, n_off, nch);
+    }
+    return hipErrorInvalidValue;
+}
+
+template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
+{
+    const int NT = 256;
+    const size_t lds = (size_t)NT * sizeof(double) + (size_t)c->ermax * sizeof(float) +
+                       (size_t)NI * c->ermax * sizeof(uint32_t) + (size_t)NT * 2 * sizeof(float);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    if (lds > 65536)
+        (void)hipFuncSetAttribute((const void *)k_hist<NI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const bool soln = c->prm.slttype == 1;
+    HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice};
+    const int ntile = (nch + NI - 1) / NI;
+    hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha, c->d_U.p,
+                       soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_spb.p, c->d_spe.p, c->d_idmin.p,
+                       c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p, c->d_minmax.p, soln ? c->d_flpart.p : (double *)nullptr,
+                       c->d_errkey.p, c->d_warn.p);
+    return hipGetLastError();
+}
+
+static hipError_t prep_frame(er_ctx *c)
+{
+    const int B = 256;
+    if (c->cell_changed) {
+        const double *i = c->invcl_d;
+        hipLaunchKernelGGL(k_engfac, dim3((unsigned)((c->Gc + B - 1) / B)), dim3(B), 0, c->stream, c->K1, c->K2, c->K3,
+                           c->ord, c->d_splfc1.p, c->d_splfc2.p, c->d_splfc3.p, i[0], i[1], i[2], i[3], i[4], i[5],
+                           i[6], i[7], i[8], c->volume_d, (double)c->prm.screen, c->d_engfac.p);
+        std::memcpy(c->prev_cell, c->cell, sizeof c->cell);
+        c->have_prev_cell = true;
+        c->cell_changed = false;
+    }
+    if (c->frame_dirty) {
+        hipLaunchKernelGGL(k_prep_atoms, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm, c->d_sitepos,
+                           c->d_charge.p, c->fc, c->K1, c->K2, c->K3, c->ord, c->d_posn.p, c->d_splw.p, c->d_splg.p);
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, (size_t)1,
+                           reinterpret_cast<uint32_t *>(c->d_rmolmax.p));
+        hipLaunchKernelGGL(k_prep_mols, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
+                           c->d_molbeg.p, c->d_posn.p, c->fc, c->d_molref.p, c->d_rmolmax.p);
+        // spatial order of the molecules and the cell list
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->ncell + B - 1) / B), dim3(B), 0, c->stream, (size_t)c->ncell,
+                           c->d_ccnt.p);
+        hipLaunchKernelGGL(k_cell_count, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
+                           c->d_molref.p, c->fc, c->nc, c->d_ccnt.p, c->d_keyrank.p);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->ncell, c->d_ccnt.p, c->d_cstart.p);
+        hipLaunchKernelGGL(k_cell_scatter, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
+                           c->d_keyrank.p, c->d_cstart.p, c->d_perm.p);
+        c->frame_dirty = false;
+    }
+    return hipGetLastError();
+}
+
+static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
+{
+    const size_t nslt = (size_t)c->nslt;
+    HIPCHK(c, c->d_insposn.ensure(nins * nslt));
+    HIPCHK(c, c->d_insw.ensure(nins * nslt * 3 * c->ord));
+    HIPCHK(c, c->d_insg.ensure(nins * nslt));
+    HIPCHK(c, c->d_insref.ensure(nins));
+    HIPCHK(c, c->d_selfreal.ensure(nins));
+    HIPCHK(c, c->d_resself.ensure(nins));
+    HIPCHK(c, c->d_selfpart.ensure((size_t)nins * c->H));
+    HIPCHK(c, c->d_T.ensure((size_t)chunk * c->Gc));
+    const size_t ntile = (size_t)(chunk + NI - 1) / NI;
+    HIPCHK(c, c->d_phi.ensure(ntile * NI * c->G));
+    const size_t ntile_all = (size_t)(nins + NI - 1) / NI;
+    HIPCHK(c, c->d_U.ensure(ntile_all * NI * c->nummol));
+    HIPCHK(c, c->d_u0.ensure(nins));
+    HIPCHK(c, c->d_C.ensure((size_t)nins * c->ermax));
+    HIPCHK(c, c->d_w.ensure(nins));
+    HIPCHK(c, c->d_perm.ensure(c->nummol));
+    HIPCHK(c, c->d_keyrank.ensure(2 * (size_t)c->nummol));
+    HIPCHK(c, c->d_ccnt.ensure(c->ncell));
+    HIPCHK(c, c->d_cstart.ensure(c->ncell + 1));
+    // enough histogram workgroups to fill the chip
+    const int ntile_chunk = (chunk + NI - 1) / NI;
+    c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
+    if (c->prm.slttype == 1) {
+        HIPCHK(c, c->d_flceng.ensure((size_t)nins * c->numslv));
+        HIPCHK(c, c->d_flpart.ensure((size_t)nins * c->nslice * c->numslv));
+    }
+    return ER_OK;
+}
+
+// energies and histograms of insertions [n_off, n_off+nch); NI = 16 for the
+// batched path, 1 for the plain path
+template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
+{
+    hipError_t e;
+    {
+        ProfScope ps(c, ER_K_SPEC2D);
+        if ((e = launch_spec2d(c, n_off, nch)) != hipSuccess) return e;
+    }
+    {
+        ProfScope ps(c, ER_K_AXIS1);
+        if ((e = launch_axis1<NI>(c, nch)) != hipSuccess) return e;
+    }
+    {
+        ProfScope ps(c, ER_K_GATHER);
+        if constexpr (NI == 1) e = launch_gather<1, 1>(c, n_off, nch);
+        else e = launch_gather<NI, NI>(c, n_off, nch);
+    }
+    if (e != hipSuccess) return e;
+    const bool soln = c->prm.slttype == 1;
+    {
+        ProfScope ps(c, ER_K_PAIRS);
+        PairArgs pa{};
+        pa.nmol = c->nummol;
+        pa.nchunk = nch;
+        pa.n_off = n_off;
+        pa.nslt = c->nslt;
+        pa.NI = NI;
+        pa.ntype = c->ntype;
+        pa.nc = c->nc;
+        pa.pp = c->pp;
+        pa.cell_w = c->cell_w;
+        hipLaunchKernelGGL(k_pairs, dim3(nch), dim3(256), 0, c->stream, pa, c->fc, c->d_molbeg.p, c->d_posn.p,
+                           c->d_ljtype.p, c->d_molref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p,
+                           c->d_insref.p, c->d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
+                           c->d_ljlensq.p, c->d_U.p);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    {
+        ProfScope ps(c, ER_K_HIST);
+        if ((e = launch_hist<NI>(c, n_off, nch)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// weights, edens and ecorr of insertions [n_off, n_off+nch) (after their
+// histograms); update_histogram's accumulation part
+static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
+{
+    const bool soln = c->prm.slttype == 1;
+    {
+        ProfScope ps(c, ER_K_WEIGHTS);
+        WeightArgs wa{};
+        wa.nchunk = nch;
+        wa.n_off = n_off;
+        wa.H = c->H;
+        wa.slttype = c->prm.slttype;
+        wa.wgtslf = c->prm.wgtslf;
+        wa.estype = c->prm.estype;
+        wa.selfcal = c->prm.selfcal;
+        wa.esmax = c->esmax;
+        wa.rigid = c->prm.slttype == 2;
+        wa.numslv = c->numslv;
+        wa.nslice = c->nslice;
+        wa.w_sys = c->w_sys;
+        wa.volume = c->fc.volume;
+        wa.screen = c->prm.screen;
+        wa.qtot = c->qtot;
+        wa.temp = c->prm.temp;
+        hipLaunchKernelGGL(k_weights, dim3(1), dim3(256), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
+                           c->d_resself.p, c->wslt_given ? c->d_wslt.p : (const float *)nullptr, c->d_state.p,
+                           c->d_flags.p, c->d_fstate.p, c->d_u0.p, c->d_w.p, c->d_escrd.p, c->d_eself.p, c->d_minmax.p,
+                           soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
+                           c->d_slnuv.p, c->d_errkey.p, c->nummol);
+        hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256), dim3(256), 0, c->stream, nch, n_off, c->ermax,
+                           c->d_C.p, c->d_w.p, c->d_edens.p);
+    }
+    if (c->prm.corrcal == 1) {
+        ProfScope ps(c, ER_K_ECORR);
+        if (c->use_mfma) {
+            const int nb = (c->ermax + 63) / 64;
+            hipLaunchKernelGGL(k_ecorr, dim3(nb, nb), dim3(256), 0, c->stream, nch, n_off, c->ermax, c->d_C.p,
+                               c->d_w.p, c->d_dmap.p, c->d_ecorr.p);
+        } else {
+            const int nb = (c->ermax + 15) / 16;
+            hipLaunchKernelGGL(k_ecorr_valu, dim3(nb, nb), dim3(16, 16), 0, c->stream, nch, n_off, c->ermax, c->d_C.p,
+                               c->d_w.p, c->d_ecorr.p);
+        }
+    }
+    return hipGetLastError();
+}
+
+extern "C" int er_run(er_ctx *c, int32_t mode)
+{
+    if (!c) return ER_E_ARG;
+    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_run: er_set_mesh first");
+    if (!c->have_frame) return fail(c, ER_E_STATE, "er_run: no frame set");
+    const bool soln = c->prm.slttype == 1;
+    int nins;
+    if (soln) {
+        if (c->nsltmol <= 0) return fail(c, ER_E_STATE, "er_run: er_set_solute_molecules first");
+        nins = c->nsltmol;
+    } else {
+        if (!c->have_ins) return fail(c, ER_E_STATE, "er_run: no insertions set");
+        nins = c->nins;
+    }
+    const int NI = (mode == ER_BATCHED) ? NI_B : 1;
+    const int chunk = (mode == ER_BATCHED) ? c->chunk : 1;
+    int rc = ensure_run_buffers(c, nins, chunk, NI);
+    if (rc) return rc;
+    {
+        ProfScope ps(c, ER_K_PREP);
+        HIPCHK(c, prep_frame(c));
+        const int B = 128;
+        hipLaunchKernelGGL(k_prep_ins, dim3((nins + B - 1) / B), dim3(B), 0, c->stream, nins, c->nslt,
+                           soln ? (const float *)nullptr : c->d_sltpos, c->d_sltq.p, c->d_sitepos, c->d_charge.p,
+                           c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3, c->ord,
+                           c->prm.screen, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_insref.p, c->d_selfreal.p,
+                           c->d_resself.p);
+        HIPCHK(c, hipGetLastError());
+        const size_t nC = (size_t)nins * c->ermax;
+        hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nC + 255) / 256)), dim3(256), 0, c->stream, nC, c->d_C.p);
+        HIPCHK(c, hipGetLastError());
+    }
+    if (mode == ER_BATCHED) {
+        for (int n0 = 0; n0 < nins; n0 += chunk) HIPCHK(c, run_chunk<NI_B>(c, n0, std::min(chunk, nins - n0)));
+        HIPCHK(c, run_accumulate(c, 0, nins));
+    } else {
+        // one insertion per launch sequence, synchronised like the reference's
+        // per-insertion host loop
+        for (int n0 = 0; n0 < nins; ++n0) {
+            HIPCHK(c, run_chunk<1>(c, n0, 1));
+            HIPCHK(c, run_accumulate(c, n0, 1));
+            HIPCHK(c, hipStreamSynchronize(c->stream));
+        }
+    }
+    c->last_NI = NI;
+    c->last_nins = nins;
+    return ER_OK;
+}
+
+static void decode_halt(er_ctx *c, unsigned long long key)
+{
+    const unsigned long long per = (unsigned long long)c->nummol + 1;
+    const int n = (int)(key / per);
+    const int j = (int)(key % per) - 1;
+    c->halt_code[0] = 0;
+    if (j < 0) {
+        float u = 0.f;
+        (void)hipMemcpy(&u, c->d_u0.p + n, sizeof(float), hipMemcpyDeviceToHost);
+        std::strcpy(c->halt_code, "eng_min");
+        c->halt_species = 0;
+        c->halt_energy = u;
+        return;
+    }
+    const int NI = c->last_NI;
+    float v = 0.f;
+    (void)hipMemcpy(&v, c->d_U.p + ((size_t)(n / NI) * c->nummol + j) * NI + (n % NI), sizeof(float),
+                    hipMemcpyDeviceToHost);
+    const int sp = c->uvspec[j];
+    c->halt_species = sp;
+    c->halt_energy = v;
+    int lo = 0, hi = c->uvmax[sp - 1];
+    const double *mesh = c->uvcrd.data() + c->idmin[sp - 1];
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((double)v > mesh[mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    std::strcpy(c->halt_code, lo <= 0 ? "eng_min" : "eng_sft");
+}
+
+extern "C" int er_sync(er_ctx *c)
+{
+    if (!c) return ER_E_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    unsigned long long key = ~0ull;
+    HIPCHK(c, hipMemcpy(&key, c->d_errkey.p, sizeof key, hipMemcpyDeviceToHost));
+    if (key != ~0ull) {
+        decode_halt(c, key);
+        return fail(c, ER_E_HALT, "halt condition %s (species %d, energy %g)", c->halt_code, c->halt_species,
+                    (double)c->halt_energy);
+    }
+    return ER_OK;
+}
+
+extern "C" int er_get_uvengy(er_ctx *c, float *uvengy, float *uvengy0)
+{
+    if (!c) return ER_E_ARG;
+    if (c->last_nins <= 0) return fail(c, ER_E_STATE, "er_get_uvengy: no run yet");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    const int nins = c->last_nins, NI = c->last_NI, nmol = c->nummol;
+    if (uvengy) {
+        const size_t ntile = (size_t)(nins + NI - 1) / NI;
+        std::vector<float> u(ntile * NI * nmol);
+        HIPCHK(c, hipMemcpy(u.data(), c->d_U.p, u.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int n = 0; n < nins; ++n)
+            for (int j = 0; j < nmol; ++j) uvengy[(size_t)n * nmol + j] = u[((size_t)(n / NI) * nmol + j) * NI + n % NI];
+    }
+    if (uvengy0) HIPCHK(c, hipMemcpy(uvengy0, c->d_u0.p, nins * sizeof(float), hipMemcpyDeviceToHost));
+    return ER_OK;
+}
+
+extern "C" int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, double *scal, float *minuv,
+                        float *maxuv, double *slnuv)
+{
+    if (!c) return ER_E_ARG;
+    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_fetch: no mesh");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (edens) HIPCHK(c, hipMemcpy(edens, c->d_edens.p, c->ermax * sizeof(double), hipMemcpyDeviceToHost));
+    if (ecorr)
+        HIPCHK(c, hipMemcpy(ecorr, c->d_ecorr.p, (size_t)c->ermax * c->ermax * sizeof(double), hipMemcpyDeviceToHost));
+    if (eself && c->esmax > 0) HIPCHK(c, hipMemcpy(eself, c->d_eself.p, c->esmax * sizeof(double), hipMemcpyDeviceToHost));
+    if (scal) HIPCHK(c, hipMemcpy(scal, c->d_state.p, 4 * sizeof(double), hipMemcpyDeviceToHost));
+    if (minuv || maxuv) {
+        std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
+        HIPCHK(c, hipMemcpy(mm.data(), c->d_minmax.p, mm.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
+        auto unord = [](unsigned int u) {
+            u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+            float f;
+            std::memcpy(&f, &u, 4);
+            return f;
+        };
+        for (int k = 0; k <= c->numslv; ++k) {
+            if (minuv) minuv[k] = unord(mm[2 * k]);
+            if (maxuv) maxuv[k] = unord(mm[2 * k + 1]);
+        }
+    }
+    if (slnuv) HIPCHK(c, hipMemcpy(slnuv, c->d_slnuv.p, c->numslv * sizeof(double), hipMemcpyDeviceToHost));
+    return ER_OK;
+}
+
+extern "C" int er_get_flceng(er_ctx *c, double *flceng)
+{
+    if (!c || !flceng) return ER_E_ARG;
+    if (c->prm.slttype != 1 || c->last_nins <= 0) return fail(c, ER_E_STATE, "er_get_flceng: no soln run");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(flceng, c->d_flceng.p, (size_t)c->last_nins * c->numslv * sizeof(double), hipMemcpyDeviceToHost));
+    return ER_OK;
+}
+
+extern "C" int er_clear(er_ctx *c)
+{
+    if (!c) return ER_E_ARG;
+    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_clear: no mesh");
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemsetAsync(c->d_edens.p, 0, c->ermax * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(c->d_ecorr.p, 0, (size_t)c->ermax * c->ermax * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(c->d_eself.p, 0, std::max(1, c->esmax) * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(c->d_slnuv.p, 0, c->numslv * sizeof(double), s));
+    // engnorm, engsmpl, avslf (voffset survives, engproc.F90:336-338)
+    HIPCHK(c, hipMemsetAsync(c->d_state.p, 0, 3 * sizeof(double), s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    return ER_OK;
+}
+
+extern "C" int er_last_error(er_ctx *c, char *code, int32_t *species, float *energy)
+{
+    if (!c) return ER_E_ARG;
+    if (code) std::memcpy(code, c->halt_code, 8);
+    if (species) *species = c->halt_species;
+    if (energy) *energy = c->halt_energy;
+    return ER_OK;
+}
+
+extern "C" const char *er_error_string(er_ctx *c)
+{
+    if (!c) return "null context";
+    return c->errmsg.c_str();
+}
+
+extern "C" int er_profile(er_ctx *c, int32_t enable)
+{
+    if (!c) return ER_E_ARG;
+    prof_collect(c);
+    c->prof = enable != 0;
+    return ER_OK;
+}
+
+extern "C" int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int64_t *launches)
+{
+    if (!c) return ER_E_ARG;
+    if (kernel < 0 || kernel >= ER_K_COUNT) return fail(c, ER_E_ARG, "er_profile_read: kernel id");
+    prof_collect(c);
+    if (total_ms) *total_ms = c->prof_ms[kernel];
+    if (launches) *launches = c->prof_n[kernel];
+    c->prof_ms[kernel] = 0.0;
+    c->prof_n[kernel] = 0;
+    return ER_OK;
+}
+
+extern "C" void er_destroy(er_ctx *c)
+{
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    prof_collect(c);
+    for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
+    c->prof_pool.clear();
+    c->d_charge.release(); c->d_ljene.release(); c->d_ljlensq.release(); c->d_molq.release();
+    c->d_ljtype.release(); c->d_molbeg.release(); c->d_spb.release(); c->d_spe.release();
+    c->d_sltq.release(); c->d_sltlj.release(); c->d_tag.release();
+    c->d_uvcrd.release(); c->d_escrd.release(); c->d_uvmax.release(); c->d_uvsoft.release(); c->d_idmin.release();
+    c->d_sitepos_own.release(); c->d_posn.release(); c->d_molref.release(); c->d_splw.release(); c->d_splg.release();
+    c->d_sltpos_own.release(); c->d_wslt.release(); c->d_insposn.release(); c->d_insref.release();
+    c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
+    c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
+    c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release();
+    c->d_selfpart.release(); c->d_T.release(); c->d_phi.release();
+    c->d_U.release(); c->d_u0.release(); c->d_C.release(); c->d_w.release(); c->d_flceng.release();
+    c->d_edens.release(); c->d_ecorr.release(); c->d_eself.release(); c->d_state.release(); c->d_slnuv.release();
+    c->d_flags.release(); c->d_warn.release(); c->d_fstate.release(); c->d_minmax.release(); c->d_errkey.release();
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}

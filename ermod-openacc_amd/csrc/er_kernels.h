@@ -1,0 +1,1098 @@
+// er_kernels.h -- gfx950 kernels of the erdst per-frame hot path.
+//
+// Data layout in HBM (one context, one frame in flight):
+//   posn   [numatm] float4 (x,y,z wrapped as realcal_prepare; w = charge)
+//   splw   [numatm][3][ORD] float    B-spline weights (recpcal.F90:504-538)
+//   splg   [numatm] int4             base grid index per axis
+//   T      [chunk][H][K2][K3] float2 spectrum after the axis-2/3 inverse DFT
+//   phi    [chunk/NI][K3][K2][K1][NI] float  solute potential grids, NI
+//          insertions interleaved so one wave reads 64 insertions of one
+//          grid point as four 64-byte runs
+//   U      [nins/NI][nummol][NI] float  uvengy, same interleave
+//   C      [nins][ermax] uint32      instantaneous histograms (insdst)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "er_fft.h"
+
+#define ER_PI_F 3.14159265358979323846f
+#define ER_MAXORD 12
+
+struct FrameConst {
+    float cellN[9];  // cell_normal, column-major (realcal.F90:24)
+    float invN[3];   // invcell_normal
+    float lenN[3];   // cell_len_normal
+    float invcl[9];  // inverse of the raw cell (engproc.F90:1049-1070)
+    int is_cuboid;
+    float volume;
+};
+
+struct PairParams {
+    float elecut, lwljcut, upljcut, screen;
+    int ljswitch;
+    // force-switch constants (realcal.F90:82-93, 899-915)
+    float lw3, up3, lw6, up6;
+    float repA, repB, repC, attA, attB, attC;
+};
+
+__device__ __forceinline__ int pmod(int a, int K)
+{
+    int r = a % K;
+    return r < 0 ? r + K : r;
+}
+
+// minimum image of a difference vector (realcal.F90:271-284)
+__device__ __forceinline__ void min_image(const FrameConst &fc, float &dx, float &dy, float &dz)
+{
+    if (fc.is_cuboid) {
+        float h0 = 0.5f * fc.lenN[0], h1 = 0.5f * fc.lenN[1], h2 = 0.5f * fc.lenN[2];
+        dx = h0 - fabsf(h0 - fabsf(dx));
+        dy = h1 - fabsf(h1 - fabsf(dy));
+        dz = h2 - fabsf(h2 - fabsf(dz));
+    } else {
+        float f = roundf(dz * fc.invN[2]);
+        dx -= fc.cellN[6] * f; dy -= fc.cellN[7] * f; dz -= fc.cellN[8] * f;
+        f = roundf(dy * fc.invN[1]);
+        dx -= fc.cellN[3] * f; dy -= fc.cellN[4] * f; dz -= fc.cellN[5] * f;
+        f = roundf(dx * fc.invN[0]);
+        dx -= fc.cellN[0] * f; dy -= fc.cellN[1] * f; dz -= fc.cellN[2] * f;
+    }
+}
+
+// wrap into the cell with Z, then Y, then X shifts (realcal.F90:885-895)
+__device__ __forceinline__ void wrap_site(const FrameConst &fc, float &x, float &y, float &z)
+{
+    float f = floorf(fc.invN[2] * z);
+    x -= fc.cellN[6] * f; y -= fc.cellN[7] * f; z -= fc.cellN[8] * f;
+    f = floorf(fc.invN[1] * y);
+    x -= fc.cellN[3] * f; y -= fc.cellN[4] * f; z -= fc.cellN[5] * f;
+    f = floorf(fc.invN[0] * x);
+    x -= fc.cellN[0] * f; y -= fc.cellN[1] * f; z -= fc.cellN[2] * f;
+}
+
+// B-spline weights w[c] = M_ord(t + c), c = 0..ord-1, by the Cox-de Boor
+// recursion M_k(x) = (x M_{k-1}(x) + (k-x) M_{k-1}(x-1)) / (k-1).  Same
+// values as the explicit sum of spline.F90:46-58 without its cancellation.
+__device__ __forceinline__ void bspline_weights(float t, int ord, float *w)
+{
+    w[0] = t;
+    w[1] = 1.0f - t;
+    for (int k = 3; k <= ord; ++k) {
+        const float inv = 1.0f / (float)(k - 1);
+        w[k - 1] = (1.0f - t) * w[k - 2] * inv;
+        for (int c = k - 2; c >= 1; --c) w[c] = ((t + (float)c) * w[c] + ((float)(k - c) - t) * w[c - 1]) * inv;
+        w[0] = t * w[0] * inv;
+    }
+}
+
+// fractional grid coordinate of one site (recpcal.F90:517-536)
+__device__ __forceinline__ void site_spline(const FrameConst &fc, float x, float y, float z, int K1, int K2,
+                                            int K3, int ord, float *w /*[3*ord]*/, int4 &g)
+{
+    const int Ks[3] = {K1, K2, K3};
+    int gg[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float f = fc.invcl[k] * x + fc.invcl[k + 3] * y + fc.invcl[k + 6] * z;
+        f = f - floorf(f);
+        const float factor = f * (float)Ks[k];
+        const int rci = (int)factor;
+        bspline_weights(factor - (float)rci, ord, w + k * ord);
+        gg[k] = rci;
+    }
+    g = make_int4(gg[0], gg[1], gg[2], 0);
+}
+
+// ---------------------------------------------------------------------------
+// per-frame preparation
+// ---------------------------------------------------------------------------
+
+// realcal_prepare (realcal.F90:30-48) + recpcal_prepare_solvent
+// (recpcal.F90:340-357) for every physical atom
+__global__ void k_prep_atoms(int natom, const float *__restrict__ pos, const float *__restrict__ charge,
+                             FrameConst fc, int K1, int K2, int K3, int ord, float4 *__restrict__ posn,
+                             float *__restrict__ splw, int4 *__restrict__ splg)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    const float x = pos[3 * (size_t)a], y = pos[3 * (size_t)a + 1], z = pos[3 * (size_t)a + 2];
+    float wx = x, wy = y, wz = z;
+    wrap_site(fc, wx, wy, wz);
+    posn[a] = make_float4(wx, wy, wz, charge[a]);
+    float w[3 * ER_MAXORD];
+    int4 g;
+    site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
+    for (int k = 0; k < 3 * ord; ++k) splw[(size_t)a * 3 * ord + k] = w[k];
+    splg[a] = g;
+}
+
+// reference site and reach radius of each molecule (for cutoff pruning)
+__global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const float4 *__restrict__ posn,
+                            FrameConst fc, float4 *__restrict__ molref, float *__restrict__ rmolmax)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nmol) return;
+    const int a0 = molbeg[j], a1 = molbeg[j + 1];
+    const float4 r = posn[a0];
+    float R = 0.0f;
+    for (int a = a0 + 1; a < a1; ++a) {
+        const float4 p = posn[a];
+        float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        min_image(fc, dx, dy, dz);
+        R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+    molref[j] = make_float4(r.x, r.y, r.z, R);
+    // frame maximum (non-negative floats order like their bit patterns)
+    atomicMax(reinterpret_cast<unsigned int *>(rmolmax), __float_as_uint(R));
+}
+
+// Insertion (refs) or solute molecule (soln) sites: wrapped position, spline,
+// reach radius and the real-space self term realcal_self[_refs]
+// (realcal.F90:669-771).  For soln the sites come from the frame.
+__global__ void k_prep_ins(int nins, int nslt, const float *__restrict__ sltpos, const float *__restrict__ sltq,
+                           const float *__restrict__ frame_pos, const float *__restrict__ charge,
+                           const int *__restrict__ molbeg, const int *__restrict__ tag, FrameConst fc,
+                           int K1, int K2, int K3, int ord, float screen, float4 *__restrict__ insposn,
+                           float *__restrict__ insw, int4 *__restrict__ insg, float4 *__restrict__ insref,
+                           float *__restrict__ selfreal, float *__restrict__ resself)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= nins) return;
+    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f);
+    float R = 0.0f;
+    for (int s = 0; s < nslt; ++s) {
+        float x, y, z, q;
+        if (tag) {
+            const size_t a = (size_t)molbeg[tag[n]] + s;
+            x = frame_pos[3 * a]; y = frame_pos[3 * a + 1]; z = frame_pos[3 * a + 2];
+            q = charge[a];
+        } else {
+            const size_t o = ((size_t)n * nslt + s) * 3;
+            x = sltpos[o]; y = sltpos[o + 1]; z = sltpos[o + 2];
+            q = sltq[s];
+        }
+        float w[3 * ER_MAXORD];
+        int4 g;
+        site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
+        const size_t is = (size_t)n * nslt + s;
+        for (int k = 0; k < 3 * ord; ++k) insw[is * 3 * ord + k] = w[k];
+        insg[is] = g;
+        wrap_site(fc, x, y, z);
+        insposn[is] = make_float4(x, y, z, q);
+        if (s == 0) c0 = make_float4(x, y, z, 0.f);
+        else {
+            float dx = x - c0.x, dy = y - c0.y, dz = z - c0.z;
+            min_image(fc, dx, dy, dz);
+            R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
+        }
+    }
+    insref[n] = make_float4(c0.x, c0.y, c0.z, R);
+    // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r
+    float e = 0.0f;
+    const float rsqpi = 0.564189583547756286948079451560772586f;  // 1/sqrt(pi)
+    for (int s = 0; s < nslt; ++s) {
+        const float4 a = insposn[(size_t)n * nslt + s];
+        e += -a.w * a.w * screen * rsqpi;
+        for (int t = s + 1; t < nslt; ++t) {
+            const float4 b = insposn[(size_t)n * nslt + t];
+            float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+            min_image(fc, dx, dy, dz);
+            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
+            e += -a.w * b.w * erff(screen * r) / r;
+        }
+    }
+    selfreal[n] = e;
+    // residual_self_ene (engproc.F90:967-977): -(pi Q^2 / alpha^2 / V) / 2 with
+    // Q = mol_charge, a plain sum of the site charges (setconf.F90:838-840)
+    float Q = 0.0f;
+    for (int s = 0; s < nslt; ++s) Q += insposn[(size_t)n * nslt + s].w;
+    float epcl = ER_PI_F * Q * Q / screen / screen / fc.volume;
+    epcl = epcl / 2.0f;
+    resself[n] = 0.0f - epcl;
+}
+
+// SPME influence function engfac (recpcal.F90:121-163), stored [m1][m3][m2]
+__global__ void k_engfac(int K1, int K2, int K3, int ord, const double *__restrict__ s1,
+                         const double *__restrict__ s2, const double *__restrict__ s3, double i00,
+                         double i10, double i20, double i01, double i11, double i21, double i02, double i12,
+                         double i22, double volume, double screen, float *__restrict__ engfac)
+{
+    const int H = K1 / 2 + 1;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)H * K2 * K3) return;
+    const int m2 = idx % K2, m3 = (idx / K2) % K3, m1 = idx / ((size_t)K2 * K3);
+    double f = 0.0;
+    const int rc[3] = {m1, m2, m3}, Km[3] = {K1, K2, K3};
+    bool skip = (m1 == 0 && m2 == 0 && m3 == 0);
+    double inm[3];
+    for (int m = 0; m < 3; ++m) {
+        if ((ord % 2 == 1) && (2 * rc[m] == Km[m])) skip = true;
+        inm[m] = (rc[m] <= Km[m] / 2) ? (double)rc[m] : (double)(rc[m] - Km[m]);
+    }
+    if (!skip) {
+        // xs(m) = dot_product(invcl(:, m), inm)
+        const double x0 = i00 * inm[0] + i10 * inm[1] + i20 * inm[2];
+        const double x1 = i01 * inm[0] + i11 * inm[1] + i21 * inm[2];
+        const double x2 = i02 * inm[0] + i12 * inm[1] + i22 * inm[2];
+        const double r2 = x0 * x0 + x1 * x1 + x2 * x2;
+        const double pi = 3.14159265358979323846;
+        f = exp(-pi * pi * r2 / (screen * screen)) / r2 / pi / volume;
+        f = f / (s1[m1] * s2[m2] * s3[m3]);
+    }
+    engfac[idx] = (float)f;
+}
+
+// ---------------------------------------------------------------------------
+// PME, stage A: per (insertion, m1) plane.  The solute's structure factor is
+// generated directly from its separable B-spline factors,
+//   rho^(m) = sum_s q_s F1_s(m1) F2_s(m2) F3_s(m3),
+//   Fd_s(m) = sum_c W_s[d][c] exp(-2 pi i m (g_d - c)/K_d),
+// which equals the R2C transform of the spread grid (recpcal.F90:453-475).
+// The FP64 half-plane self energy (recpcal.F90:482-491) and the engfac
+// multiply (recpcal.F90:493-497) are fused, then the inverse DFT runs along
+// axis 3 (one row per thread) and axis 2 (one column per thread) in LDS.
+// ---------------------------------------------------------------------------
+#define ER_SCH 8  // solute sites per LDS table chunk
+
+template <int K2, int K3>
+__global__ void __launch_bounds__(128) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
+                                                    const float4 *__restrict__ insposn,
+                                                    const float *__restrict__ insw, const int4 *__restrict__ insg,
+                                                    const float *__restrict__ engfac, const float2 *__restrict__ tw1,
+                                                    const float2 *__restrict__ tw2, const float2 *__restrict__ tw3,
+                                                    float2 *__restrict__ T, double *__restrict__ selfpart)
+{
+    constexpr int PS = K3 + 1;  // padded plane row stride
+    extern __shared__ __align__(16) unsigned char smem[];
+    float2 *plane = reinterpret_cast<float2 *>(smem);   // [K2][PS]
+    float2 *F2 = plane + K2 * PS;                        // [SCH][K2]
+    float2 *F3 = F2 + ER_SCH * K2;                       // [SCH][K3]
+    float2 *A1 = F3 + ER_SCH * K3;                       // [SCH]
+    double *red = reinterpret_cast<double *>(A1 + ER_SCH);  // [blockDim/64]
+
+    const int H = K1 / 2 + 1;
+    const int m1 = blockIdx.x;
+    const int nl = blockIdx.y;
+    const int n = n_off + nl;
+    const int tid = threadIdx.x, NT = blockDim.x;
+
+    float2 rho[K3];
+#pragma unroll
+    for (int i = 0; i < K3; ++i) rho[i] = make_float2(0.f, 0.f);
+
+    for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
+        const int ns = min(ER_SCH, nslt - s0);
+        const int per = K2 + K3 + 1;
+        for (int f = tid; f < ns * per; f += NT) {
+            const int s = f / per, r = f % per;
+            const size_t site = (size_t)n * nslt + s0 + s;
+            const int4 g = insg[site];
+            int axis, m, K, gd;
+            const float2 *tw;
+            if (r < K2) { axis = 1; m = r; K = K2; gd = g.y; tw = tw2; }
+            else if (r < K2 + K3) { axis = 2; m = r - K2; K = K3; gd = g.z; tw = tw3; }
+            else { axis = 0; m = m1; K = K1; gd = g.x; tw = tw1; }
+            const float *w = insw + site * 3 * ord + axis * ord;
+            float2 acc = make_float2(0.f, 0.f);
+            for (int c = 0; c < ord; ++c) {
+                const int e = (int)(((long long)m * pmod(gd - c, K)) % K);
+                const float2 t = tw[e];  // exp(+2 pi i e/K); forward uses the conjugate
+                acc.x += w[c] * t.x;
+                acc.y -= w[c] * t.y;
+            }
+            if (r < K2) F2[s * K2 + m] = acc;
+            else if (r < K2 + K3) F3[s * K3 + m] = acc;
+            else A1[s] = erfft::cscale(acc, insposn[site].w);
+        }
+        __syncthreads();
+        if (tid < K2) {
+            for (int s = 0; s < ns; ++s) {
+                const float2 B = erfft::cmul(A1[s], F2[s * K2 + tid]);
+#pragma unroll
+                for (int m3 = 0; m3 < K3; ++m3) {
+                    const float2 f3 = F3[s * K3 + m3];
+                    rho[m3].x += B.x * f3.x - B.y * f3.y;
+                    rho[m3].y += B.x * f3.y + B.y * f3.x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    double sp = 0.0;
+    if (tid < K2) {
+        const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
+        const float *ef = engfac + (size_t)m1 * K3 * K2 + tid;
+#pragma unroll
+        for (int m3 = 0; m3 < K3; ++m3) {
+            const float e = ef[(size_t)m3 * K2];
+            sp += wt * (double)e * ((double)rho[m3].x * rho[m3].x + (double)rho[m3].y * rho[m3].y);
+            rho[m3] = erfft::cscale(rho[m3], e);
+        }
+        erfft::fft<K3, +1>(rho, tw3);
+#pragma unroll
+        for (int x3 = 0; x3 < K3; ++x3) plane[tid * PS + x3] = rho[x3];
+    }
+    __syncthreads();
+    if (tid < K3) {
+        float2 col[K2];
+#pragma unroll
+        for (int m2 = 0; m2 < K2; ++m2) col[m2] = plane[m2 * PS + tid];
+        erfft::fft<K2, +1>(col, tw2);
+#pragma unroll
+        for (int x2 = 0; x2 < K2; ++x2) plane[x2 * PS + tid] = col[x2];
+    }
+    __syncthreads();
+    float2 *out = T + ((size_t)nl * H + m1) * (size_t)(K2 * K3);
+    for (int f = tid; f < K2 * K3; f += NT) out[f] = plane[(f / K3) * PS + (f % K3)];
+
+    // deterministic block reduction of the self-energy partial
+    for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = sp;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv];
+        selfpart[(size_t)n * H + m1] = s;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PME, stage B: complex-to-real inverse DFT along axis 1 for NI insertions x
+// R lines, via a half-length complex transform:
+//   Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/K1} (X[k] - conj X[M-k]),
+//   z = IDFT_M(Z),  phi[2j] = Re z[j],  phi[2j+1] = Im z[j],  M = K1/2.
+// Output is written with the NI insertions interleaved.
+// ---------------------------------------------------------------------------
+template <int K1, int NI>
+__global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, int R,
+                                                   const float2 *__restrict__ T, float *__restrict__ phi,
+                                                   const float2 *__restrict__ tw1, const float2 *__restrict__ twm)
+{
+    constexpr int M = K1 / 2;
+    constexpr int H = M + 1;
+    constexpr int LS = (H % 2 == 1) ? H : H + 1;  // odd stride in float2
+    extern __shared__ __align__(16) unsigned char smem[];
+    float2 *buf = reinterpret_cast<float2 *>(smem);  // [R*NI][LS]
+    float *obuf = reinterpret_cast<float *>(smem);   // [R][K1][NI] (reuses buf)
+
+    const int tile = blockIdx.z, x2 = blockIdx.y, x3_0 = blockIdx.x * R;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const size_t plane = (size_t)K2 * K3;
+
+    for (int f = tid; f < NI * H * R; f += NT) {
+        const int r = f % R, m1 = (f / R) % H, i = f / (R * H);
+        const int nl = tile * NI + i;
+        float2 v = make_float2(0.f, 0.f);
+        if (nl < nchunk) v = T[((size_t)nl * H + m1) * plane + (size_t)x2 * K3 + x3_0 + r];
+        buf[(r * NI + i) * LS + m1] = v;
+    }
+    __syncthreads();
+    float2 z[M];
+    const int line = tid;  // = r*NI + i
+    if (line < R * NI) {
+        const float2 *X = buf + line * LS;
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float2 a = X[k], b = X[M - k];
+            const float2 e = make_float2(a.x + b.x, a.y - b.y);            // X[k] + conj X[M-k]
+            const float2 d = make_float2(a.x - b.x, a.y + b.y);            // X[k] - conj X[M-k]
+            const float2 o = (k == 0) ? d : erfft::cmul(d, tw1[k]);
+            z[k] = make_float2(e.x - o.y, e.y + o.x);                     // e + i o
+        }
+        erfft::fft<M, +1>(z, twm);
+    }
+    __syncthreads();
+    if (line < R * NI) {
+        const int r = line / NI, i = line % NI;
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            obuf[(r * K1 + 2 * j) * NI + i] = z[j].x;
+            obuf[(r * K1 + 2 * j + 1) * NI + i] = z[j].y;
+        }
+    }
+    __syncthreads();
+    const size_t G = plane * K1;
+    float *dst = phi + (size_t)tile * G * NI;
+    for (int f = tid; f < R * K1 * NI; f += NT) {
+        const int r = f / (K1 * NI), rem = f % (K1 * NI);
+        dst[(((size_t)(x3_0 + r) * K2 + x2) * K1) * NI + rem] = obuf[f];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Spatial order of the molecules (per frame): counting sort of the molecule
+// reference sites into NC^3 cells keyed by the Morton code of the cell, so
+// that consecutive molecules are neighbours in space.  The gather walks the
+// molecules in this order (neighbouring stencils hit the same L2 lines) and
+// the pair search uses the cells as a cell list.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned int morton_spread(unsigned int v)  // 10 bits -> every third bit
+{
+    v &= 0x3ffu;
+    v = (v | (v << 16)) & 0x030000ffu;
+    v = (v | (v << 8)) & 0x0300f00fu;
+    v = (v | (v << 4)) & 0x030c30c3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+__device__ __forceinline__ unsigned int cell_key(int cx, int cy, int cz)
+{
+    return morton_spread(cx) | (morton_spread(cy) << 1) | (morton_spread(cz) << 2);
+}
+// fractional position of a wrapped site -> cell coordinate in [0, nc)
+__device__ __forceinline__ void site_cell(const FrameConst &fc, float x, float y, float z, int nc, int &cx, int &cy,
+                                          int &cz)
+{
+    float u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        float f = fc.invcl[k] * x + fc.invcl[k + 3] * y + fc.invcl[k + 6] * z;
+        u[k] = f - floorf(f);
+    }
+    cx = min(nc - 1, (int)(u[0] * nc));
+    cy = min(nc - 1, (int)(u[1] * nc));
+    cz = min(nc - 1, (int)(u[2] * nc));
+}
+
+__global__ void k_cell_count(int nmol, const float4 *__restrict__ molref, FrameConst fc, int nc,
+                             unsigned int *__restrict__ cnt, unsigned int *__restrict__ key_rank)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nmol) return;
+    const float4 m = molref[j];
+    int cx, cy, cz;
+    site_cell(fc, m.x, m.y, m.z, nc, cx, cy, cz);
+    const unsigned int key = cell_key(cx, cy, cz);
+    const unsigned int rank = atomicAdd(&cnt[key], 1u);
+    key_rank[2 * j] = key;
+    key_rank[2 * j + 1] = rank;
+}
+
+// exclusive scan of ncell counts in one block -> start[ncell + 1]
+__global__ void __launch_bounds__(1024) k_cell_scan(int ncell, const unsigned int *__restrict__ cnt,
+                                                    unsigned int *__restrict__ start)
+{
+    __shared__ unsigned int part[1024];
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int per = (ncell + NT - 1) / NT;
+    const int b0 = tid * per, b1 = min(ncell, b0 + per);
+    unsigned int s = 0;
+    for (int b = b0; b < b1; ++b) s += cnt[b];
+    part[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < NT; off <<= 1) {
+        unsigned int v = (tid >= off) ? part[tid - off] : 0u;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    unsigned int run = part[tid] - s;
+    for (int b = b0; b < b1; ++b) {
+        start[b] = run;
+        run += cnt[b];
+    }
+    if (tid == NT - 1) start[ncell] = part[NT - 1];
+}
+
+__global__ void k_cell_scatter(int nmol, const unsigned int *__restrict__ key_rank,
+                               const unsigned int *__restrict__ start, int *__restrict__ perm)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nmol) return;
+    perm[start[key_rank[2 * j]] + key_rank[2 * j + 1]] = j;
+}
+
+// ---------------------------------------------------------------------------
+// PME gather (recpcal.F90:651-710 / 592-649) + residual background term
+// (engproc.F90:979-1020).  A wave holds MPW = 64/LPM molecules x LPM
+// insertions; with the NI-interleaved grid a load instruction reads MPW runs
+// of LPM*4 bytes.  Molecules are visited in the spatial order perm[], and the
+// workgroups are remapped so that each of the 8 XCDs sweeps one contiguous
+// eighth of that order tile after tile (its L2 then holds one neighbourhood
+// of one tile's grids).  Writes U.
+// ---------------------------------------------------------------------------
+template <int ORD, int NI, int LPM>
+__global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off, int K1, int K2, int K3,
+                                                const int *__restrict__ perm, int ngrp_x, int nxcd,
+                                                const int *__restrict__ molbeg, const float4 *__restrict__ posn,
+                                                const float *__restrict__ splw, const int4 *__restrict__ splg,
+                                                const float *__restrict__ molq, const float *__restrict__ phi,
+                                                const int *__restrict__ tag, float qslt_refs, float screen,
+                                                float volume, float *__restrict__ U)
+{
+    constexpr int MPW = 64 / LPM;
+    // XCD-aware remap: workgroup b runs on XCD b % nxcd; give each XCD a
+    // contiguous range of molecule groups (ngrp_x is a multiple of nxcd)
+    const int b = blockIdx.x;
+    const int per = ngrp_x / nxcd;
+    const int xcd = b % nxcd, k = b / nxcd;
+    const int gx = xcd * per + (k % per);
+    const int gy = k / per;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int pos = (gx * (blockDim.x >> 6) + wave) * MPW + lane / LPM;
+    const int nl = gy * LPM + lane % LPM;
+    if (pos >= nmol || nl >= nchunk) return;
+    const int mol = perm[pos];
+    const int tile = nl / NI, i = nl % NI;
+    const size_t G = (size_t)K1 * K2 * K3;
+    const float *ph = phi + (size_t)tile * G * NI + i;
+    float acc = 0.0f;
+    const int a0 = molbeg[mol], a1 = molbeg[mol + 1];
+    for (int a = a0; a < a1; ++a) {
+        const float q = posn[a].w;
+        const float *w = splw + (size_t)a * 3 * ORD;
+        const int4 g = splg[a];
+        int i1[ORD], i2[ORD], i3[ORD];
+        float w1[ORD], w2[ORD], w3[ORD];
+#pragma unroll
+        for (int c = 0; c < ORD; ++c) {
+            i1[c] = pmod(g.x - c, K1);
+            i2[c] = pmod(g.y - c, K2);
+            i3[c] = pmod(g.z - c, K3);
+            w1[c] = w[c];
+            w2[c] = w[ORD + c];
+            w3[c] = w[2 * ORD + c];
+        }
+        float ea = 0.0f;
+#pragma unroll
+        for (int c3 = 0; c3 < ORD; ++c3) {
+#pragma unroll
+            for (int c2 = 0; c2 < ORD; ++c2) {
+                const size_t row = ((size_t)i3[c3] * K2 + i2[c2]) * K1;
+                float s = 0.0f;
+#pragma unroll
+                for (int c1 = 0; c1 < ORD; ++c1) s += w1[c1] * ph[(row + i1[c1]) * NI];
+                ea += (w3[c3] * w2[c2]) * s;
+            }
+        }
+        acc += q * ea;
+    }
+    const int n = n_off + nl;
+    float qs = qslt_refs;
+    if (tag) {
+        if (mol == tag[n]) acc = 0.0f;
+        qs = molq[tag[n]];
+    }
+    if (!tag || mol != tag[n]) acc -= ER_PI_F * qs * molq[mol] / screen / screen / volume;
+    const int gt = n / NI;  // n_off is a multiple of NI
+    U[((size_t)gt * nmol + mol) * NI + (n % NI)] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Real-space LJ + erfc-Coulomb pairs (realcal.F90:207-364 / 51-205) with
+// molecule-level pruning: a molecule whose reference site is further than
+// cutoff + R_solute + R_molecule from the solute's reference site has every
+// site pair beyond both cutoffs (triangle inequality on minimum images), so
+// the per-pair cutoff semantics are kept exactly.  Candidates come from the
+// cells within reach; each (insertion, molecule) sum is owned by one thread
+// and added once to U (deterministic).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float pair_energy(const PairParams &p, float dis2, float qa, float qb, float ljeps,
+                                             float ljsgm2)
+{
+    const float rst = sqrtf(dis2);
+    float eplj = 0.0f, epcl = 0.0f;
+    if (!(rst > p.upljcut)) {
+        const float invr2 = ljsgm2 / dis2;
+        const float invr6 = invr2 * invr2 * invr2;
+        switch (p.ljswitch) {
+        case 0:
+        case 1:
+            eplj = 4.0f * ljeps * invr6 * (invr6 - 1.0f);
+            if (rst > p.lwljcut) {
+                float swth;
+                if (p.ljswitch == 0) {
+                    const float lw2 = p.lwljcut * p.lwljcut, up2 = p.upljcut * p.upljcut;
+                    const float t = dis2 - up2, den = up2 - lw2;
+                    swth = (2.0f * dis2 + up2 - 3.0f * lw2) * (t * t) / (den * den * den);
+                } else {
+                    const float s = (rst - p.lwljcut) / (p.upljcut - p.lwljcut);
+                    swth = 1.0f - 10.0f * s * s * s + 15.0f * s * s * s * s - 6.0f * s * s * s * s * s;
+                }
+                eplj = swth * eplj;
+            }
+            break;
+        case 2: {
+            const float sg6 = ljsgm2 * ljsgm2 * ljsgm2;
+            float vdwa, vdwb;
+            if (rst <= p.lwljcut) {
+                vdwa = invr6 * invr6 - sg6 * sg6 / (p.lw6 * p.up6);
+                vdwb = invr6 - sg6 / (p.lw3 * p.up3);
+            } else {
+                const float invr3 = sqrtf(invr6), sg3 = sqrtf(sg6);
+                const float ta = invr6 - sg6 / p.up6, tb = invr3 - sg3 / p.up3;
+                vdwa = p.up6 / (p.up6 - p.lw6) * (ta * ta);
+                vdwb = p.up3 / (p.up3 - p.lw3) * (tb * tb);
+            }
+            eplj = 4.0f * ljeps * (vdwa - vdwb);
+            break;
+        }
+        default: {
+            const float sg6 = ljsgm2 * ljsgm2 * ljsgm2;
+            float vdwa, vdwb;
+            if (rst <= p.lwljcut) {
+                vdwa = invr6 * invr6 - sg6 * sg6 * p.repC;
+                vdwb = invr6 - sg6 * p.attC;
+            } else {
+                const float s = rst - p.lwljcut, s3 = s * s * s, s4 = s3 * s;
+                vdwa = invr6 * invr6 - sg6 * sg6 * (p.repA * s3 + p.repB * s4 + p.repC);
+                vdwb = invr6 - sg6 * (p.attA * s3 + p.attB * s4 + p.attC);
+            }
+            eplj = 4.0f * ljeps * (vdwa - vdwb);
+        }
+        }
+    }
+    if (!(rst >= p.elecut)) epcl = qa * qb * erfcf(p.screen * rst) / rst;
+    return eplj + epcl;
+}
+
+struct PairArgs {
+    int nmol, nchunk, n_off, nslt, NI, ntype, nc;
+    float cell_w;  // smallest perpendicular cell width
+    PairParams pp;
+};
+
+__device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const FrameConst &fc, int n, int jm,
+                                                   const int *__restrict__ molbeg, const float4 *__restrict__ posn,
+                                                   const int *__restrict__ ljtype,
+                                                   const float4 *__restrict__ insposn, const int *__restrict__ sltlj,
+                                                   int slt_a0, bool soln, const float *__restrict__ ljene,
+                                                   const float *__restrict__ ljlensq)
+{
+    float e = 0.0f;
+    for (int s = 0; s < pa.nslt; ++s) {
+        const float4 xa = insposn[(size_t)n * pa.nslt + s];
+        const int ti = soln ? ljtype[slt_a0 + s] : sltlj[s];
+        for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
+            const float4 xb = posn[b];
+            float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
+            min_image(fc, dx, dy, dz);
+            const int tj = ljtype[b];
+            e += pair_energy(pa.pp, dx * dx + dy * dy + dz * dz, xa.w, xb.w, ljene[ti + pa.ntype * tj],
+                             ljlensq[ti + pa.ntype * tj]);
+        }
+    }
+    return e;
+}
+
+#define ER_PAIR_CAP 2048
+
+__global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const int *__restrict__ molbeg,
+                                               const float4 *__restrict__ posn, const int *__restrict__ ljtype,
+                                               const float4 *__restrict__ molref,
+                                               const float *__restrict__ rmolmax,
+                                               const unsigned int *__restrict__ cstart,
+                                               const int *__restrict__ perm, const float4 *__restrict__ insposn,
+                                               const float4 *__restrict__ insref, const int *__restrict__ sltlj,
+                                               const int *__restrict__ tag, const float *__restrict__ ljene,
+                                               const float *__restrict__ ljlensq, float *__restrict__ U)
+{
+    __shared__ int cand[ER_PAIR_CAP];
+    __shared__ int ncand;
+    const int nl = blockIdx.x;
+    const int n = pa.n_off + nl;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const float4 ref = insref[n];
+    const float cut = fmaxf(pa.pp.upljcut, pa.pp.elecut) + ref.w + 1e-3f;
+    const bool soln = tag != nullptr;
+    const int skipmol = soln ? tag[n] : -1;
+    const int slt_a0 = soln ? molbeg[skipmol] : 0;
+    const size_t urow = ((size_t)(n / pa.NI) * pa.nmol) * pa.NI + (n % pa.NI);
+    if (tid == 0) ncand = 0;
+    __syncthreads();
+    int cx, cy, cz;
+    site_cell(fc, ref.x, ref.y, ref.z, pa.nc, cx, cy, cz);
+    // cells within reach of the reference site: a site in cell c lies within
+    // one cell width of the cell's boundary, so ceil(reach / width) + 1 cells
+    const int nreach = (int)ceilf((cut + rmolmax[0]) / pa.cell_w) + 1;
+    const int span = min(2 * nreach + 1, pa.nc);  // distinct cells per axis
+    const int lo = (span == pa.nc) ? 0 : -nreach;
+    const int ncell = span * span * span;
+    for (int c = tid; c < ncell; c += NT) {
+        const int dx = c % span, dy = (c / span) % span, dz = c / (span * span);
+        const int qx = pmod(cx + lo + dx, pa.nc), qy = pmod(cy + lo + dy, pa.nc), qz = pmod(cz + lo + dz, pa.nc);
+        const unsigned int key = cell_key(qx, qy, qz);
+        for (unsigned int p = cstart[key]; p < cstart[key + 1]; ++p) {
+            const int j = perm[p];
+            if (j == skipmol) continue;
+            const float4 m = molref[j];
+            float ddx = m.x - ref.x, ddy = m.y - ref.y, ddz = m.z - ref.z;
+            min_image(fc, ddx, ddy, ddz);
+            const float lim = cut + m.w;
+            if (ddx * ddx + ddy * ddy + ddz * ddz <= lim * lim) {
+                const int slot = atomicAdd(&ncand, 1);
+                if (slot < ER_PAIR_CAP) cand[slot] = j;
+                else  // list full: evaluate in place (same arithmetic, same single owner)
+                    U[urow + (size_t)j * pa.NI] += molecule_pair_sum(pa, fc, n, j, molbeg, posn, ljtype, insposn,
+                                                                     sltlj, slt_a0, soln, ljene, ljlensq);
+            }
+        }
+    }
+    __syncthreads();
+    const int nc = min(ncand, ER_PAIR_CAP);
+    for (int c = tid; c < nc; c += NT) {
+        const int jm = cand[c];
+        U[urow + (size_t)jm * pa.NI] += molecule_pair_sum(pa, fc, n, jm, molbeg, posn, ljtype, insposn, sltlj,
+                                                          slt_a0, soln, ljene, ljlensq);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Energy-coordinate binning (update_histogram's molecule loop,
+// engproc.F90:917-937, with getiduv/binsearch engproc.F90:1078-1171):
+// bin = #{mesh points strictly below v} in the species' mesh.  The mesh is
+// compared in FP64 in the reference; for an FP32 v, v > c holds exactly when
+// v > RD32(c) (c rounded toward -inf to FP32), so the thresholds are held as
+// such floats.  Grid: (NI-insertion tile) x (molecule slice); LDS
+// histograms, merged into the counts with integer atomics (exact).
+// Per-species min/max via order-preserving integer atomics, per-species
+// energy sums per slice (reduced in a fixed order later).  The earliest
+// (insertion, molecule) that hits a halt condition is kept as a key.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned int f2ord(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+struct HistArgs {
+    int nmol, nchunk, n_off, ermax, numslv, slttype, nslice;
+};
+
+template <int NI>
+__global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restrict__ U, const int *__restrict__ tag,
+                                              const float *__restrict__ meshf, const int *__restrict__ sp_begin,
+                                              const int *__restrict__ sp_end, const int *__restrict__ idmin,
+                                              const int *__restrict__ uvmax, const int *__restrict__ uvsoft,
+                                              uint32_t *__restrict__ Cnt, unsigned int *__restrict__ minmax,
+                                              double *__restrict__ flpart,
+                                              unsigned long long *__restrict__ errkey, int *__restrict__ warn)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *rsum = reinterpret_cast<double *>(smem);                 // [NT]
+    float *mesh = reinterpret_cast<float *>(rsum + blockDim.x);      // [ermax]
+    uint32_t *hist = reinterpret_cast<uint32_t *>(mesh + ha.ermax);  // [NI][ermax]
+    float *rmin = reinterpret_cast<float *>(hist + NI * ha.ermax);   // [NT]
+    float *rmax = rmin + blockDim.x;                                  // [NT]
+
+    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tile = blockIdx.x, slice = blockIdx.y;
+    for (int b = tid; b < ha.ermax; b += NT) mesh[b] = meshf[b];
+    for (int b = tid; b < NI * ha.ermax; b += NT) hist[b] = 0u;
+    __syncthreads();
+
+    const int i = tid % NI, jj = tid / NI, jstep = NT / NI;
+    const int nl = tile * NI + i;
+    const bool valid = nl < ha.nchunk;
+    const int n = ha.n_off + nl;
+    const int skip = (tag && valid) ? tag[n] : -1;
+    const size_t gt = (size_t)(ha.n_off / NI + tile);
+    for (int sp = 1; sp <= ha.numslv; ++sp) {
+        float lmin = 3.402823466e38f, lmax = -3.402823466e38f;
+        double lsum = 0.0;
+        // this block's slice of the species' molecules
+        const int s0 = sp_begin[sp - 1], s1 = sp_end[sp - 1];
+        const int len = (s1 - s0 + ha.nslice - 1) / ha.nslice;
+        const int j0 = s0 + slice * len, j1 = min(s1, j0 + len);
+        const int base = idmin[sp - 1], num = uvmax[sp - 1], soft = uvsoft[sp - 1];
+        if (valid) {
+            for (int j = j0 + jj; j < j1; j += jstep) {
+                if (j == skip) continue;
+                const float v = U[(gt * ha.nmol + j) * NI + i];
+                int lo = 0, hi = num;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (v > mesh[base + mid]) lo = mid + 1;
+                    else hi = mid;
+                }
+                const int idpti = lo;
+                if (idpti <= 0 || (ha.slttype == 1 && idpti > soft)) {
+                    const unsigned long long key =
+                        (unsigned long long)n * (unsigned long long)(ha.nmol + 1) + j + 1;
+                    atomicMin(errkey, key);
+                }
+                if (idpti == num && v < 1.0e3f) atomicOr(warn, 1);
+                atomicAdd(&hist[i * ha.ermax + base + max(idpti, 1) - 1], 1u);
+                lmin = fminf(lmin, v);
+                lmax = fmaxf(lmax, v);
+                lsum += (double)v;
+            }
+        }
+        rmin[tid] = lmin;
+        rmax[tid] = lmax;
+        rsum[tid] = lsum;
+        __syncthreads();
+        if (tid < NI) {
+            float mn = rmin[tid], mx = rmax[tid];
+            double sm = rsum[tid];
+            for (int k = 1; k < jstep; ++k) {
+                mn = fminf(mn, rmin[tid + k * NI]);
+                mx = fmaxf(mx, rmax[tid + k * NI]);
+                sm += rsum[tid + k * NI];
+            }
+            if (tile * NI + tid < ha.nchunk) {
+                if (mn <= mx) {
+                    atomicMin(&minmax[2 * sp], f2ord(mn));
+                    atomicMax(&minmax[2 * sp + 1], f2ord(mx));
+                }
+                if (flpart)
+                    flpart[((size_t)(ha.n_off + tile * NI + tid) * ha.nslice + slice) * ha.numslv + sp - 1] = sm;
+            }
+        }
+        __syncthreads();
+    }
+    for (int f = tid; f < NI * ha.ermax; f += NT) {
+        const int ii = f / ha.ermax, b = f % ha.ermax;
+        const uint32_t h = hist[f];
+        if (h && tile * NI + ii < ha.nchunk) atomicAdd(&Cnt[(size_t)(ha.n_off + tile * NI + ii) * ha.ermax + b], h);
+    }
+}
+
+__global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Weights (update_histogram engproc.F90:883-915 + volcorrect :1022-1038),
+// uvengy0 (get_uv_energy_refs :828-855) and the scalar accumulators, for
+// every insertion of a frame.
+// state: [0] engnorm [1] engsmpl [2] avslf [3] voffset (double)
+// flags: [0] voffset_initialized [1] usreal_set; usreal in fstate[0]
+// ---------------------------------------------------------------------------
+struct WeightArgs {
+    int nchunk, n_off, H, slttype, wgtslf, estype, selfcal, esmax, rigid, numslv, nslice;
+    float w_sys, volume, screen, qtot;
+    double temp;
+};
+
+__global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__restrict__ selfpart,
+                                                 const float *__restrict__ selfreal,
+                                                 const float *__restrict__ resself, const float *__restrict__ wslt,
+                                                 double *__restrict__ state, int *__restrict__ flags,
+                                                 float *__restrict__ fstate, float *__restrict__ u0,
+                                                 double *__restrict__ wout, const double *__restrict__ escrd,
+                                                 double *__restrict__ eself, unsigned int *__restrict__ minmax,
+                                                 const double *__restrict__ flpart, double *__restrict__ flceng,
+                                                 double *__restrict__ slnuv,
+                                                 unsigned long long *__restrict__ errkey, int nmol)
+{
+    __shared__ double r_w[256], r_t[256];
+    const int tid = threadIdx.x, NT = blockDim.x;
+    auto uv0_of = [&](int n, float real) -> float {
+        double rec = 0.0;
+        for (int m = 0; m < wa.H; ++m) rec += selfpart[(size_t)n * wa.H + m];
+        // uvrecp + pairep + residual in the reference's real arithmetic
+        return (float)rec + real + resself[n];
+    };
+    if (tid == 0) {
+        if (wa.rigid && !flags[1]) {
+            fstate[0] = selfreal[wa.n_off];
+            flags[1] = 1;
+        }
+        if (wa.wgtslf && !flags[0]) {
+            const float real0 = wa.rigid ? fstate[0] : selfreal[wa.n_off];
+            state[3] = (double)uv0_of(wa.n_off, real0);
+            flags[0] = 1;
+        }
+    }
+    __syncthreads();
+    const double voffset = state[3];
+    const float usreal = fstate[0];
+    double sw = 0.0, st = 0.0;
+    for (int nl = tid; nl < wa.nchunk; nl += NT) {
+        const int n = wa.n_off + nl;
+        const float real = wa.rigid ? usreal : selfreal[n];
+        const float u = uv0_of(n, real);
+        u0[n] = u;
+        double engnmfc = 1.0;
+        if (wa.wgtslf) {
+            const float factor = (float)((double)u - voffset);
+            engnmfc = (wa.slttype == 1) ? exp((double)factor / wa.temp) : exp(-(double)factor / wa.temp);
+        }
+        float tw = wa.w_sys * (wslt ? wslt[n] : 1.0f);
+        if (wa.estype == 2) {
+            tw = tw * wa.volume;
+            const float f2 = ER_PI_F * (wa.qtot * wa.qtot) / (wa.screen * wa.screen) / wa.volume / 2.0f;
+            tw = (float)((double)tw * exp((double)f2 / wa.temp));
+        }
+        engnmfc *= (double)tw;
+        wout[n] = engnmfc;
+        sw += engnmfc;
+        st += (double)tw;
+        if (flpart) {  // flceng(sp, n): slices summed in a fixed order
+            for (int sp = 0; sp < wa.numslv; ++sp) {
+                double s = 0.0;
+                for (int sl = 0; sl < wa.nslice; ++sl) s += flpart[((size_t)n * wa.nslice + sl) * wa.numslv + sp];
+                flceng[(size_t)n * wa.numslv + sp] = s;
+            }
+        }
+    }
+    // ordered tree reduction (fixed order -> deterministic)
+    r_w[tid] = sw;
+    r_t[tid] = st;
+    __syncthreads();
+    for (int s = NT / 2; s > 0; s >>= 1) {
+        if (tid < s) {
+            r_w[tid] += r_w[tid + s];
+            r_t[tid] += r_t[tid + s];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        state[0] += r_w[0];
+        state[1] += (double)wa.nchunk;
+        state[2] += r_t[0];
+        // solute self-energy extremes (engproc.F90:914-915) and histogram
+        float mn = 3.402823466e38f, mx = -3.402823466e38f;
+        for (int nl = 0; nl < wa.nchunk; ++nl) {
+            const int n = wa.n_off + nl;
+            const float u = u0[n];
+            mn = fminf(mn, u);
+            mx = fmaxf(mx, u);
+            if (wa.selfcal) {
+                int lo = 0, hi = wa.esmax;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if ((double)u > escrd[mid]) lo = mid + 1;
+                    else hi = mid;
+                }
+                if (lo <= 0) atomicMin(errkey, (unsigned long long)n * (unsigned long long)(nmol + 1));
+                eself[max(lo, 1) - 1] += wout[n];
+            }
+        }
+        if (wa.nchunk > 0) {
+            atomicMin(&minmax[0], f2ord(mn));
+            atomicMax(&minmax[1], f2ord(mx));
+        }
+    }
+    __syncthreads();
+    // slnuv += flceng * engnmfc (engproc.F90:939-941), sequential in n
+    if (flpart && tid < wa.numslv) {
+        double s = slnuv[tid];
+        for (int nl = 0; nl < wa.nchunk; ++nl) {
+            const int n = wa.n_off + nl;
+            s += flceng[(size_t)n * wa.numslv + tid] * wout[n];
+        }
+        slnuv[tid] = s;
+    }
+}
+
+// edens(b) += engnmfc * count (engproc.F90:943-947), sequential in n per bin
+__global__ void k_edens(int nchunk, int n_off, int ermax, const uint32_t *__restrict__ Cnt,
+                        const double *__restrict__ w, double *__restrict__ edens)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= ermax) return;
+    double e = edens[b];
+    for (int nl = 0; nl < nchunk; ++nl) {
+        const int n = n_off + nl;
+        const uint32_t k = Cnt[(size_t)n * ermax + b];
+        if (k) e += w[n] * (double)k;
+    }
+    edens[b] = e;
+}
+
+// ---------------------------------------------------------------------------
+// ecorr += C^T diag(w) C on FP64 MFMA (engproc.F90:948-962):
+// ecorr(b1,b2) += sum_n w_n c_n(b1) c_n(b2).  Counts are integers and w is
+// FP64, so every product is formed as the reference forms it, up to the
+// summation order.  Block tile 64x64 (upper block triangle, mirrored), wave
+// tile 32x32 = 2x2 MFMA 16x16x4.  The accumulator's (row, col) per
+// (lane, register) comes from dmap, measured once per context by
+// k_mfma_layout.
+// ---------------------------------------------------------------------------
+typedef double er_double4 __attribute__((ext_vector_type(4)));
+
+// D = A B with A[i][0] = i, A[i][1] = 1, B[0][j] = 1, B[1][j] = 16 j
+// (operand lane l holding A[l%16][l/16] and B[l/16][l%16]) gives
+// D[i][j] = i + 16 j: each output register names its own element.
+__global__ void k_mfma_layout(double *__restrict__ out)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = lane & 15, q = lane >> 4;
+    const double a = (q == 0) ? (double)r : (q == 1 ? 1.0 : 0.0);
+    const double b = (q == 0) ? 1.0 : (q == 1 ? 16.0 * r : 0.0);
+    er_double4 acc = {0.0, 0.0, 0.0, 0.0};
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    for (int t = 0; t < 4; ++t) out[lane * 4 + t] = acc[t];
+}
+
+__global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax, const uint32_t *__restrict__ Cnt,
+                                               const double *__restrict__ w, const int *__restrict__ dmap,
+                                               double *__restrict__ ecorr)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int b1w = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int b2w = blockIdx.x * 64 + (wave & 1) * 32;
+    if (blockIdx.y > blockIdx.x) return;  // lower block triangle mirrored below
+    er_double4 acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = (er_double4){0.0, 0.0, 0.0, 0.0};
+    const int kq = lane >> 4, mr = lane & 15;
+    for (int k0 = 0; k0 < nchunk; k0 += 4) {
+        const int nl = k0 + kq;
+        const bool ok = nl < nchunk;
+        const int n = n_off + (ok ? nl : 0);
+        const double wn = ok ? w[n] : 0.0;
+        const uint32_t *row = Cnt + (size_t)n * ermax;
+        double av[2], bv[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const int b1 = b1w + a * 16 + mr;
+            av[a] = (ok && b1 < ermax) ? wn * (double)row[b1] : 0.0;
+            const int b2 = b2w + a * 16 + mr;
+            bv[a] = (ok && b2 < ermax) ? (double)row[b2] : 0.0;
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b)
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int ij = dmap[lane * 4 + r];
+        const int oi = ij & 15, oj = ij >> 4;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+            for (int b = 0; b < 2; ++b) {
+                const int b1 = b1w + a * 16 + oi;
+                const int b2 = b2w + b * 16 + oj;
+                if (b1 < ermax && b2 < ermax) {
+                    const double v = acc[a][b][r];
+                    if (blockIdx.y == blockIdx.x) {
+                        if (b1 <= b2) {
+                            ecorr[(size_t)b1 * ermax + b2] += v;
+                            if (b1 != b2) ecorr[(size_t)b2 * ermax + b1] += v;
+                        }
+                    } else {
+                        ecorr[(size_t)b1 * ermax + b2] += v;
+                        ecorr[(size_t)b2 * ermax + b1] += v;
+                    }
+                }
+            }
+    }
+}
+
+// VALU fallback of the same update (one thread per upper-triangle element)
+__global__ void k_ecorr_valu(int nchunk, int n_off, int ermax, const uint32_t *__restrict__ Cnt,
+                             const double *__restrict__ w, double *__restrict__ ecorr)
+{
+    const int b1 = blockIdx.y * 16 + threadIdx.y, b2 = blockIdx.x * 16 + threadIdx.x;
+    if (b1 >= ermax || b2 >= ermax || b1 > b2) return;
+    double s = 0.0;
+    for (int nl = 0; nl < nchunk; ++nl) {
+        const int n = n_off + nl;
+        const uint32_t k1 = Cnt[(size_t)n * ermax + b1];
+        if (!k1) continue;
+        s += w[n] * (double)k1 * (double)Cnt[(size_t)n * ermax + b2];
+    }
+    ecorr[(size_t)b1 * ermax + b2] += s;
+    if (b1 != b2) ecorr[(size_t)b2 * ermax + b1] += s;
+}

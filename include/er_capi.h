@@ -1,0 +1,169 @@
+/*
+ * er_capi.h -- C ABI of the MI355X erdst hot path (per-frame solute-solvent
+ * energies + energy-coordinate distributions).
+ *
+ * The reference exposes this path only as Fortran module procedures that
+ * engconst/engstore call on shared module arrays.  Each entry point below
+ * names the reference procedure(s) it replaces, so a Fortran host binds it
+ * with iso_c_binding (fortran/er_capi_mod.f90 in the package is that
+ * interface module).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero ER_E_* status on
+ *    failure; er_last_error() then gives the reference's 7-character halt
+ *    code (erdst/mpiproc.F90:184-230) plus the species and energy involved.
+ *    The library never stops the process.
+ *  - Host arrays are owned by the caller, copied in, and never retained
+ *    past the call (the *_device variants are the documented exception).
+ *  - Index arrays are 1-based and arrays are column-major, exactly as the
+ *    Fortran module engmain holds them (erdst/engmain.F90:314-348).
+ *  - A context is used by one host thread.  It owns one HIP stream and all
+ *    device memory.  er_run() is asynchronous; er_sync(), er_fetch() and
+ *    er_get_uvengy() wait for it.
+ *  - Arithmetic: pair and PME energies in FP32 (the reference's default
+ *    build); the reciprocal-grid self energy, the weights and every
+ *    distribution accumulator (edens, ecorr, eself, engnorm, ...) in FP64.
+ */
+#ifndef ER_CAPI_H
+#define ER_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct er_ctx er_ctx;
+
+/* status codes */
+enum {
+    ER_OK = 0,
+    ER_E_ARG = 1,        /* invalid argument / inconsistent sizes            */
+    ER_E_UNSUPPORTED = 2,/* option outside what this build implements        */
+    ER_E_HIP = 3,        /* HIP runtime error                                */
+    ER_E_HALT = 4,       /* a reference halt condition (see er_last_error)   */
+    ER_E_STATE = 5       /* call out of order (e.g. er_run before a frame)   */
+};
+
+/* run modes for er_run */
+enum { ER_PLAIN = 0, ER_BATCHED = 1 };
+
+/* Topology of the physical system, as setparam leaves it in module engmain
+ * (erdst/setconf.F90:516-844).  "Physical" = molecules whose coordinates are
+ * in the trajectory (solvent, plus the solute molecules in soln mode). */
+typedef struct {
+    int32_t numatm;             /* atoms of the physical system              */
+    int32_t nummol;             /* molecules of the physical system (slvmax) */
+    int32_t ljtype_max;
+    int32_t numslv;             /* number of solvent species                 */
+    const int32_t *mol_begin_index; /* [nummol+1], 1-based                   */
+    const int32_t *ljtype;      /* [numatm], 1-based                         */
+    const int32_t *uvspec;      /* [nummol]: species 1..numslv, 0 = solute   */
+    const float *charge;        /* [numatm], already x 18.22261721           */
+    const float *ljene_mat;     /* [ljtype_max^2], column-major              */
+    const float *ljlensq_mat;   /* [ljtype_max^2], column-major              */
+} er_topology;
+
+/* The &ene_param values the hot path reads (erdst/engmain.F90:376-388),
+ * after iniparam's defaulting (erdst/setconf.F90:133-438). */
+typedef struct {
+    float elecut, lwljcut, upljcut, screen;
+    int32_t ljswitch;           /* 0..3, erdst/engmain.F90:358              */
+    int32_t cltype;             /* only 2 (PME) is implemented              */
+    int32_t splodr;             /* B-spline order, 2..12                    */
+    int32_t ms[3];              /* ms1max, ms2max, ms3max                    */
+    int32_t boxshp;             /* only 1 (periodic) is implemented         */
+    int32_t slttype;            /* 1 soln, 2 refs rigid, 3 refs flexible    */
+    int32_t wgtslf, estype, corrcal, selfcal;
+    double temp;                /* kT in kcal/mol (setconf.F90:304)         */
+} er_params;
+
+/* Replaces: setparam's device uploads (setconf.F90:656-841), recpcal_init
+ * (recpcal.F90:40-95), enginit's ecorr allocation (engproc.F90:283-287). */
+int er_create(const er_topology *topo, const er_params *prm, er_ctx **out);
+
+/* Replaces: enginit's mesh (engproc.F90:127-281).  uvmax/uvsoft: [numslv];
+ * uvcrd: [sum(uvmax)] FP64 concatenated per species; escrd: [esmax] (only
+ * read when selfcal = 1). */
+int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsoft,
+                const double *uvcrd, int32_t esmax, const double *escrd);
+
+/* Test particle of a refs run: the last molecule in the reference's system
+ * (engproc.F90:99-101).  ljtype 1-based, charge x 18.22261721. */
+int er_set_solute(er_ctx *c, int32_t nsite, const int32_t *ljtype, const float *charge);
+
+/* Solute molecules of a soln run (sltlist, engproc.F90:72-96), 1-based. */
+int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *tagslt);
+
+/* Replaces: the frame arrays of getconf_parallel (setconf.F90:849-986) that
+ * engconst consumes.  sitepos: [3*numatm] column-major; cell: [9]
+ * column-major (column j = box vector j). */
+int er_set_frame(er_ctx *c, const float *sitepos, const float *cell,
+                 float stat_weight_system);
+/* Same, but sitepos is a device pointer that stays valid until the next
+ * er_run() has completed (no copy is made). */
+int er_set_frame_device(er_ctx *c, const float *sitepos_dev, const float *cell,
+                        float stat_weight_system);
+
+/* Replaces: instslt('proc') x maxdst (engproc.F90:810-812) -- the caller
+ * places the insertions.  sltpos: [3*nsite*nins] (insertion-major, i.e. the
+ * sitepos extension slots of erdst/setconf.F90:607-611); stat_weight_solute:
+ * [nins] or NULL for all 1. */
+int er_set_insertions(er_ctx *c, int32_t nins, const float *sltpos,
+                      const float *stat_weight_solute);
+int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sltpos_dev,
+                             const float *stat_weight_solute);
+
+/* Replaces: the body of engconst for one frame (engproc.F90:419-458):
+ * realcal_prepare + recpcal_spline_greenfunc + recpcal_prepare_solvent +
+ * get_uv_energy_refs|soln + update_histogram x maxdst.  Results accumulate
+ * into the block accumulators.  ER_PLAIN launches one insertion at a time;
+ * ER_BATCHED runs every insertion of the frame in one launch sequence. */
+int er_run(er_ctx *c, int32_t mode);
+
+/* Waits for the context's stream; returns ER_E_HALT if a run hit a halt
+ * condition. */
+int er_sync(er_ctx *c);
+
+/* Per-insertion energies of the last er_run, for parity checks.
+ * uvengy: [nins*nummol] (= uvengy(slvmax, maxdst) column-major); uvengy0:
+ * [nins].  Either pointer may be NULL. */
+int er_get_uvengy(er_ctx *c, float *uvengy, float *uvengy0);
+
+/* Replaces: the data engstore reads (engproc.F90:520-725) before its
+ * normalisation.  edens [ermax], ecorr [ermax*ermax] (may be NULL),
+ * eself [esmax] (may be NULL), scal = {engnorm, engsmpl, avslf, voffset},
+ * minuv/maxuv [numslv+1] (index 0 = solute self energy), slnuv [numslv]
+ * (soln; may be NULL). */
+int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, double *scal,
+             float *minuv, float *maxuv, double *slnuv);
+
+/* Per-solute species sums of the last soln er_run (flceng, engproc.F90:918-932);
+ * flceng: [numslv*nslt]. */
+int er_get_flceng(er_ctx *c, double *flceng);
+
+/* Replaces: engclear (engproc.F90:321-340). */
+int er_clear(er_ctx *c);
+
+/* code: 8 bytes, NUL-terminated 7-character halt code ("" if none). */
+int er_last_error(er_ctx *c, char *code, int32_t *species, float *energy);
+
+/* Human-readable text of the last failure (static storage of the context). */
+const char *er_error_string(er_ctx *c);
+
+void er_destroy(er_ctx *c);
+
+/* Diagnostics, no counterpart in the reference: per-kernel device time of
+ * the runs issued while enabled, measured with HIP events recorded on the
+ * context's stream around each launch.  er_profile_read waits for the stream
+ * and returns (then resets) the accumulated time and launch count of one
+ * kernel class. */
+enum { ER_K_PREP = 0, ER_K_SPEC2D, ER_K_AXIS1, ER_K_GATHER, ER_K_PAIRS, ER_K_HIST, ER_K_WEIGHTS,
+       ER_K_ECORR, ER_K_COUNT };
+int er_profile(er_ctx *c, int32_t enable);
+int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

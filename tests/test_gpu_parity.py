@@ -1,0 +1,210 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 oracle.
+
+Energies: per (insertion, molecule) within |dE| <= RTOL*|E| + ATOL (see
+erutil).  Histograms: the GPU's counts must equal the oracle's binning of
+the GPU's own FP32 energies exactly, so edens/ecorr/engnorm agree to FP64
+summation-order rounding; bins that differ when the oracle bins its FP64
+energies are counted and must all lie within the energy tolerance of an
+edge.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import erutil as U
+from erdst_gpu import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(w, params, chunk=None):
+    if chunk is not None:
+        os.environ["ER_CHUNK"] = str(chunk)
+    try:
+        ctx = capi.Context(w.topo, params)
+    finally:
+        os.environ.pop("ER_CHUNK", None)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    if params["slttype"] == 1:
+        ctx.set_solute_molecules(w.topo.tagslt)
+    else:
+        ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    return ctx
+
+
+def _run_refs(ctx, w, f, mode, nins=None):
+    ins = w.insertions(f)
+    if nins is not None:
+        ins = ins[:nins]
+    ctx.set_frame(w.frame(f), w.cell)
+    ctx.set_insertions(ins)
+    ctx.run(mode)
+    ctx.sync()
+    return ctx.uvengy(ins.shape[0])
+
+
+@pytest.fixture(scope="module")
+def cfg0():
+    return synth.config0()
+
+
+@pytest.mark.parametrize("mode", [capi.ER_BATCHED, capi.ER_PLAIN])
+def test_refs_energies_cfg0(cfg0, mode):
+    p = U.workload_params(cfg0)
+    ctx = _ctx(cfg0, p)
+    u, u0 = _run_refs(ctx, cfg0, 0, mode)
+    uo, u0o = U.oracle_refs(cfg0, 0, p)
+    nbad, worst, at = U.energy_mismatch(u, uo)
+    print(f"mode {mode}: max |dE| {np.abs(u - uo).max():.3e}, worst excess {worst:.3e} at {at}, "
+          f"E_gpu {u[at]:.6f} E_64 {uo[at]:.6f}")
+    assert nbad == 0
+    nbad0, _, at0 = U.energy_mismatch(u0, u0o)
+    assert nbad0 == 0, (u0[at0], u0o[at0])
+
+
+def test_plain_matches_batched(cfg0):
+    p = U.workload_params(cfg0)
+    ub, u0b = _run_refs(_ctx(cfg0, p), cfg0, 1, capi.ER_BATCHED, nins=40)
+    up, u0p = _run_refs(_ctx(cfg0, p), cfg0, 1, capi.ER_PLAIN, nins=40)
+    np.testing.assert_allclose(ub, up, rtol=1e-6, atol=1e-6)
+    np.testing.assert_allclose(u0b, u0p, rtol=1e-6, atol=1e-6)
+
+
+def test_multi_chunk_equals_single_chunk(cfg0):
+    p = U.workload_params(cfg0)
+    u1, _ = _run_refs(_ctx(cfg0, p, chunk=32), cfg0, 2, capi.ER_BATCHED)
+    u2, _ = _run_refs(_ctx(cfg0, p, chunk=128), cfg0, 2, capi.ER_BATCHED)
+    np.testing.assert_array_equal(u1, u2)
+
+
+def _distribution_check(ctx, w, p, frames, mode):
+    """Accumulate over frames on the GPU; feed the GPU energies to the oracle's
+    update_histogram and compare every accumulator."""
+    d_gpu_e = U.oracle_distribution(w, p)   # oracle binning of GPU energies
+    d_64 = U.oracle_distribution(w, p)      # oracle binning of FP64 energies
+    flips = 0
+    for f in frames:
+        u, u0 = _run_refs(ctx, w, f, mode)
+        uo, u0o = U.oracle_refs(w, f, p)
+        for n in range(u.shape[0]):
+            d_gpu_e.update(w.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+            d_64.update(w.topo.uvspec, uo[n], u0o[n])
+        # bins of GPU vs FP64 energies
+        mesh = w.uvmax_uvsoft_uvcrd()[2]
+        b_gpu = np.searchsorted(mesh, u.astype(np.float64), side="left")
+        b_64 = np.searchsorted(mesh, uo, side="left")
+        diff = b_gpu != b_64
+        if diff.any():
+            # every flip must be explained by the energy tolerance straddling an edge
+            lo = np.minimum(u.astype(np.float64), uo)[diff]
+            hi = np.maximum(u.astype(np.float64), uo)[diff]
+            tol = U.energy_tolerance(uo[diff])
+            edge_between = np.searchsorted(mesh, hi, side="left") > np.searchsorted(mesh, lo, side="left")
+            assert edge_between.all()
+            assert (hi - lo <= tol).all()
+        flips += int(diff.sum())
+    g = ctx.fetch()
+    return g, d_gpu_e, d_64, flips
+
+
+def _assert_same_distribution(g, d):
+    shape = g["ecorr"].shape if g["ecorr"] is not None else None
+    np.testing.assert_allclose(g["edens"] / g["engnorm"], d.edens / d.d.engnorm, rtol=1e-12, atol=0)
+    if shape is not None:
+        np.testing.assert_allclose(g["ecorr"] / g["engnorm"], d.ecorr.reshape(shape) / d.d.engnorm,
+                                   rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["engnorm"], d.d.engnorm, rtol=1e-7)
+    np.testing.assert_allclose(g["avslf"], d.d.avslf, rtol=1e-7)
+
+
+def test_histograms_cfg0(cfg0):
+    p = U.workload_params(cfg0)
+    ctx = _ctx(cfg0, p)
+    g, dg, d64, flips = _distribution_check(ctx, cfg0, p, [0, 1], capi.ER_BATCHED)
+    print(f"bin flips vs FP64 energies: {flips} of {2 * 100 * cfg0.topo.nmol}")
+    # Exact binning of the GPU's own energies.  The GPU forms total_weight in
+    # FP32 like the reference's single build (engproc.F90:873,901-903,
+    # 1026-1036), the oracle in FP64: one common factor (1 + O(1e-8)) scales
+    # every weight here (w_sys, w_slt, V and Q are constant), so it cancels
+    # in the normalised distributions, which must then agree to FP64
+    # summation order -- a single miscounted molecule would show at >= 1e-6.
+    _assert_same_distribution(g, dg)
+    assert g["engsmpl"] == dg.d.engsmpl == 200
+    assert g["voffset"] == dg.d.voffset
+    np.testing.assert_array_equal(g["minuv"].astype(np.float64), dg.minuv)
+    np.testing.assert_array_equal(g["maxuv"].astype(np.float64), dg.maxuv)
+    # symmetric ecorr and per-species row sums N_p * edens (sum_j c_j = N_p)
+    np.testing.assert_array_equal(g["ecorr"], g["ecorr"].T)
+    np.testing.assert_allclose(g["ecorr"].sum(1), cfg0.topo.nmol * g["edens"], rtol=1e-12)
+    # against the FP64 oracle end to end: weights agree to FP32-energy rounding
+    np.testing.assert_allclose(g["engnorm"], d64.d.engnorm, rtol=1e-4)
+
+
+def test_plain_histograms_equal_batched(cfg0):
+    p = U.workload_params(cfg0)
+    res = []
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx = _ctx(cfg0, p)
+        _run_refs(ctx, cfg0, 3, mode, nins=50)
+        res.append(ctx.fetch())
+    np.testing.assert_allclose(res[0]["edens"], res[1]["edens"], rtol=1e-12)
+    np.testing.assert_allclose(res[0]["ecorr"], res[1]["ecorr"], rtol=1e-12)
+
+
+@pytest.mark.parametrize("ljswitch", [1, 2, 3])
+def test_lj_switch_forms(cfg0, ljswitch):
+    p = U.workload_params(cfg0, ljswitch=ljswitch)
+    u, _ = _run_refs(_ctx(cfg0, p), cfg0, 0, capi.ER_BATCHED, nins=20)
+    uo, _ = U.oracle_refs(cfg0, 0, p, nins=20)
+    assert U.energy_mismatch(u, uo)[0] == 0
+
+
+def test_noncubic_grid_and_order6():
+    w = synth.refs_water(0, 1000, 31.04, 24, 1, 32)
+    w.params["ms"] = (40, 32, 32)
+    p = U.workload_params(w, splodr=6)
+    u, u0 = _run_refs(_ctx(w, p), w, 0, capi.ER_BATCHED)
+    uo, u0o = U.oracle_refs(w, 0, p)
+    assert U.energy_mismatch(u, uo)[0] == 0
+    assert U.energy_mismatch(u0, u0o)[0] == 0
+
+
+def test_soln_config1_two_frames():
+    w = synth.config1()
+    p = U.workload_params(w, corrcal=1)
+    ctx = _ctx(w, p)
+    d_gpu_e = U.oracle_distribution(w, p)
+    for f in (0, 1):
+        ctx.set_frame(w.frame(f), w.cell)
+        ctx.run(capi.ER_BATCHED)
+        ctx.sync()
+        u, u0 = ctx.uvengy(1)
+        uo, u0o = U.oracle_soln(w, f, p)
+        assert u[0, w.topo.tagslt[0]] == 0.0
+        nbad, worst, at = U.energy_mismatch(u, uo)
+        assert nbad == 0, (worst, at, u[at], uo[at])
+        assert U.energy_mismatch(u0, u0o)[0] == 0
+        sp = U.species_of(w.topo, 1, w.topo.tagslt)
+        d_gpu_e.update(sp[0], u[0].astype(np.float64), float(u0[0]))
+        fl = ctx.flceng(1)
+        np.testing.assert_allclose(fl[0], d_gpu_e.flceng, rtol=1e-6)
+    g = ctx.fetch()
+    _assert_same_distribution(g, d_gpu_e)
+    np.testing.assert_allclose(g["slnuv"], d_gpu_e.slnuv, rtol=1e-6)
+
+
+def test_eng_min_halt_is_reported(cfg0):
+    p = U.workload_params(cfg0)
+    ctx = capi.Context(cfg0.topo, p)
+    uvmax, uvsoft, uvcrd = cfg0.uvmax_uvsoft_uvcrd()
+    ctx.set_mesh(uvmax, uvsoft, uvcrd + 1.0e3)   # every energy below the mesh
+    ctx.set_solute(cfg0.topo.slt_ljtype, cfg0.topo.slt_charge32)
+    ctx.set_frame(cfg0.frame(0), cfg0.cell)
+    ctx.set_insertions(cfg0.insertions(0)[:4])
+    ctx.run(capi.ER_BATCHED)
+    with pytest.raises(capi.ErError) as ei:
+        ctx.sync()
+    assert ei.value.status == capi.ER_E_HALT
+    code, species, energy = ctx.last_error()
+    assert code == "eng_min" and species == 1
