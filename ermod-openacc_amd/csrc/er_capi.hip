@@ -116,6 +116,12 @@ struct er_ctx {
     DBuf<unsigned int> d_keyrank, d_ccnt, d_cstart;
     DBuf<float> d_rmolmax;
 
+    // atoms sorted by PME brick (per frame) for the brick gather
+    int nbx = 1, nby = 1, nbz = 1, nbrick = 1;
+    DBuf<int> d_aperm;
+    DBuf<unsigned int> d_akeyrank, d_bcnt, d_bstart;
+    DBuf<float> d_Eat;
+
     // MFMA accumulator layout (lane, register) -> (row, col), measured once
     bool use_mfma = false;
     DBuf<int> d_dmap;
@@ -309,6 +315,10 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     c->ord = prm->splodr;
     c->G = (size_t)c->K1 * c->K2 * c->K3;
     c->Gc = (size_t)c->H * c->K2 * c->K3;
+    c->nbx = (c->K1 + ER_BRICK - 1) / ER_BRICK;
+    c->nby = (c->K2 + ER_BRICK - 1) / ER_BRICK;
+    c->nbz = (c->K3 + ER_BRICK - 1) / ER_BRICK;
+    c->nbrick = c->nbx * c->nby * c->nbz;
     if (const char *e = getenv("ER_CHUNK")) c->chunk = std::max(NI_B, atoi(e) / NI_B * NI_B);
 
     c->molbeg.resize(c->nummol + 1);
@@ -794,6 +804,32 @@ template <int NI, int LPM> static hipError_t launch_gather(er_ctx *c, int n_off,
     return hipErrorInvalidValue;
 }
 
+template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, int n_off, int nch)
+{
+    constexpr int NXCD = 8;
+    constexpr int RS = ER_BRICK + ORD - 1;
+    const size_t lds = (size_t)RS * RS * RS * NI * sizeof(float);
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_gather_brick<ORD, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  (int)lds);
+        attr = true;
+    }
+    const int ntile = (nch + NI - 1) / NI;
+    const int nbpad = (c->nbrick + NXCD - 1) / NXCD * NXCD;
+    hipLaunchKernelGGL((k_gather_brick<ORD, NI>), dim3(nbpad * ntile), dim3(512), lds, c->stream, c->K1, c->K2, c->K3,
+                       c->nbx, c->nby, nbpad, NXCD, nch, c->d_bstart.p, c->d_aperm.p, c->d_posn.p, c->d_splw.p,
+                       c->d_splg.p, c->d_phi.p, c->numatm, c->d_Eat.p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const bool soln = c->prm.slttype == 1;
+    const size_t nthr = (size_t)ntile * c->nummol * NI;
+    hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream, c->nummol,
+                       nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
+                       soln ? c->d_tag.p : (const int *)nullptr, c->qslt, c->prm.screen, c->fc.volume, c->d_U.p);
+    return hipGetLastError();
+}
+
 template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
 {
     const int NT = 256;
@@ -839,6 +875,14 @@ static hipError_t prep_frame(er_ctx *c)
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->ncell, c->d_ccnt.p, c->d_cstart.p);
         hipLaunchKernelGGL(k_cell_scatter, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
                            c->d_keyrank.p, c->d_cstart.p, c->d_perm.p);
+        // atoms by PME brick for the brick gather
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->nbrick + B - 1) / B), dim3(B), 0, c->stream, (size_t)c->nbrick,
+                           c->d_bcnt.p);
+        hipLaunchKernelGGL(k_atom_brick_count, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+                           c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, c->d_bcnt.p, c->d_akeyrank.p);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
+        hipLaunchKernelGGL(k_cell_scatter, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+                           c->d_akeyrank.p, c->d_bstart.p, c->d_aperm.p);
         c->frame_dirty = false;
     }
     return hipGetLastError();
@@ -866,6 +910,11 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     HIPCHK(c, c->d_keyrank.ensure(2 * (size_t)c->nummol));
     HIPCHK(c, c->d_ccnt.ensure(c->ncell));
     HIPCHK(c, c->d_cstart.ensure(c->ncell + 1));
+    HIPCHK(c, c->d_aperm.ensure(c->numatm));
+    HIPCHK(c, c->d_akeyrank.ensure(2 * (size_t)c->numatm));
+    HIPCHK(c, c->d_bcnt.ensure(c->nbrick));
+    HIPCHK(c, c->d_bstart.ensure(c->nbrick + 1));
+    if (NI > 1 && c->ord <= 6) HIPCHK(c, c->d_Eat.ensure(ntile * NI * (size_t)c->numatm));
     // enough histogram workgroups to fill the chip
     const int ntile_chunk = (chunk + NI - 1) / NI;
     c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
@@ -892,7 +941,11 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
     {
         ProfScope ps(c, ER_K_GATHER);
         if constexpr (NI == 1) e = launch_gather<1, 1>(c, n_off, nch);
-        else e = launch_gather<NI, NI>(c, n_off, nch);
+        else {
+            if (c->ord == 4) e = launch_gather_brick_k<4, NI>(c, n_off, nch);
+            else if (c->ord == 6) e = launch_gather_brick_k<6, NI>(c, n_off, nch);
+            else e = launch_gather<NI, NI>(c, n_off, nch);
+        }
     }
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;

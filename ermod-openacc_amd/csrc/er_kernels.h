@@ -580,6 +580,123 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
 }
 
 // ---------------------------------------------------------------------------
+// Brick gather (the batched path's form of recpcal.F90:651-710).  Atoms are
+// sorted per frame by the 8x8x8 brick of their base grid index; a workgroup
+// stages the (8+ORD-1)^3 region of NI interleaved potential grids that its
+// brick's stencils touch in LDS once, then every atom of the brick reads its
+// ORD^3 stencil from LDS (each grid point is read by ~15 atoms, so L2->CU
+// traffic drops by the reuse factor).  Per-atom energies go to Eat and are
+// summed per molecule by k_mol_reduce in the same atom order as k_gather.
+// ---------------------------------------------------------------------------
+#define ER_BRICK 8
+
+__global__ void k_atom_brick_count(int natom, const int4 *__restrict__ splg, int K1, int K2, int K3, int nbx,
+                                   int nby, unsigned int *__restrict__ cnt, unsigned int *__restrict__ key_rank)
+{
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    const int4 g = splg[a];
+    const unsigned int key = (unsigned int)((pmod(g.z, K3) / ER_BRICK * nby + pmod(g.y, K2) / ER_BRICK) * nbx +
+                                            pmod(g.x, K1) / ER_BRICK);
+    key_rank[2 * a] = key;
+    key_rank[2 * a + 1] = atomicAdd(&cnt[key], 1u);
+}
+
+template <int ORD, int NI>
+__global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick_pad,
+                                                      int nxcd, int nchunk, const unsigned int *__restrict__ bstart,
+                                                      const int *__restrict__ aperm,
+                                                      const float4 *__restrict__ posn,
+                                                      const float *__restrict__ splw,
+                                                      const int4 *__restrict__ splg, const float *__restrict__ phi,
+                                                      int natom, float *__restrict__ Eat)
+{
+    constexpr int RS = ER_BRICK + ORD - 1;  // region side
+    extern __shared__ __align__(16) float reg[];  // [RS][RS][RS][NI]
+    // XCD-aware remap: each XCD sweeps a contiguous range of bricks, tile by tile
+    const int b = blockIdx.x;
+    const int per = nbrick_pad / nxcd;
+    const int xcd = b % nxcd, k = b / nxcd;
+    const int brick = xcd * per + (k % per);
+    const int tile = k / per;
+    const int nbz = (K3 + ER_BRICK - 1) / ER_BRICK;
+    if (brick >= nbx * nby * nbz || tile * NI >= nchunk) return;
+    const unsigned int a0 = bstart[brick], a1 = bstart[brick + 1];
+    if (a0 == a1) return;
+    const int bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
+    const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * ER_BRICK - (ORD - 1);
+    const size_t G = (size_t)K1 * K2 * K3;
+    const float *ph = phi + (size_t)tile * G * NI;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    // stage the region: one 16-insertion run per grid point, x fastest
+    for (int f = tid; f < RS * RS * RS * NI; f += NT) {
+        const int i = f % NI, p = f / NI;
+        const int rx = p % RS, ry = (p / RS) % RS, rz = p / (RS * RS);
+        const int gx = pmod(ox + rx, K1), gy = pmod(oy + ry, K2), gz = pmod(oz + rz, K3);
+        reg[f] = ph[(((size_t)gz * K2 + gy) * K1 + gx) * NI + i];
+    }
+    __syncthreads();
+    const int i = tid % NI, s = tid / NI, sstep = NT / NI;
+    for (unsigned int pos = a0 + s; pos < a1; pos += sstep) {
+        const int a = aperm[pos];
+        const float q = posn[a].w;
+        const float *w = splw + (size_t)a * 3 * ORD;
+        const int4 g = splg[a];
+        // local region coordinates of the stencil points g - c
+        const int lx = pmod(g.x, K1) - bx * ER_BRICK + (ORD - 1);
+        const int ly = pmod(g.y, K2) - by * ER_BRICK + (ORD - 1);
+        const int lz = pmod(g.z, K3) - bz * ER_BRICK + (ORD - 1);
+        float w1[ORD], w2[ORD], w3[ORD];
+#pragma unroll
+        for (int c = 0; c < ORD; ++c) {
+            w1[c] = w[c];
+            w2[c] = w[ORD + c];
+            w3[c] = w[2 * ORD + c];
+        }
+        float ea = 0.0f;
+#pragma unroll
+        for (int c3 = 0; c3 < ORD; ++c3) {
+#pragma unroll
+            for (int c2 = 0; c2 < ORD; ++c2) {
+                const float *row = reg + (((lz - c3) * RS + (ly - c2)) * RS) * NI + i;
+                float sum = 0.0f;
+#pragma unroll
+                for (int c1 = 0; c1 < ORD; ++c1) sum += w1[c1] * row[(lx - c1) * NI];
+                ea += (w3[c3] * w2[c2]) * sum;
+            }
+        }
+        Eat[((size_t)tile * natom + a) * NI + i] = q * ea;
+    }
+}
+
+// per-molecule sum of the atom energies (atom order as in k_gather) plus the
+// residual background term (engproc.F90:979-1020); writes U
+template <int NI>
+__global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const int *__restrict__ molbeg,
+                             const float *__restrict__ molq, const float *__restrict__ Eat,
+                             const int *__restrict__ tag, float qslt_refs, float screen, float volume,
+                             float *__restrict__ U)
+{
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = f % NI;
+    const size_t rest = f / NI;
+    const int mol = rest % nmol;
+    const int tile = rest / nmol;
+    const int nl = tile * NI + i;
+    if (tile * NI >= nchunk || nl >= nchunk) return;
+    float acc = 0.0f;
+    for (int a = molbeg[mol]; a < molbeg[mol + 1]; ++a) acc += Eat[((size_t)tile * natom + a) * NI + i];
+    const int n = n_off + nl;
+    float qs = qslt_refs;
+    if (tag) {
+        if (mol == tag[n]) acc = 0.0f;
+        qs = molq[tag[n]];
+    }
+    if (!tag || mol != tag[n]) acc -= ER_PI_F * qs * molq[mol] / screen / screen / volume;
+    U[((size_t)(n / NI) * nmol + mol) * NI + (n % NI)] = acc;
+}
+
+// ---------------------------------------------------------------------------
 // Real-space LJ + erfc-Coulomb pairs (realcal.F90:207-364 / 51-205) with
 // molecule-level pruning: a molecule whose reference site is further than
 // cutoff + R_solute + R_molecule from the solute's reference site has every
@@ -703,9 +820,9 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     __syncthreads();
     int cx, cy, cz;
     site_cell(fc, ref.x, ref.y, ref.z, pa.nc, cx, cy, cz);
-    // cells within reach of the reference site: a site in cell c lies within
-    // one cell width of the cell's boundary, so ceil(reach / width) + 1 cells
-    const int nreach = (int)ceilf((cut + rmolmax[0]) / pa.cell_w) + 1;
+    // a site within distance d of the reference site lies at most
+    // ceil(d / width) cells away along every axis
+    const int nreach = (int)ceilf((cut + rmolmax[0]) / pa.cell_w);
     const int span = min(2 * nreach + 1, pa.nc);  // distinct cells per axis
     const int lo = (span == pa.nc) ? 0 : -nreach;
     const int ncell = span * span * span;

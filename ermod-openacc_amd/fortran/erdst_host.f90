@@ -1,0 +1,233 @@
+! erdst_host.f90 -- a Fortran erdst driver over the C ABI: the engdiv x frame
+! loop of trjmain (enganal.F90:86-95), engconst's device work through
+! er_run, and engstore's normalisation and writers (engproc.F90:520-725).
+!
+! Input: one stream file (argument 1) holding the engmain topology, the
+! &ene_param values, the energy-coordinate mesh and, per frame, the frame's
+! coordinates and the insertion coordinates (layout: read_workload below).
+! Output (current directory): engref.XX / corref.XX (engsln / corsln for
+! soln runs), weight_refs|weight_soln, uvrange.tt, as engstore writes them.
+program erdst_host
+  use iso_c_binding
+  use er_capi
+  implicit none
+  character(len=512) :: path
+  type(c_ptr) :: ctx
+  type(er_topology) :: topo
+  type(er_params) :: prm
+  integer(c_int32_t) :: numatm, nummol, ntype, numslv, nslt, nins, nframes, ermax, engdiv, mode
+  integer(c_int32_t), allocatable, target :: mol_begin(:), ljtype(:), uvspec(:), slt_lj(:), tagslt(:)
+  integer(c_int32_t), allocatable :: uvmax(:), uvsoft(:)
+  real(c_float), allocatable, target :: charge(:), ljene(:), ljlensq(:), slt_q(:)
+  real(c_float), allocatable :: sitepos(:,:), sltpos(:,:), wslt(:), minuv(:), maxuv(:)
+  real(c_float) :: cell(3,3)
+  real(c_double), allocatable :: uvcrd(:), edens(:), ecorr(:), slnuv(:), avediv(:,:), escrd(:), eself(:)
+  real(c_double) :: scal(4), dummy(1)
+  integer :: iu, rc, f, division, frames_per_div, k
+  integer, parameter :: stdout = 6
+
+  call get_command_argument(1, path)
+  if (len_trim(path) == 0) stop "usage: erdst_host workload.bin [plain]"
+  mode = ER_BATCHED
+  if (command_argument_count() >= 2) then
+     block
+       character(len=16) :: arg
+       call get_command_argument(2, arg)
+       if (trim(arg) == "plain") mode = ER_PLAIN
+     end block
+  end if
+  open(newunit=iu, file=trim(path), access="stream", form="unformatted", status="old", action="read")
+  call read_workload()
+
+  topo%numatm = numatm; topo%nummol = nummol; topo%ljtype_max = ntype; topo%numslv = numslv
+  topo%mol_begin_index = c_loc(mol_begin); topo%ljtype = c_loc(ljtype); topo%uvspec = c_loc(uvspec)
+  topo%charge = c_loc(charge); topo%ljene_mat = c_loc(ljene); topo%ljlensq_mat = c_loc(ljlensq)
+  rc = er_create(topo, prm, ctx)
+  call check(rc, "er_create")
+  rc = er_set_mesh(ctx, uvmax, uvsoft, uvcrd, 0_c_int32_t, dummy)
+  call check(rc, "er_set_mesh")
+  if (prm%slttype == 1) then
+     rc = er_set_solute_molecules(ctx, nins, tagslt)
+  else
+     rc = er_set_solute(ctx, nslt, slt_lj, slt_q)
+  end if
+  call check(rc, "er_set_solute")
+
+  allocate(edens(ermax), ecorr(int(ermax, 8) * ermax), slnuv(numslv), minuv(0:numslv), maxuv(0:numslv))
+  allocate(avediv(engdiv, 2))
+  frames_per_div = nframes / engdiv
+  do f = 1, nframes
+     read(iu) sitepos
+     rc = er_set_frame(ctx, sitepos, cell, 1.0_c_float)
+     call check(rc, "er_set_frame")
+     if (prm%slttype /= 1) then
+        read(iu) sltpos
+        rc = er_set_insertions(ctx, nins, sltpos, wslt)
+        call check(rc, "er_set_insertions")
+     end if
+     rc = er_run(ctx, mode)                                   ! engconst
+     call check(rc, "er_run")
+     if (mod(f, frames_per_div) == 0) then
+        division = f / frames_per_div
+        call engstore(division)
+        rc = er_clear(ctx)                                    ! engclear
+        call check(rc, "er_clear")
+     end if
+  end do
+  close(iu)
+  call er_destroy(ctx)
+
+contains
+
+  subroutine check(rc, what)
+    integer, intent(in) :: rc
+    character(len=*), intent(in) :: what
+    character(kind=c_char) :: code(8)
+    integer(c_int32_t) :: sp
+    real(c_float) :: en
+    integer :: i, r2
+    character(len=7) :: c7
+    if (rc == ER_OK) return
+    r2 = er_last_error(ctx, code, sp, en)
+    do i = 1, 7
+       c7(i:i) = code(i)
+    end do
+    write(stdout, '(A,A,A,I0)') " ", what, " failed, status ", rc
+    if (rc == ER_E_HALT) then
+       write(stdout, '(A,g12.4,A,i3,A)') '  energy of ', en, ' for ', sp, '-th species'
+       write(stdout, '(A,A)') " halt code ", c7
+    end if
+    stop 1
+  end subroutine check
+
+  subroutine read_workload()
+    integer(c_int32_t) :: hdr(20)
+    real(c_float) :: rp(4)
+    real(c_double) :: t
+    read(iu) hdr
+    numatm = hdr(1); nummol = hdr(2); ntype = hdr(3); numslv = hdr(4); nslt = hdr(5)
+    nins = hdr(6); nframes = hdr(7); ermax = hdr(8)
+    prm%slttype = hdr(9); prm%ms = hdr(10:12); prm%splodr = hdr(13); prm%ljswitch = hdr(14)
+    prm%estype = hdr(15); prm%corrcal = hdr(16); prm%wgtslf = hdr(17); prm%selfcal = hdr(18)
+    engdiv = hdr(19)
+    prm%cltype = 2; prm%boxshp = 1
+    read(iu) rp, t
+    prm%elecut = rp(1); prm%lwljcut = rp(2); prm%upljcut = rp(3); prm%screen = rp(4); prm%temp = t
+    allocate(mol_begin(nummol + 1), ljtype(numatm), uvspec(nummol), charge(numatm))
+    allocate(ljene(ntype * ntype), ljlensq(ntype * ntype), uvmax(numslv), uvsoft(numslv), uvcrd(ermax))
+    read(iu) mol_begin, ljtype, uvspec, charge, ljene, ljlensq, uvmax, uvsoft, uvcrd
+    if (prm%slttype == 1) then
+       allocate(tagslt(nins))
+       read(iu) tagslt
+    else
+       allocate(slt_lj(nslt), slt_q(nslt))
+       read(iu) slt_lj, slt_q
+       allocate(sltpos(3, nslt * nins), wslt(nins))
+       wslt(:) = 1.0
+    end if
+    read(iu) cell
+    allocate(sitepos(3, numatm))
+  end subroutine read_workload
+
+  ! representative_bin_info (engproc.F90:1355-1387)
+  subroutine bin_info(iduv, engleft, engmiddle, enginvwidth, spec)
+    integer, intent(in) :: iduv
+    real(c_double), intent(out) :: engleft, engmiddle, enginvwidth
+    integer, intent(out) :: spec
+    integer :: idmin, idpt, cnt
+    idmin = 0
+    do cnt = 1, numslv
+       idpt = idmin + uvmax(cnt)
+       if (iduv <= idpt) exit
+       idmin = idpt
+    end do
+    spec = cnt
+    idpt = iduv - idmin
+    engleft = uvcrd(iduv)
+    if (idpt <= uvsoft(spec)) then
+       engmiddle = (uvcrd(iduv) + uvcrd(iduv + 1)) / 2.0
+       enginvwidth = 1. / (uvcrd(iduv + 1) - uvcrd(iduv))
+    else if (idpt < uvmax(spec)) then
+       engmiddle = sqrt(uvcrd(iduv) * uvcrd(iduv + 1))
+       enginvwidth = 1. / (uvcrd(iduv + 1) - uvcrd(iduv))
+    else
+       engmiddle = uvcrd(iduv)
+       enginvwidth = 0.
+    end if
+  end subroutine bin_info
+
+  ! engstore for one block (engproc.F90:520-725, single process)
+  subroutine engstore(division)
+    integer, intent(in) :: division
+    character(len=3) :: suffeng
+    character(len=10), parameter :: numbers = '0123456789'
+    real(c_double) :: engnorm, engsmpl, avslf, voffset, leftbin, middlebin, invwt
+    integer :: iduv, pti, io, j, kk
+    rc = er_fetch(ctx, edens, ecorr, dummy, scal, minuv, maxuv, slnuv)
+    call check(rc, "er_fetch")
+    rc = er_sync(ctx)
+    call check(rc, "er_sync")
+    engnorm = scal(1); engsmpl = scal(2); avslf = scal(3); voffset = scal(4)
+    edens(:) = edens(:) / engnorm
+    if (prm%corrcal == 1) ecorr(:) = ecorr(:) / engnorm
+    avslf = avslf / engnorm
+    avediv(division, 1) = engnorm / engsmpl
+    if (prm%slttype == 1) then
+       avediv(division, 2) = voffset - prm%temp * log(avslf)
+    else
+       avediv(division, 2) = voffset + prm%temp * log(avslf)
+    end if
+    if (division == engdiv) then
+       if (prm%slttype == 1) then
+          open(newunit=io, file='weight_soln', action='write')
+       else
+          open(newunit=io, file='weight_refs', action='write')
+       end if
+       do kk = 1, engdiv
+          if (prm%wgtslf == 0) then
+             write(io, '(i5,e20.8)') kk, avediv(kk, 1)
+          else
+             write(io, '(i5,e20.8,e20.7)') kk, avediv(kk, 1), avediv(kk, 2)
+          end if
+       end do
+       close(io)
+       open(newunit=io, file='uvrange.tt', action='write')
+       write(io, '(A)') ' species     minimum        maximum'
+       do pti = 0, numslv
+          if (maxuv(pti) < 1.0e5) then
+             write(io, '(i5,2f15.5)') pti, minuv(pti), maxuv(pti)
+          else
+             write(io, '(i5,f15.5,g18.5)') pti, minuv(pti), maxuv(pti)
+          end if
+       end do
+       close(io)
+    end if
+    if (engdiv == 1) then
+       suffeng = '.tt'
+    else
+       j = division / 10
+       kk = mod(division, 10)
+       suffeng = '.' // numbers(j+1:j+1) // numbers(kk+1:kk+1)
+    end if
+    if (prm%slttype == 1) then
+       open(newunit=io, file='engsln' // suffeng, form="FORMATTED", action='write')
+    else
+       open(newunit=io, file='engref' // suffeng, form="FORMATTED", action='write')
+    end if
+    write(io, '(A2,A13,A15,A5,A25,A25)') "# ", "bin_left", "bin_middle", "spec", "ratio", "dens"
+    do iduv = 1, ermax
+       call bin_info(iduv, leftbin, middlebin, invwt, pti)
+       write(io, '(g15.7,g15.7,i5,g25.15,g25.15)') leftbin, middlebin, pti, edens(iduv), edens(iduv) * invwt
+    end do
+    close(io)
+    if (prm%corrcal == 1) then
+       if (prm%slttype == 1) then
+          open(newunit=io, file='corsln' // suffeng, form="UNFORMATTED", action='write')
+       else
+          open(newunit=io, file='corref' // suffeng, form="UNFORMATTED", action='write')
+       end if
+       write(io) ecorr
+       close(io)
+    end if
+  end subroutine engstore
+end program erdst_host

@@ -1,0 +1,63 @@
+"""The Fortran erdst driver (fortran/erdst_host.f90) over the C ABI: per-block
+engref/corref/weight_refs/uvrange files in the reference's formats, equal to
+the same run through the Python binding."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import erutil as U
+from erdst_gpu import capi, synth
+from erdst_gpu.workload_io import write_workload
+
+HOST = os.path.join(os.path.dirname(capi.LIB_PATH), "erdst_host")
+
+
+def read_fortran_record(path):
+    raw = np.fromfile(path, dtype=np.uint8)
+    n = int(raw[:4].view(np.int32)[0])
+    assert int(raw[4 + n:8 + n].view(np.int32)[0]) == n and raw.size == n + 8
+    return raw[4:4 + n].view(np.float64)
+
+
+def read_engfile(path):
+    lines = open(path).read().splitlines()
+    assert lines[0].split() == ["#", "bin_left", "bin_middle", "spec", "ratio", "dens"]
+    rows = []
+    for ln in lines[1:]:
+        assert len(ln) == 85           # (g15.7,g15.7,i5,g25.15,g25.15)
+        rows.append([float(ln[0:15]), float(ln[15:30]), int(ln[30:35]), float(ln[35:60]), float(ln[60:85])])
+    return np.array(rows)
+
+
+@pytest.mark.gpu
+def test_fortran_host_two_blocks(tmp_path):
+    w = synth.config0()
+    p = U.workload_params(w)
+    frames = [0, 1, 2, 3]
+    wl = tmp_path / "workload.bin"
+    write_workload(wl, w, p, frames, engdiv=2, nins=40)
+    r = subprocess.run([HOST, str(wl)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    # the same run through the Python binding
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    mesh = w.uvmax_uvsoft_uvcrd()[2]
+    for blk in (1, 2):
+        for f in frames[2 * (blk - 1):2 * blk]:
+            ctx.set_frame(w.frame(f), w.cell)
+            ctx.set_insertions(w.insertions(f)[:40])
+            ctx.run(capi.ER_BATCHED)
+        g = ctx.fetch()
+        ctx.clear()
+        eng = read_engfile(tmp_path / f"engref.0{blk}")
+        np.testing.assert_allclose(eng[:, 0], mesh, rtol=1e-6)
+        np.testing.assert_allclose(eng[:, 3], g["edens"] / g["engnorm"], rtol=1e-13, atol=1e-300)
+        cor = read_fortran_record(tmp_path / f"corref.0{blk}").reshape(g["ecorr"].shape)
+        np.testing.assert_allclose(cor, g["ecorr"] / g["engnorm"], rtol=1e-13, atol=1e-300)
+    wt = open(tmp_path / "weight_refs").read().splitlines()
+    assert len(wt) == 2 and all(len(x) == 45 for x in wt)
+    uvr = open(tmp_path / "uvrange.tt").read().splitlines()
+    assert uvr[0] == " species     minimum        maximum" and len(uvr) == 3
