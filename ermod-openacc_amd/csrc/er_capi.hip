@@ -103,7 +103,7 @@ struct er_ctx {
     std::vector<double> splfc[3];
     DBuf<double> d_splfc1, d_splfc2, d_splfc3;
     DBuf<float> d_engfac;
-    DBuf<float2> d_tw1, d_tw2, d_tw3, d_twm;
+    DBuf<float2> d_tw1, d_tw2, d_tw3, d_twm, d_tws;
     DBuf<double> d_selfpart;
     DBuf<float2> d_T;
     DBuf<float> d_phi;
@@ -404,6 +404,16 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     HIPCHK(c, upload_tw(c->d_tw2, c->K2, s));
     HIPCHK(c, upload_tw(c->d_tw3, c->K3, s));
     HIPCHK(c, upload_tw(c->d_twm, c->K1 / 2, s));
+    {
+        std::vector<float2> t(ER_TWS_TOTAL);
+        for (int r = 4; r <= 10; ++r)
+            for (int k = 0; k < r; ++k) {
+                const double a = 2.0 * M_PI * (double)k / (double)r;
+                t[tws_off(r) + k] = make_float2((float)std::cos(a), (float)std::sin(a));
+            }
+        HIPCHK(c, c->d_tws.ensure(ER_TWS_TOTAL));
+        HIPCHK(c, hipMemcpy(c->d_tws.p, t.data(), ER_TWS_TOTAL * sizeof(float2), hipMemcpyHostToDevice));
+    }
     HIPCHK(c, c->d_engfac.ensure(c->Gc));
 
     // per-atom frame buffers
@@ -709,19 +719,31 @@ extern "C" int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sl
 
 // ---- kernel dispatch on compile-time grid sizes and spline order ----
 
+// line split N = R1 * R2 of the supported lengths for the four-step plane FFT
+template <int K> struct Split;
+template <> struct Split<32> { static constexpr int A = 8, B = 4; };
+template <> struct Split<36> { static constexpr int A = 9, B = 4; };
+template <> struct Split<40> { static constexpr int A = 8, B = 5; };
+template <> struct Split<48> { static constexpr int A = 8, B = 6; };
+template <> struct Split<64> { static constexpr int A = 8, B = 8; };
+template <> struct Split<72> { static constexpr int A = 8, B = 9; };
+template <> struct Split<80> { static constexpr int A = 8, B = 10; };
+
 template <int K> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch)
 {
-    const int NT = ((K + 63) / 64) * 64;
+    constexpr int R1 = Split<K>::A, R2 = Split<K>::B;
+    const int NT = 256;
     const size_t lds = (size_t)K * (K + 1) * sizeof(float2) + (size_t)ER_SCH * 2 * K * sizeof(float2) +
-                       ER_SCH * sizeof(float2) + (NT / 64) * sizeof(double);
+                       ER_SCH * sizeof(float2) + 2 * K * sizeof(float2) + (NT / 64) * sizeof(double);
     static bool attr = false;
     if (!attr && lds > 65536) {
-        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K, K>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K, K, R1, R2, R1, R2>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_pme_spec2d<K, K>), dim3(c->H, nch), dim3(NT), lds, c->stream, c->nslt, c->ord, c->K1,
-                       n_off, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_engfac.p, c->d_tw1.p, c->d_tw2.p,
-                       c->d_tw3.p, c->d_T.p, c->d_selfpart.p);
+    hipLaunchKernelGGL((k_pme_spec2d<K, K, R1, R2, R1, R2>), dim3(c->H, nch), dim3(NT), lds, c->stream, c->nslt,
+                       c->ord, c->K1, n_off, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_engfac.p, c->d_tw1.p,
+                       c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
     return hipGetLastError();
 }
 
@@ -748,18 +770,10 @@ static int divisor_le(int K, int cap)
 
 template <int K1, int NI> static hipError_t launch_axis1_k(er_ctx *c, int nch)
 {
-    const int R = (NI == 1) ? c->K3 : divisor_le(c->K3, 256 / NI);
-    const int H = K1 / 2 + 1;
-    const int LS = (H % 2 == 1) ? H : H + 1;
-    const size_t lds = std::max((size_t)R * NI * LS * sizeof(float2), (size_t)R * K1 * NI * sizeof(float));
+    const int R = divisor_le(c->K2, 256 / NI);  // x2 lines per block
     const int nt = ((R * NI + 63) / 64) * 64;
-    static bool attr = false;
-    if (!attr && lds > 65536) {
-        (void)hipFuncSetAttribute((const void *)k_pme_axis1<K1, NI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        attr = true;
-    }
     const int ntile = (nch + NI - 1) / NI;
-    hipLaunchKernelGGL((k_pme_axis1<K1, NI>), dim3(c->K3 / R, c->K2, ntile), dim3(nt), lds, c->stream, nch, c->K2,
+    hipLaunchKernelGGL((k_pme_axis1<K1, NI>), dim3(c->K2 / R, c->K3, ntile), dim3(nt), 0, c->stream, nch, c->K2,
                        c->K3, R, c->d_T.p, c->d_phi.p, c->d_tw1.p, c->d_twm.p);
     return hipGetLastError();
 }
@@ -1235,7 +1249,7 @@ extern "C" void er_destroy(er_ctx *c)
     c->d_sltpos_own.release(); c->d_wslt.release(); c->d_insposn.release(); c->d_insref.release();
     c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
-    c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release();
+    c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_T.release(); c->d_phi.release();
     c->d_U.release(); c->d_u0.release(); c->d_C.release(); c->d_w.release(); c->d_flceng.release();
     c->d_edens.release(); c->d_ecorr.release(); c->d_eself.release(); c->d_state.release(); c->d_slnuv.release();

@@ -138,7 +138,12 @@ template <int N> struct Plan;
             Stages<N, 1, SIGN, __VA_ARGS__>::run(a, tw);                           \
         }                                                                          \
     };
+ER_FFT_PLAN(4, 4)
+ER_FFT_PLAN(5, 5)
+ER_FFT_PLAN(6, 2, 3)
 ER_FFT_PLAN(8, 4, 2)
+ER_FFT_PLAN(9, 3, 3)
+ER_FFT_PLAN(10, 2, 5)
 ER_FFT_PLAN(12, 4, 3)
 ER_FFT_PLAN(16, 4, 4)
 ER_FFT_PLAN(18, 2, 3, 3)

@@ -249,37 +249,54 @@ __global__ void k_engfac(int K1, int K2, int K3, int ord, const double *__restri
 //   Fd_s(m) = sum_c W_s[d][c] exp(-2 pi i m (g_d - c)/K_d),
 // which equals the R2C transform of the spread grid (recpcal.F90:453-475).
 // The FP64 half-plane self energy (recpcal.F90:482-491) and the engfac
-// multiply (recpcal.F90:493-497) are fused, then the inverse DFT runs along
-// axis 3 (one row per thread) and axis 2 (one column per thread) in LDS.
+// multiply (recpcal.F90:493-497) are fused, then the plane is inverse
+// transformed along axis 3 and axis 2 in LDS by a two-pass (four-step)
+// split N = R1*R2 of each line: pass A runs R2-strided R1-point DFTs and the
+// W_N^{n2 k1} twiddle, pass B contiguous R2-point DFTs; every pass keeps all
+// 256 threads busy on small register DFTs.
+// T layout: [chunk][H][K3][K2] (x2 fastest).
 // ---------------------------------------------------------------------------
 #define ER_SCH 8  // solute sites per LDS table chunk
 
-template <int K2, int K3>
-__global__ void __launch_bounds__(128) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
+// offsets of the small-radix tables exp(+2 pi i k/r), r = 4..10, concatenated
+__host__ __device__ constexpr int tws_off(int r) { return r * (r - 1) / 2 - 6; }
+#define ER_TWS_TOTAL 49
+
+template <int K2, int K3, int R1, int R2, int C1, int C2>
+__global__ void __launch_bounds__(256) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
                                                     const float4 *__restrict__ insposn,
                                                     const float *__restrict__ insw, const int4 *__restrict__ insg,
                                                     const float *__restrict__ engfac, const float2 *__restrict__ tw1,
                                                     const float2 *__restrict__ tw2, const float2 *__restrict__ tw3,
-                                                    float2 *__restrict__ T, double *__restrict__ selfpart)
+                                                    const float2 *__restrict__ tws, float2 *__restrict__ T,
+                                                    double *__restrict__ selfpart)
 {
+    static_assert(R1 * R2 == K3 && C1 * C2 == K2, "line splits");
     constexpr int PS = K3 + 1;  // padded plane row stride
+    constexpr int NT = 256;
+    constexpr int NE = K2 * K3;
+    constexpr int EPT = (NE + NT - 1) / NT;
     extern __shared__ __align__(16) unsigned char smem[];
-    float2 *plane = reinterpret_cast<float2 *>(smem);   // [K2][PS]
-    float2 *F2 = plane + K2 * PS;                        // [SCH][K2]
-    float2 *F3 = F2 + ER_SCH * K2;                       // [SCH][K3]
-    float2 *A1 = F3 + ER_SCH * K3;                       // [SCH]
-    double *red = reinterpret_cast<double *>(A1 + ER_SCH);  // [blockDim/64]
+    float2 *plane = reinterpret_cast<float2 *>(smem);  // [K2][PS]
+    float2 *F2 = plane + K2 * PS;                       // [SCH][K2]
+    float2 *F3 = F2 + ER_SCH * K2;                      // [SCH][K3]
+    float2 *A1 = F3 + ER_SCH * K3;                      // [SCH]
+    float2 *t2 = A1 + ER_SCH;                           // [K2]
+    float2 *t3 = t2 + K2;                               // [K3]
+    double *red = reinterpret_cast<double *>(t3 + K3);  // [NT/64]
 
     const int H = K1 / 2 + 1;
     const int m1 = blockIdx.x;
     const int nl = blockIdx.y;
     const int n = n_off + nl;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tid = threadIdx.x;
+    for (int k = tid; k < K2; k += NT) t2[k] = tw2[k];
+    for (int k = tid; k < K3; k += NT) t3[k] = tw3[k];
 
-    float2 rho[K3];
+    // ---- generation of rho^ over the plane, sites in LDS chunks ----
+    float2 acc[EPT];
 #pragma unroll
-    for (int i = 0; i < K3; ++i) rho[i] = make_float2(0.f, 0.f);
-
+    for (int k = 0; k < EPT; ++k) acc[k] = make_float2(0.f, 0.f);
     for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
         const int ns = min(ER_SCH, nslt - s0);
         const int per = K2 + K3 + 1;
@@ -293,76 +310,149 @@ __global__ void __launch_bounds__(128) k_pme_spec2d(int nslt, int ord, int K1, i
             else if (r < K2 + K3) { axis = 2; m = r - K2; K = K3; gd = g.z; tw = tw3; }
             else { axis = 0; m = m1; K = K1; gd = g.x; tw = tw1; }
             const float *w = insw + site * 3 * ord + axis * ord;
-            float2 acc = make_float2(0.f, 0.f);
+            float2 a = make_float2(0.f, 0.f);
             for (int c = 0; c < ord; ++c) {
                 const int e = (int)(((long long)m * pmod(gd - c, K)) % K);
-                const float2 t = tw[e];  // exp(+2 pi i e/K); forward uses the conjugate
-                acc.x += w[c] * t.x;
-                acc.y -= w[c] * t.y;
+                const float2 t = tw[e];  // exp(+2 pi i e/K); the forward sum uses the conjugate
+                a.x += w[c] * t.x;
+                a.y -= w[c] * t.y;
             }
-            if (r < K2) F2[s * K2 + m] = acc;
-            else if (r < K2 + K3) F3[s * K3 + m] = acc;
-            else A1[s] = erfft::cscale(acc, insposn[site].w);
+            if (r < K2) F2[s * K2 + m] = a;
+            else if (r < K2 + K3) F3[s * K3 + m] = a;
+            else A1[s] = erfft::cscale(a, insposn[site].w);
         }
         __syncthreads();
-        if (tid < K2) {
-            for (int s = 0; s < ns; ++s) {
-                const float2 B = erfft::cmul(A1[s], F2[s * K2 + tid]);
 #pragma unroll
-                for (int m3 = 0; m3 < K3; ++m3) {
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NT;
+            if (e < NE) {
+                const int m2 = e % K2, m3 = e / K2;
+                for (int s = 0; s < ns; ++s) {
+                    const float2 B = erfft::cmul(A1[s], F2[s * K2 + m2]);
                     const float2 f3 = F3[s * K3 + m3];
-                    rho[m3].x += B.x * f3.x - B.y * f3.y;
-                    rho[m3].y += B.x * f3.y + B.y * f3.x;
+                    acc[k].x += B.x * f3.x - B.y * f3.y;
+                    acc[k].y += B.x * f3.y + B.y * f3.x;
                 }
             }
         }
         __syncthreads();
     }
-
+    // ---- engfac multiply + FP64 half-plane self energy ----
     double sp = 0.0;
-    if (tid < K2) {
+    {
         const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
-        const float *ef = engfac + (size_t)m1 * K3 * K2 + tid;
+        const float *ef = engfac + (size_t)m1 * NE;
 #pragma unroll
-        for (int m3 = 0; m3 < K3; ++m3) {
-            const float e = ef[(size_t)m3 * K2];
-            sp += wt * (double)e * ((double)rho[m3].x * rho[m3].x + (double)rho[m3].y * rho[m3].y);
-            rho[m3] = erfft::cscale(rho[m3], e);
+        for (int k = 0; k < EPT; ++k) {
+            const int e = tid + k * NT;
+            if (e < NE) {
+                const int m2 = e % K2, m3 = e / K2;
+                const float f = ef[e];  // [m1][m3][m2]
+                sp += wt * (double)f * ((double)acc[k].x * acc[k].x + (double)acc[k].y * acc[k].y);
+                plane[m2 * PS + m3] = erfft::cscale(acc[k], f);
+            }
         }
-        erfft::fft<K3, +1>(rho, tw3);
-#pragma unroll
-        for (int x3 = 0; x3 < K3; ++x3) plane[tid * PS + x3] = rho[x3];
     }
     __syncthreads();
-    if (tid < K3) {
-        float2 col[K2];
+    // ---- rows (axis 3, length K3 = R1*R2), pass A in place ----
+    for (int it = tid; it < K2 * R2; it += NT) {
+        const int m2 = it / R2, n2 = it % R2;
+        float2 v[R1];
 #pragma unroll
-        for (int m2 = 0; m2 < K2; ++m2) col[m2] = plane[m2 * PS + tid];
-        erfft::fft<K2, +1>(col, tw2);
+        for (int a = 0; a < R1; ++a) v[a] = plane[m2 * PS + R2 * a + n2];
+        erfft::fft<R1, +1>(v, tws + tws_off(R1));
 #pragma unroll
-        for (int x2 = 0; x2 < K2; ++x2) plane[x2 * PS + tid] = col[x2];
+        for (int k1 = 1; k1 < R1; ++k1) v[k1] = erfft::cmul(v[k1], t3[(n2 * k1) % K3]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) plane[m2 * PS + R2 * k1 + n2] = v[k1];
     }
     __syncthreads();
-    float2 *out = T + ((size_t)nl * H + m1) * (size_t)(K2 * K3);
-    for (int f = tid; f < K2 * K3; f += NT) out[f] = plane[(f / K3) * PS + (f % K3)];
+    // ---- rows, pass B (outputs held until every thread has read) ----
+    {
+        constexpr int IB = (K2 * R1 + NT - 1) / NT;
+        float2 hold[IB][R2];
+#pragma unroll
+        for (int j = 0; j < IB; ++j) {
+            const int it = tid + j * NT;
+            if (it < K2 * R1) {
+                const int m2 = it / R1, k1 = it % R1;
+#pragma unroll
+                for (int b = 0; b < R2; ++b) hold[j][b] = plane[m2 * PS + R2 * k1 + b];
+                erfft::fft<R2, +1>(hold[j], tws + tws_off(R2));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IB; ++j) {
+            const int it = tid + j * NT;
+            if (it < K2 * R1) {
+                const int m2 = it / R1, k1 = it % R1;
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) plane[m2 * PS + k1 + R1 * k2] = hold[j][k2];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- columns (axis 2, length K2 = C1*C2), pass A in place ----
+    for (int it = tid; it < C2 * K3; it += NT) {
+        const int n2 = it / K3, x3 = it % K3;
+        float2 v[C1];
+#pragma unroll
+        for (int a = 0; a < C1; ++a) v[a] = plane[(C2 * a + n2) * PS + x3];
+        erfft::fft<C1, +1>(v, tws + tws_off(C1));
+#pragma unroll
+        for (int k1 = 1; k1 < C1; ++k1) v[k1] = erfft::cmul(v[k1], t2[(n2 * k1) % K2]);
+#pragma unroll
+        for (int k1 = 0; k1 < C1; ++k1) plane[(C2 * k1 + n2) * PS + x3] = v[k1];
+    }
+    __syncthreads();
+    // ---- columns, pass B ----
+    {
+        constexpr int IC = (C1 * K3 + NT - 1) / NT;
+        float2 hold[IC][C2];
+#pragma unroll
+        for (int j = 0; j < IC; ++j) {
+            const int it = tid + j * NT;
+            if (it < C1 * K3) {
+                const int k1 = it / K3, x3 = it % K3;
+#pragma unroll
+                for (int b = 0; b < C2; ++b) hold[j][b] = plane[(C2 * k1 + b) * PS + x3];
+                erfft::fft<C2, +1>(hold[j], tws + tws_off(C2));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IC; ++j) {
+            const int it = tid + j * NT;
+            if (it < C1 * K3) {
+                const int k1 = it / K3, x3 = it % K3;
+#pragma unroll
+                for (int k2 = 0; k2 < C2; ++k2) plane[(k1 + C1 * k2) * PS + x3] = hold[j][k2];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- out: T[nl][m1][x3][x2] ----
+    float2 *out = T + ((size_t)nl * H + m1) * (size_t)NE;
+    for (int f = tid; f < NE; f += NT) out[f] = plane[(f % K2) * PS + (f / K2)];
 
-    // deterministic block reduction of the self-energy partial
     for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = sp;
     __syncthreads();
     if (tid == 0) {
-        double s = 0.0;
-        for (int wv = 0; wv < NT / 64; ++wv) s += red[wv];
-        selfpart[(size_t)n * H + m1] = s;
+        double t = 0.0;
+        for (int wv = 0; wv < NT / 64; ++wv) t += red[wv];
+        selfpart[(size_t)n * H + m1] = t;
     }
 }
 
 // ---------------------------------------------------------------------------
-// PME, stage B: complex-to-real inverse DFT along axis 1 for NI insertions x
-// R lines, via a half-length complex transform:
+// PME, stage B: complex-to-real inverse DFT along axis 1 of NI insertions x
+// R lines (x2 range) of one x3, via a half-length complex transform:
 //   Z[k] = (X[k] + conj X[M-k]) + i e^{+2 pi i k/K1} (X[k] - conj X[M-k]),
 //   z = IDFT_M(Z),  phi[2j] = Re z[j],  phi[2j+1] = Im z[j],  M = K1/2.
-// Output is written with the NI insertions interleaved.
+// One line per thread (insertion fastest across lanes), so the output is
+// written directly as 64-byte runs of NI interleaved insertions.
 // ---------------------------------------------------------------------------
 template <int K1, int NI>
 __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, int R,
@@ -371,27 +461,18 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
 {
     constexpr int M = K1 / 2;
     constexpr int H = M + 1;
-    constexpr int LS = (H % 2 == 1) ? H : H + 1;  // odd stride in float2
-    extern __shared__ __align__(16) unsigned char smem[];
-    float2 *buf = reinterpret_cast<float2 *>(smem);  // [R*NI][LS]
-    float *obuf = reinterpret_cast<float *>(smem);   // [R][K1][NI] (reuses buf)
-
-    const int tile = blockIdx.z, x2 = blockIdx.y, x3_0 = blockIdx.x * R;
-    const int tid = threadIdx.x, NT = blockDim.x;
+    const int tile = blockIdx.z, x3 = blockIdx.y, x2_0 = blockIdx.x * R;
+    const int tid = threadIdx.x;
+    const int i = tid % NI, r = tid / NI;
+    if (r >= R) return;
+    const int nl = tile * NI + i;
     const size_t plane = (size_t)K2 * K3;
-
-    for (int f = tid; f < NI * H * R; f += NT) {
-        const int r = f % R, m1 = (f / R) % H, i = f / (R * H);
-        const int nl = tile * NI + i;
-        float2 v = make_float2(0.f, 0.f);
-        if (nl < nchunk) v = T[((size_t)nl * H + m1) * plane + (size_t)x2 * K3 + x3_0 + r];
-        buf[(r * NI + i) * LS + m1] = v;
-    }
-    __syncthreads();
     float2 z[M];
-    const int line = tid;  // = r*NI + i
-    if (line < R * NI) {
-        const float2 *X = buf + line * LS;
+    if (nl < nchunk) {
+        const float2 *src = T + (size_t)nl * H * plane + (size_t)x3 * K2 + x2_0 + r;
+        float2 X[H];
+#pragma unroll
+        for (int m = 0; m < H; ++m) X[m] = src[(size_t)m * plane];
 #pragma unroll
         for (int k = 0; k < M; ++k) {
             const float2 a = X[k], b = X[M - k];
@@ -401,22 +482,16 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
             z[k] = make_float2(e.x - o.y, e.y + o.x);                     // e + i o
         }
         erfft::fft<M, +1>(z, twm);
-    }
-    __syncthreads();
-    if (line < R * NI) {
-        const int r = line / NI, i = line % NI;
+    } else {
 #pragma unroll
-        for (int j = 0; j < M; ++j) {
-            obuf[(r * K1 + 2 * j) * NI + i] = z[j].x;
-            obuf[(r * K1 + 2 * j + 1) * NI + i] = z[j].y;
-        }
+        for (int k = 0; k < M; ++k) z[k] = make_float2(0.f, 0.f);
     }
-    __syncthreads();
     const size_t G = plane * K1;
-    float *dst = phi + (size_t)tile * G * NI;
-    for (int f = tid; f < R * K1 * NI; f += NT) {
-        const int r = f / (K1 * NI), rem = f % (K1 * NI);
-        dst[(((size_t)(x3_0 + r) * K2 + x2) * K1) * NI + rem] = obuf[f];
+    float *dst = phi + (size_t)tile * G * NI + (((size_t)x3 * K2 + x2_0 + r) * K1) * NI + i;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        dst[(2 * j) * NI] = z[j].x;
+        dst[(2 * j + 1) * NI] = z[j].y;
     }
 }
 

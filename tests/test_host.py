@@ -130,3 +130,9 @@ def test_context_without_library_fails_loudly(monkeypatch):
     monkeypatch.setattr(capi, "_lib", None)
     with pytest.raises(ImportError):
         capi.load()
+
+
+def test_fortran_host_binary_loads():
+    host = os.path.join(os.path.dirname(capi.LIB_PATH), "erdst_host")
+    r = subprocess.run([host], capture_output=True, text=True, timeout=60)
+    assert "usage: erdst_host" in (r.stdout + r.stderr)
