@@ -70,26 +70,45 @@ def run_frames(ctx, w, pos, ins, mode, capi, nins=None):
         ctx.run(mode)
 
 
-def cpu_baseline(w, params, budget_s):
-    """Oracle (FP64 restatement of the reference algorithm: brute-force pairs,
-    one PME grid per insertion, binary-search histogram, outer-product ecorr)
-    on the host cores, over a bounded sample of frame 0's insertions."""
+def cpu_baseline(soln, refs, frames_per_block=5, nblocks=2, nins=100, nthreads=None):
+    """The oracle (FP64 restatement of the reference algorithm: brute-force
+    pairs, one PME grid per insertion, binary-search histogram, outer-product
+    ecorr) on the host cores over a bounded sample of the same workload:
+    frames 0..9 with their first 100 insertions each (1000 insertions), plus
+    the solution run of the same solute (10 frames) that the free energy
+    needs.  Returns the baseline record and the oracle's block results."""
     import erutil as U
-    threads = max(1, min(os.cpu_count() or 1, 16))
-    d = U.oracle_distribution(w, params)
+    threads = nthreads or max(1, min(os.cpu_count() or 1, 16))
+    t0 = time.perf_counter()
+    sblocks, _ = U.oracle_dmu_blocks(soln, refs, frames_per_block, nblocks, 0, nthreads=threads)
+    t1 = time.perf_counter()
+    _, rblocks = U.oracle_dmu_blocks(soln, refs, frames_per_block, 0, nins, nthreads=threads,
+                                     refs_frames=frames_per_block * nblocks)
+    t2 = time.perf_counter()
+    n = frames_per_block * nblocks * nins
+    rec = dict(value=round(n / (t2 - t1), 2), unit="insertions/s", cores=threads, kind="port",
+               sample=f"frames 0-{frames_per_block * nblocks - 1}, first {nins} insertions each ({n} insertions, "
+                      f"{t2 - t1:.1f} s); solution run {frames_per_block * nblocks} frames ({t1 - t0:.1f} s)")
+    return rec, sblocks, rblocks
 
-    def run(n):
-        t0 = time.perf_counter()
-        uv, uv0 = U.oracle_refs(w, 0, params, nins=n, nthreads=threads)
-        for k in range(n):
-            d.update(w.topo.uvspec, uv[k], uv0[k])
-        return time.perf_counter() - t0
 
-    t_probe = run(threads)
-    n = int(max(threads, min(w.nins, threads * max(1, int(budget_s / max(t_probe, 1e-3))))))
-    t = run(n)
-    return dict(value=n / t, unit="insertions/s", cores=threads, kind="port",
-                sample=f"frame 0, first {n} insertions of the same workload ({t:.1f} s)")
+def dmu_error(soln, refs, sg, rg, so, ro, workdir):
+    """Solvation free energy (slvfe's merge over engstore's files) from the
+    GPU's and the oracle's distributions of the same frames and insertions."""
+    from erdst_gpu import slvfe
+    import tempfile
+    from erdst_gpu.params import hot_path_params
+    kT = hot_path_params(refs.slttype, refs.params["ms"], splodr=4)["temp"]
+    mesh = refs.uvmax_uvsoft_uvcrd()
+    with tempfile.TemporaryDirectory(dir=workdir) as d:
+        a = slvfe.dmu_from_blocks(sg, rg, *mesh, kT, os.path.join(d, "gpu"))
+        b = slvfe.dmu_from_blocks(so, ro, *mesh, kT, os.path.join(d, "oracle"))
+    err = np.abs(a.chmpt - b.chmpt)
+    return dict(value_gpu=round(a.dmu, 6), value_oracle=round(b.dmu, 6), abs_err=float(f"{err[0, a.grref].max():.3e}"),
+                max_abs_err_all_groups=float(f"{err.max():.3e}"), unit="kcal/mol", bar=0.01,
+                group=a.groups[a.grref][0], blocks=a.numrun,
+                sample="solution: 2 blocks x 5 frames of the solute in the same 30000-water box; "
+                       "reference: frames 0-9 x 100 insertions; slvfe defaults (reg, eczr, py-hnc)")
 
 
 def main():
@@ -98,7 +117,6 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--plain-insertions", type=int, default=200)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -191,9 +209,15 @@ def main():
         "cpu_baseline": None,
     }
     ctx.close()
+    out["dmu"] = None
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(w, params, args.cpu_seconds)
-        out["config"]["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        import erutil as U
+        soln, refs = synth.dmu_pair(2)
+        sg, rg = U.gpu_dmu_blocks(soln, refs, 5, 2, 100)
+        rec, so, ro = cpu_baseline(soln, refs)
+        out["cpu_baseline"] = rec
+        out["config"]["gpu_over_cpu"] = round(value / rec["value"], 1)
+        out["dmu"] = dmu_error(soln, refs, sg, rg, so, ro, ROOT)
     print(json.dumps(out))
 
 

@@ -221,3 +221,49 @@ def config1() -> Workload:
                  mesh=[mesh], seed=seed, slttype=1)
     w._frame_fn = frame
     return w
+
+
+def soln_water(config: int, nwater: int, L: float, nframes: int, grid: int,
+               solute: Species | None = None, name: str = "") -> Workload:
+    """Solution run of the rigid test particle of refs_water in the same
+    water box: the solute sits at the box centre with a random orientation
+    per frame, and the waters avoid its sites (3.2 A from any solute site),
+    so every solute-water energy stays inside the linear part of the mesh
+    (erdst halts with eng_sft beyond it in solution runs,
+    erdst/engproc.F90:1150-1156).  With refs_water(config, ...) it forms the
+    (engsln, engref/corref) pair from which slvfe computes the solvation
+    free energy."""
+    seed = 20261202 + 100 + config
+    solute = solute or tetra_species()
+    topo = build_topology([solute, water_species(nwater)], solute_index=0, refs=False)
+    box = np.array([L, L, L])
+    geom_w = water_geometry()
+    centre = box / 2
+
+    def frame(f):
+        rng = _stream(seed, f)
+        R = random_rotations(rng, 1)[0]
+        sx = centre + solute.geometry @ R.T
+        wc = lattice_centres(rng, nwater, box, avoid=sx, avoid_r=3.2, spare=max(64, nwater // 50))
+        wx = place_rigid(rng, geom_w, wc)
+        return np.concatenate([sx, wx]).astype(np.float32)
+
+    mesh = linear_log_mesh(-20.0, 5.0, 0.05, 1.0e4, 100)
+    w = Workload(name or f"config{config}-soln", topo, box, nframes=nframes, nins=1,
+                 params=dict(ms=(grid, grid, grid)), mesh=[mesh], seed=seed, slttype=1)
+    w._frame_fn = frame
+    return w
+
+
+def dmu_pair(config: int = 0, nframes: int | None = None):
+    """(solution, reference) workloads of the tetrahedral solute in water at
+    the sizes of configuration 0 (1000 waters) or 2 (30 000 waters)."""
+    if config == 0:
+        refs = config0()
+        soln = soln_water(0, 1000, 31.04, nframes or 10, 32)
+    elif config == 2:
+        refs = config2()
+        soln = soln_water(2, 30000, 67.0, nframes or 200, 72)
+    else:
+        raise ValueError("dmu_pair covers configurations 0 and 2")
+    return soln, refs

@@ -6,7 +6,8 @@
 ! &ene_param values, the energy-coordinate mesh and, per frame, the frame's
 ! coordinates and the insertion coordinates (layout: read_workload below).
 ! Output (current directory): engref.XX / corref.XX (engsln / corsln for
-! soln runs), weight_refs|weight_soln, uvrange.tt, as engstore writes them.
+! soln runs), weight_refs|weight_soln, aveuv.tt (soln), uvrange.tt, as engstore
+! writes them.
 program erdst_host
   use iso_c_binding
   use er_capi
@@ -22,6 +23,7 @@ program erdst_host
   real(c_float), allocatable :: sitepos(:,:), sltpos(:,:), wslt(:), minuv(:), maxuv(:)
   real(c_float) :: cell(3,3)
   real(c_double), allocatable :: uvcrd(:), edens(:), ecorr(:), slnuv(:), avediv(:,:), escrd(:), eself(:)
+  real(c_double), allocatable :: aveuv(:,:)
   real(c_double) :: scal(4), dummy(1)
   integer :: iu, rc, f, division, frames_per_div, k
   integer, parameter :: stdout = 6
@@ -54,7 +56,7 @@ program erdst_host
   call check(rc, "er_set_solute")
 
   allocate(edens(ermax), ecorr(int(ermax, 8) * ermax), slnuv(numslv), minuv(0:numslv), maxuv(0:numslv))
-  allocate(avediv(engdiv, 2))
+  allocate(avediv(engdiv, 2), aveuv(engdiv, numslv))
   frames_per_div = nframes / engdiv
   do f = 1, nframes
      read(iu) sitepos
@@ -173,11 +175,19 @@ contains
     avslf = avslf / engnorm
     avediv(division, 1) = engnorm / engsmpl
     if (prm%slttype == 1) then
+       aveuv(division, :) = slnuv(:) / engnorm
        avediv(division, 2) = voffset - prm%temp * log(avslf)
     else
        avediv(division, 2) = voffset + prm%temp * log(avslf)
     end if
     if (division == engdiv) then
+       if (prm%slttype == 1) then
+          open(newunit=io, file='aveuv.tt', action='write')
+          do kk = 1, engdiv
+             write(io, '(i5,9999f15.5)') kk, aveuv(kk, :)
+          end do
+          close(io)
+       end if
        if (prm%slttype == 1) then
           open(newunit=io, file='weight_soln', action='write')
        else

@@ -87,3 +87,70 @@ def energy_mismatch(e_gpu, e_ref, rtol=RTOL, atol=ATOL):
     excess = np.abs(e_gpu - e_ref) - energy_tolerance(e_ref, rtol, atol)
     worst = np.unravel_index(np.argmax(excess), excess.shape)
     return int((excess > 0).sum()), float(excess.max()), worst
+
+
+def oracle_block(d) -> dict:
+    """An oracle Distribution in er_fetch's dict layout."""
+    g = dict(edens=d.edens.copy(), ecorr=d.ecorr.reshape(d.ermax, d.ermax).copy(), slnuv=d.slnuv.copy())
+    g.update(d.scalars)
+    return g
+
+
+def oracle_dmu_blocks(soln, refs, frames_per_block, nblocks_soln, nins, f32_energies=False, nthreads=0,
+                      refs_frames=None):
+    """Oracle soln and refs block results for slvfe (nblocks_soln blocks of
+    frames_per_block frames each; refs in one block over refs_frames frames,
+    by default as many as the soln run has, with nins insertions each; a
+    side with no frames or insertions is skipped).  With f32_energies the
+    FP64 energies are rounded to float32 before binning."""
+    def cast(x):
+        return x.astype(np.float32).astype(np.float64) if f32_energies else x
+    ps, pr = workload_params(soln), workload_params(refs)
+    sblocks = []
+    ds = oracle_distribution(soln, ps)
+    sp = species_of(soln.topo, 1, soln.topo.tagslt)[0]
+    for b in range(nblocks_soln):
+        ds.clear()
+        for f in range(b * frames_per_block, (b + 1) * frames_per_block):
+            uv, uv0 = oracle_soln(soln, f, ps, nthreads=nthreads)
+            ds.update(sp, cast(uv[0]), float(cast(uv0)[0]))
+        sblocks.append(oracle_block(ds))
+    nref = nblocks_soln * frames_per_block if refs_frames is None else refs_frames
+    if nref == 0 or nins == 0:
+        return sblocks, []
+    dr = oracle_distribution(refs, pr)
+    for f in range(nref):
+        uv, uv0 = oracle_refs(refs, f, pr, nins=nins, nthreads=nthreads)
+        uv, uv0 = cast(uv), cast(uv0)
+        for n in range(uv.shape[0]):
+            dr.update(refs.topo.uvspec, uv[n], float(uv0[n]))
+    return sblocks, [oracle_block(dr)]
+
+
+def gpu_dmu_blocks(soln, refs, frames_per_block, nblocks_soln, nins, mode=None):
+    """The GPU path's soln and refs block results for slvfe, on the same
+    frames and insertions as oracle_dmu_blocks."""
+    from erdst_gpu import capi
+    mode = capi.ER_BATCHED if mode is None else mode
+    ps, pr = workload_params(soln), workload_params(refs)
+    ctx = capi.Context(soln.topo, ps)
+    ctx.set_mesh(*soln.uvmax_uvsoft_uvcrd())
+    ctx.set_solute_molecules(soln.topo.tagslt)
+    sblocks = []
+    for b in range(nblocks_soln):
+        for f in range(b * frames_per_block, (b + 1) * frames_per_block):
+            ctx.set_frame(soln.frame(f), soln.cell)
+            ctx.run(mode)
+        sblocks.append(ctx.fetch(want_ecorr=False))
+        ctx.clear()
+    ctx.close()
+    ctx = capi.Context(refs.topo, pr)
+    ctx.set_mesh(*refs.uvmax_uvsoft_uvcrd())
+    ctx.set_solute(refs.topo.slt_ljtype, refs.topo.slt_charge32)
+    for f in range(nblocks_soln * frames_per_block):
+        ctx.set_frame(refs.frame(f), refs.cell)
+        ctx.set_insertions(refs.insertions(f)[:nins])
+        ctx.run(mode)
+    rblocks = [ctx.fetch()]
+    ctx.close()
+    return sblocks, rblocks

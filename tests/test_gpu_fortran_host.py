@@ -61,3 +61,25 @@ def test_fortran_host_two_blocks(tmp_path):
     assert len(wt) == 2 and all(len(x) == 45 for x in wt)
     uvr = open(tmp_path / "uvrange.tt").read().splitlines()
     assert uvr[0] == " species     minimum        maximum" and len(uvr) == 3
+
+
+@pytest.mark.gpu
+def test_fortran_host_files_feed_slvfe(tmp_path):
+    """erdst (Fortran host, soln and refs runs) -> files -> slvfe merge: the
+    solvation free energy equals the one from the Python binding's blocks."""
+    from erdst_gpu import slvfe
+    soln, refs = synth.dmu_pair(0)
+    for sub, w, engdiv in (("soln", soln, 2), ("refs", refs, 1)):
+        d = tmp_path / sub
+        d.mkdir()
+        p = U.workload_params(w)
+        write_workload(d / "workload.bin", w, p, range(10), engdiv=engdiv, nins=100)
+        r = subprocess.run([HOST, str(d / "workload.bin")], cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+    assert (tmp_path / "soln" / "aveuv.tt").exists()
+    res = slvfe.run_merge(slvfe.FeVars(), str(tmp_path))
+    sg, rg = U.gpu_dmu_blocks(soln, refs, 5, 2, 100)
+    kT = U.workload_params(refs)["temp"]
+    ref = slvfe.dmu_from_blocks(sg, rg, *refs.uvmax_uvsoft_uvcrd(), kT, str(tmp_path / "py"))
+    print(f"dmu from host files {res.dmu:.6f}, from the binding {ref.dmu:.6f}")
+    np.testing.assert_allclose(res.chmpt, ref.chmpt, rtol=0, atol=1e-9)
