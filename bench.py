@@ -104,8 +104,8 @@ def dmu_error(soln, refs, sg, rg, so, ro, workdir):
         a = slvfe.dmu_from_blocks(sg, rg, *mesh, kT, os.path.join(d, "gpu"))
         b = slvfe.dmu_from_blocks(so, ro, *mesh, kT, os.path.join(d, "oracle"))
     err = np.abs(a.chmpt - b.chmpt)
-    return dict(value_gpu=round(a.dmu, 6), value_oracle=round(b.dmu, 6), abs_err=float(f"{err[0, a.grref].max():.3e}"),
-                max_abs_err_all_groups=float(f"{err.max():.3e}"), unit="kcal/mol", bar=0.01,
+    return dict(value_gpu=round(a.dmu, 6), value_oracle=round(b.dmu, 6), abs_err=float(f"{abs(a.dmu - b.dmu):.3e}"),
+                max_abs_err_any_block_or_group=float(f"{err.max():.3e}"), unit="kcal/mol", bar=0.01,
                 group=a.groups[a.grref][0], blocks=a.numrun,
                 sample="solution: 2 blocks x 5 frames of the solute in the same 30000-water box; "
                        "reference: frames 0-9 x 100 insertions; slvfe defaults (reg, eczr, py-hnc)")

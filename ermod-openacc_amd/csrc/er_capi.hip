@@ -17,6 +17,7 @@
 namespace {
 
 constexpr int NI_B = 16;  // insertion interleave of the batched path
+constexpr int ER_KSLICES = 16;  // insertion slices of the split-K edens/ecorr sums
 
 template <typename T> struct DBuf {
     T *p = nullptr;
@@ -137,6 +138,7 @@ struct er_ctx {
 
     // accumulators
     DBuf<double> d_edens, d_ecorr, d_eself, d_state, d_slnuv;
+    DBuf<double> d_part, d_epart;  // split-K partial sums of ecorr / edens
     DBuf<int> d_flags, d_warn;
     DBuf<float> d_fstate;
     DBuf<unsigned int> d_minmax;
@@ -535,6 +537,8 @@ extern "C" int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsof
         HIPCHK(c, hipMemcpyAsync(c->d_escrd.p, c->escrd.data(), c->esmax * sizeof(double), hipMemcpyHostToDevice, s));
     HIPCHK(c, c->d_edens.ensure(tot));
     HIPCHK(c, c->d_ecorr.ensure((size_t)tot * tot));
+    HIPCHK(c, c->d_epart.ensure((size_t)ER_KSLICES * tot));
+    if (c->prm.corrcal == 1 && c->use_mfma) HIPCHK(c, c->d_part.ensure((size_t)ER_KSLICES * tot * tot));
     HIPCHK(c, c->d_eself.ensure(std::max(1, c->esmax)));
     HIPCHK(c, hipStreamSynchronize(s));
     c->have_mesh = true;
@@ -966,6 +970,8 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
     {
         ProfScope ps(c, ER_K_PAIRS);
         PairArgs pa{};
+        // enough workgroups per insertion to fill the chip (at least 4 per CU)
+        pa.nsplit = std::max(1, std::min(16, (4 * 256 + nch - 1) / nch));
         pa.nmol = c->nummol;
         pa.nchunk = nch;
         pa.n_off = n_off;
@@ -975,7 +981,7 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
         pa.nc = c->nc;
         pa.pp = c->pp;
         pa.cell_w = c->cell_w;
-        hipLaunchKernelGGL(k_pairs, dim3(nch), dim3(256), 0, c->stream, pa, c->fc, c->d_molbeg.p, c->d_posn.p,
+        hipLaunchKernelGGL(k_pairs, dim3(nch * pa.nsplit), dim3(256), 0, c->stream, pa, c->fc, c->d_molbeg.p, c->d_posn.p,
                            c->d_ljtype.p, c->d_molref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p,
                            c->d_insref.p, c->d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
                            c->d_ljlensq.p, c->d_U.p);
@@ -1017,19 +1023,29 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
                            c->d_flags.p, c->d_fstate.p, c->d_u0.p, c->d_w.p, c->d_escrd.p, c->d_eself.p, c->d_minmax.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
                            c->d_slnuv.p, c->d_errkey.p, c->nummol);
-        hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256), dim3(256), 0, c->stream, nch, n_off, c->ermax,
-                           c->d_C.p, c->d_w.p, c->d_edens.p);
     }
-    if (c->prm.corrcal == 1) {
+    // insertion slices of at most kper (a multiple of 4) for the split-K sums
+    const int kper = std::max(4, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
+    const int nks = (nch + kper - 1) / kper;
+    const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
+    {
         ProfScope ps(c, ER_K_ECORR);
-        if (c->use_mfma) {
-            const int nb = (c->ermax + 63) / 64;
-            hipLaunchKernelGGL(k_ecorr, dim3(nb, nb), dim3(256), 0, c->stream, nch, n_off, c->ermax, c->d_C.p,
-                               c->d_w.p, c->d_dmap.p, c->d_ecorr.p);
+        const int nb = (c->ermax + 63) / 64;
+        if (mfma) {
+            hipLaunchKernelGGL(k_ecorr, dim3(nb, nb, nks), dim3(256), 0, c->stream, nch, n_off, c->ermax, kper,
+                               c->d_C.p, c->d_w.p, c->d_dmap.p, c->d_part.p, c->d_epart.p);
         } else {
-            const int nb = (c->ermax + 15) / 16;
-            hipLaunchKernelGGL(k_ecorr_valu, dim3(nb, nb), dim3(16, 16), 0, c->stream, nch, n_off, c->ermax, c->d_C.p,
-                               c->d_w.p, c->d_ecorr.p);
+            hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256, nks), dim3(256), 0, c->stream, nch, n_off,
+                               c->ermax, kper, c->d_C.p, c->d_w.p, c->d_epart.p);
+        }
+        const bool full = mfma;
+        hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, full ? c->ermax : 1), dim3(256), 0,
+                           c->stream, c->ermax, nks, c->d_part.p, c->d_epart.p,
+                           full ? c->d_ecorr.p : (double *)nullptr, c->d_edens.p);
+        if (c->prm.corrcal == 1 && !c->use_mfma) {
+            const int nb16 = (c->ermax + 15) / 16;
+            hipLaunchKernelGGL(k_ecorr_valu, dim3(nb16, nb16), dim3(16, 16), 0, c->stream, nch, n_off, c->ermax,
+                               c->d_C.p, c->d_w.p, c->d_ecorr.p);
         }
     }
     return hipGetLastError();
@@ -1252,7 +1268,7 @@ extern "C" void er_destroy(er_ctx *c)
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_T.release(); c->d_phi.release();
     c->d_U.release(); c->d_u0.release(); c->d_C.release(); c->d_w.release(); c->d_flceng.release();
-    c->d_edens.release(); c->d_ecorr.release(); c->d_eself.release(); c->d_state.release(); c->d_slnuv.release();
+    c->d_edens.release(); c->d_ecorr.release(); c->d_part.release(); c->d_epart.release(); c->d_eself.release(); c->d_state.release(); c->d_slnuv.release();
     c->d_flags.release(); c->d_warn.release(); c->d_fstate.release(); c->d_minmax.release(); c->d_errkey.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;

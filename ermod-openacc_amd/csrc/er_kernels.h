@@ -677,6 +677,19 @@ __global__ void k_atom_brick_count(int natom, const int4 *__restrict__ splg, int
     key_rank[2 * a + 1] = atomicAdd(&cnt[key], 1u);
 }
 
+// wrap a region coordinate that lies in [-K, 2K) into [0, K)
+__device__ __forceinline__ int wrap1(int x, int K)
+{
+    x += (x < 0) ? K : 0;
+    return x - ((x >= K) ? K : 0);
+}
+
+// Staging: every thread issues its ceil(NP*NV/512) 16-byte loads before any
+// LDS store, so a workgroup has its whole region in flight at once.  Compute:
+// four lanes per atom, each lane owning four of the NI = 16 insertions
+// (one float4 of a grid point), so every stencil read is one ds_read_b128
+// and a wave covers 16 atoms.  Per insertion the arithmetic order is that of
+// k_gather (x-sum with w1, then (w3*w2) * sum), so results are identical.
 template <int ORD, int NI>
 __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick_pad,
                                                       int nxcd, int nchunk, const unsigned int *__restrict__ bstart,
@@ -686,8 +699,14 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
                                                       const int4 *__restrict__ splg, const float *__restrict__ phi,
                                                       int natom, float *__restrict__ Eat)
 {
+    static_assert(NI == 16, "the brick gather stages 16 interleaved insertions");
     constexpr int RS = ER_BRICK + ORD - 1;  // region side
-    extern __shared__ __align__(16) float reg[];  // [RS][RS][RS][NI]
+    constexpr int NV = NI / 4;              // float4 per grid point
+    constexpr int NP = RS * RS * RS;
+    constexpr int NF4 = NP * NV;
+    constexpr int NT = 512;
+    constexpr int PER = (NF4 + NT - 1) / NT;
+    extern __shared__ __align__(16) float4 reg4[];  // [RS][RS][RS][NV]
     // XCD-aware remap: each XCD sweeps a contiguous range of bricks, tile by tile
     const int b = blockIdx.x;
     const int per = nbrick_pad / nxcd;
@@ -701,23 +720,31 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     const int bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
     const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * ER_BRICK - (ORD - 1);
     const size_t G = (size_t)K1 * K2 * K3;
-    const float *ph = phi + (size_t)tile * G * NI;
-    const int tid = threadIdx.x, NT = blockDim.x;
-    // stage the region: one 16-insertion run per grid point, x fastest
-    for (int f = tid; f < RS * RS * RS * NI; f += NT) {
-        const int i = f % NI, p = f / NI;
+    const float4 *ph4 = reinterpret_cast<const float4 *>(phi + (size_t)tile * G * NI);
+    const int tid = threadIdx.x;
+    float4 v[PER];
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        // the tail lanes of the last round reload the last unit (in bounds) and skip the store
+        const int f = min(tid + r * NT, NF4 - 1);
+        const int j = f % NV, p = f / NV;
         const int rx = p % RS, ry = (p / RS) % RS, rz = p / (RS * RS);
-        const int gx = pmod(ox + rx, K1), gy = pmod(oy + ry, K2), gz = pmod(oz + rz, K3);
-        reg[f] = ph[(((size_t)gz * K2 + gy) * K1 + gx) * NI + i];
+        const int gx = wrap1(ox + rx, K1), gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
+        v[r] = ph4[(((size_t)gz * K2 + gy) * K1 + gx) * NV + j];
+    }
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+        const int f = tid + r * NT;
+        if (f < NF4) reg4[f] = v[r];
     }
     __syncthreads();
-    const int i = tid % NI, s = tid / NI, sstep = NT / NI;
-    for (unsigned int pos = a0 + s; pos < a1; pos += sstep) {
+    const int j = tid % NV, s = tid / NV;
+    float4 *Eat4 = reinterpret_cast<float4 *>(Eat);
+    for (unsigned int pos = a0 + s; pos < a1; pos += NT / NV) {
         const int a = aperm[pos];
         const float q = posn[a].w;
         const float *w = splw + (size_t)a * 3 * ORD;
         const int4 g = splg[a];
-        // local region coordinates of the stencil points g - c
         const int lx = pmod(g.x, K1) - bx * ER_BRICK + (ORD - 1);
         const int ly = pmod(g.y, K2) - by * ER_BRICK + (ORD - 1);
         const int lz = pmod(g.z, K3) - bz * ER_BRICK + (ORD - 1);
@@ -728,19 +755,29 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
             w2[c] = w[ORD + c];
             w3[c] = w[2 * ORD + c];
         }
-        float ea = 0.0f;
+        float4 ea = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int c3 = 0; c3 < ORD; ++c3) {
 #pragma unroll
             for (int c2 = 0; c2 < ORD; ++c2) {
-                const float *row = reg + (((lz - c3) * RS + (ly - c2)) * RS) * NI + i;
-                float sum = 0.0f;
+                const float4 *row = reg4 + (((lz - c3) * RS + (ly - c2)) * RS) * NV + j;
+                float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-                for (int c1 = 0; c1 < ORD; ++c1) sum += w1[c1] * row[(lx - c1) * NI];
-                ea += (w3[c3] * w2[c2]) * sum;
+                for (int c1 = 0; c1 < ORD; ++c1) {
+                    const float4 p = row[(lx - c1) * NV];
+                    sum.x += w1[c1] * p.x;
+                    sum.y += w1[c1] * p.y;
+                    sum.z += w1[c1] * p.z;
+                    sum.w += w1[c1] * p.w;
+                }
+                const float w23 = w3[c3] * w2[c2];
+                ea.x += w23 * sum.x;
+                ea.y += w23 * sum.y;
+                ea.z += w23 * sum.z;
+                ea.w += w23 * sum.w;
             }
         }
-        Eat[((size_t)tile * natom + a) * NI + i] = q * ea;
+        Eat4[((size_t)tile * natom + a) * NV + j] = make_float4(q * ea.x, q * ea.y, q * ea.z, q * ea.w);
     }
 }
 
@@ -840,7 +877,7 @@ __device__ __forceinline__ float pair_energy(const PairParams &p, float dis2, fl
 }
 
 struct PairArgs {
-    int nmol, nchunk, n_off, nslt, NI, ntype, nc;
+    int nmol, nchunk, n_off, nslt, NI, ntype, nc, nsplit;
     float cell_w;  // smallest perpendicular cell width
     PairParams pp;
 };
@@ -882,7 +919,9 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
 {
     __shared__ int cand[ER_PAIR_CAP];
     __shared__ int ncand;
-    const int nl = blockIdx.x;
+    // nsplit workgroups per insertion; part p owns the cells c with c % nsplit == p,
+    // so every (insertion, molecule) sum still has a single owner
+    const int nl = blockIdx.x / pa.nsplit, part = blockIdx.x % pa.nsplit;
     const int n = pa.n_off + nl;
     const int tid = threadIdx.x, NT = blockDim.x;
     const float4 ref = insref[n];
@@ -901,7 +940,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     const int span = min(2 * nreach + 1, pa.nc);  // distinct cells per axis
     const int lo = (span == pa.nc) ? 0 : -nreach;
     const int ncell = span * span * span;
-    for (int c = tid; c < ncell; c += NT) {
+    for (int c = part + tid * pa.nsplit; c < ncell; c += NT * pa.nsplit) {
         const int dx = c % span, dy = (c / span) % span, dz = c / (span * span);
         const int qx = pmod(cx + lo + dx, pa.nc), qy = pmod(cy + lo + dy, pa.nc), qz = pmod(cz + lo + dz, pa.nc);
         const unsigned int key = cell_key(qx, qy, qz);
@@ -1173,18 +1212,21 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
 }
 
 // edens(b) += engnmfc * count (engproc.F90:943-947), sequential in n per bin
-__global__ void k_edens(int nchunk, int n_off, int ermax, const uint32_t *__restrict__ Cnt,
-                        const double *__restrict__ w, double *__restrict__ edens)
+// edens partial of insertion slice blockIdx.y (used when no ecorr is formed
+// or on the VALU ecorr path); reduced by k_ecorr_reduce with ecorr = null
+__global__ void k_edens(int nchunk, int n_off, int ermax, int kper, const uint32_t *__restrict__ Cnt,
+                        const double *__restrict__ w, double *__restrict__ epart)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x, slice = blockIdx.y;
     if (b >= ermax) return;
-    double e = edens[b];
-    for (int nl = 0; nl < nchunk; ++nl) {
+    const int k_begin = slice * kper, k_end = min(nchunk, k_begin + kper);
+    double e = 0.0;
+    for (int nl = k_begin; nl < k_end; ++nl) {
         const int n = n_off + nl;
         const uint32_t k = Cnt[(size_t)n * ermax + b];
         if (k) e += w[n] * (double)k;
     }
-    edens[b] = e;
+    epart[(size_t)slice * ermax + b] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -1212,27 +1254,38 @@ __global__ void k_mfma_layout(double *__restrict__ out)
     for (int t = 0; t < 4; ++t) out[lane * 4 + t] = acc[t];
 }
 
-__global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax, const uint32_t *__restrict__ Cnt,
-                                               const double *__restrict__ w, const int *__restrict__ dmap,
-                                               double *__restrict__ ecorr)
+// Split-K form: workgroup (tile, slice) covers insertions [slice*kper,
+// (slice+1)*kper) of the chunk and writes its 64x64 block of the partial sum
+// to part[slice] (no atomics); the diagonal blocks also write the slice's
+// edens partial.  k_ecorr_reduce then adds the slices in a fixed order, so
+// results do not depend on scheduling.  Insertions are taken four per MFMA
+// step (kq = lane / 16), the next step's operands loaded before the current
+// step's MFMAs.
+__global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax, int kper,
+                                               const uint32_t *__restrict__ Cnt, const double *__restrict__ w,
+                                               const int *__restrict__ dmap, double *__restrict__ part,
+                                               double *__restrict__ epart)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int b1w = blockIdx.y * 64 + (wave >> 1) * 32;
-    const int b2w = blockIdx.x * 64 + (wave & 1) * 32;
-    if (blockIdx.y > blockIdx.x) return;  // lower block triangle mirrored below
+    const int by = blockIdx.y, bx = blockIdx.x, slice = blockIdx.z;
+    if (by > bx) return;  // lower block triangle: mirrored by the reduce
+    const int b1w = by * 64 + (wave >> 1) * 32;
+    const int b2w = bx * 64 + (wave & 1) * 32;
+    const int k_begin = slice * kper, k_end = min(nchunk, k_begin + kper);
+    const size_t plane = (size_t)ermax * ermax;
     er_double4 acc[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = (er_double4){0.0, 0.0, 0.0, 0.0};
+    double ed[2] = {0.0, 0.0};
     const int kq = lane >> 4, mr = lane & 15;
-    for (int k0 = 0; k0 < nchunk; k0 += 4) {
+    auto load = [&](int k0, double (&av)[2], double (&bv)[2]) {
         const int nl = k0 + kq;
-        const bool ok = nl < nchunk;
-        const int n = n_off + (ok ? nl : 0);
+        const bool ok = nl < k_end;
+        const int n = n_off + (ok ? nl : k_begin);
         const double wn = ok ? w[n] : 0.0;
         const uint32_t *row = Cnt + (size_t)n * ermax;
-        double av[2], bv[2];
 #pragma unroll
         for (int a = 0; a < 2; ++a) {
             const int b1 = b1w + a * 16 + mr;
@@ -1240,12 +1293,23 @@ __global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax,
             const int b2 = b2w + a * 16 + mr;
             bv[a] = (ok && b2 < ermax) ? (double)row[b2] : 0.0;
         }
+    };
+    double av[2], bv[2];
+    load(k_begin, av, bv);
+    for (int k0 = k_begin; k0 < k_end; k0 += 4) {
+        double an[2], bn[2];
+        load(k0 + 4, an, bn);
 #pragma unroll
-        for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) {
+            ed[a] += av[a];
 #pragma unroll
             for (int b = 0; b < 2; ++b)
                 acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        av[0] = an[0]; av[1] = an[1];
+        bv[0] = bn[0]; bv[1] = bn[1];
     }
+    double *P = part + (size_t)slice * plane;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ij = dmap[lane * 4 + r];
@@ -1256,19 +1320,42 @@ __global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax,
             for (int b = 0; b < 2; ++b) {
                 const int b1 = b1w + a * 16 + oi;
                 const int b2 = b2w + b * 16 + oj;
-                if (b1 < ermax && b2 < ermax) {
-                    const double v = acc[a][b][r];
-                    if (blockIdx.y == blockIdx.x) {
-                        if (b1 <= b2) {
-                            ecorr[(size_t)b1 * ermax + b2] += v;
-                            if (b1 != b2) ecorr[(size_t)b2 * ermax + b1] += v;
-                        }
-                    } else {
-                        ecorr[(size_t)b1 * ermax + b2] += v;
-                        ecorr[(size_t)b2 * ermax + b1] += v;
-                    }
-                }
+                if (b1 < ermax && b2 < ermax) P[(size_t)b1 * ermax + b2] = acc[a][b][r];
             }
+    }
+    // edens partial: sum the four kq groups in a fixed order (diagonal blocks,
+    // left-column waves own the rows)
+    if (by == bx && (wave & 1) == 0) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+            const double e1 = __shfl(ed[a], mr + 16, 64), e2 = __shfl(ed[a], mr + 32, 64),
+                         e3 = __shfl(ed[a], mr + 48, 64);
+            const double e = ((ed[a] + e1) + e2) + e3;
+            const int b1 = b1w + a * 16 + mr;
+            if (kq == 0 && b1 < ermax) epart[(size_t)slice * ermax + b1] = e;
+        }
+    }
+}
+
+// ecorr(b1,b2) += sum over slices of part (upper triangle, mirrored);
+// edens(b) += sum over slices of epart
+__global__ void k_ecorr_reduce(int ermax, int nslice, const double *__restrict__ part,
+                               const double *__restrict__ epart, double *__restrict__ ecorr,
+                               double *__restrict__ edens)
+{
+    const int b2 = blockIdx.x * blockDim.x + threadIdx.x, b1 = blockIdx.y;
+    if (b2 >= ermax) return;
+    const size_t plane = (size_t)ermax * ermax;
+    if (ecorr && b1 <= b2) {
+        double s = 0.0;
+        for (int k = 0; k < nslice; ++k) s += part[k * plane + (size_t)b1 * ermax + b2];
+        ecorr[(size_t)b1 * ermax + b2] += s;
+        if (b1 != b2) ecorr[(size_t)b2 * ermax + b1] += s;
+    }
+    if (b1 == 0) {
+        double e = 0.0;
+        for (int k = 0; k < nslice; ++k) e += epart[(size_t)k * ermax + b2];
+        edens[b2] += e;
     }
 }
 
