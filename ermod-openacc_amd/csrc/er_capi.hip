@@ -106,9 +106,10 @@ struct er_ctx {
     DBuf<float> d_engfac;
     DBuf<float2> d_tw1, d_tw2, d_tw3, d_twm, d_tws;
     DBuf<double> d_selfpart;
+    DBuf<float2> d_Ftab;           // per-insertion separable structure-factor factors
     DBuf<float2> d_T;
     DBuf<float> d_phi;
-    int chunk = 128;
+    int chunk = 1024;              // insertions per launch sequence (ER_CHUNK overrides)
 
     // spatial order / cell list (per frame)
     int nc = 1, ncell = 1;
@@ -229,11 +230,14 @@ static void prof_collect(er_ctx *c)
 
 static bool fft_len_ok(int n)
 {
-    static const int ok[] = {32, 36, 40, 48, 64, 72, 80};
+    static const int ok[] = {32, 36, 40, 48, 64, 72, 80, 96, 108, 128};
     for (int v : ok)
         if (v == n) return true;
     return false;
 }
+
+// (K2, K3) plane shapes with a compiled plane kernel
+static bool plane_ok(int k2, int k3) { return (k2 == k3 && fft_len_ok(k2)) || (k2 == 128 && k3 == 108); }
 
 // cardinal B-spline M_n (spline.F90:46-58) in double for the |b|^2 factors
 static double mn_spline(int n, double x)
@@ -300,10 +304,10 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     if (prm->ljswitch < 0 || prm->ljswitch > 3) return bad(fail(c, ER_E_ARG, "er_create: ljswitch"));
     if (!(prm->splodr == 4 || prm->splodr == 6 || prm->splodr == 8))
         return bad(fail(c, ER_E_UNSUPPORTED, "er_create: splodr must be 4, 6 or 8"));
-    if (!fft_len_ok(prm->ms[0]) || !fft_len_ok(prm->ms[1]) || prm->ms[1] != prm->ms[2])
+    if (!fft_len_ok(prm->ms[0]) || !plane_ok(prm->ms[1], prm->ms[2]))
         return bad(fail(c, ER_E_UNSUPPORTED,
-                        "er_create: PME grid %dx%dx%d not supported (each of ms1max and ms2max=ms3max "
-                        "must be one of 32 36 40 48 64 72 80)",
+                        "er_create: PME grid %dx%dx%d not supported (ms1max one of 32 36 40 48 64 72 80 96 "
+                        "108 128; ms2max = ms3max from the same list, or 128 x 108)",
                         prm->ms[0], prm->ms[1], prm->ms[2]));
     c->numatm = topo->numatm;
     c->nummol = topo->nummol;
@@ -319,7 +323,7 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     c->Gc = (size_t)c->H * c->K2 * c->K3;
     c->nbx = (c->K1 + ER_BRICK - 1) / ER_BRICK;
     c->nby = (c->K2 + ER_BRICK - 1) / ER_BRICK;
-    c->nbz = (c->K3 + ER_BRICK - 1) / ER_BRICK;
+    c->nbz = (c->K3 + brick_z(c->ord) - 1) / brick_z(c->ord);
     c->nbrick = c->nbx * c->nby * c->nbz;
     if (const char *e = getenv("ER_CHUNK")) c->chunk = std::max(NI_B, atoi(e) / NI_B * NI_B);
 
@@ -408,7 +412,7 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     HIPCHK(c, upload_tw(c->d_twm, c->K1 / 2, s));
     {
         std::vector<float2> t(ER_TWS_TOTAL);
-        for (int r = 4; r <= 10; ++r)
+        for (int r = 4; r <= 16; ++r)
             for (int k = 0; k < r; ++k) {
                 const double a = 2.0 * M_PI * (double)k / (double)r;
                 t[tws_off(r) + k] = make_float2((float)std::cos(a), (float)std::sin(a));
@@ -732,35 +736,45 @@ template <> struct Split<48> { static constexpr int A = 8, B = 6; };
 template <> struct Split<64> { static constexpr int A = 8, B = 8; };
 template <> struct Split<72> { static constexpr int A = 8, B = 9; };
 template <> struct Split<80> { static constexpr int A = 8, B = 10; };
+template <> struct Split<96> { static constexpr int A = 8, B = 12; };
+template <> struct Split<108> { static constexpr int A = 12, B = 9; };
+template <> struct Split<128> { static constexpr int A = 8, B = 16; };
 
-template <int K> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch)
+template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch)
 {
-    constexpr int R1 = Split<K>::A, R2 = Split<K>::B;
-    const int NT = 256;
-    const size_t lds = (size_t)K * (K + 1) * sizeof(float2) + (size_t)ER_SCH * 2 * K * sizeof(float2) +
-                       ER_SCH * sizeof(float2) + 2 * K * sizeof(float2) + (NT / 64) * sizeof(double);
+    constexpr int R1 = Split<K3>::A, R2 = Split<K3>::B, C1 = Split<K2>::A, C2 = Split<K2>::B;
+    constexpr int NT = (K2 * K3 > 6400) ? 512 : 256;  // large planes: more threads, fewer values per thread
+    const size_t lds = (size_t)K2 * (K3 + 1) * sizeof(float2) + (size_t)ER_SCH * (K2 + K3) * sizeof(float2) +
+                       (K2 + K3) * sizeof(float2) + (NT / 64) * sizeof(double);
     static bool attr = false;
     if (!attr && lds > 65536) {
-        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K, K, R1, R2, R1, R2>,
+        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
-    hipLaunchKernelGGL((k_pme_spec2d<K, K, R1, R2, R1, R2>), dim3(c->H, nch), dim3(NT), lds, c->stream, c->nslt,
-                       c->ord, c->K1, n_off, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_engfac.p, c->d_tw1.p,
-                       c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
+    hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT>), dim3(c->H, nch), dim3(NT), lds, c->stream,
+                       c->nslt, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p,
+                       c->d_T.p, c->d_selfpart.p);
     return hipGetLastError();
 }
 
 static hipError_t launch_spec2d(er_ctx *c, int n_off, int nch)
 {
-    switch (c->K2) {
-    case 32: return launch_spec2d_k<32>(c, n_off, nch);
-    case 36: return launch_spec2d_k<36>(c, n_off, nch);
-    case 40: return launch_spec2d_k<40>(c, n_off, nch);
-    case 48: return launch_spec2d_k<48>(c, n_off, nch);
-    case 64: return launch_spec2d_k<64>(c, n_off, nch);
-    case 72: return launch_spec2d_k<72>(c, n_off, nch);
-    case 80: return launch_spec2d_k<80>(c, n_off, nch);
+    if (c->K2 == c->K3) {
+        switch (c->K2) {
+        case 32: return launch_spec2d_k<32, 32>(c, n_off, nch);
+        case 36: return launch_spec2d_k<36, 36>(c, n_off, nch);
+        case 40: return launch_spec2d_k<40, 40>(c, n_off, nch);
+        case 48: return launch_spec2d_k<48, 48>(c, n_off, nch);
+        case 64: return launch_spec2d_k<64, 64>(c, n_off, nch);
+        case 72: return launch_spec2d_k<72, 72>(c, n_off, nch);
+        case 80: return launch_spec2d_k<80, 80>(c, n_off, nch);
+        case 96: return launch_spec2d_k<96, 96>(c, n_off, nch);
+        case 108: return launch_spec2d_k<108, 108>(c, n_off, nch);
+        case 128: return launch_spec2d_k<128, 128>(c, n_off, nch);
+        }
+    } else if (c->K2 == 128 && c->K3 == 108) {
+        return launch_spec2d_k<128, 108>(c, n_off, nch);
     }
     return hipErrorInvalidValue;
 }
@@ -792,6 +806,9 @@ template <int NI> static hipError_t launch_axis1(er_ctx *c, int nch)
     case 64: return launch_axis1_k<64, NI>(c, nch);
     case 72: return launch_axis1_k<72, NI>(c, nch);
     case 80: return launch_axis1_k<80, NI>(c, nch);
+    case 96: return launch_axis1_k<96, NI>(c, nch);
+    case 108: return launch_axis1_k<108, NI>(c, nch);
+    case 128: return launch_axis1_k<128, NI>(c, nch);
     }
     return hipErrorInvalidValue;
 }
@@ -825,8 +842,8 @@ template <int NI, int LPM> static hipError_t launch_gather(er_ctx *c, int n_off,
 template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, int n_off, int nch)
 {
     constexpr int NXCD = 8;
-    constexpr int RS = ER_BRICK + ORD - 1;
-    const size_t lds = (size_t)RS * RS * RS * NI * sizeof(float);
+    constexpr int RS = ER_BRICK + ORD - 1, RZ = brick_z(ORD) + ORD - 1;
+    const size_t lds = (size_t)RS * RS * RZ * NI * sizeof(float);
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_gather_brick<ORD, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -897,7 +914,8 @@ static hipError_t prep_frame(er_ctx *c)
         hipLaunchKernelGGL(k_zero_u32, dim3((c->nbrick + B - 1) / B), dim3(B), 0, c->stream, (size_t)c->nbrick,
                            c->d_bcnt.p);
         hipLaunchKernelGGL(k_atom_brick_count, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
-                           c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, c->d_bcnt.p, c->d_akeyrank.p);
+                           c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, brick_z(c->ord), c->d_bcnt.p,
+                           c->d_akeyrank.p);
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
         hipLaunchKernelGGL(k_cell_scatter, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
                            c->d_akeyrank.p, c->d_bstart.p, c->d_aperm.p);
@@ -916,6 +934,7 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     HIPCHK(c, c->d_selfreal.ensure(nins));
     HIPCHK(c, c->d_resself.ensure(nins));
     HIPCHK(c, c->d_selfpart.ensure((size_t)nins * c->H));
+    HIPCHK(c, c->d_Ftab.ensure((size_t)nins * nslt * (c->H + c->K2 + c->K3)));
     HIPCHK(c, c->d_T.ensure((size_t)chunk * c->Gc));
     const size_t ntile = (size_t)(chunk + NI - 1) / NI;
     HIPCHK(c, c->d_phi.ensure(ntile * NI * c->G));
@@ -1078,6 +1097,11 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
                            c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3, c->ord,
                            c->prm.screen, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_insref.p, c->d_selfreal.p,
                            c->d_resself.p);
+        HIPCHK(c, hipGetLastError());
+        const size_t nf = (size_t)nins * c->nslt * (c->H + c->K2 + c->K3);
+        hipLaunchKernelGGL(k_ins_factors, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, c->stream,
+                           nins * c->nslt, c->ord, c->K1, c->K2, c->K3, c->H, c->d_insposn.p, c->d_insw.p,
+                           c->d_insg.p, c->d_tw1.p, c->d_tw2.p, c->d_tw3.p, c->d_Ftab.p);
         HIPCHK(c, hipGetLastError());
         const size_t nC = (size_t)nins * c->ermax;
         hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nC + 255) / 256)), dim3(256), 0, c->stream, nC, c->d_C.p);
@@ -1266,7 +1290,7 @@ extern "C" void er_destroy(er_ctx *c)
     c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
-    c->d_selfpart.release(); c->d_T.release(); c->d_phi.release();
+    c->d_selfpart.release(); c->d_Ftab.release(); c->d_T.release(); c->d_phi.release();
     c->d_U.release(); c->d_u0.release(); c->d_C.release(); c->d_w.release(); c->d_flceng.release();
     c->d_edens.release(); c->d_ecorr.release(); c->d_part.release(); c->d_epart.release(); c->d_eself.release(); c->d_state.release(); c->d_slnuv.release();
     c->d_flags.release(); c->d_warn.release(); c->d_fstate.release(); c->d_minmax.release(); c->d_errkey.release();

@@ -260,28 +260,58 @@ __global__ void k_engfac(int K1, int K2, int K3, int ord, const double *__restri
 
 // offsets of the small-radix tables exp(+2 pi i k/r), r = 4..10, concatenated
 __host__ __device__ constexpr int tws_off(int r) { return r * (r - 1) / 2 - 6; }
-#define ER_TWS_TOTAL 49
+#define ER_TWS_TOTAL 130  // radices 4..16
 
-template <int K2, int K3, int R1, int R2, int C1, int C2>
-__global__ void __launch_bounds__(256) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
-                                                    const float4 *__restrict__ insposn,
-                                                    const float *__restrict__ insw, const int4 *__restrict__ insg,
-                                                    const float *__restrict__ engfac, const float2 *__restrict__ tw1,
+// Separable structure-factor factors of every insertion site, computed once
+// per run instead of once per (insertion, m1) plane:
+//   F_axis(m) = sum_c w_c exp(-2 pi i m (g - c) / K)   (axis 1 scaled by q)
+// Ftab: [insertion][site][H + K2 + K3]: m1 = 0..H-1, then m2, then m3.
+__global__ void k_ins_factors(int nrows, int ord, int K1, int K2, int K3, int H,
+                              const float4 *__restrict__ insposn, const float *__restrict__ insw,
+                              const int4 *__restrict__ insg, const float2 *__restrict__ tw1,
+                              const float2 *__restrict__ tw2, const float2 *__restrict__ tw3,
+                              float2 *__restrict__ Ftab)
+{
+    const int per = H + K2 + K3;
+    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= (size_t)nrows * per) return;
+    const int r = (int)(f % per);
+    const size_t site = f / per;
+    const int4 g = insg[site];
+    int axis, m, K, gd;
+    const float2 *tw;
+    if (r < H) { axis = 0; m = r; K = K1; gd = g.x; tw = tw1; }
+    else if (r < H + K2) { axis = 1; m = r - H; K = K2; gd = g.y; tw = tw2; }
+    else { axis = 2; m = r - H - K2; K = K3; gd = g.z; tw = tw3; }
+    const float *w = insw + site * 3 * ord + axis * ord;
+    float2 a = make_float2(0.f, 0.f);
+    for (int c = 0; c < ord; ++c) {
+        const int e = (int)(((long long)m * pmod(gd - c, K)) % K);
+        const float2 t = tw[e];  // exp(+2 pi i e/K); the forward sum uses the conjugate
+        a.x += w[c] * t.x;
+        a.y -= w[c] * t.y;
+    }
+    if (axis == 0) a = erfft::cscale(a, insposn[site].w);
+    Ftab[f] = a;
+}
+
+template <int K2, int K3, int R1, int R2, int C1, int C2, int NT>
+__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
+                                                    const float2 *__restrict__ Ftab,
+                                                    const float *__restrict__ engfac,
                                                     const float2 *__restrict__ tw2, const float2 *__restrict__ tw3,
                                                     const float2 *__restrict__ tws, float2 *__restrict__ T,
                                                     double *__restrict__ selfpart)
 {
     static_assert(R1 * R2 == K3 && C1 * C2 == K2, "line splits");
     constexpr int PS = K3 + 1;  // padded plane row stride
-    constexpr int NT = 256;
     constexpr int NE = K2 * K3;
     constexpr int EPT = (NE + NT - 1) / NT;
     extern __shared__ __align__(16) unsigned char smem[];
     float2 *plane = reinterpret_cast<float2 *>(smem);  // [K2][PS]
-    float2 *F2 = plane + K2 * PS;                       // [SCH][K2]
+    float2 *F2 = plane + K2 * PS;                       // [SCH][K2]: F1(m1) * F2(m2)
     float2 *F3 = F2 + ER_SCH * K2;                      // [SCH][K3]
-    float2 *A1 = F3 + ER_SCH * K3;                      // [SCH]
-    float2 *t2 = A1 + ER_SCH;                           // [K2]
+    float2 *t2 = F3 + ER_SCH * K3;                      // [K2]
     float2 *t3 = t2 + K2;                               // [K3]
     double *red = reinterpret_cast<double *>(t3 + K3);  // [NT/64]
 
@@ -297,29 +327,14 @@ __global__ void __launch_bounds__(256) k_pme_spec2d(int nslt, int ord, int K1, i
     float2 acc[EPT];
 #pragma unroll
     for (int k = 0; k < EPT; ++k) acc[k] = make_float2(0.f, 0.f);
+    const int per = H + K2 + K3;
     for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
         const int ns = min(ER_SCH, nslt - s0);
-        const int per = K2 + K3 + 1;
-        for (int f = tid; f < ns * per; f += NT) {
-            const int s = f / per, r = f % per;
-            const size_t site = (size_t)n * nslt + s0 + s;
-            const int4 g = insg[site];
-            int axis, m, K, gd;
-            const float2 *tw;
-            if (r < K2) { axis = 1; m = r; K = K2; gd = g.y; tw = tw2; }
-            else if (r < K2 + K3) { axis = 2; m = r - K2; K = K3; gd = g.z; tw = tw3; }
-            else { axis = 0; m = m1; K = K1; gd = g.x; tw = tw1; }
-            const float *w = insw + site * 3 * ord + axis * ord;
-            float2 a = make_float2(0.f, 0.f);
-            for (int c = 0; c < ord; ++c) {
-                const int e = (int)(((long long)m * pmod(gd - c, K)) % K);
-                const float2 t = tw[e];  // exp(+2 pi i e/K); the forward sum uses the conjugate
-                a.x += w[c] * t.x;
-                a.y -= w[c] * t.y;
-            }
-            if (r < K2) F2[s * K2 + m] = a;
-            else if (r < K2 + K3) F3[s * K3 + m] = a;
-            else A1[s] = erfft::cscale(a, insposn[site].w);
+        for (int f = tid; f < ns * (K2 + K3); f += NT) {
+            const int s = f / (K2 + K3), r = f % (K2 + K3);
+            const float2 *Fs = Ftab + ((size_t)n * nslt + s0 + s) * per;
+            if (r < K2) F2[s * K2 + r] = erfft::cmul(Fs[m1], Fs[H + r]);
+            else F3[s * K3 + (r - K2)] = Fs[H + r];
         }
         __syncthreads();
 #pragma unroll
@@ -328,7 +343,7 @@ __global__ void __launch_bounds__(256) k_pme_spec2d(int nslt, int ord, int K1, i
             if (e < NE) {
                 const int m2 = e % K2, m3 = e / K2;
                 for (int s = 0; s < ns; ++s) {
-                    const float2 B = erfft::cmul(A1[s], F2[s * K2 + m2]);
+                    const float2 B = F2[s * K2 + m2];
                     const float2 f3 = F3[s * K3 + m3];
                     acc[k].x += B.x * f3.x - B.y * f3.y;
                     acc[k].y += B.x * f3.y + B.y * f3.x;
@@ -656,22 +671,26 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
 
 // ---------------------------------------------------------------------------
 // Brick gather (the batched path's form of recpcal.F90:651-710).  Atoms are
-// sorted per frame by the 8x8x8 brick of their base grid index; a workgroup
-// stages the (8+ORD-1)^3 region of NI interleaved potential grids that its
-// brick's stencils touch in LDS once, then every atom of the brick reads its
-// ORD^3 stencil from LDS (each grid point is read by ~15 atoms, so L2->CU
-// traffic drops by the reuse factor).  Per-atom energies go to Eat and are
-// summed per molecule by k_mol_reduce in the same atom order as k_gather.
+// sorted per frame by the brick (8 x 8 x BZ grid points) of their base grid
+// index; a workgroup stages the (8+ORD-1)^2 x (BZ+ORD-1) region of NI
+// interleaved potential grids that its brick's stencils touch in LDS once,
+// then every atom of the brick reads its ORD^3 stencil from LDS (each grid
+// point is read by ~15 atoms, so L2->CU traffic drops by the reuse factor).
+// BZ is chosen so that two workgroups fit a CU's LDS at order 4 (one stages
+// while the other computes).  Per-atom energies go to Eat and are summed per
+// molecule by k_mol_reduce in the same atom order as k_gather.
 // ---------------------------------------------------------------------------
 #define ER_BRICK 8
+__host__ __device__ constexpr int brick_z(int ord) { return ord <= 4 ? 6 : 4; }
 
 __global__ void k_atom_brick_count(int natom, const int4 *__restrict__ splg, int K1, int K2, int K3, int nbx,
-                                   int nby, unsigned int *__restrict__ cnt, unsigned int *__restrict__ key_rank)
+                                   int nby, int bzs, unsigned int *__restrict__ cnt,
+                                   unsigned int *__restrict__ key_rank)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natom) return;
     const int4 g = splg[a];
-    const unsigned int key = (unsigned int)((pmod(g.z, K3) / ER_BRICK * nby + pmod(g.y, K2) / ER_BRICK) * nbx +
+    const unsigned int key = (unsigned int)((pmod(g.z, K3) / bzs * nby + pmod(g.y, K2) / ER_BRICK) * nbx +
                                             pmod(g.x, K1) / ER_BRICK);
     key_rank[2 * a] = key;
     key_rank[2 * a + 1] = atomicAdd(&cnt[key], 1u);
@@ -700,37 +719,87 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
                                                       int natom, float *__restrict__ Eat)
 {
     static_assert(NI == 16, "the brick gather stages 16 interleaved insertions");
-    constexpr int RS = ER_BRICK + ORD - 1;  // region side
+    constexpr int BZ = brick_z(ORD);
+    constexpr int RS = ER_BRICK + ORD - 1;  // region side along x and y
+    constexpr int RZ = BZ + ORD - 1;        // region side along z
     constexpr int NV = NI / 4;              // float4 per grid point
-    constexpr int NP = RS * RS * RS;
+    constexpr int NP = RS * RS * RZ;
     constexpr int NF4 = NP * NV;
     constexpr int NT = 512;
     constexpr int PER = (NF4 + NT - 1) / NT;
-    extern __shared__ __align__(16) float4 reg4[];  // [RS][RS][RS][NV]
+    extern __shared__ __align__(16) float4 reg4[];  // [RZ][RS][RS][NV]
     // XCD-aware remap: each XCD sweeps a contiguous range of bricks, tile by tile
     const int b = blockIdx.x;
     const int per = nbrick_pad / nxcd;
     const int xcd = b % nxcd, k = b / nxcd;
     const int brick = xcd * per + (k % per);
     const int tile = k / per;
-    const int nbz = (K3 + ER_BRICK - 1) / ER_BRICK;
+    const int nbz = (K3 + BZ - 1) / BZ;
     if (brick >= nbx * nby * nbz || tile * NI >= nchunk) return;
     const unsigned int a0 = bstart[brick], a1 = bstart[brick + 1];
     if (a0 == a1) return;
     const int bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
-    const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * ER_BRICK - (ORD - 1);
+    const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * BZ - (ORD - 1);
     const size_t G = (size_t)K1 * K2 * K3;
     const float4 *ph4 = reinterpret_cast<const float4 *>(phi + (size_t)tile * G * NI);
     const int tid = threadIdx.x;
+    const int j = tid % NV, s = tid / NV;
+    // the first round's atom data is requested before the staging loads so
+    // that both latencies overlap
+    struct AtomIn {
+        int a, lx, ly, lz;
+        float q, w[3 * ORD];
+    };
+    auto load_atom = [&](unsigned int pos, AtomIn &t) {
+        t.a = aperm[pos];
+        t.q = posn[t.a].w;
+        const float *w = splw + (size_t)t.a * 3 * ORD;
+#pragma unroll
+        for (int c = 0; c < 3 * ORD; ++c) t.w[c] = w[c];
+        const int4 g = splg[t.a];
+        t.lx = pmod(g.x, K1) - bx * ER_BRICK + (ORD - 1);
+        t.ly = pmod(g.y, K2) - by * ER_BRICK + (ORD - 1);
+        t.lz = pmod(g.z, K3) - bz * BZ + (ORD - 1);
+    };
+    float4 *Eat4 = reinterpret_cast<float4 *>(Eat);
+    auto compute_atom = [&](const AtomIn &t) {
+        float4 ea = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int c3 = 0; c3 < ORD; ++c3) {
+#pragma unroll
+            for (int c2 = 0; c2 < ORD; ++c2) {
+                const float4 *row = reg4 + (((t.lz - c3) * RS + (t.ly - c2)) * RS) * NV + j;
+                float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int c1 = 0; c1 < ORD; ++c1) {
+                    const float4 p = row[(t.lx - c1) * NV];
+                    const float w1 = t.w[c1];
+                    sum.x += w1 * p.x;
+                    sum.y += w1 * p.y;
+                    sum.z += w1 * p.z;
+                    sum.w += w1 * p.w;
+                }
+                const float w23 = t.w[2 * ORD + c3] * t.w[ORD + c2];
+                ea.x += w23 * sum.x;
+                ea.y += w23 * sum.y;
+                ea.z += w23 * sum.z;
+                ea.w += w23 * sum.w;
+            }
+        }
+        Eat4[((size_t)tile * natom + t.a) * NV + j] = make_float4(t.q * ea.x, t.q * ea.y, t.q * ea.z, t.q * ea.w);
+    };
+    AtomIn first;
+    const unsigned int pos0 = a0 + s;
+    if (pos0 < a1) load_atom(pos0, first);
     float4 v[PER];
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
         // the tail lanes of the last round reload the last unit (in bounds) and skip the store
         const int f = min(tid + r * NT, NF4 - 1);
-        const int j = f % NV, p = f / NV;
+        const int jj = f % NV, p = f / NV;
         const int rx = p % RS, ry = (p / RS) % RS, rz = p / (RS * RS);
         const int gx = wrap1(ox + rx, K1), gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
-        v[r] = ph4[(((size_t)gz * K2 + gy) * K1 + gx) * NV + j];
+        v[r] = ph4[(((size_t)gz * K2 + gy) * K1 + gx) * NV + jj];
     }
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
@@ -738,46 +807,11 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
         if (f < NF4) reg4[f] = v[r];
     }
     __syncthreads();
-    const int j = tid % NV, s = tid / NV;
-    float4 *Eat4 = reinterpret_cast<float4 *>(Eat);
-    for (unsigned int pos = a0 + s; pos < a1; pos += NT / NV) {
-        const int a = aperm[pos];
-        const float q = posn[a].w;
-        const float *w = splw + (size_t)a * 3 * ORD;
-        const int4 g = splg[a];
-        const int lx = pmod(g.x, K1) - bx * ER_BRICK + (ORD - 1);
-        const int ly = pmod(g.y, K2) - by * ER_BRICK + (ORD - 1);
-        const int lz = pmod(g.z, K3) - bz * ER_BRICK + (ORD - 1);
-        float w1[ORD], w2[ORD], w3[ORD];
-#pragma unroll
-        for (int c = 0; c < ORD; ++c) {
-            w1[c] = w[c];
-            w2[c] = w[ORD + c];
-            w3[c] = w[2 * ORD + c];
-        }
-        float4 ea = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-        for (int c3 = 0; c3 < ORD; ++c3) {
-#pragma unroll
-            for (int c2 = 0; c2 < ORD; ++c2) {
-                const float4 *row = reg4 + (((lz - c3) * RS + (ly - c2)) * RS) * NV + j;
-                float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-                for (int c1 = 0; c1 < ORD; ++c1) {
-                    const float4 p = row[(lx - c1) * NV];
-                    sum.x += w1[c1] * p.x;
-                    sum.y += w1[c1] * p.y;
-                    sum.z += w1[c1] * p.z;
-                    sum.w += w1[c1] * p.w;
-                }
-                const float w23 = w3[c3] * w2[c2];
-                ea.x += w23 * sum.x;
-                ea.y += w23 * sum.y;
-                ea.z += w23 * sum.z;
-                ea.w += w23 * sum.w;
-            }
-        }
-        Eat4[((size_t)tile * natom + a) * NV + j] = make_float4(q * ea.x, q * ea.y, q * ea.z, q * ea.w);
+    if (pos0 < a1) compute_atom(first);
+    for (unsigned int pos = pos0 + NT / NV; pos < a1; pos += NT / NV) {
+        AtomIn t;
+        load_atom(pos, t);
+        compute_atom(t);
     }
 }
 
@@ -1111,6 +1145,7 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
                                                  unsigned long long *__restrict__ errkey, int nmol)
 {
     __shared__ double r_w[256], r_t[256];
+    __shared__ float r_mn[256], r_mx[256];
     const int tid = threadIdx.x, NT = blockDim.x;
     auto uv0_of = [&](int n, float real) -> float {
         double rec = 0.0;
@@ -1133,11 +1168,14 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
     const double voffset = state[3];
     const float usreal = fstate[0];
     double sw = 0.0, st = 0.0;
+    float mn = 3.402823466e38f, mx = -3.402823466e38f;
     for (int nl = tid; nl < wa.nchunk; nl += NT) {
         const int n = wa.n_off + nl;
         const float real = wa.rigid ? usreal : selfreal[n];
         const float u = uv0_of(n, real);
         u0[n] = u;
+        mn = fminf(mn, u);
+        mx = fmaxf(mx, u);
         double engnmfc = 1.0;
         if (wa.wgtslf) {
             const float factor = (float)((double)u - voffset);
@@ -1164,11 +1202,15 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
     // ordered tree reduction (fixed order -> deterministic)
     r_w[tid] = sw;
     r_t[tid] = st;
+    r_mn[tid] = mn;
+    r_mx[tid] = mx;
     __syncthreads();
     for (int s = NT / 2; s > 0; s >>= 1) {
         if (tid < s) {
             r_w[tid] += r_w[tid + s];
             r_t[tid] += r_t[tid + s];
+            r_mn[tid] = fminf(r_mn[tid], r_mn[tid + s]);
+            r_mx[tid] = fmaxf(r_mx[tid], r_mx[tid + s]);
         }
         __syncthreads();
     }
@@ -1176,14 +1218,16 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
         state[0] += r_w[0];
         state[1] += (double)wa.nchunk;
         state[2] += r_t[0];
-        // solute self-energy extremes (engproc.F90:914-915) and histogram
-        float mn = 3.402823466e38f, mx = -3.402823466e38f;
-        for (int nl = 0; nl < wa.nchunk; ++nl) {
-            const int n = wa.n_off + nl;
-            const float u = u0[n];
-            mn = fminf(mn, u);
-            mx = fmaxf(mx, u);
-            if (wa.selfcal) {
+        // solute self-energy extremes (engproc.F90:914-915)
+        if (wa.nchunk > 0) {
+            atomicMin(&minmax[0], f2ord(r_mn[0]));
+            atomicMax(&minmax[1], f2ord(r_mx[0]));
+        }
+        // self-energy distribution, sequential in n as the reference adds it
+        if (wa.selfcal) {
+            for (int nl = 0; nl < wa.nchunk; ++nl) {
+                const int n = wa.n_off + nl;
+                const float u = u0[n];
                 int lo = 0, hi = wa.esmax;
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
@@ -1193,10 +1237,6 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
                 if (lo <= 0) atomicMin(errkey, (unsigned long long)n * (unsigned long long)(nmol + 1));
                 eself[max(lo, 1) - 1] += wout[n];
             }
-        }
-        if (wa.nchunk > 0) {
-            atomicMin(&minmax[0], f2ord(mn));
-            atomicMax(&minmax[1], f2ord(mx));
         }
     }
     __syncthreads();

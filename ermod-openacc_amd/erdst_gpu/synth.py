@@ -267,3 +267,162 @@ def dmu_pair(config: int = 0, nframes: int | None = None):
     else:
         raise ValueError("dmu_pair covers configurations 0 and 2")
     return soln, refs
+
+
+def region_lattice(rng: np.random.Generator, n: int, lo, hi, jitter: float = 0.5) -> np.ndarray:
+    """n centres on a jittered lattice filling the box region [lo, hi) (first
+    n of a seeded permutation of the lattice sites)."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    ext = hi - lo
+    # lattice counts per axis proportional to the extents
+    h = (ext.prod() / n) ** (1.0 / 3.0)
+    m = np.maximum(1, np.floor(ext / h).astype(int))
+    while m.prod() < n:
+        m[np.argmax(ext / m)] += 1
+    idx = np.stack(np.meshgrid(*[np.arange(k) for k in m], indexing="ij"), -1).reshape(-1, 3)
+    pts = lo + (idx + 0.5) * (ext / m)
+    pts = pts[rng.permutation(len(pts))[:n]]
+    return pts + rng.uniform(-jitter, jitter, pts.shape)
+
+
+def ethanol_like(rng: np.random.Generator, nmol: int) -> Species:
+    """Rigid 9-site molecule: a random 9-site chain (1.4 A bonds), random
+    charges summing to zero, sigma U[2.5, 3.8] A, epsilon U[0.03, 0.2]."""
+    q = rng.normal(0.0, 0.3, 9)
+    q -= q.mean()
+    geom = random_chain(rng, 9, bond=1.4, min_sep=1.9)
+    return Species("ethanol-like", nmol, q, rng.uniform(0.03, 0.2, 9), rng.uniform(2.5, 3.8, 9), geom)
+
+
+@dataclass
+class MultiSoluteWorkload:
+    """One solvent trajectory and several test particles inserted into it
+    (configuration 3).  topo_for(k) is the topology of the run of solute k;
+    insertions(f, k) its insertions of frame f."""
+    name: str
+    solvent: list
+    solutes: list
+    box: np.ndarray
+    nframes: int
+    nins: int
+    params: dict
+    mesh: list
+    seed: int
+    slttype: int = 2
+    _frame_fn: object = None
+    _ins_fn: object = None
+
+    @property
+    def cell(self) -> np.ndarray:
+        return np.diag(self.box).astype(np.float32)
+
+    def topo_for(self, k: int) -> Topology:
+        return build_topology(list(self.solvent) + [self.solutes[k]], solute_index=len(self.solvent), refs=True)
+
+    def frame(self, f: int) -> np.ndarray:
+        return self._frame_fn(f)
+
+    def insertions(self, f: int, k: int) -> np.ndarray:
+        return self._ins_fn(f, k)
+
+    def workload(self, k: int) -> Workload:
+        """The single-solute view of solute k (a Workload)."""
+        w = Workload(f"{self.name}-solute{k}", self.topo_for(k), self.box, nframes=self.nframes, nins=self.nins,
+                     params=dict(self.params), mesh=self.mesh, seed=self.seed, slttype=self.slttype)
+        w._frame_fn = self._frame_fn
+        w._ins_fn = lambda f, _k=k: self._ins_fn(f, _k)
+        return w
+
+    def uvmax_uvsoft_uvcrd(self):
+        return Workload.uvmax_uvsoft_uvcrd(self)
+
+
+def config3(nins: int = 500, nframes: int = 100) -> MultiSoluteWorkload:
+    """Two-species solvent (12 000 waters + 2000 rigid 9-site ethanol-like
+    molecules) in an 80 A cube; 16 rigid solutes with log-uniform site counts
+    in [1, 60] and zero-sum random charges; 500 insertions per solute per
+    frame; 800 bins per species (500 linear from -60 kcal/mol, 300
+    log-spaced up to 1e4); PME 80^3."""
+    seed = 20261202 + 3
+    rng0 = np.random.Generator(np.random.Philox(key=seed))
+    eth = ethanol_like(rng0, 2000)
+    nw = 12000
+    solutes = []
+    for k in range(16):
+        ns = int(np.clip(np.round(np.exp(rng0.uniform(0.0, np.log(60.0)))), 1, 60))
+        sp = random_solute(rng0, f"solute{k}", ns)
+        if ns == 1:
+            sp.charge[:] = 0.0
+        solutes.append(sp)
+    solvent = [water_species(nw), eth]
+    L = 80.0
+    box = np.array([L, L, L])
+    geom_w = water_geometry()
+
+    def frame(f):
+        rng = _stream(seed, 2 * f)
+        centres = lattice_centres(rng, nw + eth.nmol, box)
+        wx = place_rigid(rng, geom_w, centres[:nw])
+        ex = place_rigid(rng, eth.geometry, centres[nw:])
+        return np.concatenate([wx, ex]).astype(np.float32)
+
+    def ins(f, k):
+        rng = _stream(seed, 2 * f + 1 + 1000 * (k + 1))
+        sp = solutes[k]
+        centres = rng.uniform(0.0, 1.0, (nins, 3)) * box
+        return place_rigid(rng, sp.geometry, centres).reshape(nins, sp.nsite, 3).astype(np.float32)
+
+    # the 45-site solute reaches below -20 kcal/mol against ethanol-like molecules
+    mesh = linear_log_mesh(-60.0, 5.0, 0.13, 1.0e4, 300)
+    w = MultiSoluteWorkload("config3", solvent, solutes, box, nframes, nins, dict(ms=(80, 80, 80)),
+                            [mesh, mesh], seed)
+    w._frame_fn, w._ins_fn = frame, ins
+    return w
+
+
+def config4(nins: int = 500, nframes: int = 50) -> Workload:
+    """Orthorhombic 120 x 120 x 100 A box: a central 40 A slab (z in [30, 70))
+    of 3000 rigid random 50-site chains (zero-sum charges N(0, 0.2 e), sigma
+    U[3, 4] A) flanked by 16 667 rigid waters; a flexible 150-site solute
+    (each insertion a jittered conformation) inserted at 500 positions per
+    frame inside the slab; PME 128 x 128 x 108 order 4; 600 bins per species
+    (500 linear from -60 kcal/mol, 100 log-spaced up to 1e4)."""
+    seed = 20261202 + 4
+    rng0 = np.random.Generator(np.random.Philox(key=seed))
+    q = rng0.normal(0.0, 0.2, 50)
+    q -= q.mean()
+    chain = Species("chain50", 3000, q, rng0.uniform(0.03, 0.2, 50), rng0.uniform(3.0, 4.0, 50),
+                    random_chain(rng0, 50))
+    solute = random_solute(rng0, "chain150", 150)
+    nw = 16667
+    box = np.array([120.0, 120.0, 100.0])
+    topo = build_topology([chain, water_species(nw), solute], solute_index=2, refs=True)
+    geom_w = water_geometry()
+    z0, z1 = 30.0, 70.0
+
+    def frame(f):
+        rng = _stream(seed, 2 * f)
+        cc = region_lattice(rng, chain.nmol, (0.0, 0.0, z0), (box[0], box[1], z1))
+        cx = place_rigid(rng, chain.geometry, cc)
+        nlo = nw // 2
+        wlo = region_lattice(rng, nlo, (0.0, 0.0, 0.0), (box[0], box[1], z0))
+        whi = region_lattice(rng, nw - nlo, (0.0, 0.0, z1), (box[0], box[1], box[2]))
+        wx = place_rigid(rng, geom_w, np.concatenate([wlo, whi]))
+        return np.concatenate([cx, wx]).astype(np.float32)
+
+    def ins(f):
+        rng = _stream(seed, 2 * f + 1)
+        centres = np.column_stack([rng.uniform(0.0, box[0], nins), rng.uniform(0.0, box[1], nins),
+                                   rng.uniform(z0, z1, nins)])
+        R = random_rotations(rng, nins)
+        conf = solute.geometry[None, :, :] + rng.normal(0.0, 0.1, (nins, solute.nsite, 3))
+        conf -= conf.mean(axis=1, keepdims=True)
+        x = centres[:, None, :] + np.einsum("nij,nsj->nsi", R, conf)
+        return x.astype(np.float32)
+
+    # chain-solute energies reach below -30 kcal/mol in the slab
+    mesh = linear_log_mesh(-60.0, 5.0, 0.13, 1.0e4, 100)
+    w = Workload("config4", topo, box, nframes=nframes, nins=nins, params=dict(ms=(128, 128, 108)),
+                 mesh=[mesh, mesh], seed=seed, slttype=3)
+    w._frame_fn, w._ins_fn = frame, ins
+    return w

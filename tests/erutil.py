@@ -15,10 +15,17 @@ from erdst_gpu.params import hot_path_params
 # ~200-term PME gather and of sums whose terms cancel.
 RTOL = 1e-5
 ATOL = 2e-4
-# Above E_HIGH (the top of every mesh used here, so all such energies share
-# the open-ended last bin) near-overlapping sites dominate: the FP32 distance
-# of two sites at coordinates ~|x| has relative error ~eps32*|x|/r, and the
-# r^-12 term multiplies it by 12, so the bar there is RTOL_HIGH.
+# Above E_MID the energy is the LJ wall of sites in near contact: the FP32
+# distance of two sites at coordinates ~|x| carries an absolute error
+# ~eps32*|x| (5e-6 A at |x| = 80 A), i.e. a relative error eps32*|x|/r
+# (3e-6 at r = 1.6 A, where one LJ pair already gives 2000 kcal/mol), and
+# r^-12 multiplies it by 12: about 4e-5 in an 80 A box.  RTOL_MID = 1e-4
+# covers that; such energies lie in the log-spaced part of every mesh here,
+# whose bins are several percent wide.  Above E_HIGH (the top of every mesh
+# used here, so all such energies share the open-ended last bin) overlaps
+# are closer still and the bar is RTOL_HIGH.
+E_MID = 1.0e2
+RTOL_MID = 1e-4
 E_HIGH = 1.0e4
 RTOL_HIGH = 1e-3
 
@@ -78,7 +85,9 @@ def species_of(topo, nins, soln_tag=None):
 
 def energy_tolerance(e_ref, rtol=RTOL, atol=ATOL):
     e_ref = np.asarray(e_ref, np.float64)
-    return np.where(np.abs(e_ref) > E_HIGH, RTOL_HIGH, rtol) * np.abs(e_ref) + atol
+    a = np.abs(e_ref)
+    r = np.where(a > E_HIGH, RTOL_HIGH, np.where(a > E_MID, max(rtol, RTOL_MID), rtol))
+    return r * a + atol
 
 
 def energy_mismatch(e_gpu, e_ref, rtol=RTOL, atol=ATOL):

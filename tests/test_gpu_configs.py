@@ -1,0 +1,114 @@
+"""The other benchmark configurations through the C ABI against the oracle.
+
+Configuration 3: a two-species solvent (waters + 9-site ethanol-like
+molecules, 54 000 atoms, 80 A cube, PME 80^3) with 16 rigid solutes of 1 to
+60 sites; each solute is its own run over the same frames.  Energies per
+(insertion, molecule) to the erutil bar; the distributions (two species,
+800 bins each, 1600^2 ecorr) exactly against the oracle's binning of the
+GPU's own energies."""
+import numpy as np
+import pytest
+
+import erutil as U
+from erdst_gpu import capi, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _ctx(w, p):
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    return ctx
+
+
+@pytest.fixture(scope="module")
+def cfg3():
+    return synth.config3()
+
+
+@pytest.mark.parametrize("k", [6, 5, 1, 0])
+def test_config3_solute_energies(cfg3, k):
+    w = cfg3.workload(k)
+    p = U.workload_params(w)
+    ctx = _ctx(w, p)
+    ins = w.insertions(0)[:8]
+    ctx.set_frame(w.frame(0), w.cell)
+    ctx.set_insertions(ins)
+    ctx.run(capi.ER_BATCHED)
+    ctx.sync()
+    u, u0 = ctx.uvengy(8)
+    uo, u0o = U.oracle_refs(w, 0, p, nins=8)
+    nbad, worst, at = U.energy_mismatch(u, uo)
+    print(f"solute {k} ({w.topo.slt_charge32.size} sites): max |dE| {np.abs(u - uo).max():.3e}, "
+          f"worst excess {worst:.3e}")
+    assert nbad == 0, (worst, at, u[at], uo[at])
+    assert U.energy_mismatch(u0, u0o)[0] == 0
+
+
+def test_config3_two_species_distribution(cfg3):
+    w = cfg3.workload(0)
+    p = U.workload_params(w)
+    ctx = _ctx(w, p)
+    d = U.oracle_distribution(w, p)
+    for f in (0, 1):
+        ins = w.insertions(f)[:40]
+        ctx.set_frame(w.frame(f), w.cell)
+        ctx.set_insertions(ins)
+        ctx.run(capi.ER_BATCHED)
+        ctx.sync()
+        u, u0 = ctx.uvengy(40)
+        for n in range(40):
+            d.update(w.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+    g = ctx.fetch()
+    assert g["edens"].size == 1600
+    np.testing.assert_allclose(g["edens"] / g["engnorm"], d.edens / d.d.engnorm, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["ecorr"] / g["engnorm"], d.ecorr.reshape(1600, 1600) / d.d.engnorm,
+                               rtol=1e-12, atol=0)
+    # per species: bins of species p sum to its molecule count
+    nm = np.bincount(w.topo.uvspec, minlength=3)[1:]
+    en = g["edens"] / g["engnorm"]
+    np.testing.assert_allclose([en[:800].sum(), en[800:].sum()], nm, rtol=1e-12)
+
+
+@pytest.fixture(scope="module")
+def cfg4():
+    return synth.config4()
+
+
+def test_config4_slab_energies(cfg4):
+    """Configuration 4: 200 001 atoms in a 120 x 120 x 100 A box, PME
+    128 x 128 x 108 (K2 != K3), a flexible 150-site solute inserted in the
+    slab; per-insertion conformations change the solute self energies."""
+    w = cfg4
+    p = U.workload_params(w)
+    ctx = _ctx(w, p)
+    ins = w.insertions(0)[:6]
+    ctx.set_frame(w.frame(0), w.cell)
+    ctx.set_insertions(ins)
+    ctx.run(capi.ER_BATCHED)
+    ctx.sync()
+    u, u0 = ctx.uvengy(6)
+    uo, u0o = U.oracle_refs(w, 0, p, nins=6)
+    nbad, worst, at = U.energy_mismatch(u, uo)
+    print(f"config 4: max |dE| {np.abs(u - uo).max():.3e}, worst excess {worst:.3e}, "
+          f"uvengy0 {u0.tolist()} vs {u0o.tolist()}")
+    assert nbad == 0, (worst, at, u[at], uo[at])
+    assert U.energy_mismatch(u0, u0o)[0] == 0
+
+
+def test_config4_distribution_plain_equals_batched(cfg4):
+    w = cfg4
+    p = U.workload_params(w)
+    res = []
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx = _ctx(w, p)
+        ctx.set_frame(w.frame(1), w.cell)
+        ctx.set_insertions(w.insertions(1)[:20])
+        ctx.run(mode)
+        res.append(ctx.fetch())
+    np.testing.assert_allclose(res[0]["edens"], res[1]["edens"], rtol=1e-12)
+    np.testing.assert_allclose(res[0]["ecorr"], res[1]["ecorr"], rtol=1e-12)
+    nm = np.bincount(w.topo.uvspec, minlength=3)[1:]
+    en = res[0]["edens"] / res[0]["engnorm"]
+    np.testing.assert_allclose([en[:600].sum(), en[600:].sum()], nm, rtol=1e-12)
