@@ -1,13 +1,17 @@
 #!/usr/bin/env python
 """Benchmark of the erdst hot path on one MI355X.
 
-Workload (config 2): 30 000 rigid 3-site waters (90 000 atoms) in a 67.0 A
-cube, 1000 insertions per frame of the 5-site tetrahedral test particle,
-PME 72^3 order 4, cutoff 12 A, alpha 0.35, 600-bin energy mesh, ecorr on.
-A step is one frame: energies of 1000 insertions x 30 000 molecules,
-their histograms, edens and the 600 x 600 ecorr update.  Frames and
-insertions are generated on the host and uploaded before timing, so the
-timed region starts with every input resident in HBM.
+Default workload (configuration 2, the headline): 30 000 rigid 3-site waters
+(90 000 atoms) in a 67.0 A cube, 1000 insertions per frame of the 5-site
+tetrahedral test particle, PME 72^3 order 4, cutoff 12 A, alpha 0.35,
+600-bin energy mesh, ecorr on.  A step is one frame: energies of 1000
+insertions x 30 000 molecules, their histograms, edens and the 600 x 600
+ecorr update.  Frames and insertions are generated on the host and uploaded
+before timing, so the timed region starts with every input resident in HBM.
+
+--workload 0|1|3|4 runs the other configurations the same way (3: all 16
+solutes of a frame in one context and one er_run; 1: the solution run, one
+solute per frame).
 
 Prints one JSON line (rank 0, N = 1 only).
 """
@@ -28,21 +32,87 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6
+METRIC = "solute insertions/sec @90k atoms, batched vs plain, dmu err vs FP64 oracle"
+
+DESCRIPTION = {
+    0: "config0: refs, 1000 rigid 3-site waters (3000 atoms), 31.04 A cube, 100 insertions/frame of a rigid "
+       "5-site solute, PME 32^3 order 4, 600 energy bins, ecorr on",
+    1: "config1: soln, 8000 rigid 3-site waters + one flexible 20-site solute, 62.1 A cube, PME 64^3 order 4, "
+       "1200 energy bins, ecorr on",
+    2: "config2: refs, 30000 rigid 3-site waters (90000 atoms), 67.0 A cube, 1000 insertions/frame of a rigid "
+       "5-site solute, PME 72^3 order 4, cutoff 12 A, 600 energy bins, ecorr on",
+    3: "config3: refs, 12000 waters + 2000 rigid 9-site molecules (54000 atoms), 80 A cube, 16 rigid solutes "
+       "of 1-45 sites x 500 insertions/frame in one context, PME 80^3 order 4, 2 x 800 energy bins, ecorr on",
+    4: "config4: refs, 3000 50-site chains in a 40 A slab + 16667 waters (200001 atoms), 120x120x100 A, "
+       "500 insertions/frame of a flexible 150-site solute, PME 128x128x108 order 4, 2 x 600 energy bins, "
+       "ecorr on",
+}
 
 
-def algorithmic(kernel, w, nch, ord_):
+class Job:
+    """A configuration as a list of solute workloads sharing their frames."""
+
+    def __init__(self, n):
+        from erdst_gpu import synth
+        self.n = n
+        if n == 3:
+            self.multi = synth.config3()
+            self.solutes = [self.multi.workload(k) for k in range(len(self.multi.solutes))]
+        else:
+            self.multi = None
+            self.solutes = [{0: synth.config0, 1: synth.config1, 2: synth.config2, 4: synth.config4}[n]()]
+        self.w = self.solutes[0]
+        self.soln = self.w.slttype == 1
+        from erdst_gpu.params import hot_path_params
+        over = dict(corrcal=1) if self.soln else {}
+        self.params = hot_path_params(self.w.slttype, self.w.params["ms"], splodr=4, **over)
+
+    @property
+    def ins_per_frame(self):
+        return len(self.w.topo.tagslt) if self.soln else sum(s.nins for s in self.solutes)
+
+    def context(self, capi):
+        ctx = capi.Context(self.w.topo, self.params)
+        ctx.set_mesh(*self.w.uvmax_uvsoft_uvcrd())
+        if self.soln:
+            ctx.set_solute_molecules(self.w.topo.tagslt)
+        elif self.multi is not None:
+            ctx.set_solutes([(s.topo.slt_ljtype, s.topo.slt_charge32) for s in self.solutes])
+        else:
+            ctx.set_solute(self.w.topo.slt_ljtype, self.w.topo.slt_charge32)
+        return ctx
+
+    def inputs(self, frames, torch, device):
+        pos = [torch.from_numpy(np.ascontiguousarray(self.w.frame(f))).to(device) for f in frames]
+        ins = [[] if self.soln else
+               [torch.from_numpy(np.ascontiguousarray(s.insertions(f))).to(device) for s in self.solutes]
+               for f in frames]
+        return pos, ins
+
+    def run_frames(self, ctx, pos, ins, mode, nins=None):
+        for x, per in zip(pos, ins):
+            ctx.set_frame_device(x.data_ptr(), self.w.cell)
+            for k, s in enumerate(per):
+                if self.multi is not None:
+                    ctx.select_solute(k)
+                ctx.set_insertions_device(s.shape[0] if nins is None else nins, s.data_ptr())
+            ctx.run(mode)
+
+
+def algorithmic(kernel, job, nch):
     """(bound, algorithmic bytes or flops) of one launch over nch insertions.
     Bytes counted once per distinct array element a launch must touch."""
+    w = job.w
+    ord_ = job.params["splodr"]
     K1, K2, K3 = w.params["ms"]
     G = K1 * K2 * K3
     Gc = (K1 // 2 + 1) * K2 * K3
-    topo = w.topo
-    natom, nmol = topo.natom, topo.nmol
-    nslt = len(topo.slt_charge32)
+    natom, nmol = w.topo.natom, w.topo.nmol
+    nslt = np.mean([len(s.topo.slt_charge32) for s in job.solutes]) if not job.soln else 1
     ermax = int(w.uvmax_uvsoft_uvcrd()[0].sum())
     site_bytes = 16 + 12 * ord_ + 16        # posn + spline weights + grid index
-    if kernel == "spec2d":      # engfac read once, T written, solute sites read
-        return "hbm", Gc * 4 + nch * (Gc * 8 + nslt * site_bytes)
+    if kernel == "spec2d":      # engfac read once, T written, factor tables read
+        return "hbm", Gc * 4 + nch * (Gc * 8 + nslt * (K1 // 2 + 1 + K2 + K3) * 8)
     if kernel == "axis1":       # T read, phi written
         return "hbm", nch * (Gc * 8 + G * 4)
     if kernel == "gather":      # phi read once, per-atom data, U written
@@ -56,21 +126,7 @@ def algorithmic(kernel, w, nch, ord_):
     return None, None
 
 
-def make_inputs(w, frames, torch, device):
-    pos = [torch.from_numpy(np.ascontiguousarray(w.frame(f))).to(device) for f in frames]
-    ins = [torch.from_numpy(np.ascontiguousarray(w.insertions(f))).to(device) for f in frames]
-    return pos, ins
-
-
-def run_frames(ctx, w, pos, ins, mode, capi, nins=None):
-    for x, s in zip(pos, ins):
-        n = s.shape[0] if nins is None else nins
-        ctx.set_frame_device(x.data_ptr(), w.cell)
-        ctx.set_insertions_device(n, s.data_ptr())
-        ctx.run(mode)
-
-
-def cpu_baseline(soln, refs, frames_per_block=5, nblocks=2, nins=100, nthreads=None):
+def cpu_baseline_dmu(soln, refs, frames_per_block=5, nblocks=2, nins=100, nthreads=None):
     """The oracle (FP64 restatement of the reference algorithm: brute-force
     pairs, one PME grid per insertion, binary-search histogram, outer-product
     ecorr) on the host cores over a bounded sample of the same workload:
@@ -92,6 +148,39 @@ def cpu_baseline(soln, refs, frames_per_block=5, nblocks=2, nins=100, nthreads=N
     return rec, sblocks, rblocks
 
 
+def cpu_baseline_sample(job, budget_s=10.0, nthreads=None):
+    """The oracle on the host cores over a bounded sample of frame 0 of the
+    same workload (the first solute's first insertions, or the solution
+    frames), about budget_s seconds of work."""
+    import erutil as U
+    threads = nthreads or max(1, min(os.cpu_count() or 1, 16))
+    w, p = job.w, job.params
+    d = U.oracle_distribution(w, p)
+    if job.soln:
+        sp = U.species_of(w.topo, 1, w.topo.tagslt)[0]
+        t0, nf = time.perf_counter(), 0
+        while nf == 0 or (time.perf_counter() - t0 < budget_s and nf < 20):
+            uv, uv0 = U.oracle_soln(w, nf, p, nthreads=threads)
+            d.update(sp, uv[0], float(uv0[0]))
+            nf += 1
+        t = time.perf_counter() - t0
+        return dict(value=round(nf / t, 3), unit="insertions/s", cores=threads, kind="port",
+                    sample=f"frames 0-{nf - 1} of the solution run, one solute each ({t:.1f} s)")
+
+    def run(n):
+        t0 = time.perf_counter()
+        uv, uv0 = U.oracle_refs(w, 0, p, nins=n, nthreads=threads)
+        for k in range(n):
+            d.update(w.topo.uvspec, uv[k], float(uv0[k]))
+        return time.perf_counter() - t0
+
+    t_probe = run(threads)
+    n = int(max(threads, min(w.nins, threads * max(1, int(budget_s / max(t_probe, 1e-3))))))
+    t = run(n)
+    return dict(value=round(n / t, 2), unit="insertions/s", cores=threads, kind="port",
+                sample=f"frame 0, first {n} insertions of solute 1 ({t:.1f} s)")
+
+
 def dmu_error(soln, refs, sg, rg, so, ro, workdir):
     """Solvation free energy (slvfe's merge over engstore's files) from the
     GPU's and the oracle's distributions of the same frames and insertions."""
@@ -107,7 +196,7 @@ def dmu_error(soln, refs, sg, rg, so, ro, workdir):
     return dict(value_gpu=round(a.dmu, 6), value_oracle=round(b.dmu, 6), abs_err=float(f"{abs(a.dmu - b.dmu):.3e}"),
                 max_abs_err_any_block_or_group=float(f"{err.max():.3e}"), unit="kcal/mol", bar=0.01,
                 group=a.groups[a.grref][0], blocks=a.numrun,
-                sample="solution: 2 blocks x 5 frames of the solute in the same 30000-water box; "
+                sample="solution: 2 blocks x 5 frames of the solute in the same water box; "
                        "reference: frames 0-9 x 100 insertions; slvfe defaults (reg, eczr, py-hnc)")
 
 
@@ -116,6 +205,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--plain-insertions", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -126,52 +216,63 @@ def main():
 
     import torch
     from erdst_gpu import capi, synth
-    from erdst_gpu.params import hot_path_params
 
     device = torch.device("cuda:0")
-    w = synth.config2()
-    params = hot_path_params(w.slttype, w.params["ms"], splodr=4)
+    job = Job(args.workload)
     W, K = args.warmup, args.steps
-    pos, ins = make_inputs(w, range(W + K), torch, device)
+    pos, ins = job.inputs(range(W + K), torch, device)
     torch.cuda.synchronize()
-
-    ctx = capi.Context(w.topo, params)
-    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
-    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    ctx = job.context(capi)
 
     # warm-up (compiles nothing; first-touch of buffers, caches, clocks)
-    run_frames(ctx, w, pos[:W], ins[:W], capi.ER_BATCHED, capi)
+    job.run_frames(ctx, pos[:W], ins[:W], capi.ER_BATCHED)
     ctx.sync()
     ctx.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    run_frames(ctx, w, pos[W:], ins[W:], capi.ER_BATCHED, capi)
+    job.run_frames(ctx, pos[W:], ins[W:], capi.ER_BATCHED)
     ctx.sync()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = ctx.profile_read()
     ctx.profile(False)
     elapsed = t1 - t0
-    nins_total = K * w.nins
+    nins_total = K * job.ins_per_frame
     value = nins_total / elapsed
 
-    # plain path: one insertion per launch sequence, same inputs
-    nplain = min(args.plain_insertions, w.nins)
-    run_frames(ctx, w, pos[:1], ins[:1], capi.ER_PLAIN, capi, nins=8)
-    ctx.sync()
+    # plain path: one insertion per launch sequence, same inputs (the first
+    # solute, in a single-solute context when the workload has several)
+    pjob = job
+    if job.multi is not None:
+        pjob = Job.__new__(Job)
+        pjob.__dict__.update(job.__dict__)
+        pjob.multi, pjob.solutes = None, job.solutes[:1]
+    pctx = pjob.context(capi) if pjob is not job else ctx
+    if job.soln:
+        pl_pos, pl_ins, nplain = pos[W:W + 1], ins[W:W + 1], job.ins_per_frame
+        pjob.run_frames(pctx, pos[:1], ins[:1], capi.ER_PLAIN)
+    else:
+        nplain = min(args.plain_insertions, job.solutes[0].nins)
+        pl_pos, pl_ins = pos[W:W + 1], [ins[W][:1]]
+        pjob.run_frames(pctx, pos[:1], [ins[0][:1]], capi.ER_PLAIN, nins=8)
+    pctx.sync()
     torch.cuda.synchronize()
+    nrep = 20 if job.soln else 1
     tp0 = time.perf_counter()
-    run_frames(ctx, w, pos[W:W + 1], ins[W:W + 1], capi.ER_PLAIN, capi, nins=nplain)
-    ctx.sync()
+    for _ in range(nrep):
+        pjob.run_frames(pctx, pl_pos, pl_ins, capi.ER_PLAIN, nins=None if job.soln else nplain)
+    pctx.sync()
     torch.cuda.synchronize()
-    plain = nplain / (time.perf_counter() - tp0)
+    plain = nrep * nplain / (time.perf_counter() - tp0)
+    if pctx is not ctx:
+        pctx.close()
 
     # dominant kernel and its roofline
     busy = {k: v for k, v in prof.items() if v[1] > 0}
-    dom = max((k for k in busy if algorithmic(k, w, 1, params["splodr"])[0]), key=lambda k: busy[k][0])
+    dom = max((k for k in busy if algorithmic(k, job, 1)[0]), key=lambda k: busy[k][0])
     ms, launches = busy[dom]
     nch = nins_total / launches
-    bound, amount = algorithmic(dom, w, nch, params["splodr"])
+    bound, amount = algorithmic(dom, job, nch)
     avg_s = ms / launches / 1e3
     if bound == "hbm":
         achieved, peak, unit = amount / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
@@ -180,7 +281,7 @@ def main():
     kernels = {k: dict(ms_per_step=v[0] / K, launches=v[1]) for k, v in busy.items()}
 
     out = {
-        "metric": "solute insertions/sec @90k atoms, batched vs plain, dmu err vs FP64 oracle",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "insertions/s",
         "n_gpus": 1,
@@ -193,10 +294,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "config2: refs, 30000 rigid 3-site waters (90000 atoms), 67.0 A cube, "
-                        "1000 insertions/frame of a rigid 5-site solute, PME 72^3 order 4, cutoff 12 A, "
-                        "600 energy bins, ecorr on; one step = one frame",
-            "insertions_per_step": w.nins,
+            "workload": DESCRIPTION[args.workload] + "; one step = one frame",
+            "insertions_per_step": job.ins_per_frame,
             "frames_per_s": round(K / elapsed, 3),
             "plain_insertions_per_s": round(plain, 1),
             "batched_over_plain": round(value / plain, 2),
@@ -207,17 +306,20 @@ def main():
                      "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
                      "launch_ms": round(avg_s * 1e3, 5), "units_per_launch": round(nch, 2)},
         "cpu_baseline": None,
+        "dmu": None,
     }
     ctx.close()
-    out["dmu"] = None
     if not args.no_cpu:
-        import erutil as U
-        soln, refs = synth.dmu_pair(2)
-        sg, rg = U.gpu_dmu_blocks(soln, refs, 5, 2, 100)
-        rec, so, ro = cpu_baseline(soln, refs)
+        if args.workload in (0, 2):
+            import erutil as U
+            soln, refs = synth.dmu_pair(args.workload)
+            sg, rg = U.gpu_dmu_blocks(soln, refs, 5, 2, 100)
+            rec, so, ro = cpu_baseline_dmu(soln, refs)
+            out["dmu"] = dmu_error(soln, refs, sg, rg, so, ro, ROOT)
+        else:
+            rec = cpu_baseline_sample(job)
         out["cpu_baseline"] = rec
         out["config"]["gpu_over_cpu"] = round(value / rec["value"], 1)
-        out["dmu"] = dmu_error(soln, refs, sg, rg, so, ro, ROOT)
     print(json.dumps(out))
 
 

@@ -18,6 +18,7 @@ namespace {
 
 constexpr int NI_B = 16;  // insertion interleave of the batched path
 constexpr int ER_KSLICES = 16;  // insertion slices of the split-K edens/ecorr sums
+constexpr int ER_SPARSE_SITES = 12;  // above this many sites the plane kernel spreads + transforms
 
 template <typename T> struct DBuf {
     T *p = nullptr;
@@ -42,6 +43,36 @@ template <typename T> struct DBuf {
 
 }  // namespace
 
+// One solute analysed in the context: its sites, its insertions of the
+// current frame, its per-insertion energies and its block accumulators.  A
+// context has one slot unless er_set_solutes gives it several test
+// particles; the frame-level state is shared by all slots.
+struct Slot {
+    int nslt = 0;                  // sites per insertion / per solute molecule
+    float qslt = 0.f;              // refs test-particle charge
+    DBuf<float> d_sltq;
+    DBuf<int> d_sltlj;
+    int nins = 0;
+    bool have_ins = false, wslt_given = false;
+    DBuf<float> d_sltpos_own, d_wslt;
+    const float *d_sltpos = nullptr;
+    DBuf<float> d_U, d_u0;         // uvengy / uvengy0 of the last run
+    int last_NI = 16, last_nins = 0;
+    // accumulators
+    DBuf<double> d_edens, d_ecorr, d_eself, d_state, d_slnuv;
+    DBuf<int> d_flags, d_warn;
+    DBuf<float> d_fstate;
+    DBuf<unsigned int> d_minmax;
+    DBuf<unsigned long long> d_errkey;
+    void release()
+    {
+        d_sltq.release(); d_sltlj.release(); d_sltpos_own.release(); d_wslt.release(); d_U.release();
+        d_u0.release(); d_edens.release(); d_ecorr.release(); d_eself.release(); d_state.release();
+        d_slnuv.release(); d_flags.release(); d_warn.release(); d_fstate.release(); d_minmax.release();
+        d_errkey.release();
+    }
+};
+
 struct er_ctx {
     // sizes and parameters
     int numatm = 0, nummol = 0, ntype = 0, numslv = 0;
@@ -59,11 +90,11 @@ struct er_ctx {
     DBuf<float> d_charge, d_ljene, d_ljlensq, d_molq;
     DBuf<int> d_ljtype, d_molbeg, d_spb, d_spe;
 
-    // solute
-    int nslt = 0;                  // sites per insertion / per solute molecule
-    float qslt = 0.f;              // refs test-particle charge
-    DBuf<float> d_sltq;
-    DBuf<int> d_sltlj;
+    // solutes (one slot per test particle; slot 0 in soln runs)
+    std::vector<Slot> slots = std::vector<Slot>(1);
+    int cur = 0;
+    Slot &S() { return slots[cur]; }
+    int halt_slot = 0;
     int nsltmol = 0;               // soln: number of solute molecules
     std::vector<int> tagslt;       // 0-based
     DBuf<int> d_tag;
@@ -91,11 +122,6 @@ struct er_ctx {
     std::vector<float> h_pin;      // host staging (pageable)
 
     // insertions
-    int nins = 0;
-    bool have_ins = false;
-    DBuf<float> d_sltpos_own, d_wslt;
-    const float *d_sltpos = nullptr;
-    bool wslt_given = false;
     DBuf<float4> d_insposn, d_insref;
     DBuf<float> d_insw, d_selfreal;
     DBuf<int4> d_insg;
@@ -129,21 +155,14 @@ struct er_ctx {
     DBuf<int> d_dmap;
 
     // energies / histograms
-    DBuf<float> d_U, d_u0, d_meshf, d_resself;
+    DBuf<float> d_meshf, d_resself;
     DBuf<double> d_flpart;
     int nslice = 1;
     DBuf<uint32_t> d_C;
     DBuf<double> d_w, d_flceng;
-    int last_NI = NI_B;
-    int last_nins = 0;
 
     // accumulators
-    DBuf<double> d_edens, d_ecorr, d_eself, d_state, d_slnuv;
     DBuf<double> d_part, d_epart;  // split-K partial sums of ecorr / edens
-    DBuf<int> d_flags, d_warn;
-    DBuf<float> d_fstate;
-    DBuf<unsigned int> d_minmax;
-    DBuf<unsigned long long> d_errkey;
 
     // diagnostics: event pairs recorded around launches while profiling
     bool prof = false;
@@ -284,6 +303,59 @@ static void gmx_params(int pw, float lw, float up, float &A, float &B, float &C)
     A = -(float)pw * ((float)(pw + 4) * up - (float)(pw + 1) * lw) / (std::pow(up, (float)(pw + 2)) * df * df) / 3.0f;
     B = (float)pw * ((float)(pw + 3) * up - (float)(pw + 1) * lw) / (std::pow(up, (float)(pw + 2)) * df * df * df) / 4.0f;
     C = 1.0f / std::pow(up, (float)pw) - A * df * df * df - B * df * df * df * df;
+}
+
+// mesh-independent per-slot state: weights' state (voffset survives engclear),
+// flags, halt key and minuv/maxuv
+static int slot_state_init(er_ctx *c, Slot &sl)
+{
+    hipStream_t s = c->stream;
+    HIPCHK(c, sl.d_state.ensure(4));
+    HIPCHK(c, sl.d_flags.ensure(4));
+    HIPCHK(c, sl.d_fstate.ensure(4));
+    HIPCHK(c, sl.d_warn.ensure(1));
+    HIPCHK(c, sl.d_errkey.ensure(1));
+    HIPCHK(c, sl.d_minmax.ensure(2 * (size_t)(c->numslv + 1)));
+    HIPCHK(c, sl.d_slnuv.ensure(c->numslv));
+    HIPCHK(c, hipMemsetAsync(sl.d_state.p, 0, 4 * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_flags.p, 0, 4 * sizeof(int), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_fstate.p, 0, 4 * sizeof(float), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_warn.p, 0, sizeof(int), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_errkey.p, 0xff, sizeof(unsigned long long), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_slnuv.p, 0, c->numslv * sizeof(double), s));
+    std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
+    auto ord = [](float f) {
+        unsigned int u;
+        std::memcpy(&u, &f, 4);
+        return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+    };
+    for (int k = 0; k <= c->numslv; ++k) {
+        mm[2 * k] = ord(3.402823466e38f);       // minuv = huge (engproc.F90:299-302)
+        mm[2 * k + 1] = ord(-3.402823466e38f);  // maxuv = -huge
+    }
+    HIPCHK(c, hipMemcpy(sl.d_minmax.p, mm.data(), mm.size() * sizeof(unsigned int), hipMemcpyHostToDevice));
+    return ER_OK;
+}
+
+// mesh-sized per-slot accumulators (contents cleared by er_clear)
+static int slot_acc_alloc(er_ctx *c, Slot &sl)
+{
+    HIPCHK(c, sl.d_edens.ensure(c->ermax));
+    HIPCHK(c, sl.d_ecorr.ensure((size_t)c->ermax * c->ermax));
+    HIPCHK(c, sl.d_eself.ensure(std::max(1, c->esmax)));
+    return ER_OK;
+}
+
+static int slot_clear(er_ctx *c, Slot &sl)
+{
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemsetAsync(sl.d_edens.p, 0, c->ermax * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_ecorr.p, 0, (size_t)c->ermax * c->ermax * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_eself.p, 0, std::max(1, c->esmax) * sizeof(double), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_slnuv.p, 0, c->numslv * sizeof(double), s));
+    // engnorm, engsmpl, avslf (voffset survives, engproc.F90:336-338)
+    HIPCHK(c, hipMemsetAsync(sl.d_state.p, 0, 3 * sizeof(double), s));
+    return ER_OK;
 }
 
 extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx **out)
@@ -429,32 +501,9 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     HIPCHK(c, c->d_molref.ensure(c->nummol));
     HIPCHK(c, c->d_sitepos_own.ensure((size_t)3 * c->numatm));
 
-    // accumulators (state survives engclear: voffset, minuv/maxuv)
-    HIPCHK(c, c->d_state.ensure(4));
-    HIPCHK(c, c->d_flags.ensure(4));
-    HIPCHK(c, c->d_fstate.ensure(4));
-    HIPCHK(c, c->d_warn.ensure(1));
-    HIPCHK(c, c->d_errkey.ensure(1));
-    HIPCHK(c, c->d_minmax.ensure(2 * (size_t)(c->numslv + 1)));
-    HIPCHK(c, c->d_slnuv.ensure(c->numslv));
-    HIPCHK(c, hipMemsetAsync(c->d_state.p, 0, 4 * sizeof(double), s));
-    HIPCHK(c, hipMemsetAsync(c->d_flags.p, 0, 4 * sizeof(int), s));
-    HIPCHK(c, hipMemsetAsync(c->d_fstate.p, 0, 4 * sizeof(float), s));
-    HIPCHK(c, hipMemsetAsync(c->d_warn.p, 0, sizeof(int), s));
-    HIPCHK(c, hipMemsetAsync(c->d_errkey.p, 0xff, sizeof(unsigned long long), s));
-    HIPCHK(c, hipMemsetAsync(c->d_slnuv.p, 0, c->numslv * sizeof(double), s));
     {
-        std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
-        auto ord = [](float f) {
-            unsigned int u;
-            std::memcpy(&u, &f, 4);
-            return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-        };
-        for (int k = 0; k <= c->numslv; ++k) {
-            mm[2 * k] = ord(3.402823466e38f);       // minuv = huge (engproc.F90:299-302)
-            mm[2 * k + 1] = ord(-3.402823466e38f);  // maxuv = -huge
-        }
-        HIPCHK(c, hipMemcpy(c->d_minmax.p, mm.data(), mm.size() * sizeof(unsigned int), hipMemcpyHostToDevice));
+        const int rc = slot_state_init(c, c->slots[0]);
+        if (rc) return bad(rc);
     }
     // Accumulator layout of the FP64 MFMA: each output register of a known
     // product names its own (row, col); use the MFMA path only if the
@@ -539,33 +588,77 @@ extern "C" int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsof
     HIPCHK(c, c->d_escrd.ensure(std::max(1, c->esmax)));
     if (c->esmax > 0)
         HIPCHK(c, hipMemcpyAsync(c->d_escrd.p, c->escrd.data(), c->esmax * sizeof(double), hipMemcpyHostToDevice, s));
-    HIPCHK(c, c->d_edens.ensure(tot));
-    HIPCHK(c, c->d_ecorr.ensure((size_t)tot * tot));
+    c->ermax = tot;
+    for (Slot &sl : c->slots) {
+        const int rc = slot_acc_alloc(c, sl);
+        if (rc) return rc;
+    }
     HIPCHK(c, c->d_epart.ensure((size_t)ER_KSLICES * tot));
     if (c->prm.corrcal == 1 && c->use_mfma) HIPCHK(c, c->d_part.ensure((size_t)ER_KSLICES * tot * tot));
-    HIPCHK(c, c->d_eself.ensure(std::max(1, c->esmax)));
     HIPCHK(c, hipStreamSynchronize(s));
     c->have_mesh = true;
     return er_clear(c);
+}
+
+static int set_slot_solute(er_ctx *c, Slot &sl, int32_t nsite, const int32_t *ljtype, const float *charge)
+{
+    if (nsite <= 0 || !ljtype || !charge) return fail(c, ER_E_ARG, "er_set_solute: bad arguments");
+    std::vector<int> lj(nsite);
+    float q = 0.f;
+    for (int s = 0; s < nsite; ++s) {
+        lj[s] = ljtype[s] - 1;
+        if (lj[s] < 0 || lj[s] >= c->ntype) return fail(c, ER_E_ARG, "er_set_solute: ljtype out of range");
+        q += charge[s];
+    }
+    sl.qslt = q;
+    if (sl.nslt != nsite) sl.have_ins = false;  // insertions of another shape are void
+    sl.nslt = nsite;
+    HIPCHK(c, sl.d_sltq.ensure(nsite));
+    HIPCHK(c, hipMemcpy(sl.d_sltq.p, charge, nsite * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, sl.d_sltlj.ensure(nsite));
+    HIPCHK(c, hipMemcpy(sl.d_sltlj.p, lj.data(), nsite * sizeof(int), hipMemcpyHostToDevice));
+    return ER_OK;
 }
 
 extern "C" int er_set_solute(er_ctx *c, int32_t nsite, const int32_t *ljtype, const float *charge)
 {
     if (!c) return ER_E_ARG;
     if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "er_set_solute: soln runs use er_set_solute_molecules");
-    if (nsite <= 0 || !ljtype || !charge) return fail(c, ER_E_ARG, "er_set_solute: bad arguments");
-    std::vector<int> lj(nsite);
-    c->qslt = 0.f;
-    for (int s = 0; s < nsite; ++s) {
-        lj[s] = ljtype[s] - 1;
-        if (lj[s] < 0 || lj[s] >= c->ntype) return fail(c, ER_E_ARG, "er_set_solute: ljtype out of range");
-        c->qslt += charge[s];
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return set_slot_solute(c, c->S(), nsite, ljtype, charge);
+}
+
+extern "C" int er_set_solutes(er_ctx *c, int32_t nsolute, const int32_t *nsite, const int32_t *ljtype,
+                              const float *charge)
+{
+    if (!c) return ER_E_ARG;
+    if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "er_set_solutes: only for refs runs");
+    if (nsolute <= 0 || !nsite || !ljtype || !charge) return fail(c, ER_E_ARG, "er_set_solutes: bad arguments");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    for (size_t k = (size_t)nsolute; k < c->slots.size(); ++k) c->slots[k].release();
+    c->slots.resize(nsolute);
+    c->cur = 0;
+    size_t off = 0;
+    for (int k = 0; k < nsolute; ++k) {
+        Slot &sl = c->slots[k];
+        int rc = set_slot_solute(c, sl, nsite[k], ljtype + off, charge + off);
+        if (!rc) rc = slot_state_init(c, sl);
+        if (!rc && c->have_mesh) rc = slot_acc_alloc(c, sl);
+        if (!rc && c->have_mesh) rc = slot_clear(c, sl);
+        if (rc) return rc;
+        sl.have_ins = false;
+        off += (size_t)nsite[k];
     }
-    c->nslt = nsite;
-    HIPCHK(c, c->d_sltq.ensure(nsite));
-    HIPCHK(c, hipMemcpy(c->d_sltq.p, charge, nsite * sizeof(float), hipMemcpyHostToDevice));
-    HIPCHK(c, c->d_sltlj.ensure(nsite));
-    HIPCHK(c, hipMemcpy(c->d_sltlj.p, lj.data(), nsite * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return ER_OK;
+}
+
+extern "C" int er_select_solute(er_ctx *c, int32_t k)
+{
+    if (!c) return ER_E_ARG;
+    if (k < 1 || k > (int)c->slots.size()) return fail(c, ER_E_ARG, "er_select_solute: %d out of 1..%d", k,
+                                                       (int)c->slots.size());
+    c->cur = k - 1;
     return ER_OK;
 }
 
@@ -585,7 +678,7 @@ extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *t
         c->tagslt[k] = m;
     }
     c->nsltmol = nslt;
-    c->nslt = ns;
+    c->S().nslt = ns;
     HIPCHK(c, c->d_tag.ensure(nslt));
     HIPCHK(c, hipMemcpy(c->d_tag.p, c->tagslt.data(), nslt * sizeof(int), hipMemcpyHostToDevice));
     return ER_OK;
@@ -689,15 +782,15 @@ extern "C" int er_set_frame_device(er_ctx *c, const float *sitepos_dev, const fl
 static int ins_common(er_ctx *c, int32_t nins, const float *wslt, bool copies)
 {
     if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "insertions are for refs runs");
-    if (c->nslt <= 0) return fail(c, ER_E_STATE, "er_set_solute must precede insertions");
+    if (c->S().nslt <= 0) return fail(c, ER_E_STATE, "er_set_solute must precede insertions");
     if (nins <= 0) return fail(c, ER_E_ARG, "nins must be positive");
     // a run still in flight may read the buffers about to be overwritten
     if (copies || wslt) HIPCHK(c, hipStreamSynchronize(c->stream));
-    c->nins = nins;
-    HIPCHK(c, c->d_wslt.ensure(nins));
-    c->wslt_given = wslt != nullptr;
-    if (wslt) HIPCHK(c, hipMemcpy(c->d_wslt.p, wslt, nins * sizeof(float), hipMemcpyHostToDevice));
-    c->have_ins = true;
+    c->S().nins = nins;
+    HIPCHK(c, c->S().d_wslt.ensure(nins));
+    c->S().wslt_given = wslt != nullptr;
+    if (wslt) HIPCHK(c, hipMemcpy(c->S().d_wslt.p, wslt, nins * sizeof(float), hipMemcpyHostToDevice));
+    c->S().have_ins = true;
     return ER_OK;
 }
 
@@ -707,10 +800,10 @@ extern "C" int er_set_insertions(er_ctx *c, int32_t nins, const float *sltpos, c
     if (!sltpos) return fail(c, ER_E_ARG, "er_set_insertions: null sltpos");
     int rc = ins_common(c, nins, stat_weight_solute, true);
     if (rc) return rc;
-    const size_t n = (size_t)3 * c->nslt * nins;
-    HIPCHK(c, c->d_sltpos_own.ensure(n));
-    HIPCHK(c, hipMemcpy(c->d_sltpos_own.p, sltpos, n * sizeof(float), hipMemcpyHostToDevice));
-    c->d_sltpos = c->d_sltpos_own.p;
+    const size_t n = (size_t)3 * c->S().nslt * nins;
+    HIPCHK(c, c->S().d_sltpos_own.ensure(n));
+    HIPCHK(c, hipMemcpy(c->S().d_sltpos_own.p, sltpos, n * sizeof(float), hipMemcpyHostToDevice));
+    c->S().d_sltpos = c->S().d_sltpos_own.p;
     return ER_OK;
 }
 
@@ -721,7 +814,7 @@ extern "C" int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sl
     if (!sltpos_dev) return fail(c, ER_E_ARG, "er_set_insertions_device: null sltpos");
     int rc = ins_common(c, nins, stat_weight_solute, false);
     if (rc) return rc;
-    c->d_sltpos = sltpos_dev;
+    c->S().d_sltpos = sltpos_dev;
     return ER_OK;
 }
 
@@ -752,9 +845,11 @@ template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
+    // many sites: scatter + forward plane DFT beats nslt * K2 * K3 products
+    const int sparse = c->S().nslt > ER_SPARSE_SITES ? 1 : 0;
     hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT>), dim3(c->H, nch), dim3(NT), lds, c->stream,
-                       c->nslt, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p,
-                       c->d_T.p, c->d_selfpart.p);
+                       c->S().nslt, c->ord, c->K1, n_off, sparse, c->d_Ftab.p, c->d_insw.p, c->d_insg.p,
+                       c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
     return hipGetLastError();
 }
 
@@ -824,8 +919,8 @@ template <int ORD, int NI, int LPM> static hipError_t launch_gather_k(er_ctx *c,
     const bool soln = c->prm.slttype == 1;
     hipLaunchKernelGGL((k_gather<ORD, NI, LPM>), dim3(ngx * ngy), dim3(256), 0, c->stream, c->nummol, nch, n_off,
                        c->K1, c->K2, c->K3, c->d_perm.p, ngx, NXCD, c->d_molbeg.p, c->d_posn.p, c->d_splw.p,
-                       c->d_splg.p, c->d_molq.p, c->d_phi.p, soln ? c->d_tag.p : (const int *)nullptr, c->qslt,
-                       c->prm.screen, c->fc.volume, c->d_U.p);
+                       c->d_splg.p, c->d_molq.p, c->d_phi.p, soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt,
+                       c->prm.screen, c->fc.volume, c->S().d_U.p);
     return hipGetLastError();
 }
 
@@ -861,7 +956,7 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
     const size_t nthr = (size_t)ntile * c->nummol * NI;
     hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream, c->nummol,
                        nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
-                       soln ? c->d_tag.p : (const int *)nullptr, c->qslt, c->prm.screen, c->fc.volume, c->d_U.p);
+                       soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt, c->prm.screen, c->fc.volume, c->S().d_U.p);
     return hipGetLastError();
 }
 
@@ -876,10 +971,10 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
     const bool soln = c->prm.slttype == 1;
     HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice};
     const int ntile = (nch + NI - 1) / NI;
-    hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha, c->d_U.p,
+    hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha, c->S().d_U.p,
                        soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_spb.p, c->d_spe.p, c->d_idmin.p,
-                       c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p, c->d_minmax.p, soln ? c->d_flpart.p : (double *)nullptr,
-                       c->d_errkey.p, c->d_warn.p);
+                       c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p, c->S().d_minmax.p, soln ? c->d_flpart.p : (double *)nullptr,
+                       c->S().d_errkey.p, c->S().d_warn.p);
     return hipGetLastError();
 }
 
@@ -926,7 +1021,7 @@ static hipError_t prep_frame(er_ctx *c)
 
 static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
 {
-    const size_t nslt = (size_t)c->nslt;
+    const size_t nslt = (size_t)c->S().nslt;
     HIPCHK(c, c->d_insposn.ensure(nins * nslt));
     HIPCHK(c, c->d_insw.ensure(nins * nslt * 3 * c->ord));
     HIPCHK(c, c->d_insg.ensure(nins * nslt));
@@ -939,8 +1034,8 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     const size_t ntile = (size_t)(chunk + NI - 1) / NI;
     HIPCHK(c, c->d_phi.ensure(ntile * NI * c->G));
     const size_t ntile_all = (size_t)(nins + NI - 1) / NI;
-    HIPCHK(c, c->d_U.ensure(ntile_all * NI * c->nummol));
-    HIPCHK(c, c->d_u0.ensure(nins));
+    HIPCHK(c, c->S().d_U.ensure(ntile_all * NI * c->nummol));
+    HIPCHK(c, c->S().d_u0.ensure(nins));
     HIPCHK(c, c->d_C.ensure((size_t)nins * c->ermax));
     HIPCHK(c, c->d_w.ensure(nins));
     HIPCHK(c, c->d_perm.ensure(c->nummol));
@@ -994,7 +1089,7 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
         pa.nmol = c->nummol;
         pa.nchunk = nch;
         pa.n_off = n_off;
-        pa.nslt = c->nslt;
+        pa.nslt = c->S().nslt;
         pa.NI = NI;
         pa.ntype = c->ntype;
         pa.nc = c->nc;
@@ -1002,8 +1097,8 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
         pa.cell_w = c->cell_w;
         hipLaunchKernelGGL(k_pairs, dim3(nch * pa.nsplit), dim3(256), 0, c->stream, pa, c->fc, c->d_molbeg.p, c->d_posn.p,
                            c->d_ljtype.p, c->d_molref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p,
-                           c->d_insref.p, c->d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
-                           c->d_ljlensq.p, c->d_U.p);
+                           c->d_insref.p, c->S().d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
+                           c->d_ljlensq.p, c->S().d_U.p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     {
@@ -1038,10 +1133,10 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         wa.qtot = c->qtot;
         wa.temp = c->prm.temp;
         hipLaunchKernelGGL(k_weights, dim3(1), dim3(256), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
-                           c->d_resself.p, c->wslt_given ? c->d_wslt.p : (const float *)nullptr, c->d_state.p,
-                           c->d_flags.p, c->d_fstate.p, c->d_u0.p, c->d_w.p, c->d_escrd.p, c->d_eself.p, c->d_minmax.p,
+                           c->d_resself.p, c->S().wslt_given ? c->S().d_wslt.p : (const float *)nullptr, c->S().d_state.p,
+                           c->S().d_flags.p, c->S().d_fstate.p, c->S().d_u0.p, c->d_w.p, c->d_escrd.p, c->S().d_eself.p, c->S().d_minmax.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
-                           c->d_slnuv.p, c->d_errkey.p, c->nummol);
+                           c->S().d_slnuv.p, c->S().d_errkey.p, c->nummol);
     }
     // insertion slices of at most kper (a multiple of 4) for the split-K sums
     const int kper = std::max(4, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
@@ -1060,47 +1155,38 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         const bool full = mfma;
         hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, full ? c->ermax : 1), dim3(256), 0,
                            c->stream, c->ermax, nks, c->d_part.p, c->d_epart.p,
-                           full ? c->d_ecorr.p : (double *)nullptr, c->d_edens.p);
+                           full ? c->S().d_ecorr.p : (double *)nullptr, c->S().d_edens.p);
         if (c->prm.corrcal == 1 && !c->use_mfma) {
             const int nb16 = (c->ermax + 15) / 16;
             hipLaunchKernelGGL(k_ecorr_valu, dim3(nb16, nb16), dim3(16, 16), 0, c->stream, nch, n_off, c->ermax,
-                               c->d_C.p, c->d_w.p, c->d_ecorr.p);
+                               c->d_C.p, c->d_w.p, c->S().d_ecorr.p);
         }
     }
     return hipGetLastError();
 }
 
-extern "C" int er_run(er_ctx *c, int32_t mode)
+// one slot's insertions of the frame: per-insertion prep, chunks, accumulation
+static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
 {
-    if (!c) return ER_E_ARG;
-    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_run: er_set_mesh first");
-    if (!c->have_frame) return fail(c, ER_E_STATE, "er_run: no frame set");
     const bool soln = c->prm.slttype == 1;
-    int nins;
-    if (soln) {
-        if (c->nsltmol <= 0) return fail(c, ER_E_STATE, "er_run: er_set_solute_molecules first");
-        nins = c->nsltmol;
-    } else {
-        if (!c->have_ins) return fail(c, ER_E_STATE, "er_run: no insertions set");
-        nins = c->nins;
-    }
+    const int nins = soln ? c->nsltmol : c->S().nins;
     const int NI = (mode == ER_BATCHED) ? NI_B : 1;
     const int chunk = (mode == ER_BATCHED) ? c->chunk : 1;
     int rc = ensure_run_buffers(c, nins, chunk, NI);
     if (rc) return rc;
     {
         ProfScope ps(c, ER_K_PREP);
-        HIPCHK(c, prep_frame(c));
+        if (prep_frame_now) HIPCHK(c, prep_frame(c));
         const int B = 128;
-        hipLaunchKernelGGL(k_prep_ins, dim3((nins + B - 1) / B), dim3(B), 0, c->stream, nins, c->nslt,
-                           soln ? (const float *)nullptr : c->d_sltpos, c->d_sltq.p, c->d_sitepos, c->d_charge.p,
-                           c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3, c->ord,
-                           c->prm.screen, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_insref.p, c->d_selfreal.p,
-                           c->d_resself.p);
+        hipLaunchKernelGGL(k_prep_ins, dim3((nins + B - 1) / B), dim3(B), 0, c->stream, nins, c->S().nslt,
+                           soln ? (const float *)nullptr : c->S().d_sltpos, c->S().d_sltq.p, c->d_sitepos,
+                           c->d_charge.p, c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1,
+                           c->K2, c->K3, c->ord, c->prm.screen, c->d_insposn.p, c->d_insw.p, c->d_insg.p,
+                           c->d_insref.p, c->d_selfreal.p, c->d_resself.p);
         HIPCHK(c, hipGetLastError());
-        const size_t nf = (size_t)nins * c->nslt * (c->H + c->K2 + c->K3);
+        const size_t nf = (size_t)nins * c->S().nslt * (c->H + c->K2 + c->K3);
         hipLaunchKernelGGL(k_ins_factors, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, c->stream,
-                           nins * c->nslt, c->ord, c->K1, c->K2, c->K3, c->H, c->d_insposn.p, c->d_insw.p,
+                           nins * c->S().nslt, c->ord, c->K1, c->K2, c->K3, c->H, c->d_insposn.p, c->d_insw.p,
                            c->d_insg.p, c->d_tw1.p, c->d_tw2.p, c->d_tw3.p, c->d_Ftab.p);
         HIPCHK(c, hipGetLastError());
         const size_t nC = (size_t)nins * c->ermax;
@@ -1119,9 +1205,35 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
     }
-    c->last_NI = NI;
-    c->last_nins = nins;
+    c->S().last_NI = NI;
+    c->S().last_nins = nins;
     return ER_OK;
+}
+
+extern "C" int er_run(er_ctx *c, int32_t mode)
+{
+    if (!c) return ER_E_ARG;
+    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_run: er_set_mesh first");
+    if (!c->have_frame) return fail(c, ER_E_STATE, "er_run: no frame set");
+    if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run: mode");
+    std::vector<int> todo;
+    if (c->prm.slttype == 1) {
+        if (c->nsltmol <= 0) return fail(c, ER_E_STATE, "er_run: er_set_solute_molecules first");
+        todo.push_back(0);
+    } else {
+        for (int k = 0; k < (int)c->slots.size(); ++k)
+            if (c->slots[k].have_ins) todo.push_back(k);
+        if (todo.empty()) return fail(c, ER_E_STATE, "er_run: no insertions set");
+    }
+    // every solute of the frame in turn; the frame's preparation is shared
+    const int keep = c->cur;
+    int rc = ER_OK;
+    for (size_t t = 0; t < todo.size() && rc == ER_OK; ++t) {
+        c->cur = todo[t];
+        rc = run_slot(c, mode, t == 0);
+    }
+    c->cur = keep;
+    return rc;
 }
 
 static void decode_halt(er_ctx *c, unsigned long long key)
@@ -1132,15 +1244,15 @@ static void decode_halt(er_ctx *c, unsigned long long key)
     c->halt_code[0] = 0;
     if (j < 0) {
         float u = 0.f;
-        (void)hipMemcpy(&u, c->d_u0.p + n, sizeof(float), hipMemcpyDeviceToHost);
+        (void)hipMemcpy(&u, c->S().d_u0.p + n, sizeof(float), hipMemcpyDeviceToHost);
         std::strcpy(c->halt_code, "eng_min");
         c->halt_species = 0;
         c->halt_energy = u;
         return;
     }
-    const int NI = c->last_NI;
+    const int NI = c->S().last_NI;
     float v = 0.f;
-    (void)hipMemcpy(&v, c->d_U.p + ((size_t)(n / NI) * c->nummol + j) * NI + (n % NI), sizeof(float),
+    (void)hipMemcpy(&v, c->S().d_U.p + ((size_t)(n / NI) * c->nummol + j) * NI + (n % NI), sizeof(float),
                     hipMemcpyDeviceToHost);
     const int sp = c->uvspec[j];
     c->halt_species = sp;
@@ -1159,12 +1271,18 @@ extern "C" int er_sync(er_ctx *c)
 {
     if (!c) return ER_E_ARG;
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    unsigned long long key = ~0ull;
-    HIPCHK(c, hipMemcpy(&key, c->d_errkey.p, sizeof key, hipMemcpyDeviceToHost));
-    if (key != ~0ull) {
-        decode_halt(c, key);
-        return fail(c, ER_E_HALT, "halt condition %s (species %d, energy %g)", c->halt_code, c->halt_species,
-                    (double)c->halt_energy);
+    const int keep = c->cur;
+    for (int k = 0; k < (int)c->slots.size(); ++k) {
+        unsigned long long key = ~0ull;
+        HIPCHK(c, hipMemcpy(&key, c->slots[k].d_errkey.p, sizeof key, hipMemcpyDeviceToHost));
+        if (key != ~0ull) {
+            c->cur = k;
+            decode_halt(c, key);
+            c->cur = keep;
+            c->halt_slot = k;
+            return fail(c, ER_E_HALT, "halt condition %s (solute %d, species %d, energy %g)", c->halt_code, k + 1,
+                        c->halt_species, (double)c->halt_energy);
+        }
     }
     return ER_OK;
 }
@@ -1172,17 +1290,17 @@ extern "C" int er_sync(er_ctx *c)
 extern "C" int er_get_uvengy(er_ctx *c, float *uvengy, float *uvengy0)
 {
     if (!c) return ER_E_ARG;
-    if (c->last_nins <= 0) return fail(c, ER_E_STATE, "er_get_uvengy: no run yet");
+    if (c->S().last_nins <= 0) return fail(c, ER_E_STATE, "er_get_uvengy: no run yet");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    const int nins = c->last_nins, NI = c->last_NI, nmol = c->nummol;
+    const int nins = c->S().last_nins, NI = c->S().last_NI, nmol = c->nummol;
     if (uvengy) {
         const size_t ntile = (size_t)(nins + NI - 1) / NI;
         std::vector<float> u(ntile * NI * nmol);
-        HIPCHK(c, hipMemcpy(u.data(), c->d_U.p, u.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(u.data(), c->S().d_U.p, u.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (int n = 0; n < nins; ++n)
             for (int j = 0; j < nmol; ++j) uvengy[(size_t)n * nmol + j] = u[((size_t)(n / NI) * nmol + j) * NI + n % NI];
     }
-    if (uvengy0) HIPCHK(c, hipMemcpy(uvengy0, c->d_u0.p, nins * sizeof(float), hipMemcpyDeviceToHost));
+    if (uvengy0) HIPCHK(c, hipMemcpy(uvengy0, c->S().d_u0.p, nins * sizeof(float), hipMemcpyDeviceToHost));
     return ER_OK;
 }
 
@@ -1192,14 +1310,14 @@ extern "C" int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, 
     if (!c) return ER_E_ARG;
     if (!c->have_mesh) return fail(c, ER_E_STATE, "er_fetch: no mesh");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    if (edens) HIPCHK(c, hipMemcpy(edens, c->d_edens.p, c->ermax * sizeof(double), hipMemcpyDeviceToHost));
+    if (edens) HIPCHK(c, hipMemcpy(edens, c->S().d_edens.p, c->ermax * sizeof(double), hipMemcpyDeviceToHost));
     if (ecorr)
-        HIPCHK(c, hipMemcpy(ecorr, c->d_ecorr.p, (size_t)c->ermax * c->ermax * sizeof(double), hipMemcpyDeviceToHost));
-    if (eself && c->esmax > 0) HIPCHK(c, hipMemcpy(eself, c->d_eself.p, c->esmax * sizeof(double), hipMemcpyDeviceToHost));
-    if (scal) HIPCHK(c, hipMemcpy(scal, c->d_state.p, 4 * sizeof(double), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(ecorr, c->S().d_ecorr.p, (size_t)c->ermax * c->ermax * sizeof(double), hipMemcpyDeviceToHost));
+    if (eself && c->esmax > 0) HIPCHK(c, hipMemcpy(eself, c->S().d_eself.p, c->esmax * sizeof(double), hipMemcpyDeviceToHost));
+    if (scal) HIPCHK(c, hipMemcpy(scal, c->S().d_state.p, 4 * sizeof(double), hipMemcpyDeviceToHost));
     if (minuv || maxuv) {
         std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
-        HIPCHK(c, hipMemcpy(mm.data(), c->d_minmax.p, mm.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(mm.data(), c->S().d_minmax.p, mm.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
         auto unord = [](unsigned int u) {
             u = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
             float f;
@@ -1211,16 +1329,16 @@ extern "C" int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, 
             if (maxuv) maxuv[k] = unord(mm[2 * k + 1]);
         }
     }
-    if (slnuv) HIPCHK(c, hipMemcpy(slnuv, c->d_slnuv.p, c->numslv * sizeof(double), hipMemcpyDeviceToHost));
+    if (slnuv) HIPCHK(c, hipMemcpy(slnuv, c->S().d_slnuv.p, c->numslv * sizeof(double), hipMemcpyDeviceToHost));
     return ER_OK;
 }
 
 extern "C" int er_get_flceng(er_ctx *c, double *flceng)
 {
     if (!c || !flceng) return ER_E_ARG;
-    if (c->prm.slttype != 1 || c->last_nins <= 0) return fail(c, ER_E_STATE, "er_get_flceng: no soln run");
+    if (c->prm.slttype != 1 || c->S().last_nins <= 0) return fail(c, ER_E_STATE, "er_get_flceng: no soln run");
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    HIPCHK(c, hipMemcpy(flceng, c->d_flceng.p, (size_t)c->last_nins * c->numslv * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(flceng, c->d_flceng.p, (size_t)c->S().last_nins * c->numslv * sizeof(double), hipMemcpyDeviceToHost));
     return ER_OK;
 }
 
@@ -1228,14 +1346,11 @@ extern "C" int er_clear(er_ctx *c)
 {
     if (!c) return ER_E_ARG;
     if (!c->have_mesh) return fail(c, ER_E_STATE, "er_clear: no mesh");
-    hipStream_t s = c->stream;
-    HIPCHK(c, hipMemsetAsync(c->d_edens.p, 0, c->ermax * sizeof(double), s));
-    HIPCHK(c, hipMemsetAsync(c->d_ecorr.p, 0, (size_t)c->ermax * c->ermax * sizeof(double), s));
-    HIPCHK(c, hipMemsetAsync(c->d_eself.p, 0, std::max(1, c->esmax) * sizeof(double), s));
-    HIPCHK(c, hipMemsetAsync(c->d_slnuv.p, 0, c->numslv * sizeof(double), s));
-    // engnorm, engsmpl, avslf (voffset survives, engproc.F90:336-338)
-    HIPCHK(c, hipMemsetAsync(c->d_state.p, 0, 3 * sizeof(double), s));
-    HIPCHK(c, hipStreamSynchronize(s));
+    for (Slot &sl : c->slots) {
+        const int rc = slot_clear(c, sl);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return ER_OK;
 }
 
@@ -1283,17 +1398,15 @@ extern "C" void er_destroy(er_ctx *c)
     c->prof_pool.clear();
     c->d_charge.release(); c->d_ljene.release(); c->d_ljlensq.release(); c->d_molq.release();
     c->d_ljtype.release(); c->d_molbeg.release(); c->d_spb.release(); c->d_spe.release();
-    c->d_sltq.release(); c->d_sltlj.release(); c->d_tag.release();
+    for (Slot &sl : c->slots) sl.release();
+    c->d_tag.release();
     c->d_uvcrd.release(); c->d_escrd.release(); c->d_uvmax.release(); c->d_uvsoft.release(); c->d_idmin.release();
     c->d_sitepos_own.release(); c->d_posn.release(); c->d_molref.release(); c->d_splw.release(); c->d_splg.release();
-    c->d_sltpos_own.release(); c->d_wslt.release(); c->d_insposn.release(); c->d_insref.release();
-    c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
+    c->d_insposn.release(); c->d_insref.release(); c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_Ftab.release(); c->d_T.release(); c->d_phi.release();
-    c->d_U.release(); c->d_u0.release(); c->d_C.release(); c->d_w.release(); c->d_flceng.release();
-    c->d_edens.release(); c->d_ecorr.release(); c->d_part.release(); c->d_epart.release(); c->d_eself.release(); c->d_state.release(); c->d_slnuv.release();
-    c->d_flags.release(); c->d_warn.release(); c->d_fstate.release(); c->d_minmax.release(); c->d_errkey.release();
+    c->d_C.release(); c->d_w.release(); c->d_flceng.release(); c->d_part.release(); c->d_epart.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

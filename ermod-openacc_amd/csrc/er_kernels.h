@@ -295,9 +295,110 @@ __global__ void k_ins_factors(int nrows, int ord, int K1, int K2, int K3, int H,
     Ftab[f] = a;
 }
 
+// In-place 2-D DFT of the LDS plane [K2][PS] (x3/m3 fastest) along axis 3
+// (K3 = R1*R2) and then axis 2 (K2 = C1*C2), each as a four-step transform:
+// pass A = R1-point DFTs over stride-R2 elements + twiddles, pass B = R2-point
+// DFTs with the results written in natural order.  SIGN = +1 is the inverse
+// (e^{+i}) transform, -1 the forward one; both unnormalised.
+template <int K2, int K3, int R1, int R2, int C1, int C2, int NT, int SIGN>
+__device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const float2 *t3,
+                                          const float2 *__restrict__ tws, int tid)
+{
+    constexpr int PS = K3 + 1;
+    // ---- rows (axis 3), pass A in place ----
+    for (int it = tid; it < K2 * R2; it += NT) {
+        const int m2 = it / R2, n2 = it % R2;
+        float2 v[R1];
+#pragma unroll
+        for (int a = 0; a < R1; ++a) v[a] = plane[m2 * PS + R2 * a + n2];
+        erfft::fft<R1, SIGN>(v, tws + tws_off(R1));
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1) v[k1] = erfft::twmul<SIGN>(v[k1], t3[(n2 * k1) % K3]);
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) plane[m2 * PS + R2 * k1 + n2] = v[k1];
+    }
+    __syncthreads();
+    // ---- rows, pass B (outputs held until every thread has read) ----
+    {
+        constexpr int IB = (K2 * R1 + NT - 1) / NT;
+        float2 hold[IB][R2];
+#pragma unroll
+        for (int j = 0; j < IB; ++j) {
+            const int it = tid + j * NT;
+            if (it < K2 * R1) {
+                const int m2 = it / R1, k1 = it % R1;
+#pragma unroll
+                for (int b = 0; b < R2; ++b) hold[j][b] = plane[m2 * PS + R2 * k1 + b];
+                erfft::fft<R2, SIGN>(hold[j], tws + tws_off(R2));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IB; ++j) {
+            const int it = tid + j * NT;
+            if (it < K2 * R1) {
+                const int m2 = it / R1, k1 = it % R1;
+#pragma unroll
+                for (int k2 = 0; k2 < R2; ++k2) plane[m2 * PS + k1 + R1 * k2] = hold[j][k2];
+            }
+        }
+    }
+    __syncthreads();
+    // ---- columns (axis 2), pass A in place ----
+    for (int it = tid; it < C2 * K3; it += NT) {
+        const int n2 = it / K3, x3 = it % K3;
+        float2 v[C1];
+#pragma unroll
+        for (int a = 0; a < C1; ++a) v[a] = plane[(C2 * a + n2) * PS + x3];
+        erfft::fft<C1, SIGN>(v, tws + tws_off(C1));
+#pragma unroll
+        for (int k1 = 1; k1 < C1; ++k1) v[k1] = erfft::twmul<SIGN>(v[k1], t2[(n2 * k1) % K2]);
+#pragma unroll
+        for (int k1 = 0; k1 < C1; ++k1) plane[(C2 * k1 + n2) * PS + x3] = v[k1];
+    }
+    __syncthreads();
+    // ---- columns, pass B ----
+    {
+        constexpr int IC = (C1 * K3 + NT - 1) / NT;
+        float2 hold[IC][C2];
+#pragma unroll
+        for (int j = 0; j < IC; ++j) {
+            const int it = tid + j * NT;
+            if (it < C1 * K3) {
+                const int k1 = it / K3, x3 = it % K3;
+#pragma unroll
+                for (int b = 0; b < C2; ++b) hold[j][b] = plane[(C2 * k1 + b) * PS + x3];
+                erfft::fft<C2, SIGN>(hold[j], tws + tws_off(C2));
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IC; ++j) {
+            const int it = tid + j * NT;
+            if (it < C1 * K3) {
+                const int k1 = it / K3, x3 = it % K3;
+#pragma unroll
+                for (int k2 = 0; k2 < C2; ++k2) plane[(k1 + C1 * k2) * PS + x3] = hold[j][k2];
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// PME, stage A: one (insertion, m1) plane of the half spectrum per workgroup.
+//   1. rho^(m1, m2, m3) = FFT of the solute's spread charges.  Few sites:
+//      sum_s F1_s(m1) F2_s(m2) F3_s(m3) from the separable factor table
+//      (nslt * K2 * K3 complex products).  Many sites (sparse = 1): the
+//      (x2, x3) plane of sum_s F1_s(m1) w2_s(x2) w3_s(x3) is scattered in
+//      LDS (ORD^2 points per site, sites in order) and transformed forward
+//      along axes 3 and 2 (the same transform as the spread + forward FFT
+//      of recpcal.F90:446-478, done per plane).
+//   2. FP64 half-plane self-energy partial and the engfac multiply.
+//   3. inverse DFT along axes 3 and 2; T[nl][m1][x3][x2] for stage B.
 template <int K2, int K3, int R1, int R2, int C1, int C2, int NT>
-__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
+__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off, int sparse,
                                                     const float2 *__restrict__ Ftab,
+                                                    const float *__restrict__ insw, const int4 *__restrict__ insg,
                                                     const float *__restrict__ engfac,
                                                     const float2 *__restrict__ tw2, const float2 *__restrict__ tw3,
                                                     const float2 *__restrict__ tws, float2 *__restrict__ T,
@@ -322,36 +423,65 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
     const int tid = threadIdx.x;
     for (int k = tid; k < K2; k += NT) t2[k] = tw2[k];
     for (int k = tid; k < K3; k += NT) t3[k] = tw3[k];
-
-    // ---- generation of rho^ over the plane, sites in LDS chunks ----
-    float2 acc[EPT];
-#pragma unroll
-    for (int k = 0; k < EPT; ++k) acc[k] = make_float2(0.f, 0.f);
     const int per = H + K2 + K3;
-    for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
-        const int ns = min(ER_SCH, nslt - s0);
-        for (int f = tid; f < ns * (K2 + K3); f += NT) {
-            const int s = f / (K2 + K3), r = f % (K2 + K3);
-            const float2 *Fs = Ftab + ((size_t)n * nslt + s0 + s) * per;
-            if (r < K2) F2[s * K2 + r] = erfft::cmul(Fs[m1], Fs[H + r]);
-            else F3[s * K3 + (r - K2)] = Fs[H + r];
+
+    if (!sparse) {
+        // ---- separable generation, sites in LDS chunks ----
+        float2 acc[EPT];
+#pragma unroll
+        for (int k = 0; k < EPT; ++k) acc[k] = make_float2(0.f, 0.f);
+        for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
+            const int ns = min(ER_SCH, nslt - s0);
+            for (int f = tid; f < ns * (K2 + K3); f += NT) {
+                const int s = f / (K2 + K3), r = f % (K2 + K3);
+                const float2 *Fs = Ftab + ((size_t)n * nslt + s0 + s) * per;
+                if (r < K2) F2[s * K2 + r] = erfft::cmul(Fs[m1], Fs[H + r]);
+                else F3[s * K3 + (r - K2)] = Fs[H + r];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < EPT; ++k) {
+                const int e = tid + k * NT;
+                if (e < NE) {
+                    const int m2 = e % K2, m3 = e / K2;
+                    for (int s = 0; s < ns; ++s) {
+                        const float2 B = F2[s * K2 + m2];
+                        const float2 f3 = F3[s * K3 + m3];
+                        acc[k].x += B.x * f3.x - B.y * f3.y;
+                        acc[k].y += B.x * f3.y + B.y * f3.x;
+                    }
+                }
+            }
+            __syncthreads();
         }
-        __syncthreads();
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NT;
-            if (e < NE) {
-                const int m2 = e % K2, m3 = e / K2;
-                for (int s = 0; s < ns; ++s) {
-                    const float2 B = F2[s * K2 + m2];
-                    const float2 f3 = F3[s * K3 + m3];
-                    acc[k].x += B.x * f3.x - B.y * f3.y;
-                    acc[k].y += B.x * f3.y + B.y * f3.x;
-                }
+            if (e < NE) plane[(e % K2) * PS + e / K2] = acc[k];
+        }
+    } else {
+        // ---- sparse spread of the plane, then forward DFT along axes 3, 2 ----
+        for (int e = tid; e < K2 * PS; e += NT) plane[e] = make_float2(0.f, 0.f);
+        __syncthreads();
+        // one wave, one lane per stencil point (c2, c3): the ORD^2 points of a
+        // site are distinct, and the wave's LDS updates stay in site order
+        if (tid < ord * ord) {
+            const int c2 = tid / ord, c3 = tid % ord;
+            for (int s = 0; s < nslt; ++s) {
+                const size_t site = (size_t)n * nslt + s;
+                const int4 g = insg[site];
+                const float *w = insw + site * 3 * ord;
+                const float2 A = Ftab[site * per + m1];
+                const int x2 = pmod(g.y - c2, K2), x3 = pmod(g.z - c3, K3);
+                const float2 v = erfft::cscale(A, w[ord + c2] * w[2 * ord + c3]);
+                float2 *q = plane + x2 * PS + x3;
+                *q = make_float2(q->x + v.x, q->y + v.y);
             }
         }
         __syncthreads();
+        plane_dft<K2, K3, R1, R2, C1, C2, NT, -1>(plane, t2, t3, tws, tid);
     }
+    __syncthreads();
     // ---- engfac multiply + FP64 half-plane self energy ----
     double sp = 0.0;
     {
@@ -363,90 +493,15 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
             if (e < NE) {
                 const int m2 = e % K2, m3 = e / K2;
                 const float f = ef[e];  // [m1][m3][m2]
-                sp += wt * (double)f * ((double)acc[k].x * acc[k].x + (double)acc[k].y * acc[k].y);
-                plane[m2 * PS + m3] = erfft::cscale(acc[k], f);
+                float2 *q = plane + m2 * PS + m3;
+                const float2 a = *q;
+                sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+                *q = erfft::cscale(a, f);
             }
         }
     }
     __syncthreads();
-    // ---- rows (axis 3, length K3 = R1*R2), pass A in place ----
-    for (int it = tid; it < K2 * R2; it += NT) {
-        const int m2 = it / R2, n2 = it % R2;
-        float2 v[R1];
-#pragma unroll
-        for (int a = 0; a < R1; ++a) v[a] = plane[m2 * PS + R2 * a + n2];
-        erfft::fft<R1, +1>(v, tws + tws_off(R1));
-#pragma unroll
-        for (int k1 = 1; k1 < R1; ++k1) v[k1] = erfft::cmul(v[k1], t3[(n2 * k1) % K3]);
-#pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) plane[m2 * PS + R2 * k1 + n2] = v[k1];
-    }
-    __syncthreads();
-    // ---- rows, pass B (outputs held until every thread has read) ----
-    {
-        constexpr int IB = (K2 * R1 + NT - 1) / NT;
-        float2 hold[IB][R2];
-#pragma unroll
-        for (int j = 0; j < IB; ++j) {
-            const int it = tid + j * NT;
-            if (it < K2 * R1) {
-                const int m2 = it / R1, k1 = it % R1;
-#pragma unroll
-                for (int b = 0; b < R2; ++b) hold[j][b] = plane[m2 * PS + R2 * k1 + b];
-                erfft::fft<R2, +1>(hold[j], tws + tws_off(R2));
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IB; ++j) {
-            const int it = tid + j * NT;
-            if (it < K2 * R1) {
-                const int m2 = it / R1, k1 = it % R1;
-#pragma unroll
-                for (int k2 = 0; k2 < R2; ++k2) plane[m2 * PS + k1 + R1 * k2] = hold[j][k2];
-            }
-        }
-    }
-    __syncthreads();
-    // ---- columns (axis 2, length K2 = C1*C2), pass A in place ----
-    for (int it = tid; it < C2 * K3; it += NT) {
-        const int n2 = it / K3, x3 = it % K3;
-        float2 v[C1];
-#pragma unroll
-        for (int a = 0; a < C1; ++a) v[a] = plane[(C2 * a + n2) * PS + x3];
-        erfft::fft<C1, +1>(v, tws + tws_off(C1));
-#pragma unroll
-        for (int k1 = 1; k1 < C1; ++k1) v[k1] = erfft::cmul(v[k1], t2[(n2 * k1) % K2]);
-#pragma unroll
-        for (int k1 = 0; k1 < C1; ++k1) plane[(C2 * k1 + n2) * PS + x3] = v[k1];
-    }
-    __syncthreads();
-    // ---- columns, pass B ----
-    {
-        constexpr int IC = (C1 * K3 + NT - 1) / NT;
-        float2 hold[IC][C2];
-#pragma unroll
-        for (int j = 0; j < IC; ++j) {
-            const int it = tid + j * NT;
-            if (it < C1 * K3) {
-                const int k1 = it / K3, x3 = it % K3;
-#pragma unroll
-                for (int b = 0; b < C2; ++b) hold[j][b] = plane[(C2 * k1 + b) * PS + x3];
-                erfft::fft<C2, +1>(hold[j], tws + tws_off(C2));
-            }
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IC; ++j) {
-            const int it = tid + j * NT;
-            if (it < C1 * K3) {
-                const int k1 = it / K3, x3 = it % K3;
-#pragma unroll
-                for (int k2 = 0; k2 < C2; ++k2) plane[(k1 + C1 * k2) * PS + x3] = hold[j][k2];
-            }
-        }
-    }
-    __syncthreads();
+    plane_dft<K2, K3, R1, R2, C1, C2, NT, +1>(plane, t2, t3, tws, tid);
     // ---- out: T[nl][m1][x3][x2] ----
     float2 *out = T + ((size_t)nl * H + m1) * (size_t)NE;
     for (int f = tid; f < NE; f += NT) out[f] = plane[(f % K2) * PS + (f / K2)];

@@ -16,7 +16,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "liberdst_hip.so")
 ER_OK, ER_E_ARG, ER_E_UNSUPPORTED, ER_E_HIP, ER_E_HALT, ER_E_STATE = range(6)
 ER_PLAIN, ER_BATCHED = 0, 1
 
-EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solute_molecules", "er_set_frame",
+EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_select_solute",
+            "er_set_solute_molecules", "er_set_frame",
             "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_sync",
             "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
             "er_destroy", "er_profile", "er_profile_read"]
@@ -58,6 +59,8 @@ def load() -> C.CDLL:
     L.er_create.argtypes = [C.POINTER(ErTopology), C.POINTER(ErParams), C.POINTER(vp)]
     L.er_set_mesh.argtypes = [vp, vp, vp, vp, i32, vp]
     L.er_set_solute.argtypes = [vp, i32, vp, vp]
+    L.er_set_solutes.argtypes = [vp, i32, vp, vp, vp]
+    L.er_select_solute.argtypes = [vp, i32]
     L.er_set_solute_molecules.argtypes = [vp, i32, vp]
     L.er_set_frame.argtypes = [vp, vp, vp, f32]
     L.er_set_frame_device.argtypes = [vp, vp, vp, f32]
@@ -147,6 +150,17 @@ class Context:
         lj = np.ascontiguousarray(np.asarray(ljtype0) + 1, np.int32)
         q = np.ascontiguousarray(charge32, np.float32)
         self._check(self.L.er_set_solute(self.h, lj.size, _ptr(lj), _ptr(q)))
+
+    def set_solutes(self, solutes):
+        """solutes: [(ljtype0, charge32), ...], one per test particle."""
+        ns = np.array([len(q) for _, q in solutes], np.int32)
+        lj = np.ascontiguousarray(np.concatenate([np.asarray(t) + 1 for t, _ in solutes]), np.int32)
+        q = np.ascontiguousarray(np.concatenate([np.asarray(c) for _, c in solutes]), np.float32)
+        self._check(self.L.er_set_solutes(self.h, ns.size, _ptr(ns), _ptr(lj), _ptr(q)))
+
+    def select_solute(self, k):
+        """Select solute k (0-based here, 1-based in the C ABI)."""
+        self._check(self.L.er_select_solute(self.h, int(k) + 1))
 
     def set_solute_molecules(self, tagslt0):
         t = np.ascontiguousarray(np.asarray(tagslt0) + 1, np.int32)

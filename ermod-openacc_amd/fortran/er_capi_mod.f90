@@ -24,7 +24,7 @@ module er_capi
      real(c_double) :: temp
   end type er_params
 
-  public :: er_create, er_set_mesh, er_set_solute, er_set_solute_molecules, &
+  public :: er_create, er_set_mesh, er_set_solute, er_set_solutes, er_select_solute, er_set_solute_molecules, &
        er_set_frame, er_set_insertions, er_run, er_sync, er_get_uvengy, er_fetch, &
        er_get_flceng, er_clear, er_last_error, er_destroy
 
@@ -53,6 +53,20 @@ module er_capi
        integer(c_int32_t), intent(in) :: ljtype(*)
        real(c_float), intent(in) :: charge(*)
      end function er_set_solute
+
+     integer(c_int) function er_set_solutes(ctx, nsolute, nsite, ljtype, charge) bind(C, name="er_set_solutes")
+       import :: c_int, c_int32_t, c_float, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: nsolute
+       integer(c_int32_t), intent(in) :: nsite(*), ljtype(*)
+       real(c_float), intent(in) :: charge(*)
+     end function er_set_solutes
+
+     integer(c_int) function er_select_solute(ctx, k) bind(C, name="er_select_solute")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: k
+     end function er_select_solute
 
      integer(c_int) function er_set_solute_molecules(ctx, nslt, tagslt) &
           bind(C, name="er_set_solute_molecules")

@@ -92,6 +92,22 @@ int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsoft,
  * (engproc.F90:99-101).  ljtype 1-based, charge x 18.22261721. */
 int er_set_solute(er_ctx *c, int32_t nsite, const int32_t *ljtype, const float *charge);
 
+/* Several test particles analysed over the same frames in one context (refs
+ * runs).  The reference runs erdst once per solute (one test particle per
+ * run, engproc.F90:99-101); here each solute k = 1..nsolute is a slot with
+ * its own insertions, energies and block accumulators, and one er_run
+ * processes every slot that has insertions, sharing the frame's
+ * preparation.  nsite: [nsolute]; ljtype (1-based) and charge (x 18.22261721):
+ * concatenated in solute order.  Resets every slot's accumulators; slot 1
+ * is selected afterwards. */
+int er_set_solutes(er_ctx *c, int32_t nsolute, const int32_t *nsite, const int32_t *ljtype,
+                   const float *charge);
+
+/* Selects slot k (1-based) for the calls that address one solute:
+ * er_set_solute, er_set_insertions[_device], er_get_uvengy, er_fetch.
+ * er_clear clears every slot; er_sync reports the first slot that halted. */
+int er_select_solute(er_ctx *c, int32_t k);
+
 /* Solute molecules of a soln run (sltlist, engproc.F90:72-96), 1-based. */
 int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *tagslt);
 

@@ -112,3 +112,45 @@ def test_config4_distribution_plain_equals_batched(cfg4):
     nm = np.bincount(w.topo.uvspec, minlength=3)[1:]
     en = res[0]["edens"] / res[0]["engnorm"]
     np.testing.assert_allclose([en[:600].sum(), en[600:].sum()], nm, rtol=1e-12)
+
+
+def test_config3_all_solutes_one_context(cfg3):
+    """All 16 solutes of configuration 3 in one context (er_set_solutes):
+    one er_run per frame processes every solute; each solute's energies and
+    distributions equal those of its own single-solute context."""
+    nins = 24
+    ws = [cfg3.workload(k) for k in range(16)]
+    p = U.workload_params(ws[0])
+    multi = capi.Context(ws[0].topo, p)
+    multi.set_mesh(*cfg3.uvmax_uvsoft_uvcrd())
+    multi.set_solutes([(w.topo.slt_ljtype, w.topo.slt_charge32) for w in ws])
+    frames = (0, 1)
+    singles = []
+    for k, w in enumerate(ws):
+        ctx = _ctx(w, p)
+        singles.append(ctx)
+    for f in frames:
+        x = cfg3.frame(f)
+        multi.set_frame(x, cfg3.cell)
+        for k, w in enumerate(ws):
+            multi.select_solute(k)
+            multi.set_insertions(w.insertions(f)[:nins])
+        multi.run(capi.ER_BATCHED)
+        multi.sync()
+        for k, w in enumerate(ws):
+            singles[k].set_frame(x, cfg3.cell)
+            singles[k].set_insertions(w.insertions(f)[:nins])
+            singles[k].run(capi.ER_BATCHED)
+            singles[k].sync()
+            multi.select_solute(k)
+            um, u0m = multi.uvengy(nins)
+            us, u0s = singles[k].uvengy(nins)
+            np.testing.assert_array_equal(um, us)
+            np.testing.assert_array_equal(u0m, u0s)
+    for k in range(16):
+        multi.select_solute(k)
+        gm, gs = multi.fetch(), singles[k].fetch()
+        for key in ("edens", "ecorr"):
+            np.testing.assert_array_equal(gm[key], gs[key])
+        for key in ("engnorm", "engsmpl", "avslf", "voffset"):
+            assert gm[key] == gs[key]
