@@ -63,13 +63,14 @@ struct Slot {
     DBuf<int> d_flags, d_warn;
     DBuf<float> d_fstate;
     DBuf<unsigned int> d_minmax;
-    DBuf<unsigned long long> d_errkey;
+    DBuf<unsigned long long> d_errkey, d_halt;  // halt key; captured key + energy
+    unsigned int seq = 0;                         // runs of this slot
     void release()
     {
         d_sltq.release(); d_sltlj.release(); d_sltpos_own.release(); d_wslt.release(); d_U.release();
         d_u0.release(); d_edens.release(); d_ecorr.release(); d_eself.release(); d_state.release();
         d_slnuv.release(); d_flags.release(); d_warn.release(); d_fstate.release(); d_minmax.release();
-        d_errkey.release();
+        d_errkey.release(); d_halt.release();
     }
 };
 
@@ -315,6 +316,7 @@ static int slot_state_init(er_ctx *c, Slot &sl)
     HIPCHK(c, sl.d_fstate.ensure(4));
     HIPCHK(c, sl.d_warn.ensure(1));
     HIPCHK(c, sl.d_errkey.ensure(1));
+    HIPCHK(c, sl.d_halt.ensure(2));
     HIPCHK(c, sl.d_minmax.ensure(2 * (size_t)(c->numslv + 1)));
     HIPCHK(c, sl.d_slnuv.ensure(c->numslv));
     HIPCHK(c, hipMemsetAsync(sl.d_state.p, 0, 4 * sizeof(double), s));
@@ -322,6 +324,8 @@ static int slot_state_init(er_ctx *c, Slot &sl)
     HIPCHK(c, hipMemsetAsync(sl.d_fstate.p, 0, 4 * sizeof(float), s));
     HIPCHK(c, hipMemsetAsync(sl.d_warn.p, 0, sizeof(int), s));
     HIPCHK(c, hipMemsetAsync(sl.d_errkey.p, 0xff, sizeof(unsigned long long), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_halt.p, 0xff, 2 * sizeof(unsigned long long), s));
+    sl.seq = 0;
     HIPCHK(c, hipMemsetAsync(sl.d_slnuv.p, 0, c->numslv * sizeof(double), s));
     std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
     auto ord = [](float f) {
@@ -969,7 +973,7 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)k_hist<NI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const bool soln = c->prm.slttype == 1;
-    HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice};
+    HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice, c->S().seq};
     const int ntile = (nch + NI - 1) / NI;
     hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha, c->S().d_U.p,
                        soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_spb.p, c->d_spe.p, c->d_idmin.p,
@@ -1127,6 +1131,7 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         wa.rigid = c->prm.slttype == 2;
         wa.numslv = c->numslv;
         wa.nslice = c->nslice;
+        wa.seq = c->S().seq;
         wa.w_sys = c->w_sys;
         wa.volume = c->fc.volume;
         wa.screen = c->prm.screen;
@@ -1170,6 +1175,9 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
 {
     const bool soln = c->prm.slttype == 1;
     const int nins = soln ? c->nsltmol : c->S().nins;
+    if (nins >= (1 << 21) || c->nummol >= (1 << 21) - 1)
+        return fail(c, ER_E_UNSUPPORTED, "er_run: at most 2^21-1 insertions and 2^21-2 molecules per run");
+    c->S().seq += 1;
     const int NI = (mode == ER_BATCHED) ? NI_B : 1;
     const int chunk = (mode == ER_BATCHED) ? c->chunk : 1;
     int rc = ensure_run_buffers(c, nins, chunk, NI);
@@ -1205,6 +1213,9 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
     }
+    hipLaunchKernelGGL(k_halt_capture, dim3(1), dim3(1), 0, c->stream, c->S().seq, NI, c->nummol,
+                       c->S().d_errkey.p, c->S().d_U.p, c->S().d_u0.p, c->S().d_halt.p);
+    HIPCHK(c, hipGetLastError());
     c->S().last_NI = NI;
     c->S().last_nins = nins;
     return ER_OK;
@@ -1236,27 +1247,28 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
     return rc;
 }
 
+// the halt of slot c->S(): (insertion, molecule) from the key, the energy as
+// captured at the end of the run that produced it
 static void decode_halt(er_ctx *c, unsigned long long key)
 {
-    const unsigned long long per = (unsigned long long)c->nummol + 1;
-    const int n = (int)(key / per);
-    const int j = (int)(key % per) - 1;
-    c->halt_code[0] = 0;
-    if (j < 0) {
-        float u = 0.f;
-        (void)hipMemcpy(&u, c->S().d_u0.p + n, sizeof(float), hipMemcpyDeviceToHost);
+    const int jp1 = (int)(key & 0x1fffffu);
+    unsigned long long cap[2] = {~0ull, 0ull};
+    (void)hipMemcpy(cap, c->S().d_halt.p, sizeof cap, hipMemcpyDeviceToHost);
+    float v = 0.f;
+    if (cap[0] == key) {
+        const unsigned int bits = (unsigned int)cap[1];
+        std::memcpy(&v, &bits, 4);
+    } else {
+        v = NAN;  // not captured (cannot happen after a completed run)
+    }
+    c->halt_energy = v;
+    if (jp1 == 0) {
         std::strcpy(c->halt_code, "eng_min");
         c->halt_species = 0;
-        c->halt_energy = u;
         return;
     }
-    const int NI = c->S().last_NI;
-    float v = 0.f;
-    (void)hipMemcpy(&v, c->S().d_U.p + ((size_t)(n / NI) * c->nummol + j) * NI + (n % NI), sizeof(float),
-                    hipMemcpyDeviceToHost);
-    const int sp = c->uvspec[j];
+    const int sp = c->uvspec[jp1 - 1];
     c->halt_species = sp;
-    c->halt_energy = v;
     int lo = 0, hi = c->uvmax[sp - 1];
     const double *mesh = c->uvcrd.data() + c->idmin[sp - 1];
     while (lo < hi) {

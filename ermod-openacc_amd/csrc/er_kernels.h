@@ -187,21 +187,23 @@ __global__ void k_prep_ins(int nins, int nslt, const float *__restrict__ sltpos,
         }
     }
     insref[n] = make_float4(c0.x, c0.y, c0.z, R);
-    // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r
-    float e = 0.0f;
+    // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r;
+    // FP32 terms as in the reference's single build, summed in FP64 (a
+    // 150-site solute has 11 175 terms that largely cancel)
+    double e = 0.0;
     const float rsqpi = 0.564189583547756286948079451560772586f;  // 1/sqrt(pi)
     for (int s = 0; s < nslt; ++s) {
         const float4 a = insposn[(size_t)n * nslt + s];
-        e += -a.w * a.w * screen * rsqpi;
+        e += (double)(-a.w * a.w * screen * rsqpi);
         for (int t = s + 1; t < nslt; ++t) {
             const float4 b = insposn[(size_t)n * nslt + t];
             float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
             min_image(fc, dx, dy, dz);
             const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            e += -a.w * b.w * erff(screen * r) / r;
+            e += (double)(-a.w * b.w * erff(screen * r) / r);
         }
     }
-    selfreal[n] = e;
+    selfreal[n] = (float)e;
     // residual_self_ene (engproc.F90:967-977): -(pi Q^2 / alpha^2 / V) / 2 with
     // Q = mol_charge, a plain sum of the site charges (setconf.F90:838-840)
     float Q = 0.0f;
@@ -1078,7 +1080,16 @@ __device__ __forceinline__ unsigned int f2ord(float f)
 
 struct HistArgs {
     int nmol, nchunk, n_off, ermax, numslv, slttype, nslice;
+    unsigned int seq;  // run number of the slot (halt keys order runs first)
 };
+
+// Halt key: the earliest (run, insertion, molecule) that hit a halt condition
+// wins the 64-bit atomicMin.  Bits 63..42 run number, 41..21 insertion, 20..0
+// molecule + 1 (0: the solute's own uvengy0).
+__device__ __forceinline__ unsigned long long halt_key(unsigned int seq, int n, int jp1)
+{
+    return ((unsigned long long)(seq & 0x3fffffu) << 42) | ((unsigned long long)n << 21) | (unsigned long long)jp1;
+}
 
 template <int NI>
 __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restrict__ U, const int *__restrict__ tag,
@@ -1127,11 +1138,7 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
                     else hi = mid;
                 }
                 const int idpti = lo;
-                if (idpti <= 0 || (ha.slttype == 1 && idpti > soft)) {
-                    const unsigned long long key =
-                        (unsigned long long)n * (unsigned long long)(ha.nmol + 1) + j + 1;
-                    atomicMin(errkey, key);
-                }
+                if (idpti <= 0 || (ha.slttype == 1 && idpti > soft)) atomicMin(errkey, halt_key(ha.seq, n, j + 1));
                 if (idpti == num && v < 1.0e3f) atomicOr(warn, 1);
                 atomicAdd(&hist[i * ha.ermax + base + max(idpti, 1) - 1], 1u);
                 lmin = fminf(lmin, v);
@@ -1175,6 +1182,21 @@ __global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
     if (i < n) p[i] = 0u;
 }
 
+// At the end of a run: if the halt key belongs to this run and has not been
+// captured yet, record the offending energy while U / uvengy0 still hold it
+// (cap[0] = captured key, cap[1] = the energy's bits).
+__global__ void k_halt_capture(unsigned int seq, int NI, int nmol, const unsigned long long *__restrict__ errkey,
+                               const float *__restrict__ U, const float *__restrict__ u0,
+                               unsigned long long *__restrict__ cap)
+{
+    const unsigned long long key = *errkey;
+    if (key == ~0ull || (unsigned int)(key >> 42) != (seq & 0x3fffffu) || cap[0] == key) return;
+    const int n = (int)((key >> 21) & 0x1fffffu), jp1 = (int)(key & 0x1fffffu);
+    const float v = jp1 == 0 ? u0[n] : U[((size_t)(n / NI) * nmol + (jp1 - 1)) * NI + (n % NI)];
+    cap[0] = key;
+    cap[1] = (unsigned long long)__float_as_uint(v);
+}
+
 // ---------------------------------------------------------------------------
 // Weights (update_histogram engproc.F90:883-915 + volcorrect :1022-1038),
 // uvengy0 (get_uv_energy_refs :828-855) and the scalar accumulators, for
@@ -1184,6 +1206,7 @@ __global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
 // ---------------------------------------------------------------------------
 struct WeightArgs {
     int nchunk, n_off, H, slttype, wgtslf, estype, selfcal, esmax, rigid, numslv, nslice;
+    unsigned int seq;
     float w_sys, volume, screen, qtot;
     double temp;
 };
@@ -1289,7 +1312,7 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
                     if ((double)u > escrd[mid]) lo = mid + 1;
                     else hi = mid;
                 }
-                if (lo <= 0) atomicMin(errkey, (unsigned long long)n * (unsigned long long)(nmol + 1));
+                if (lo <= 0) atomicMin(errkey, halt_key(wa.seq, n, 0));
                 eself[max(lo, 1) - 1] += wout[n];
             }
         }

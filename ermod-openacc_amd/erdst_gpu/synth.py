@@ -317,7 +317,10 @@ class MultiSoluteWorkload:
         return np.diag(self.box).astype(np.float32)
 
     def topo_for(self, k: int) -> Topology:
-        return build_topology(list(self.solvent) + [self.solutes[k]], solute_index=len(self.solvent), refs=True)
+        """Topology of solute k's run; the LJ type table covers every solute,
+        so all runs (and one multi-solute context) share it."""
+        return build_topology(list(self.solvent) + [self.solutes[k]], solute_index=len(self.solvent), refs=True,
+                              extra_lj_species=self.solutes)
 
     def frame(self, f: int) -> np.ndarray:
         return self._frame_fn(f)
@@ -386,7 +389,7 @@ def config4(nins: int = 500, nframes: int = 50) -> Workload:
     U[3, 4] A) flanked by 16 667 rigid waters; a flexible 150-site solute
     (each insertion a jittered conformation) inserted at 500 positions per
     frame inside the slab; PME 128 x 128 x 108 order 4; 600 bins per species
-    (500 linear from -60 kcal/mol, 100 log-spaced up to 1e4)."""
+    (500 linear from -150 kcal/mol, 100 log-spaced up to 1e4)."""
     seed = 20261202 + 4
     rng0 = np.random.Generator(np.random.Philox(key=seed))
     q = rng0.normal(0.0, 0.2, 50)
@@ -420,8 +423,8 @@ def config4(nins: int = 500, nframes: int = 50) -> Workload:
         x = centres[:, None, :] + np.einsum("nij,nsj->nsi", R, conf)
         return x.astype(np.float32)
 
-    # chain-solute energies reach below -30 kcal/mol in the slab
-    mesh = linear_log_mesh(-60.0, 5.0, 0.13, 1.0e4, 100)
+    # chain-solute energies reach -61 kcal/mol within 5 frames in the slab
+    mesh = linear_log_mesh(-150.0, 5.0, 0.31, 1.0e4, 100)
     w = Workload("config4", topo, box, nframes=nframes, nins=nins, params=dict(ms=(128, 128, 108)),
                  mesh=[mesh, mesh], seed=seed, slttype=3)
     w._frame_fn, w._ins_fn = frame, ins

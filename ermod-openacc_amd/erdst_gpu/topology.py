@@ -69,27 +69,38 @@ class Topology:
 
 
 def build_topology(species: list[Species], solute_index: int, refs: bool,
-                   ljformat: int = LJFMT_EPS_CAL_SGM_A, cmbrule: int = LJCMB_ARITH) -> Topology:
+                   ljformat: int = LJFMT_EPS_CAL_SGM_A, cmbrule: int = LJCMB_ARITH,
+                   extra_lj_species: list[Species] | None = None) -> Topology:
     """setparam for species listed in MDinfo order.  For refs runs the solute
-    (test particle) must be the last species (setconf.F90:598-600)."""
+    (test particle) must be the last species (setconf.F90:598-600).
+
+    extra_lj_species: further species whose LJ types are appended to the type
+    table after those of `species` (not part of the system), so that several
+    test particles analysed in one context share one table."""
     if refs and solute_index != len(species) - 1:
         raise ValueError("the test particle must be the last species")
     t = Topology(species=species, solute_index=solute_index, refs=refs)
     # LJ type table in order of first appearance (setconf.F90:756-777)
+    # (with extra species: the system's solvent species, then the extras in
+    # their order, so the table does not depend on which extra is the solute)
     table: list[tuple[float, float]] = []
-    site_types: list[np.ndarray] = []
-    for sp in species:
+
+    def site_keys(sp):
         sig = np.asarray(sp.sigma, np.float32).copy()
         eps = np.asarray(sp.eps, np.float32)
         if ljformat == LJFMT_EPS_RMINH:
             sig = (np.float32(SGMCNV) * sig).astype(np.float32)
-        types = []
-        for s, e in zip(sig.tolist(), eps.tolist()):
-            key = (s, e)
+        return list(zip(sig.tolist(), eps.tolist()))
+
+    order = list(species)
+    if extra_lj_species:
+        order = [sp for i, sp in enumerate(species) if not (refs and i == solute_index)] + list(extra_lj_species)
+        order += [species[solute_index]] if refs else []
+    for sp in order:
+        for key in site_keys(sp):
             if key not in table:
                 table.append(key)
-            types.append(table.index(key))
-        site_types.append(np.array(types, np.int32))
+    site_types = [np.array([table.index(key) for key in site_keys(sp)], np.int32) for sp in species]
     ntype = len(table)
     sig = np.array([k[0] for k in table], np.float32)
     eps = np.array([k[1] for k in table], np.float32)

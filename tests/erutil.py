@@ -15,16 +15,16 @@ from erdst_gpu.params import hot_path_params
 # ~200-term PME gather and of sums whose terms cancel.
 RTOL = 1e-5
 ATOL = 2e-4
-# Above E_MID the energy is the LJ wall of sites in near contact: the FP32
-# distance of two sites at coordinates ~|x| carries an absolute error
-# ~eps32*|x| (5e-6 A at |x| = 80 A), i.e. a relative error eps32*|x|/r
-# (3e-6 at r = 1.6 A, where one LJ pair already gives 2000 kcal/mol), and
-# r^-12 multiplies it by 12: about 4e-5 in an 80 A box.  RTOL_MID = 1e-4
-# covers that; such energies lie in the log-spaced part of every mesh here,
-# whose bins are several percent wide.  Above E_HIGH (the top of every mesh
-# used here, so all such energies share the open-ended last bin) overlaps
-# are closer still and the bar is RTOL_HIGH.
-E_MID = 1.0e2
+# Above E_MID the energy is the LJ wall of sites inside their contact
+# distance: the FP32 distance of two sites at coordinates ~|x| carries an
+# absolute error ~eps32*|x| (5e-6 A at |x| = 80 A), i.e. a relative error
+# eps32*|x|/r (2.5e-6 at r = 2 A, where one LJ pair gives ~100 kcal/mol), and
+# r^-12 multiplies it by 12: 3e-5 in an 80 A box, measured 2.9e-5 at
+# 93 kcal/mol.  RTOL_MID = 1e-4 covers that; such energies lie in the
+# log-spaced part of every mesh here, whose bins are several percent wide.
+# Above E_HIGH (the top of every mesh used here, so all such energies share
+# the open-ended last bin) overlaps are closer still and the bar is RTOL_HIGH.
+E_MID = 20.0
 RTOL_MID = 1e-4
 E_HIGH = 1.0e4
 RTOL_HIGH = 1e-3

@@ -208,3 +208,42 @@ def test_eng_min_halt_is_reported(cfg0):
     assert ei.value.status == capi.ER_E_HALT
     code, species, energy = ctx.last_error()
     assert code == "eng_min" and species == 1
+
+
+def test_halt_reports_the_first_offending_energy_across_runs(cfg0):
+    """Several frames run before er_sync: the halt reported is the earliest
+    (frame, insertion, molecule) below the mesh, with its own energy."""
+    p = U.workload_params(cfg0)
+    uvmax, uvsoft, uvcrd = cfg0.uvmax_uvsoft_uvcrd()
+    ref = _ctx(cfg0, p)
+    u0_frames = []
+    for f in (0, 1, 2):
+        u, _ = _run_refs(ref, cfg0, f, capi.ER_BATCHED, nins=30)
+        u0_frames.append(u)
+    # lowest mesh point = the largest of the frames' minimum energies, so that
+    # every frame has at least one energy at or below it
+    t = max(float(u.min()) for u in u0_frames)
+    shifted = uvcrd + (t - uvcrd[0])
+    shifted[0] = t
+    first = None
+    for f, u in enumerate(u0_frames):
+        bad = np.argwhere(u.astype(np.float64) <= shifted[0])
+        if bad.size:
+            n, j = bad[0]                          # row-major: smallest insertion, then molecule
+            first = (f, int(n), int(j), float(u[n, j]))
+            break
+    assert first is not None
+    ctx = capi.Context(cfg0.topo, p)
+    ctx.set_mesh(uvmax, uvsoft, shifted)
+    ctx.set_solute(cfg0.topo.slt_ljtype, cfg0.topo.slt_charge32)
+    for f in (0, 1, 2):
+        ctx.set_frame(cfg0.frame(f), cfg0.cell)
+        ctx.set_insertions(cfg0.insertions(f)[:30])
+        ctx.run(capi.ER_BATCHED)
+    with pytest.raises(capi.ErError) as ei:
+        ctx.sync()
+    assert ei.value.status == capi.ER_E_HALT
+    code, species, energy = ctx.last_error()
+    print("first offending (frame, insertion, molecule, energy):", first, "reported", code, species, energy)
+    assert code == "eng_min" and species == 1
+    assert np.float32(energy) == np.float32(first[3])

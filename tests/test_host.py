@@ -136,3 +136,19 @@ def test_fortran_host_binary_loads():
     host = os.path.join(os.path.dirname(capi.LIB_PATH), "erdst_host")
     r = subprocess.run([host], capture_output=True, text=True, timeout=60)
     assert "usage: erdst_host" in (r.stdout + r.stderr)
+
+
+def test_multi_solute_topologies_share_one_lj_table():
+    from erdst_gpu import synth
+    w3 = synth.config3()
+    topos = [w3.topo_for(k) for k in (0, 5, 6, 15)]
+    for t in topos[1:]:
+        assert t.ntype == topos[0].ntype
+        np.testing.assert_array_equal(t.ljene32, topos[0].ljene32)
+        np.testing.assert_array_equal(t.ljlensq32, topos[0].ljlensq32)
+        np.testing.assert_array_equal(t.ljtype, topos[0].ljtype)
+    # each solute's own sites keep their own (sigma, epsilon)
+    for k, t in zip((0, 5, 6, 15), topos):
+        sp = w3.solutes[k]
+        sig = np.sqrt(t.ljlensq32.reshape(t.ntype, t.ntype)[t.slt_ljtype, t.slt_ljtype])
+        np.testing.assert_allclose(sig, sp.sigma.astype(np.float32), rtol=1e-6)
