@@ -147,7 +147,7 @@ struct er_ctx {
 
     // atoms sorted by PME brick (per frame) for the brick gather
     int nbx = 1, nby = 1, nbz = 1, nbrick = 1;
-    DBuf<int> d_aperm;
+    DBuf<int> d_aperm, d_border;  // atoms by brick; bricks in Morton order
     DBuf<unsigned int> d_akeyrank, d_bcnt, d_bstart;
     DBuf<float> d_Eat;
 
@@ -401,6 +401,26 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     c->nby = (c->K2 + ER_BRICK - 1) / ER_BRICK;
     c->nbz = (c->K3 + brick_z(c->ord) - 1) / brick_z(c->ord);
     c->nbrick = c->nbx * c->nby * c->nbz;
+    {
+        // bricks ranked along a Morton curve (bit-interleaved bx, by, bz)
+        auto spread = [](unsigned v) {
+            unsigned r = 0;
+            for (int bit = 0; bit < 10; ++bit) r |= ((v >> bit) & 1u) << (3 * bit);
+            return r;
+        };
+        std::vector<std::pair<unsigned, int>> key(c->nbrick);
+        for (int z = 0; z < c->nbz; ++z)
+            for (int y = 0; y < c->nby; ++y)
+                for (int x = 0; x < c->nbx; ++x) {
+                    const int id = (z * c->nby + y) * c->nbx + x;
+                    key[id] = {spread(x) | (spread(y) << 1) | (spread(z) << 2), id};
+                }
+        std::sort(key.begin(), key.end());
+        std::vector<int> order(c->nbrick);
+        for (int k = 0; k < c->nbrick; ++k) order[k] = key[k].second;
+        HIPCHK(c, c->d_border.ensure(c->nbrick));
+        HIPCHK(c, hipMemcpy(c->d_border.p, order.data(), c->nbrick * sizeof(int), hipMemcpyHostToDevice));
+    }
     if (const char *e = getenv("ER_CHUNK")) c->chunk = std::max(NI_B, atoi(e) / NI_B * NI_B);
 
     c->molbeg.resize(c->nummol + 1);
@@ -845,15 +865,21 @@ template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off
                        (K2 + K3) * sizeof(float2) + (NT / 64) * sizeof(double);
     static bool attr = false;
     if (!attr && lds > 65536) {
-        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT>,
+        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT, false>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        (void)hipFuncSetAttribute((const void *)k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT, true>,
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
     // many sites: scatter + forward plane DFT beats nslt * K2 * K3 products
-    const int sparse = c->S().nslt > ER_SPARSE_SITES ? 1 : 0;
-    hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT>), dim3(c->H, nch), dim3(NT), lds, c->stream,
-                       c->S().nslt, c->ord, c->K1, n_off, sparse, c->d_Ftab.p, c->d_insw.p, c->d_insg.p,
-                       c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
+    if (c->S().nslt > ER_SPARSE_SITES)
+        hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT, true>), dim3(c->H, nch), dim3(NT), lds,
+                           c->stream, c->S().nslt, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_insw.p, c->d_insg.p,
+                           c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
+    else
+        hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT, false>), dim3(c->H, nch), dim3(NT), lds,
+                           c->stream, c->S().nslt, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_insw.p, c->d_insg.p,
+                           c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
     return hipGetLastError();
 }
 
@@ -942,7 +968,9 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
 {
     constexpr int NXCD = 8;
     constexpr int RS = ER_BRICK + ORD - 1, RZ = brick_z(ORD) + ORD - 1;
-    const size_t lds = (size_t)RS * RS * RZ * NI * sizeof(float);
+    // the region, padded to whole rounds of 512 16-byte units (the staging's tail)
+    constexpr int NF4 = RS * RS * RZ * (NI / 4), PER = (NF4 + 511) / 512;
+    const size_t lds = (size_t)PER * 512 * 16;
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_gather_brick<ORD, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -950,10 +978,10 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
         attr = true;
     }
     const int ntile = (nch + NI - 1) / NI;
-    const int nbpad = (c->nbrick + NXCD - 1) / NXCD * NXCD;
-    hipLaunchKernelGGL((k_gather_brick<ORD, NI>), dim3(nbpad * ntile), dim3(512), lds, c->stream, c->K1, c->K2, c->K3,
-                       c->nbx, c->nby, nbpad, NXCD, nch, c->d_bstart.p, c->d_aperm.p, c->d_posn.p, c->d_splw.p,
-                       c->d_splg.p, c->d_phi.p, c->numatm, c->d_Eat.p);
+    const size_t nwg = (size_t)NXCD * ((c->nbrick + NXCD - 1) / NXCD) * ntile;
+    hipLaunchKernelGGL((k_gather_brick<ORD, NI>), dim3((unsigned)nwg), dim3(512), lds, c->stream, c->K1, c->K2, c->K3,
+                       c->nbx, c->nby, c->nbrick, c->d_border.p, nch, c->d_bstart.p, c->d_aperm.p, c->d_posn.p,
+                       c->d_splw.p, c->d_splg.p, c->d_phi.p, c->numatm, c->d_Eat.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
@@ -1418,7 +1446,7 @@ extern "C" void er_destroy(er_ctx *c)
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_Ftab.release(); c->d_T.release(); c->d_phi.release();
-    c->d_C.release(); c->d_w.release(); c->d_flceng.release(); c->d_part.release(); c->d_epart.release();
+    c->d_C.release(); c->d_w.release(); c->d_border.release(); c->d_flceng.release(); c->d_part.release(); c->d_epart.release();
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }

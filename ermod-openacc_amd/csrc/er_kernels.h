@@ -397,8 +397,8 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
 //      of recpcal.F90:446-478, done per plane).
 //   2. FP64 half-plane self-energy partial and the engfac multiply.
 //   3. inverse DFT along axes 3 and 2; T[nl][m1][x3][x2] for stage B.
-template <int K2, int K3, int R1, int R2, int C1, int C2, int NT>
-__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off, int sparse,
+template <int K2, int K3, int R1, int R2, int C1, int C2, int NT, bool SPARSE>
+__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
                                                     const float2 *__restrict__ Ftab,
                                                     const float *__restrict__ insw, const int4 *__restrict__ insg,
                                                     const float *__restrict__ engfac,
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
     for (int k = tid; k < K3; k += NT) t3[k] = tw3[k];
     const int per = H + K2 + K3;
 
-    if (!sparse) {
+    if constexpr (!SPARSE) {
         // ---- separable generation, sites in LDS chunks ----
         float2 acc[EPT];
 #pragma unroll
@@ -767,8 +767,9 @@ __device__ __forceinline__ int wrap1(int x, int K)
 // and a wave covers 16 atoms.  Per insertion the arithmetic order is that of
 // k_gather (x-sum with w1, then (w3*w2) * sum), so results are identical.
 template <int ORD, int NI>
-__global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick_pad,
-                                                      int nxcd, int nchunk, const unsigned int *__restrict__ bstart,
+__global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick,
+                                                      const int *__restrict__ border, int nchunk,
+                                                      const unsigned int *__restrict__ bstart,
                                                       const int *__restrict__ aperm,
                                                       const float4 *__restrict__ posn,
                                                       const float *__restrict__ splw,
@@ -785,17 +786,20 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     constexpr int NT = 512;
     constexpr int PER = (NF4 + NT - 1) / NT;
     extern __shared__ __align__(16) float4 reg4[];  // [RZ][RS][RS][NV]
-    // XCD-aware remap: each XCD sweeps a contiguous range of bricks, tile by tile
+    // XCD-aware remap: workgroup b runs on XCD b % 8.  The bricks are ranked
+    // along a Morton curve (border) and each XCD owns an equal contiguous
+    // range of ranks, swept tile by tile: a compact block of bricks whose
+    // lower neighbours (the data a region's halo holds) were staged recently
+    // on the same XCD and are still in its L2.
     const int b = blockIdx.x;
-    const int per = nbrick_pad / nxcd;
-    const int xcd = b % nxcd, k = b / nxcd;
-    const int brick = xcd * per + (k % per);
-    const int tile = k / per;
-    const int nbz = (K3 + BZ - 1) / BZ;
-    if (brick >= nbx * nby * nbz || tile * NI >= nchunk) return;
+    const int per = (nbrick + 7) / 8;
+    const int xcd = b & 7, k = b >> 3;
+    const int rank = xcd * per + k % per, tile = k / per;
+    if (rank >= nbrick || tile * NI >= nchunk) return;
+    const int brick = border[rank];
+    const int bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
     const unsigned int a0 = bstart[brick], a1 = bstart[brick + 1];
     if (a0 == a1) return;
-    const int bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
     const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * BZ - (ORD - 1);
     const size_t G = (size_t)K1 * K2 * K3;
     const float4 *ph4 = reinterpret_cast<const float4 *>(phi + (size_t)tile * G * NI);
@@ -848,20 +852,21 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     AtomIn first;
     const unsigned int pos0 = a0 + s;
     if (pos0 < a1) load_atom(pos0, first);
-    float4 v[PER];
+    // staging straight into LDS (global_load_lds_dwordx4): lane l of a wave
+    // writes its 16 bytes at the wave's base + 16 l, so unit f lands at
+    // reg4[f]; the tail lanes of the last round reload the last unit (in
+    // bounds) into the padding behind the region (the LDS allocation is
+    // PER * NT units)
+    const int wbase = tid & ~63;
 #pragma unroll
     for (int r = 0; r < PER; ++r) {
-        // the tail lanes of the last round reload the last unit (in bounds) and skip the store
         const int f = min(tid + r * NT, NF4 - 1);
         const int jj = f % NV, p = f / NV;
         const int rx = p % RS, ry = (p / RS) % RS, rz = p / (RS * RS);
         const int gx = wrap1(ox + rx, K1), gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
-        v[r] = ph4[(((size_t)gz * K2 + gy) * K1 + gx) * NV + jj];
-    }
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int f = tid + r * NT;
-        if (f < NF4) reg4[f] = v[r];
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void *)(ph4 + (((size_t)gz * K2 + gy) * K1 + gx) * NV + jj),
+            (__attribute__((address_space(3))) void *)(reg4 + r * NT + wbase), 16, 0, 0);
     }
     __syncthreads();
     if (pos0 < a1) compute_atom(first);
