@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -913,6 +914,25 @@ static int divisor_le(int K, int cap)
 
 template <int K1, int NI> static hipError_t launch_axis1_k(er_ctx *c, int nch)
 {
+    if constexpr (NI == 16) {
+        // transposed-store form: 8 (or 4) lines per workgroup
+        const int ntile = (nch + NI - 1) / NI;
+        auto go = [&](auto rl) -> hipError_t {
+            constexpr int RL = decltype(rl)::value;
+            const size_t lds = (size_t)RL * (K1 * 16 + 8) * sizeof(float);
+            static bool attr = false;
+            if (!attr && lds > 65536) {
+                (void)hipFuncSetAttribute((const void *)k_pme_axis1_t<K1, RL>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+                attr = true;
+            }
+            hipLaunchKernelGGL((k_pme_axis1_t<K1, RL>), dim3(c->K2 / RL, c->K3, ntile), dim3(RL * 16), lds,
+                               c->stream, nch, c->K2, c->K3, c->d_T.p, c->d_phi.p, c->d_tw1.p, c->d_twm.p);
+            return hipGetLastError();
+        };
+        if (c->K2 % 8 == 0) return go(std::integral_constant<int, 8>{});
+        if (c->K2 % 4 == 0) return go(std::integral_constant<int, 4>{});
+    }
     const int R = divisor_le(c->K2, 256 / NI);  // x2 lines per block
     const int nt = ((R * NI + 63) / 64) * 64;
     const int ntile = (nch + NI - 1) / NI;

@@ -567,6 +567,60 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
     }
 }
 
+// Batched form of stage B: RL lines x 16 insertions per workgroup.  Lanes
+// run along x2 first, so a wave reads RL * 8-byte runs of T; the lines are
+// transposed through LDS ([line][x1][16], rows padded by 8 floats: the
+// transposed writes hit distinct banks) and leave as one contiguous
+// RL * K1 * 16-float block of phi written with 16-byte stores.  Per-line
+// arithmetic as in k_pme_axis1.
+template <int K1, int RL>
+__global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int K3,
+                                                          const float2 *__restrict__ T, float *__restrict__ phi,
+                                                          const float2 *__restrict__ tw1,
+                                                          const float2 *__restrict__ twm)
+{
+    constexpr int NI = 16, M = K1 / 2, H = M + 1, LS = K1 * NI + 8;
+    extern __shared__ __align__(16) float sml[];  // [RL][LS]
+    const int tile = blockIdx.z, x3 = blockIdx.y, x2_0 = blockIdx.x * RL;
+    const int tid = threadIdx.x;
+    const int r = tid % RL, i = tid / RL;
+    const int nl = tile * NI + i;
+    const size_t plane = (size_t)K2 * K3;
+    float2 z[M];
+    if (nl < nchunk) {
+        const float2 *src = T + (size_t)nl * H * plane + (size_t)x3 * K2 + x2_0 + r;
+        float2 X[H];
+#pragma unroll
+        for (int m = 0; m < H; ++m) X[m] = src[(size_t)m * plane];
+#pragma unroll
+        for (int k = 0; k < M; ++k) {
+            const float2 a = X[k], b = X[M - k];
+            const float2 e = make_float2(a.x + b.x, a.y - b.y);            // X[k] + conj X[M-k]
+            const float2 d = make_float2(a.x - b.x, a.y + b.y);            // X[k] - conj X[M-k]
+            const float2 o = (k == 0) ? d : erfft::cmul(d, tw1[k]);
+            z[k] = make_float2(e.x - o.y, e.y + o.x);                     // e + i o
+        }
+        erfft::fft<M, +1>(z, twm);
+    } else {
+#pragma unroll
+        for (int k = 0; k < M; ++k) z[k] = make_float2(0.f, 0.f);
+    }
+    float *o = sml + r * LS + i;
+#pragma unroll
+    for (int j = 0; j < M; ++j) {
+        o[(2 * j) * NI] = z[j].x;
+        o[(2 * j + 1) * NI] = z[j].y;
+    }
+    __syncthreads();
+    const size_t G = plane * K1;
+    float4 *dst = reinterpret_cast<float4 *>(phi + (size_t)tile * G * NI + ((size_t)x3 * K2 + x2_0) * K1 * NI);
+    constexpr int PERLINE = K1 * NI / 4;
+    for (int f = tid; f < RL * PERLINE; f += RL * 16) {
+        const int line = f / PERLINE, rem = f % PERLINE;
+        dst[f] = *reinterpret_cast<const float4 *>(sml + line * LS + rem * 4);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Spatial order of the molecules (per frame): counting sort of the molecule
 // reference sites into NC^3 cells keyed by the Morton code of the cell, so
