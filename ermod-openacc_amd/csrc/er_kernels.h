@@ -307,17 +307,32 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
                                           const float2 *__restrict__ tws, int tid)
 {
     constexpr int PS = K3 + 1;
-    // ---- rows (axis 3), pass A in place ----
-    for (int it = tid; it < K2 * R2; it += NT) {
-        const int m2 = it / R2, n2 = it % R2;
-        float2 v[R1];
+    // ---- rows (axis 3), pass A in place: each thread's items are loaded
+    // together, then transformed, then stored (their LDS latencies overlap) ----
+    {
+        constexpr int IA = (K2 * R2 + NT - 1) / NT;
+        float2 v[IA][R1];
 #pragma unroll
-        for (int a = 0; a < R1; ++a) v[a] = plane[m2 * PS + R2 * a + n2];
-        erfft::fft<R1, SIGN>(v, tws + tws_off(R1));
+        for (int j = 0; j < IA; ++j) {
+            const int it = tid + j * NT;
+            if (it < K2 * R2) {
+                const int m2 = it / R2, n2 = it % R2;
 #pragma unroll
-        for (int k1 = 1; k1 < R1; ++k1) v[k1] = erfft::twmul<SIGN>(v[k1], t3[(n2 * k1) % K3]);
+                for (int a = 0; a < R1; ++a) v[j][a] = plane[m2 * PS + R2 * a + n2];
+            }
+        }
 #pragma unroll
-        for (int k1 = 0; k1 < R1; ++k1) plane[m2 * PS + R2 * k1 + n2] = v[k1];
+        for (int j = 0; j < IA; ++j) {
+            const int it = tid + j * NT;
+            if (it < K2 * R2) {
+                const int m2 = it / R2, n2 = it % R2;
+                erfft::fft<R1, SIGN>(v[j], tws + tws_off(R1));
+#pragma unroll
+                for (int k1 = 1; k1 < R1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t3[(n2 * k1) % K3]);
+#pragma unroll
+                for (int k1 = 0; k1 < R1; ++k1) plane[m2 * PS + R2 * k1 + n2] = v[j][k1];
+            }
+        }
     }
     __syncthreads();
     // ---- rows, pass B (outputs held until every thread has read) ----
@@ -347,16 +362,30 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
     }
     __syncthreads();
     // ---- columns (axis 2), pass A in place ----
-    for (int it = tid; it < C2 * K3; it += NT) {
-        const int n2 = it / K3, x3 = it % K3;
-        float2 v[C1];
+    {
+        constexpr int IA = (C2 * K3 + NT - 1) / NT;
+        float2 v[IA][C1];
 #pragma unroll
-        for (int a = 0; a < C1; ++a) v[a] = plane[(C2 * a + n2) * PS + x3];
-        erfft::fft<C1, SIGN>(v, tws + tws_off(C1));
+        for (int j = 0; j < IA; ++j) {
+            const int it = tid + j * NT;
+            if (it < C2 * K3) {
+                const int n2 = it / K3, x3 = it % K3;
 #pragma unroll
-        for (int k1 = 1; k1 < C1; ++k1) v[k1] = erfft::twmul<SIGN>(v[k1], t2[(n2 * k1) % K2]);
+                for (int a = 0; a < C1; ++a) v[j][a] = plane[(C2 * a + n2) * PS + x3];
+            }
+        }
 #pragma unroll
-        for (int k1 = 0; k1 < C1; ++k1) plane[(C2 * k1 + n2) * PS + x3] = v[k1];
+        for (int j = 0; j < IA; ++j) {
+            const int it = tid + j * NT;
+            if (it < C2 * K3) {
+                const int n2 = it / K3, x3 = it % K3;
+                erfft::fft<C1, SIGN>(v[j], tws + tws_off(C1));
+#pragma unroll
+                for (int k1 = 1; k1 < C1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t2[(n2 * k1) % K2]);
+#pragma unroll
+                for (int k1 = 0; k1 < C1; ++k1) plane[(C2 * k1 + n2) * PS + x3] = v[j][k1];
+            }
+        }
     }
     __syncthreads();
     // ---- columns, pass B ----
@@ -767,7 +796,8 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
                 ea += (w3[c3] * w2[c2]) * s;
             }
         }
-        acc += q * ea;
+        // the rounded per-atom product, then the sum (as the batched path's Eat + k_mol_reduce)
+        acc += __fmul_rn(q, ea);
     }
     const int n = n_off + nl;
     float qs = qslt_refs;
@@ -819,7 +849,8 @@ __device__ __forceinline__ int wrap1(int x, int K)
 // four lanes per atom, each lane owning four of the NI = 16 insertions
 // (one float4 of a grid point), so every stencil read is one ds_read_b128
 // and a wave covers 16 atoms.  Per insertion the arithmetic order is that of
-// k_gather (x-sum with w1, then (w3*w2) * sum), so results are identical.
+// k_gather (x-sum with w1, then (w3*w2) * sum); the two paths still differ
+// in FP32 rounding (contraction choices), not in the terms summed.
 template <int ORD, int NI>
 __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick,
                                                       const int *__restrict__ border, int nchunk,

@@ -163,3 +163,46 @@ def gpu_dmu_blocks(soln, refs, frames_per_block, nblocks_soln, nins, mode=None):
     rblocks = [ctx.fetch()]
     ctx.close()
     return sblocks, rblocks
+
+
+def plain_and_batched(w, params, f, nins, capi):
+    """Run frame f's first nins insertions in both modes, each in a fresh
+    context.  Returns [(uvengy, uvengy0, fetch, oracle Distribution of that
+    mode's own energies)] for (batched, plain)."""
+    out = []
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx = capi.Context(w.topo, params)
+        ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+        ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+        ctx.set_frame(w.frame(f), w.cell)
+        ctx.set_insertions(w.insertions(f)[:nins])
+        ctx.run(mode)
+        ctx.sync()
+        u, u0 = ctx.uvengy(nins)
+        g = ctx.fetch()
+        ctx.close()
+        d = oracle_distribution(w, params)
+        for n in range(nins):
+            d.update(w.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+        out.append((u, u0, g, d))
+    return out
+
+
+def assert_plain_matches_batched(res):
+    """The plain path (one insertion per launch sequence) and the batched
+    path add the same terms in different orders (per-atom products stored
+    before the molecule sum, different FMA contraction), so their FP32
+    energies agree to rounding; each path's distributions
+    are exact for its own energies, so the two differ only where an energy
+    sits within that rounding of a bin edge."""
+    (ub, u0b, gb, db), (up, u0p, gp, dp) = res
+    # each is held to the energy bar against the FP64 oracle, so they are held
+    # to it against each other (sums over 50-site molecules cancel to ~1e-2
+    # from terms ~1e1, so the agreement is absolute there, not relative)
+    assert energy_mismatch(ub, up)[0] == 0
+    assert energy_mismatch(u0b, u0p)[0] == 0
+    for g, d in ((gb, db), (gp, dp)):
+        np.testing.assert_allclose(g["edens"] / g["engnorm"], d.edens / d.d.engnorm, rtol=1e-12, atol=0)
+        shape = g["ecorr"].shape
+        np.testing.assert_allclose(g["ecorr"] / g["engnorm"], d.ecorr.reshape(shape) / d.d.engnorm,
+                                   rtol=1e-12, atol=0)

@@ -100,17 +100,11 @@ def test_config4_slab_energies(cfg4):
 def test_config4_distribution_plain_equals_batched(cfg4):
     w = cfg4
     p = U.workload_params(w)
-    res = []
-    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
-        ctx = _ctx(w, p)
-        ctx.set_frame(w.frame(1), w.cell)
-        ctx.set_insertions(w.insertions(1)[:20])
-        ctx.run(mode)
-        res.append(ctx.fetch())
-    np.testing.assert_allclose(res[0]["edens"], res[1]["edens"], rtol=1e-12)
-    np.testing.assert_allclose(res[0]["ecorr"], res[1]["ecorr"], rtol=1e-12)
+    res = U.plain_and_batched(w, p, 1, 20, capi)
+    U.assert_plain_matches_batched(res)
     nm = np.bincount(w.topo.uvspec, minlength=3)[1:]
-    en = res[0]["edens"] / res[0]["engnorm"]
+    g = res[0][2]
+    en = g["edens"] / g["engnorm"]
     np.testing.assert_allclose([en[:600].sum(), en[600:].sum()], nm, rtol=1e-12)
 
 

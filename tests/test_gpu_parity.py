@@ -143,13 +143,7 @@ def test_histograms_cfg0(cfg0):
 
 def test_plain_histograms_equal_batched(cfg0):
     p = U.workload_params(cfg0)
-    res = []
-    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
-        ctx = _ctx(cfg0, p)
-        _run_refs(ctx, cfg0, 3, mode, nins=50)
-        res.append(ctx.fetch())
-    np.testing.assert_allclose(res[0]["edens"], res[1]["edens"], rtol=1e-12)
-    np.testing.assert_allclose(res[0]["ecorr"], res[1]["ecorr"], rtol=1e-12)
+    U.assert_plain_matches_batched(U.plain_and_batched(cfg0, p, 3, 50, capi))
 
 
 @pytest.mark.parametrize("ljswitch", [1, 2, 3])
