@@ -126,6 +126,22 @@ def algorithmic(kernel, job, nch):
     return None, None
 
 
+def pmc_traffic(kernel, workload, nch):
+    """HBM bytes of one launch of `kernel` over nch insertions, from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (each counter scaled by what it reads for a device copy of known size),
+    or None when no such summary exists."""
+    path = os.path.join(ROOT, "profiles", f"hbm_traffic_config{workload}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    k = rec["kernels"].get(kernel)
+    if k is None:
+        return None
+    return k["traffic_bytes"] * nch / rec["units_per_launch"]
+
+
 def cpu_baseline_dmu(soln, refs, frames_per_block=5, nblocks=2, nins=100, nthreads=None):
     """The oracle (FP64 restatement of the reference algorithm: brute-force
     pairs, one PME grid per insertion, binary-search histogram, outer-product
@@ -279,6 +295,7 @@ def main():
     else:
         achieved, peak, unit = amount / avg_s / 1e12, FP64_MFMA_PEAK_TFS, "TFLOP/s"
     kernels = {k: dict(ms_per_step=v[0] / K, launches=v[1]) for k, v in busy.items()}
+    traffic = pmc_traffic(dom, args.workload, nch)
 
     out = {
         "metric": METRIC,
@@ -303,7 +320,9 @@ def main():
         },
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": bound, "achieved": round(achieved, 2), "peak": peak,
-                     "unit": unit, "frac": round(achieved / peak, 4), "traffic": None,
+                     "unit": unit, "frac": round(achieved / peak, 4),
+                     "traffic": None if traffic is None else round(traffic),
+                     "algorithmic_bytes" if bound == "hbm" else "algorithmic_flops": round(amount),
                      "launch_ms": round(avg_s * 1e3, 5), "units_per_launch": round(nch, 2)},
         "cpu_baseline": None,
         "dmu": None,
