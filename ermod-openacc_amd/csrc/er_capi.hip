@@ -21,9 +21,30 @@ constexpr int NI_B = 16;  // insertion interleave of the batched path
 constexpr int ER_KSLICES = 16;  // insertion slices of the split-K edens/ecorr sums
 constexpr int ER_SPARSE_SITES = 12;  // above this many sites the plane kernel spreads + transforms
 
+// device buffer owned by one context (freed with it); movable, not copyable
 template <typename T> struct DBuf {
     T *p = nullptr;
     size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n)
+    {
+        o.p = nullptr;
+        o.n = 0;
+    }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+        if (this != &o) {
+            release();
+            p = o.p;
+            n = o.n;
+            o.p = nullptr;
+            o.n = 0;
+        }
+        return *this;
+    }
+    ~DBuf() { release(); }
     hipError_t ensure(size_t count)
     {
         if (count <= n && p) return hipSuccess;
@@ -148,7 +169,8 @@ struct er_ctx {
 
     // atoms sorted by PME brick (per frame) for the brick gather
     int nbx = 1, nby = 1, nbz = 1, nbrick = 1;
-    DBuf<int> d_aperm, d_border;  // atoms by brick; bricks in Morton order
+    DBuf<int> d_border;           // bricks in Morton order
+    DBuf<float> d_brec;           // atom records in brick order
     DBuf<unsigned int> d_akeyrank, d_bcnt, d_bstart;
     DBuf<float> d_Eat;
 
@@ -926,7 +948,8 @@ template <int K1, int NI> static hipError_t launch_axis1_k(er_ctx *c, int nch)
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 attr = true;
             }
-            hipLaunchKernelGGL((k_pme_axis1_t<K1, RL>), dim3(c->K2 / RL, c->K3, ntile), dim3(RL * 16), lds,
+            const size_t nblk = (size_t)(c->K2 / RL) * c->K3 * ntile;  // one flat, XCD-remapped range
+            hipLaunchKernelGGL((k_pme_axis1_t<K1, RL>), dim3((unsigned)(8 * ((nblk + 7) / 8))), dim3(RL * 16), lds,
                                c->stream, nch, c->K2, c->K3, c->d_T.p, c->d_phi.p, c->d_tw1.p, c->d_twm.p);
             return hipGetLastError();
         };
@@ -988,9 +1011,7 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
 {
     constexpr int NXCD = 8;
     constexpr int RS = ER_BRICK + ORD - 1, RZ = brick_z(ORD) + ORD - 1;
-    // the region, padded to whole rounds of 512 16-byte units (the staging's tail)
-    constexpr int NF4 = RS * RS * RZ * (NI / 4), PER = (NF4 + 511) / 512;
-    const size_t lds = (size_t)PER * 512 * 16;
+    const size_t lds = (size_t)RS * RS * RZ * (NI / 4) * 16;  // the staged region
     static bool attr = false;
     if (!attr) {
         (void)hipFuncSetAttribute((const void *)k_gather_brick<ORD, NI>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1000,13 +1021,13 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
     const int ntile = (nch + NI - 1) / NI;
     const size_t nwg = (size_t)NXCD * ((c->nbrick + NXCD - 1) / NXCD) * ntile;
     hipLaunchKernelGGL((k_gather_brick<ORD, NI>), dim3((unsigned)nwg), dim3(512), lds, c->stream, c->K1, c->K2, c->K3,
-                       c->nbx, c->nby, c->nbrick, c->d_border.p, nch, c->d_bstart.p, c->d_aperm.p, c->d_posn.p,
-                       c->d_splw.p, c->d_splg.p, c->d_phi.p, c->numatm, c->d_Eat.p);
+                       c->nbx, c->nby, c->nbrick, c->d_border.p, nch, c->d_bstart.p, c->d_brec.p, c->d_phi.p,
+                       c->numatm, c->d_Eat.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
-    const size_t nthr = (size_t)ntile * c->nummol * NI;
-    hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((nthr + 255) / 256)), dim3(256), 0, c->stream, c->nummol,
+    hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + 256 / NI - 1) / (256 / NI)), ntile), dim3(256),
+                       0, c->stream, c->nummol,
                        nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
                        soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt, c->prm.screen, c->fc.volume, c->S().d_U.p);
     return hipGetLastError();
@@ -1064,8 +1085,14 @@ static hipError_t prep_frame(er_ctx *c)
                            c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, brick_z(c->ord), c->d_bcnt.p,
                            c->d_akeyrank.p);
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
-        hipLaunchKernelGGL(k_cell_scatter, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
-                           c->d_akeyrank.p, c->d_bstart.p, c->d_aperm.p);
+        if (c->ord == 4)
+            hipLaunchKernelGGL(k_brick_records<4>, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+                               c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
+                               c->K3, c->nbx, c->nby, c->d_brec.p);
+        else if (c->ord == 6)
+            hipLaunchKernelGGL(k_brick_records<6>, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+                               c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
+                               c->K3, c->nbx, c->nby, c->d_brec.p);
         c->frame_dirty = false;
     }
     return hipGetLastError();
@@ -1094,7 +1121,7 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     HIPCHK(c, c->d_keyrank.ensure(2 * (size_t)c->nummol));
     HIPCHK(c, c->d_ccnt.ensure(c->ncell));
     HIPCHK(c, c->d_cstart.ensure(c->ncell + 1));
-    HIPCHK(c, c->d_aperm.ensure(c->numatm));
+    if (c->ord == 4 || c->ord == 6) HIPCHK(c, c->d_brec.ensure((size_t)c->numatm * brick_rec_floats(c->ord)));
     HIPCHK(c, c->d_akeyrank.ensure(2 * (size_t)c->numatm));
     HIPCHK(c, c->d_bcnt.ensure(c->nbrick));
     HIPCHK(c, c->d_bstart.ensure(c->nbrick + 1));
@@ -1233,12 +1260,14 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
     {
         ProfScope ps(c, ER_K_PREP);
         if (prep_frame_now) HIPCHK(c, prep_frame(c));
-        const int B = 128;
-        hipLaunchKernelGGL(k_prep_ins, dim3((nins + B - 1) / B), dim3(B), 0, c->stream, nins, c->S().nslt,
-                           soln ? (const float *)nullptr : c->S().d_sltpos, c->S().d_sltq.p, c->d_sitepos,
-                           c->d_charge.p, c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1,
-                           c->K2, c->K3, c->ord, c->prm.screen, c->d_insposn.p, c->d_insw.p, c->d_insg.p,
-                           c->d_insref.p, c->d_selfreal.p, c->d_resself.p);
+        const size_t nsite = (size_t)nins * c->S().nslt;
+        hipLaunchKernelGGL(k_prep_sites, dim3((unsigned)((nsite + 255) / 256)), dim3(256), 0, c->stream, nins,
+                           c->S().nslt, soln ? (const float *)nullptr : c->S().d_sltpos, c->S().d_sltq.p,
+                           c->d_sitepos, c->d_charge.p, c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr,
+                           c->fc, c->K1, c->K2, c->K3, c->ord, c->d_insposn.p, c->d_insw.p, c->d_insg.p);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)((nins + 3) / 4)), dim3(256), 0, c->stream, nins, c->S().nslt,
+                           c->fc, c->prm.screen, c->d_insposn.p, c->d_insref.p, c->d_selfreal.p, c->d_resself.p);
         HIPCHK(c, hipGetLastError());
         const size_t nf = (size_t)nins * c->S().nslt * (c->H + c->K2 + c->K3);
         hipLaunchKernelGGL(k_ins_factors, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, c->stream,

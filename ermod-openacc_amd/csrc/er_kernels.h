@@ -146,71 +146,86 @@ __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const floa
     atomicMax(reinterpret_cast<unsigned int *>(rmolmax), __float_as_uint(R));
 }
 
-// Insertion (refs) or solute molecule (soln) sites: wrapped position, spline,
-// reach radius and the real-space self term realcal_self[_refs]
-// (realcal.F90:669-771).  For soln the sites come from the frame.
-__global__ void k_prep_ins(int nins, int nslt, const float *__restrict__ sltpos, const float *__restrict__ sltq,
-                           const float *__restrict__ frame_pos, const float *__restrict__ charge,
-                           const int *__restrict__ molbeg, const int *__restrict__ tag, FrameConst fc,
-                           int K1, int K2, int K3, int ord, float screen, float4 *__restrict__ insposn,
-                           float *__restrict__ insw, int4 *__restrict__ insg, float4 *__restrict__ insref,
-                           float *__restrict__ selfreal, float *__restrict__ resself)
+// Insertion (refs) or solute molecule (soln) sites: wrapped position and
+// spline; k_prep_ins then adds the reach radius and the real-space self term
+// realcal_self[_refs] (realcal.F90:669-771).  For soln the sites come from
+// the frame.
+__global__ void k_prep_sites(int nins, int nslt, const float *__restrict__ sltpos, const float *__restrict__ sltq,
+                             const float *__restrict__ frame_pos, const float *__restrict__ charge,
+                             const int *__restrict__ molbeg, const int *__restrict__ tag, FrameConst fc, int K1,
+                             int K2, int K3, int ord, float4 *__restrict__ insposn, float *__restrict__ insw,
+                             int4 *__restrict__ insg)
 {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= nins) return;
-    float4 c0 = make_float4(0.f, 0.f, 0.f, 0.f);
-    float R = 0.0f;
-    for (int s = 0; s < nslt; ++s) {
-        float x, y, z, q;
-        if (tag) {
-            const size_t a = (size_t)molbeg[tag[n]] + s;
-            x = frame_pos[3 * a]; y = frame_pos[3 * a + 1]; z = frame_pos[3 * a + 2];
-            q = charge[a];
-        } else {
-            const size_t o = ((size_t)n * nslt + s) * 3;
-            x = sltpos[o]; y = sltpos[o + 1]; z = sltpos[o + 2];
-            q = sltq[s];
-        }
-        float w[3 * ER_MAXORD];
-        int4 g;
-        site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
-        const size_t is = (size_t)n * nslt + s;
-        for (int k = 0; k < 3 * ord; ++k) insw[is * 3 * ord + k] = w[k];
-        insg[is] = g;
-        wrap_site(fc, x, y, z);
-        insposn[is] = make_float4(x, y, z, q);
-        if (s == 0) c0 = make_float4(x, y, z, 0.f);
-        else {
-            float dx = x - c0.x, dy = y - c0.y, dz = z - c0.z;
-            min_image(fc, dx, dy, dz);
-            R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
-        }
+    // one thread per (insertion, site): spline weights, grid base, wrapped position
+    const size_t is = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (is >= (size_t)nins * nslt) return;
+    const int n = (int)(is / nslt), s = (int)(is % nslt);
+    float x, y, z, q;
+    if (tag) {
+        const size_t a = (size_t)molbeg[tag[n]] + s;
+        x = frame_pos[3 * a]; y = frame_pos[3 * a + 1]; z = frame_pos[3 * a + 2];
+        q = charge[a];
+    } else {
+        const size_t o = is * 3;
+        x = sltpos[o]; y = sltpos[o + 1]; z = sltpos[o + 2];
+        q = sltq[s];
     }
-    insref[n] = make_float4(c0.x, c0.y, c0.z, R);
+    float w[3 * ER_MAXORD];
+    int4 g;
+    site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
+    for (int k = 0; k < 3 * ord; ++k) insw[is * 3 * ord + k] = w[k];
+    insg[is] = g;
+    wrap_site(fc, x, y, z);
+    insposn[is] = make_float4(x, y, z, q);
+}
+
+// One wave per insertion (after k_prep_sites): reference site and reach
+// radius, real-space self energy and residual self term.
+__global__ void __launch_bounds__(256) k_prep_ins(int nins, int nslt, FrameConst fc, float screen,
+                                                  const float4 *__restrict__ insposn, float4 *__restrict__ insref,
+                                                  float *__restrict__ selfreal, float *__restrict__ resself)
+{
+    const int lane = threadIdx.x & 63;
+    const int n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    if (n >= nins) return;
+    const float4 *P = insposn + (size_t)n * nslt;
+    const float4 c0 = P[0];
+    float R = 0.0f;
+    for (int s = 1 + lane; s < nslt; s += 64) {
+        float dx = P[s].x - c0.x, dy = P[s].y - c0.y, dz = P[s].z - c0.z;
+        min_image(fc, dx, dy, dz);
+        R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+    for (int off = 32; off > 0; off >>= 1) R = fmaxf(R, __shfl_xor(R, off, 64));
     // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r;
     // FP32 terms as in the reference's single build, summed in FP64 (a
-    // 150-site solute has 11 175 terms that largely cancel)
+    // 150-site solute has 11 175 terms that largely cancel): lane l takes
+    // the pairs (s, t) with t - s - 1 = l mod 64, then a fixed-order tree
     double e = 0.0;
     const float rsqpi = 0.564189583547756286948079451560772586f;  // 1/sqrt(pi)
+    for (int s = lane; s < nslt; s += 64) e += (double)(-P[s].w * P[s].w * screen * rsqpi);
     for (int s = 0; s < nslt; ++s) {
-        const float4 a = insposn[(size_t)n * nslt + s];
-        e += (double)(-a.w * a.w * screen * rsqpi);
-        for (int t = s + 1; t < nslt; ++t) {
-            const float4 b = insposn[(size_t)n * nslt + t];
+        const float4 a = P[s];
+        for (int t = s + 1 + lane; t < nslt; t += 64) {
+            const float4 b = P[t];
             float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
             min_image(fc, dx, dy, dz);
             const float r = sqrtf(dx * dx + dy * dy + dz * dz);
             e += (double)(-a.w * b.w * erff(screen * r) / r);
         }
     }
-    selfreal[n] = (float)e;
-    // residual_self_ene (engproc.F90:967-977): -(pi Q^2 / alpha^2 / V) / 2 with
-    // Q = mol_charge, a plain sum of the site charges (setconf.F90:838-840)
-    float Q = 0.0f;
-    for (int s = 0; s < nslt; ++s) Q += insposn[(size_t)n * nslt + s].w;
-    float epcl = ER_PI_F * Q * Q / screen / screen / fc.volume;
-    epcl = epcl / 2.0f;
-    resself[n] = 0.0f - epcl;
+    for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+    if (lane == 0) {
+        insref[n] = make_float4(c0.x, c0.y, c0.z, R);
+        selfreal[n] = (float)e;
+        // residual_self_ene (engproc.F90:967-977): -(pi Q^2 / alpha^2 / V) / 2 with
+        // Q = mol_charge, a plain sum of the site charges (setconf.F90:838-840)
+        float Q = 0.0f;
+        for (int s = 0; s < nslt; ++s) Q += P[s].w;
+        float epcl = ER_PI_F * Q * Q / screen / screen / fc.volume;
+        epcl = epcl / 2.0f;
+        resself[n] = 0.0f - epcl;
+    }
 }
 
 // SPME influence function engfac (recpcal.F90:121-163), stored [m1][m3][m2]
@@ -601,7 +616,10 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
 // transposed through LDS ([line][x1][16], rows padded by 8 floats: the
 // transposed writes hit distinct banks) and leave as one contiguous
 // RL * K1 * 16-float block of phi written with 16-byte stores.  Per-line
-// arithmetic as in k_pme_axis1.
+// arithmetic as in k_pme_axis1.  XCD-aware order: workgroup b runs on XCD
+// b % 8, and each XCD sweeps one contiguous eighth of the (x2 group, x3,
+// tile) blocks with x2 groups fastest, so the groups sharing a 128-byte
+// line of T are fetched back to back through the same L2.
 template <int K1, int RL>
 __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int K3,
                                                           const float2 *__restrict__ T, float *__restrict__ phi,
@@ -610,7 +628,12 @@ __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int
 {
     constexpr int NI = 16, M = K1 / 2, H = M + 1, LS = K1 * NI + 8;
     extern __shared__ __align__(16) float sml[];  // [RL][LS]
-    const int tile = blockIdx.z, x3 = blockIdx.y, x2_0 = blockIdx.x * RL;
+    const int ngx = K2 / RL;
+    const int total = ngx * K3 * ((nchunk + NI - 1) / NI);
+    const int per = (total + 7) / 8;
+    const int blk = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (blk >= total) return;
+    const int tile = blk / (ngx * K3), x3 = (blk / ngx) % K3, x2_0 = (blk % ngx) * RL;
     const int tid = threadIdx.x;
     const int r = tid % RL, i = tid / RL;
     const int nl = tile * NI + i;
@@ -703,26 +726,32 @@ __global__ void k_cell_count(int nmol, const float4 *__restrict__ molref, FrameC
 __global__ void __launch_bounds__(1024) k_cell_scan(int ncell, const unsigned int *__restrict__ cnt,
                                                     unsigned int *__restrict__ start)
 {
-    __shared__ unsigned int part[1024];
-    const int tid = threadIdx.x, NT = blockDim.x;
-    const int per = (ncell + NT - 1) / NT;
-    const int b0 = tid * per, b1 = min(ncell, b0 + per);
-    unsigned int s = 0;
-    for (int b = b0; b < b1; ++b) s += cnt[b];
-    part[tid] = s;
-    __syncthreads();
-    for (int off = 1; off < NT; off <<= 1) {
-        unsigned int v = (tid >= off) ? part[tid - off] : 0u;
+    // exclusive scan in rounds of 1024 coalesced counts: inclusive wave scans
+    // by shuffles, wave totals through LDS, a running carry between rounds
+    __shared__ unsigned int wsum[16];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    unsigned int carry = 0;
+    for (int base = 0; base < ncell; base += 1024) {
+        const int b = base + tid;
+        const unsigned int v = (b < ncell) ? cnt[b] : 0u;
+        unsigned int x = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
         __syncthreads();
-        part[tid] += v;
+        unsigned int before = 0, total = 0;
+        for (int w = 0; w < 16; ++w) {
+            const unsigned int t = wsum[w];
+            before += (w < wave) ? t : 0u;
+            total += t;
+        }
+        if (b < ncell) start[b] = carry + before + x - v;
+        carry += total;
         __syncthreads();
     }
-    unsigned int run = part[tid] - s;
-    for (int b = b0; b < b1; ++b) {
-        start[b] = run;
-        run += cnt[b];
-    }
-    if (tid == NT - 1) start[ncell] = part[NT - 1];
+    if (tid == 0) start[ncell] = carry;
 }
 
 __global__ void k_cell_scatter(int nmol, const unsigned int *__restrict__ key_rank,
@@ -844,21 +873,61 @@ __device__ __forceinline__ int wrap1(int x, int K)
     return x - ((x >= K) ? K : 0);
 }
 
-// Staging: every thread issues its ceil(NP*NV/512) 16-byte loads before any
-// LDS store, so a workgroup has its whole region in flight at once.  Compute:
-// four lanes per atom, each lane owning four of the NI = 16 insertions
-// (one float4 of a grid point), so every stencil read is one ds_read_b128
-// and a wave covers 16 atoms.  Per insertion the arithmetic order is that of
-// k_gather (x-sum with w1, then (w3*w2) * sum); the two paths still differ
-// in FP32 rounding (contraction choices), not in the terms summed.
+// Per-frame atom records in brick order for the brick gather, RECF floats
+// each: w1[ORD] w2[ORD] w3[ORD] q, the stencil's top corner in region
+// coordinates (lx | ly << 8 | lz << 16) and the atom index (both as int
+// bits), padded to whole float4s.
+__host__ __device__ constexpr int brick_rec_floats(int ord) { return (3 * ord + 3 + 3) / 4 * 4; }
+
+template <int ORD>
+__global__ void k_brick_records(int natom, const unsigned int *__restrict__ key_rank,
+                                const unsigned int *__restrict__ bstart, const float4 *__restrict__ posn,
+                                const float *__restrict__ splw, const int4 *__restrict__ splg, int K1, int K2, int K3,
+                                int nbx, int nby, float *__restrict__ rec)
+{
+    constexpr int RECF = brick_rec_floats(ORD), BZ = brick_z(ORD);
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= natom) return;
+    const unsigned int key = key_rank[2 * a];
+    const unsigned int pos = bstart[key] + key_rank[2 * a + 1];
+    const int bx = key % nbx, by = (key / nbx) % nby, bz = key / (nbx * nby);
+    const int4 g = splg[a];
+    const int lx = pmod(g.x, K1) - bx * ER_BRICK + (ORD - 1);
+    const int ly = pmod(g.y, K2) - by * ER_BRICK + (ORD - 1);
+    const int lz = pmod(g.z, K3) - bz * BZ + (ORD - 1);
+    float *r = rec + (size_t)pos * RECF;
+    const float *w = splw + (size_t)a * 3 * ORD;
+    for (int c = 0; c < 3 * ORD; ++c) r[c] = w[c];
+    r[3 * ORD] = posn[a].w;
+    r[3 * ORD + 1] = __int_as_float(lx | (ly << 8) | (lz << 16));
+    r[3 * ORD + 2] = __int_as_float(a);
+    for (int c = 3 * ORD + 3; c < RECF; ++c) r[c] = 0.f;
+}
+
+// value of the lane R places further along the same 16-lane DPP row (row_ror:R)
+template <int R>
+__device__ __forceinline__ float row_ror(float v)
+{
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x120 + R, 0xf, 0xf, false));
+}
+
+// Staging: the region is RZ * RS rows of RS grid points x NV float4, each
+// row contiguous in phi (but for the periodic wrap along x); wave w stages
+// rows w, w + 8, ... with one global_load_lds_dwordx4 per row (lane l ->
+// point l / NV, float4 l % NV; the row's address is scalar), every load
+// issued before the first wait.  Compute: sixteen lanes per atom, lane
+// (c, j) owning the x offsets c1 = c, c+4, ... of the stencil and four of
+// the NI = 16 insertions (one float4 of a grid point).  A 16-lane group
+// reads 4 consecutive grid points x 4 float4 = 256 contiguous bytes per
+// step, and a wave covers 4 atoms.  Lane (c, j) sums w1(c1) * (w3 * (w2 *
+// phi)) over its column; the four x-partials are combined by two DPP row
+// rotations ((c=0 + c=2) + (c=1 + c=3)).  The terms are those of k_gather;
+// the association differs (FP32 rounding only).
 template <int ORD, int NI>
 __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick,
                                                       const int *__restrict__ border, int nchunk,
                                                       const unsigned int *__restrict__ bstart,
-                                                      const int *__restrict__ aperm,
-                                                      const float4 *__restrict__ posn,
-                                                      const float *__restrict__ splw,
-                                                      const int4 *__restrict__ splg, const float *__restrict__ phi,
+                                                      const float *__restrict__ rec, const float *__restrict__ phi,
                                                       int natom, float *__restrict__ Eat)
 {
     static_assert(NI == 16, "the brick gather stages 16 interleaved insertions");
@@ -866,10 +935,15 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     constexpr int RS = ER_BRICK + ORD - 1;  // region side along x and y
     constexpr int RZ = BZ + ORD - 1;        // region side along z
     constexpr int NV = NI / 4;              // float4 per grid point
-    constexpr int NP = RS * RS * RZ;
-    constexpr int NF4 = NP * NV;
-    constexpr int NT = 512;
-    constexpr int PER = (NF4 + NT - 1) / NT;
+    constexpr int RSU = RS * NV;            // float4 per region row
+    constexpr int NROW = RS * RZ;
+    constexpr int NT = 512, NW = NT / 64;
+    constexpr int RPW = (NROW + NW - 1) / NW;  // rows per wave (at most)
+    constexpr int XL = 4;                   // lanes along the stencil's x offsets
+    constexpr int NX = (ORD + XL - 1) / XL; // x offsets per lane
+    constexpr int APASS = NT / (XL * NV);   // atoms per workgroup pass
+    constexpr int RECF = brick_rec_floats(ORD);
+    static_assert(RSU <= 64, "a region row is staged by one wave instruction");
     extern __shared__ __align__(16) float4 reg4[];  // [RZ][RS][RS][NV]
     // XCD-aware remap: workgroup b runs on XCD b % 8.  The bricks are ranked
     // along a Morton curve (border) and each XCD owns an equal contiguous
@@ -886,79 +960,141 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     const unsigned int a0 = bstart[brick], a1 = bstart[brick + 1];
     if (a0 == a1) return;
     const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * BZ - (ORD - 1);
-    const size_t G = (size_t)K1 * K2 * K3;
-    const float4 *ph4 = reinterpret_cast<const float4 *>(phi + (size_t)tile * G * NI);
+    const float4 *ph4 = reinterpret_cast<const float4 *>(phi) + (size_t)tile * K1 * K2 * K3 * NV;
     const int tid = threadIdx.x;
-    const int j = tid % NV, s = tid / NV;
-    // the first round's atom data is requested before the staging loads so
-    // that both latencies overlap
+    const int j = tid % NV, cx = (tid / NV) % XL, s = tid / (NV * XL);
     struct AtomIn {
         int a, lx, ly, lz;
-        float q, w[3 * ORD];
+        float q, w1[NX], w2[ORD], w3[ORD];
     };
     auto load_atom = [&](unsigned int pos, AtomIn &t) {
-        t.a = aperm[pos];
-        t.q = posn[t.a].w;
-        const float *w = splw + (size_t)t.a * 3 * ORD;
+        const float *r = rec + (size_t)pos * RECF;
 #pragma unroll
-        for (int c = 0; c < 3 * ORD; ++c) t.w[c] = w[c];
-        const int4 g = splg[t.a];
-        t.lx = pmod(g.x, K1) - bx * ER_BRICK + (ORD - 1);
-        t.ly = pmod(g.y, K2) - by * ER_BRICK + (ORD - 1);
-        t.lz = pmod(g.z, K3) - bz * BZ + (ORD - 1);
+        for (int m = 0; m < NX; ++m) t.w1[m] = (cx + XL * m < ORD) ? r[cx + XL * m] : 0.f;
+        if constexpr (ORD == 4) {  // 64-byte records: w2, w3 and the tail are aligned float4
+            const float4 v2 = *reinterpret_cast<const float4 *>(r + 4), v3 = *reinterpret_cast<const float4 *>(r + 8);
+            const float4 tl = *reinterpret_cast<const float4 *>(r + 12);
+            t.w2[0] = v2.x; t.w2[1] = v2.y; t.w2[2] = v2.z; t.w2[3] = v2.w;
+            t.w3[0] = v3.x; t.w3[1] = v3.y; t.w3[2] = v3.z; t.w3[3] = v3.w;
+            t.q = tl.x;
+            const int li = __float_as_int(tl.y);
+            t.lx = li & 0xff; t.ly = (li >> 8) & 0xff; t.lz = li >> 16;
+            t.a = __float_as_int(tl.z);
+        } else {
+#pragma unroll
+            for (int c = 0; c < ORD; ++c) {
+                t.w2[c] = r[ORD + c];
+                t.w3[c] = r[2 * ORD + c];
+            }
+            t.q = r[3 * ORD];
+            const int li = __float_as_int(r[3 * ORD + 1]);
+            t.lx = li & 0xff; t.ly = (li >> 8) & 0xff; t.lz = li >> 16;
+            t.a = __float_as_int(r[3 * ORD + 2]);
+        }
     };
     float4 *Eat4 = reinterpret_cast<float4 *>(Eat);
-    auto compute_atom = [&](const AtomIn &t) {
+    auto compute_atom = [&](const AtomIn &t, bool live) {
         float4 ea = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) {
 #pragma unroll
-        for (int c3 = 0; c3 < ORD; ++c3) {
+            for (int m = 0; m < NX; ++m) {
+                // x offset c1 = cx + 4 m; its column stays inside the region
+                // (lx - c1 >= 0) only for c1 < ORD, so clamp the others (w1 = 0)
+                const int c1 = min(cx + XL * m, ORD - 1);
+                // lowest corner of the stencil column: every read below is at
+                // a constant non-negative offset from it (ds_read offset field)
+                const float4 *b0 =
+                    reg4 + (((t.lz - (ORD - 1)) * RS + (t.ly - (ORD - 1))) * RS + (t.lx - c1)) * NV + j;
+                // order 4: all 16 reads of the column are issued before the
+                // first use; larger orders read one (c3) row at a time
+                constexpr int RB = (ORD <= 4) ? ORD : 1;
+                float4 pr[RB][ORD];
+                float4 xs = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-            for (int c2 = 0; c2 < ORD; ++c2) {
-                const float4 *row = reg4 + (((t.lz - c3) * RS + (t.ly - c2)) * RS) * NV + j;
-                float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+                for (int c3 = 0; c3 < ORD; ++c3) {
+                    if (c3 % RB == 0) {
 #pragma unroll
-                for (int c1 = 0; c1 < ORD; ++c1) {
-                    const float4 p = row[(t.lx - c1) * NV];
-                    const float w1 = t.w[c1];
-                    sum.x += w1 * p.x;
-                    sum.y += w1 * p.y;
-                    sum.z += w1 * p.z;
-                    sum.w += w1 * p.w;
+                        for (int r = 0; r < RB; ++r)
+#pragma unroll
+                            for (int c2 = 0; c2 < ORD; ++c2)
+                                pr[r][c2] = b0[((ORD - 1 - c3 - r) * RS + (ORD - 1 - c2)) * RS * NV];
+                    }
+                    float4 zs = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                    for (int c2 = 0; c2 < ORD; ++c2) {
+                        const float4 p = pr[c3 % RB][c2];
+                        const float w2 = t.w2[c2];
+                        zs.x += w2 * p.x;
+                        zs.y += w2 * p.y;
+                        zs.z += w2 * p.z;
+                        zs.w += w2 * p.w;
+                    }
+                    const float w3 = t.w3[c3];
+                    xs.x += w3 * zs.x;
+                    xs.y += w3 * zs.y;
+                    xs.z += w3 * zs.z;
+                    xs.w += w3 * zs.w;
                 }
-                const float w23 = t.w[2 * ORD + c3] * t.w[ORD + c2];
-                ea.x += w23 * sum.x;
-                ea.y += w23 * sum.y;
-                ea.z += w23 * sum.z;
-                ea.w += w23 * sum.w;
+                const float w1 = t.w1[m];
+                ea.x += w1 * xs.x;
+                ea.y += w1 * xs.y;
+                ea.z += w1 * xs.z;
+                ea.w += w1 * xs.w;
             }
         }
-        Eat4[((size_t)tile * natom + t.a) * NV + j] = make_float4(t.q * ea.x, t.q * ea.y, t.q * ea.z, t.q * ea.w);
+        // every lane of the wave takes part in the rotations
+        ea.x += row_ror<8>(ea.x);
+        ea.y += row_ror<8>(ea.y);
+        ea.z += row_ror<8>(ea.z);
+        ea.w += row_ror<8>(ea.w);
+        ea.x += row_ror<4>(ea.x);
+        ea.y += row_ror<4>(ea.y);
+        ea.z += row_ror<4>(ea.z);
+        ea.w += row_ror<4>(ea.w);
+        if (live && cx == 0)
+            Eat4[((size_t)tile * natom + t.a) * NV + j] = make_float4(t.q * ea.x, t.q * ea.y, t.q * ea.z, t.q * ea.w);
     };
-    AtomIn first;
-    const unsigned int pos0 = a0 + s;
-    if (pos0 < a1) load_atom(pos0, first);
-    // staging straight into LDS (global_load_lds_dwordx4): lane l of a wave
-    // writes its 16 bytes at the wave's base + 16 l, so unit f lands at
-    // reg4[f]; the tail lanes of the last round reload the last unit (in
-    // bounds) into the padding behind the region (the LDS allocation is
-    // PER * NT units)
-    const int wbase = tid & ~63;
+    // the records of the first NPF passes are requested before the staging
+    // loads, so that all these latencies overlap
+    constexpr int NPF = 4;
+    AtomIn pre[NPF];
 #pragma unroll
-    for (int r = 0; r < PER; ++r) {
-        const int f = min(tid + r * NT, NF4 - 1);
-        const int jj = f % NV, p = f / NV;
-        const int rx = p % RS, ry = (p / RS) % RS, rz = p / (RS * RS);
-        const int gx = wrap1(ox + rx, K1), gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void *)(ph4 + (((size_t)gz * K2 + gy) * K1 + gx) * NV + jj),
-            (__attribute__((address_space(3))) void *)(reg4 + r * NT + wbase), 16, 0, 0);
+    for (int q = 0; q < NPF; ++q)
+        if (a0 + q * APASS + s < a1) load_atom(a0 + q * APASS + s, pre[q]);
+    // staging straight into LDS (global_load_lds_dwordx4): lane l of a wave
+    // writes its 16 bytes at the LDS address given + 16 l, so lane l of row
+    // r lands at reg4[r * RSU + l]; lanes l >= RSU stay idle
+    {
+        const int lane = tid & 63;
+        const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);  // scalar: row math stays in SGPRs
+        const int gx = wrap1(ox + lane / NV, K1);
+        const unsigned int lbyte = (unsigned int)(gx * NV + lane % NV) * 16u;
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wv + q * NW;
+            if (r < NROW) {
+                const int rz = r / RS, ry = r % RS;
+                const int gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
+                const char *row = reinterpret_cast<const char *>(ph4 + ((size_t)gz * K2 + gy) * K1 * NV);
+                if (lane < RSU)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(row + lbyte),
+                        (__attribute__((address_space(3))) void *)(reg4 + r * RSU), 16, 0, 0);
+            }
+        }
     }
     __syncthreads();
-    if (pos0 < a1) compute_atom(first);
-    for (unsigned int pos = pos0 + NT / NV; pos < a1; pos += NT / NV) {
+    // pass bounds are uniform over the workgroup: all 64 lanes of a wave run
+    // the DPP rotations together
+#pragma unroll
+    for (int q = 0; q < NPF; ++q)
+        if (a0 + q * APASS < a1) compute_atom(pre[q], a0 + q * APASS + s < a1);
+    for (unsigned int base = a0 + NPF * APASS; base < a1; base += APASS) {
+        const unsigned int pos = base + s;
         AtomIn t;
-        load_atom(pos, t);
-        compute_atom(t);
+        const bool live = pos < a1;
+        if (live) load_atom(pos, t);
+        compute_atom(t, live);
     }
 }
 
@@ -970,13 +1106,12 @@ __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const i
                              const int *__restrict__ tag, float qslt_refs, float screen, float volume,
                              float *__restrict__ U)
 {
-    const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = f % NI;
-    const size_t rest = f / NI;
-    const int mol = rest % nmol;
-    const int tile = rest / nmol;
+    // block = 256 / NI molecules x NI insertions of one tile (blockIdx.y)
+    const int i = threadIdx.x % NI;
+    const int mol = blockIdx.x * (blockDim.x / NI) + threadIdx.x / NI;
+    const int tile = blockIdx.y;
     const int nl = tile * NI + i;
-    if (tile * NI >= nchunk || nl >= nchunk) return;
+    if (mol >= nmol || nl >= nchunk) return;
     float acc = 0.0f;
     for (int a = molbeg[mol]; a < molbeg[mol + 1]; ++a) acc += Eat[((size_t)tile * natom + a) * NI + i];
     const int n = n_off + nl;
