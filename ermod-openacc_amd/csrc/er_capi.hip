@@ -180,6 +180,7 @@ struct er_ctx {
 
     // energies / histograms
     DBuf<float> d_meshf, d_resself;
+    DBuf<unsigned short> d_blut;  // bin brackets per species (k_hist)
     DBuf<double> d_flpart;
     int nslice = 1;
     DBuf<uint32_t> d_C;
@@ -625,6 +626,24 @@ extern "C" int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsof
         }
         HIPCHK(c, c->d_meshf.ensure(tot));
         HIPCHK(c, hipMemcpy(c->d_meshf.p, mf.data(), tot * sizeof(float), hipMemcpyHostToDevice));
+        // bin brackets: bucket b holds the floats whose order key has top bits b
+        std::vector<unsigned short> lut((size_t)c->numslv * ER_LUT_N);
+        for (int k = 0; k < c->numslv; ++k) {
+            if (c->uvmax[k] > 65535) return fail(c, ER_E_ARG, "er_set_mesh: more than 65535 bins for species %d", k + 1);
+            const float *m0 = mf.data() + c->idmin[k], *m1 = m0 + c->uvmax[k];
+            for (int b = 0; b < ER_LUT_N - 1; ++b) {
+                const uint32_t key = (uint32_t)b << (32 - ER_LUT_BITS);  // smallest order key of the bucket
+                const uint32_t bits = (key & 0x80000000u) ? (key & 0x7fffffffu) : ~key;
+                float fmin;
+                std::memcpy(&fmin, &bits, sizeof fmin);
+                // NaN buckets (beyond +inf) take the whole mesh; k_hist never looks them up
+                const size_t cnt = (fmin != fmin) ? (size_t)c->uvmax[k] : (size_t)(std::lower_bound(m0, m1, fmin) - m0);
+                lut[(size_t)k * ER_LUT_N + b] = (unsigned short)cnt;
+            }
+            lut[(size_t)k * ER_LUT_N + ER_LUT_N - 1] = (unsigned short)c->uvmax[k];
+        }
+        HIPCHK(c, c->d_blut.ensure(lut.size()));
+        HIPCHK(c, hipMemcpy(c->d_blut.p, lut.data(), lut.size() * sizeof(unsigned short), hipMemcpyHostToDevice));
     }
     HIPCHK(c, c->d_uvmax.ensure(c->numslv));
     HIPCHK(c, hipMemcpyAsync(c->d_uvmax.p, c->uvmax.data(), c->numslv * sizeof(int), hipMemcpyHostToDevice, s));
@@ -1037,7 +1056,8 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
 {
     const int NT = 256;
     const size_t lds = (size_t)NT * sizeof(double) + (size_t)c->ermax * sizeof(float) +
-                       (size_t)NI * c->ermax * sizeof(uint32_t) + (size_t)NT * 2 * sizeof(float);
+                       (size_t)NI * c->ermax * sizeof(uint32_t) + (size_t)NT * 2 * sizeof(float) +
+                       (size_t)c->numslv * ER_LUT_N * sizeof(unsigned short);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)k_hist<NI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
@@ -1045,7 +1065,8 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
     HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice, c->S().seq};
     const int ntile = (nch + NI - 1) / NI;
     hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha, c->S().d_U.p,
-                       soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_spb.p, c->d_spe.p, c->d_idmin.p,
+                       soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_blut.p, c->d_spb.p, c->d_spe.p,
+                       c->d_idmin.p,
                        c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p, c->S().d_minmax.p, soln ? c->d_flpart.p : (double *)nullptr,
                        c->S().d_errkey.p, c->S().d_warn.p);
     return hipGetLastError();

@@ -1303,6 +1303,14 @@ __device__ __forceinline__ unsigned int f2ord(float f)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
+// Bin brackets: the top ER_LUT_BITS of f2ord(v) select a bucket of floats;
+// lut[b] = number of a species' mesh values below the bucket's smallest
+// float, so every v of bucket b has its bin index (the count of mesh values
+// below v) in [lut[b], lut[b + 1]], and the binary search runs over that
+// bracket only (same result as over the whole species mesh).
+#define ER_LUT_BITS 12
+#define ER_LUT_N ((1 << ER_LUT_BITS) + 1)
+
 struct HistArgs {
     int nmol, nchunk, n_off, ermax, numslv, slttype, nslice;
     unsigned int seq;  // run number of the slot (halt keys order runs first)
@@ -1318,7 +1326,9 @@ __device__ __forceinline__ unsigned long long halt_key(unsigned int seq, int n, 
 
 template <int NI>
 __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restrict__ U, const int *__restrict__ tag,
-                                              const float *__restrict__ meshf, const int *__restrict__ sp_begin,
+                                              const float *__restrict__ meshf,
+                                              const unsigned short *__restrict__ lut,
+                                              const int *__restrict__ sp_begin,
                                               const int *__restrict__ sp_end, const int *__restrict__ idmin,
                                               const int *__restrict__ uvmax, const int *__restrict__ uvsoft,
                                               uint32_t *__restrict__ Cnt, unsigned int *__restrict__ minmax,
@@ -1331,11 +1341,13 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
     uint32_t *hist = reinterpret_cast<uint32_t *>(mesh + ha.ermax);  // [NI][ermax]
     float *rmin = reinterpret_cast<float *>(hist + NI * ha.ermax);   // [NT]
     float *rmax = rmin + blockDim.x;                                  // [NT]
+    unsigned short *blut = reinterpret_cast<unsigned short *>(rmax + blockDim.x);  // [numslv][ER_LUT_N]
 
     const int tid = threadIdx.x, NT = blockDim.x;
     const int tile = blockIdx.x, slice = blockIdx.y;
     for (int b = tid; b < ha.ermax; b += NT) mesh[b] = meshf[b];
     for (int b = tid; b < NI * ha.ermax; b += NT) hist[b] = 0u;
+    for (int b = tid; b < ha.numslv * ER_LUT_N; b += NT) blut[b] = lut[b];
     __syncthreads();
 
     const int i = tid % NI, jj = tid / NI, jstep = NT / NI;
@@ -1356,7 +1368,12 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
             for (int j = j0 + jj; j < j1; j += jstep) {
                 if (j == skip) continue;
                 const float v = U[(gt * ha.nmol + j) * NI + i];
-                int lo = 0, hi = num;
+                int lo = 0, hi = 0;  // NaN: bin 0 (no mesh value is below it), as the full search
+                if (v == v) {
+                    const unsigned short *bl = blut + (sp - 1) * ER_LUT_N + (f2ord(v) >> (32 - ER_LUT_BITS));
+                    lo = bl[0];
+                    hi = bl[1];
+                }
                 while (lo < hi) {
                     const int mid = (lo + hi) >> 1;
                     if (v > mesh[base + mid]) lo = mid + 1;
