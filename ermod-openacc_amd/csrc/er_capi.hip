@@ -169,6 +169,7 @@ struct er_ctx {
 
     // atoms sorted by PME brick (per frame) for the brick gather
     int nbx = 1, nby = 1, nbz = 1, nbrick = 1;
+    int ncu = 0;  // compute units of the context's device
     DBuf<int> d_border;           // bricks in Morton order
     DBuf<float> d_brec;           // atom records in brick order
     DBuf<unsigned int> d_akeyrank, d_bcnt, d_bstart;
@@ -499,6 +500,11 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     gmx_params(6, prm->lwljcut, prm->upljcut, pp.attA, pp.attB, pp.attC);
 
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    {
+        int dev = 0;
+        HIPCHK(c, hipGetDevice(&dev));
+        HIPCHK(c, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    }
     hipStream_t s = c->stream;
     HIPCHK(c, c->d_charge.ensure(c->numatm));
     HIPCHK(c, hipMemcpyAsync(c->d_charge.p, topo->charge, c->numatm * sizeof(float), hipMemcpyHostToDevice, s));
@@ -1045,10 +1051,31 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
-    hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + 256 / NI - 1) / (256 / NI)), ntile), dim3(256),
+    const int mpb = ER_MR_MPT * (256 / NI);  // molecules per block
+    hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + mpb - 1) / mpb), ntile), dim3(256),
                        0, c->stream, c->nummol,
                        nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
                        soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt, c->prm.screen, c->fc.volume, c->S().d_U.p);
+    return hipGetLastError();
+}
+
+// order 4: the streaming gather, one workgroup per CU (its two LDS buffers
+// fill a CU), a whole number of workgroups per XCD
+template <int NI> static hipError_t launch_gather_stream_k(er_ctx *c, int n_off, int nch)
+{
+    const int nwg = 8 * std::max(1, c->ncu / 8);
+    hipLaunchKernelGGL((k_gather_stream<4, NI>), dim3((unsigned)nwg), dim3(ER_GS_NT), 0, c->stream, c->K1, c->K2,
+                       c->K3, c->nbx, c->nby, c->nbrick, c->d_border.p, nch, c->d_bstart.p, c->d_brec.p, c->d_phi.p,
+                       c->numatm, c->d_Eat.p);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const bool soln = c->prm.slttype == 1;
+    const int ntile = (nch + NI - 1) / NI;
+    const int mpb = ER_MR_MPT * (256 / NI);  // molecules per block
+    hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + mpb - 1) / mpb), ntile), dim3(256), 0,
+                       c->stream, c->nummol, nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
+                       soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt, c->prm.screen, c->fc.volume,
+                       c->S().d_U.p);
     return hipGetLastError();
 }
 
@@ -1174,7 +1201,7 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
         ProfScope ps(c, ER_K_GATHER);
         if constexpr (NI == 1) e = launch_gather<1, 1>(c, n_off, nch);
         else {
-            if (c->ord == 4) e = launch_gather_brick_k<4, NI>(c, n_off, nch);
+            if (c->ord == 4) e = launch_gather_stream_k<NI>(c, n_off, nch);
             else if (c->ord == 6) e = launch_gather_brick_k<6, NI>(c, n_off, nch);
             else e = launch_gather<NI, NI>(c, n_off, nch);
         }

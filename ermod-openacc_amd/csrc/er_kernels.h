@@ -13,6 +13,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <type_traits>
 #include "er_fft.h"
 
 #define ER_PI_F 3.14159265358979323846f
@@ -726,30 +727,42 @@ __global__ void k_cell_count(int nmol, const float4 *__restrict__ molref, FrameC
 __global__ void __launch_bounds__(1024) k_cell_scan(int ncell, const unsigned int *__restrict__ cnt,
                                                     unsigned int *__restrict__ start)
 {
-    // exclusive scan in rounds of 1024 coalesced counts: inclusive wave scans
-    // by shuffles, wave totals through LDS, a running carry between rounds
+    // exclusive scan in rounds of 1024 coalesced counts (the counts of RR
+    // rounds are loaded together): inclusive wave scans by shuffles, wave
+    // totals through LDS, a running carry between rounds
+    constexpr int RR = 8;
     __shared__ unsigned int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned int carry = 0;
-    for (int base = 0; base < ncell; base += 1024) {
-        const int b = base + tid;
-        const unsigned int v = (b < ncell) ? cnt[b] : 0u;
-        unsigned int x = v;
-        for (int off = 1; off < 64; off <<= 1) {
-            const unsigned int y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
+    for (int base0 = 0; base0 < ncell; base0 += RR * 1024) {
+        unsigned int vr[RR];
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int b = base0 + r * 1024 + tid;
+            vr[r] = (b < ncell) ? cnt[b] : 0u;
         }
-        if (lane == 63) wsum[wave] = x;
-        __syncthreads();
-        unsigned int before = 0, total = 0;
-        for (int w = 0; w < 16; ++w) {
-            const unsigned int t = wsum[w];
-            before += (w < wave) ? t : 0u;
-            total += t;
+#pragma unroll
+        for (int r = 0; r < RR; ++r) {
+            const int b = base0 + r * 1024 + tid;
+            if (base0 + r * 1024 >= ncell) break;  // uniform
+            const unsigned int v = vr[r];
+            unsigned int x = v;
+            for (int off = 1; off < 64; off <<= 1) {
+                const unsigned int y = __shfl_up(x, off, 64);
+                if (lane >= off) x += y;
+            }
+            if (lane == 63) wsum[wave] = x;
+            __syncthreads();
+            unsigned int before = 0, total = 0;
+            for (int w = 0; w < 16; ++w) {
+                const unsigned int t = wsum[w];
+                before += (w < wave) ? t : 0u;
+                total += t;
+            }
+            if (b < ncell) start[b] = carry + before + x - v;
+            carry += total;
+            __syncthreads();
         }
-        if (b < ncell) start[b] = carry + before + x - v;
-        carry += total;
-        __syncthreads();
     }
     if (tid == 0) start[ncell] = carry;
 }
@@ -1098,30 +1111,248 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     }
 }
 
+// ---------------------------------------------------------------------------
+// Streaming form of the brick gather (order 4): one workgroup per CU walks
+// its XCD's (brick, tile) items (the same Morton-ranked ranges as
+// k_gather_brick, tile after tile) with two LDS buffers, each holding one
+// region and up to ER_GS_RCAP atom records.  While the atoms of item j are
+// computed from one buffer, item j + W's region and records are already on
+// their way into the other (global_load_lds_dwordx4), so the staging latency
+// hides behind the arithmetic.  The two buffers are distinct static arrays
+// and every LDS access names one of them directly, so the compiler's waits
+// for the in-flight copies apply only to the buffer being filled.  Atoms
+// beyond ER_GS_RCAP in one brick read their records from global memory.
+// Lane mapping and arithmetic are those of k_gather_brick.
+// ---------------------------------------------------------------------------
+#define ER_GS_RCAP 128
+#define ER_GS_NT 1024
+template <int ORD, int NI>
+__global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int K3, int nbx, int nby, int nbrick,
+                                                             const int *__restrict__ border, int nchunk,
+                                                             const unsigned int *__restrict__ bstart,
+                                                             const float *__restrict__ rec,
+                                                             const float *__restrict__ phi, int natom,
+                                                             float *__restrict__ Eat)
+{
+    static_assert(NI == 16 && ORD == 4, "the streaming gather handles order 4 with 16 interleaved insertions");
+    constexpr int BZ = brick_z(ORD);
+    constexpr int RS = ER_BRICK + ORD - 1, RZ = BZ + ORD - 1;
+    constexpr int NV = NI / 4, RSU = RS * NV, NROW = RS * RZ, NREG = NROW * RSU;
+    constexpr int NT = ER_GS_NT, NW = NT / 64, RPW = (NROW + NW - 1) / NW;
+    constexpr int XL = 4, APASS = NT / (XL * NV);
+    constexpr int RECF = brick_rec_floats(ORD), RECU = RECF / 4;  // float4 per record
+    static_assert(RECF == 16, "64-byte records");
+    constexpr int NBUF = NREG + ER_GS_RCAP * RECU;
+    __shared__ __align__(16) float4 bufA[NBUF];
+    __shared__ __align__(16) float4 bufB[NBUF];
+
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int j4 = tid % NV, cx = (tid / NV) % XL, s = tid / (NV * XL);
+    const int xcd = blockIdx.x & 7, W = gridDim.x >> 3, k0 = blockIdx.x >> 3;
+    const int per = (nbrick + 7) / 8;
+    const int ntile = (nchunk + NI - 1) / NI;
+    const int nitem = per * ntile;
+    const size_t G4 = (size_t)K1 * K2 * K3 * NV;  // float4 per tile of phi
+    const float4 *rec4 = reinterpret_cast<const float4 *>(rec);
+    float4 *Eat4 = reinterpret_cast<float4 *>(Eat);
+
+    struct Item {
+        int tile, bx, by, bz;
+        unsigned int a0, a1;
+        bool live;
+    };
+    auto item = [&](int jj) {
+        Item it{};
+        it.live = false;
+        if (jj >= nitem) return it;
+        const int rank = xcd * per + jj % per;
+        if (rank >= nbrick) return it;
+        const int brick = border[rank];
+        it.tile = jj / per;
+        it.bx = brick % nbx;
+        it.by = (brick / nbx) % nby;
+        it.bz = brick / (nbx * nby);
+        it.a0 = bstart[brick];
+        it.a1 = bstart[brick + 1];
+        it.live = it.a1 > it.a0;
+        return it;
+    };
+    // issue the copies of an item's region and first records into buf
+    auto stage = [&](const Item &it, float4 *buf) {
+        const int ox = it.bx * ER_BRICK - (ORD - 1), oy = it.by * ER_BRICK - (ORD - 1), oz = it.bz * BZ - (ORD - 1);
+        const float4 *ph4 = reinterpret_cast<const float4 *>(phi) + (size_t)it.tile * G4;
+        const int gx = wrap1(ox + lane / NV, K1);
+        const unsigned int lbyte = (unsigned int)(gx * NV + lane % NV) * 16u;
+#pragma unroll
+        for (int q = 0; q < RPW; ++q) {
+            const int r = wv + q * NW;
+            if (r < NROW) {
+                const int rz = r / RS, ry = r % RS;
+                const int gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
+                const char *row = reinterpret_cast<const char *>(ph4 + ((size_t)gz * K2 + gy) * K1 * NV);
+                if (lane < RSU)
+                    __builtin_amdgcn_global_load_lds(
+                        (const __attribute__((address_space(1))) void *)(row + lbyte),
+                        (__attribute__((address_space(3))) void *)(buf + r * RSU), 16, 0, 0);
+            }
+        }
+        // records [a0, a0 + min(n, RCAP)): contiguous 64-byte rows
+        const int nrec = min((int)(it.a1 - it.a0), ER_GS_RCAP);
+        const int nu = nrec * RECU;
+        const char *rbase = reinterpret_cast<const char *>(rec4 + (size_t)it.a0 * RECU);
+        for (int u0 = wv * 64; u0 < nu; u0 += NT) {  // wave-uniform
+            if (u0 + lane < nu)
+                __builtin_amdgcn_global_load_lds(
+                    (const __attribute__((address_space(1))) void *)(rbase + (unsigned int)(u0 + lane) * 16u),
+                    (__attribute__((address_space(3))) void *)(buf + NREG + u0), 16, 0, 0);
+        }
+    };
+    struct AtomIn {
+        int a, lx, ly, lz;
+        float q, w1, w2[ORD], w3[ORD];
+    };
+    auto unpack = [&](AtomIn &t, float w1, float4 v2, float4 v3, float4 tl) {
+        t.w1 = w1;
+        t.w2[0] = v2.x; t.w2[1] = v2.y; t.w2[2] = v2.z; t.w2[3] = v2.w;
+        t.w3[0] = v3.x; t.w3[1] = v3.y; t.w3[2] = v3.z; t.w3[3] = v3.w;
+        t.q = tl.x;
+        const int li = __float_as_int(tl.y);
+        t.lx = li & 0xff; t.ly = (li >> 8) & 0xff; t.lz = li >> 16;
+        t.a = __float_as_int(tl.z);
+    };
+    // one pass of APASS atoms [base, min(base + APASS, end)) of an item from
+    // buf (all lanes take part); records from LDS (FROM_LDS) or, for atoms
+    // beyond the staged ER_GS_RCAP, from global memory.  The two forms stay
+    // separate code so that LDS reads never wait on the copies in flight.
+    auto pass = [&](const Item &it, const float4 *buf, unsigned int base, unsigned int end, auto from_lds) {
+        const unsigned int pos = base + s;
+        const bool live = pos < end;
+        AtomIn t{};
+        if (live) {
+            if constexpr (decltype(from_lds)::value) {
+                const float4 *r = buf + NREG + (pos - it.a0) * RECU;
+                unpack(t, reinterpret_cast<const float *>(r)[cx], r[1], r[2], r[3]);
+            } else {
+                const float4 *r = rec4 + (size_t)pos * RECU;
+                unpack(t, reinterpret_cast<const float *>(r)[cx], r[1], r[2], r[3]);
+            }
+        }
+        float4 ea = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (live) {
+            const float4 *b0 =
+                buf + (((t.lz - (ORD - 1)) * RS + (t.ly - (ORD - 1))) * RS + (t.lx - cx)) * NV + j4;
+            float4 pr[ORD][ORD];
+#pragma unroll
+            for (int c3 = 0; c3 < ORD; ++c3)
+#pragma unroll
+                for (int c2 = 0; c2 < ORD; ++c2) pr[c3][c2] = b0[((ORD - 1 - c3) * RS + (ORD - 1 - c2)) * RS * NV];
+            float4 xs = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int c3 = 0; c3 < ORD; ++c3) {
+                float4 zs = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+                for (int c2 = 0; c2 < ORD; ++c2) {
+                    const float4 p = pr[c3][c2];
+                    const float w2 = t.w2[c2];
+                    zs.x += w2 * p.x;
+                    zs.y += w2 * p.y;
+                    zs.z += w2 * p.z;
+                    zs.w += w2 * p.w;
+                }
+                const float w3 = t.w3[c3];
+                xs.x += w3 * zs.x;
+                xs.y += w3 * zs.y;
+                xs.z += w3 * zs.z;
+                xs.w += w3 * zs.w;
+            }
+            ea.x = t.w1 * xs.x;
+            ea.y = t.w1 * xs.y;
+            ea.z = t.w1 * xs.z;
+            ea.w = t.w1 * xs.w;
+        }
+        ea.x += row_ror<8>(ea.x);
+        ea.y += row_ror<8>(ea.y);
+        ea.z += row_ror<8>(ea.z);
+        ea.w += row_ror<8>(ea.w);
+        ea.x += row_ror<4>(ea.x);
+        ea.y += row_ror<4>(ea.y);
+        ea.z += row_ror<4>(ea.z);
+        ea.w += row_ror<4>(ea.w);
+        if (live && cx == 0)
+            Eat4[((size_t)it.tile * natom + t.a) * NV + j4] =
+                make_float4(t.q * ea.x, t.q * ea.y, t.q * ea.z, t.q * ea.w);
+    };
+    // pass bounds are uniform over the workgroup
+    auto compute = [&](const Item &it, const float4 *buf) {
+        if (!it.live) return;
+        const unsigned int split = min(it.a1, it.a0 + ER_GS_RCAP);
+        for (unsigned int base = it.a0; base < split; base += APASS)
+            pass(it, buf, base, split, std::integral_constant<bool, true>{});
+        for (unsigned int base = split; base < it.a1; base += APASS)
+            pass(it, buf, base, it.a1, std::integral_constant<bool, false>{});
+    };
+
+    // item descriptors run one step ahead of the copies, the copies one
+    // step ahead of the arithmetic
+    int j = k0;
+    Item cur = item(j), nxt = item(j + W);
+    if (cur.live) stage(cur, bufA);
+    __syncthreads();  // (waits for the copies)
+    for (;;) {
+        // even step: compute from bufA, fill bufB
+        Item nx2 = item(j + 2 * W);
+        if (nxt.live) stage(nxt, bufB);
+        compute(cur, bufA);
+        __syncthreads();
+        if (j + W >= nitem) break;
+        j += W;
+        cur = nxt;
+        nxt = nx2;
+        // odd step: compute from bufB, fill bufA
+        nx2 = item(j + 2 * W);
+        if (nxt.live) stage(nxt, bufA);
+        compute(cur, bufB);
+        __syncthreads();
+        if (j + W >= nitem) break;
+        j += W;
+        cur = nxt;
+        nxt = nx2;
+    }
+}
+
 // per-molecule sum of the atom energies (atom order as in k_gather) plus the
 // residual background term (engproc.F90:979-1020); writes U
+#define ER_MR_MPT 8  // molecules per thread in k_mol_reduce
 template <int NI>
 __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const int *__restrict__ molbeg,
                              const float *__restrict__ molq, const float *__restrict__ Eat,
                              const int *__restrict__ tag, float qslt_refs, float screen, float volume,
                              float *__restrict__ U)
 {
-    // block = 256 / NI molecules x NI insertions of one tile (blockIdx.y)
+    // block = ER_MR_MPT * 256 / NI molecules x NI insertions of one tile
+    // (blockIdx.y); thread (m, i) takes molecules m, m + 256 / NI, ...
     const int i = threadIdx.x % NI;
-    const int mol = blockIdx.x * (blockDim.x / NI) + threadIdx.x / NI;
+    const int mstep = blockDim.x / NI;
     const int tile = blockIdx.y;
     const int nl = tile * NI + i;
-    if (mol >= nmol || nl >= nchunk) return;
-    float acc = 0.0f;
-    for (int a = molbeg[mol]; a < molbeg[mol + 1]; ++a) acc += Eat[((size_t)tile * natom + a) * NI + i];
+    if (nl >= nchunk) return;
     const int n = n_off + nl;
-    float qs = qslt_refs;
-    if (tag) {
-        if (mol == tag[n]) acc = 0.0f;
-        qs = molq[tag[n]];
+    const int skip = tag ? tag[n] : -1;
+    const float qs = tag ? molq[skip] : qslt_refs;
+    const float *E = Eat + (size_t)tile * natom * NI + i;
+    float *Urow = U + (size_t)(n / NI) * nmol * NI + (n % NI);
+#pragma unroll
+    for (int r = 0; r < ER_MR_MPT; ++r) {
+        const int mol = (blockIdx.x * ER_MR_MPT + r) * mstep + threadIdx.x / NI;
+        if (mol < nmol) {
+            float acc = 0.0f;
+            for (int a = molbeg[mol]; a < molbeg[mol + 1]; ++a) acc += E[(size_t)a * NI];
+            if (mol == skip) acc = 0.0f;
+            else acc -= ER_PI_F * qs * molq[mol] / screen / screen / volume;
+            Urow[(size_t)mol * NI] = acc;
+        }
     }
-    if (!tag || mol != tag[n]) acc -= ER_PI_F * qs * molq[mol] / screen / screen / volume;
-    U[((size_t)(n / NI) * nmol + mol) * NI + (n % NI)] = acc;
 }
 
 // ---------------------------------------------------------------------------
