@@ -908,7 +908,7 @@ template <> struct Split<128> { static constexpr int A = 8, B = 16; };
 template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch)
 {
     constexpr int R1 = Split<K3>::A, R2 = Split<K3>::B, C1 = Split<K2>::A, C2 = Split<K2>::B;
-    constexpr int NT = (K2 * K3 > 6400) ? 512 : 256;  // large planes: more threads, fewer values per thread
+    constexpr int NT = (K2 * K3 >= 5184) ? 512 : 256;  // large planes: more threads, fewer values per thread
     const size_t lds = (size_t)K2 * (K3 + 1) * sizeof(float2) + (size_t)ER_SCH * (K2 + K3) * sizeof(float2) +
                        (K2 + K3) * sizeof(float2) + (NT / 64) * sizeof(double);
     static bool attr = false;
