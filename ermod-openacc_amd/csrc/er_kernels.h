@@ -1342,14 +1342,37 @@ __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const i
     const float qs = tag ? molq[skip] : qslt_refs;
     const float *E = Eat + (size_t)tile * natom * NI + i;
     float *Urow = U + (size_t)(n / NI) * nmol * NI + (n % NI);
+    // every bound, charge and the rows of molecules of up to four atoms are
+    // requested before any sum, so a thread has all its loads in flight
+    int b0[ER_MR_MPT], b1[ER_MR_MPT];
+    float mq[ER_MR_MPT], e[ER_MR_MPT][4];
+#pragma unroll
+    for (int r = 0; r < ER_MR_MPT; ++r) {
+        const int mol = (blockIdx.x * ER_MR_MPT + r) * mstep + threadIdx.x / NI;
+        b0[r] = b1[r] = 0;
+        mq[r] = 0.0f;
+        if (mol < nmol) {
+            b0[r] = molbeg[mol];
+            b1[r] = molbeg[mol + 1];
+            mq[r] = molq[mol];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < ER_MR_MPT; ++r)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) e[r][k] = (b0[r] + k < b1[r]) ? E[(size_t)(b0[r] + k) * NI] : 0.0f;
 #pragma unroll
     for (int r = 0; r < ER_MR_MPT; ++r) {
         const int mol = (blockIdx.x * ER_MR_MPT + r) * mstep + threadIdx.x / NI;
         if (mol < nmol) {
+            // atom order: the first four from registers, any further ones loaded here
             float acc = 0.0f;
-            for (int a = molbeg[mol]; a < molbeg[mol + 1]; ++a) acc += E[(size_t)a * NI];
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (b0[r] + k < b1[r]) acc += e[r][k];
+            for (int a = b0[r] + 4; a < b1[r]; ++a) acc += E[(size_t)a * NI];
             if (mol == skip) acc = 0.0f;
-            else acc -= ER_PI_F * qs * molq[mol] / screen / screen / volume;
+            else acc -= ER_PI_F * qs * mq[r] / screen / screen / volume;
             Urow[(size_t)mol * NI] = acc;
         }
     }
