@@ -1563,6 +1563,7 @@ __device__ __forceinline__ unsigned int f2ord(float f)
 // below v) in [lut[b], lut[b + 1]], and the binary search runs over that
 // bracket only (same result as over the whole species mesh).
 #define ER_LUT_BITS 12
+#define ER_HB 8  // energies loaded together per k_hist thread
 #define ER_LUT_N ((1 << ER_LUT_BITS) + 1)
 
 struct HistArgs {
@@ -1619,9 +1620,20 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
         const int j0 = s0 + slice * len, j1 = min(s1, j0 + len);
         const int base = idmin[sp - 1], num = uvmax[sp - 1], soft = uvsoft[sp - 1];
         if (valid) {
-            for (int j = j0 + jj; j < j1; j += jstep) {
-                if (j == skip) continue;
-                const float v = U[(gt * ha.nmol + j) * NI + i];
+            // the energies of ER_HB consecutive steps are loaded together, then
+            // binned in the same order as one at a time
+            for (int jb = j0 + jj; jb < j1; jb += ER_HB * jstep) {
+              float vb[ER_HB];
+#pragma unroll
+              for (int k = 0; k < ER_HB; ++k) {
+                  const int j = jb + k * jstep;
+                  vb[k] = (j < j1) ? U[(gt * ha.nmol + j) * NI + i] : 0.0f;
+              }
+#pragma unroll
+              for (int k = 0; k < ER_HB; ++k) {
+                const int j = jb + k * jstep;
+                if (j >= j1 || j == skip) continue;
+                const float v = vb[k];
                 int lo = 0, hi = 0;  // NaN: bin 0 (no mesh value is below it), as the full search
                 if (v == v) {
                     const unsigned short *bl = blut + (sp - 1) * ER_LUT_N + (f2ord(v) >> (32 - ER_LUT_BITS));
@@ -1640,6 +1652,7 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
                 lmin = fminf(lmin, v);
                 lmax = fmaxf(lmax, v);
                 lsum += (double)v;
+              }
             }
         }
         rmin[tid] = lmin;
