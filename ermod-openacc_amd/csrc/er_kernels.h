@@ -1476,6 +1476,42 @@ __device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const Fra
 }
 
 #define ER_PAIR_CAP 2048
+#define ER_PAIR_SCAP 256   // solute sites held in LDS
+#define ER_PAIR_TCAP 1024  // LJ table entries (ntype^2) held in LDS
+
+// molecule_pair_sum with the insertion's sites and the LJ tables in LDS and
+// the molecule's first four atoms loaded together; same terms, same order
+__device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const FrameConst &fc, int jm,
+                                                       const int *__restrict__ molbeg,
+                                                       const float4 *__restrict__ posn,
+                                                       const int *__restrict__ ljtype, const float4 *sx,
+                                                       const int *stj, const float *te, const float *ts)
+{
+    const int b0 = molbeg[jm], b1 = molbeg[jm + 1];
+    float4 xb4[4];
+    int tb4[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        xb4[k] = (b0 + k < b1) ? posn[b0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        tb4[k] = (b0 + k < b1) ? ljtype[b0 + k] : 0;
+    }
+    float e = 0.0f;
+    auto term = [&](const float4 &xa, int ti, const float4 &xb, int tj) {
+        float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
+        min_image(fc, dx, dy, dz);
+        e += pair_energy(pa.pp, dx * dx + dy * dy + dz * dz, xa.w, xb.w, te[ti + pa.ntype * tj],
+                         ts[ti + pa.ntype * tj]);
+    };
+    for (int s = 0; s < pa.nslt; ++s) {
+        const float4 xa = sx[s];
+        const int ti = stj[s];
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (b0 + k < b1) term(xa, ti, xb4[k], tb4[k]);
+        for (int b = b0 + 4; b < b1; ++b) term(xa, ti, posn[b], ljtype[b]);
+    }
+    return e;
+}
 
 __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const int *__restrict__ molbeg,
                                                const float4 *__restrict__ posn, const int *__restrict__ ljtype,
@@ -1489,6 +1525,9 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
 {
     __shared__ int cand[ER_PAIR_CAP];
     __shared__ int ncand;
+    __shared__ float4 sx[ER_PAIR_SCAP];
+    __shared__ int stj[ER_PAIR_SCAP];
+    __shared__ float te[ER_PAIR_TCAP], ts[ER_PAIR_TCAP];
     // nsplit workgroups per insertion; part p owns the cells c with c % nsplit == p,
     // so every (insertion, molecule) sum still has a single owner
     const int nl = blockIdx.x / pa.nsplit, part = blockIdx.x % pa.nsplit;
@@ -1500,8 +1539,26 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     const int skipmol = soln ? tag[n] : -1;
     const int slt_a0 = soln ? molbeg[skipmol] : 0;
     const size_t urow = ((size_t)(n / pa.NI) * pa.nmol) * pa.NI + (n % pa.NI);
+    // the insertion's sites and the LJ tables in LDS when they fit
+    const int nt2 = pa.ntype * pa.ntype;
+    const bool in_lds = pa.nslt <= ER_PAIR_SCAP && nt2 <= ER_PAIR_TCAP;
     if (tid == 0) ncand = 0;
+    if (in_lds) {
+        for (int k = tid; k < pa.nslt; k += NT) {
+            sx[k] = insposn[(size_t)n * pa.nslt + k];
+            stj[k] = soln ? ljtype[slt_a0 + k] : sltlj[k];
+        }
+        for (int k = tid; k < nt2; k += NT) {
+            te[k] = ljene[k];
+            ts[k] = ljlensq[k];
+        }
+    }
     __syncthreads();
+    auto pair_sum = [&](int j) {
+        return in_lds ? molecule_pair_sum_lds(pa, fc, j, molbeg, posn, ljtype, sx, stj, te, ts)
+                      : molecule_pair_sum(pa, fc, n, j, molbeg, posn, ljtype, insposn, sltlj, slt_a0, soln, ljene,
+                                          ljlensq);
+    };
     int cx, cy, cz;
     site_cell(fc, ref.x, ref.y, ref.z, pa.nc, cx, cy, cz);
     // a site within distance d of the reference site lies at most
@@ -1514,19 +1571,29 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
         const int dx = c % span, dy = (c / span) % span, dz = c / (span * span);
         const int qx = pmod(cx + lo + dx, pa.nc), qy = pmod(cy + lo + dy, pa.nc), qz = pmod(cz + lo + dz, pa.nc);
         const unsigned int key = cell_key(qx, qy, qz);
-        for (unsigned int p = cstart[key]; p < cstart[key + 1]; ++p) {
-            const int j = perm[p];
-            if (j == skipmol) continue;
-            const float4 m = molref[j];
-            float ddx = m.x - ref.x, ddy = m.y - ref.y, ddz = m.z - ref.z;
-            min_image(fc, ddx, ddy, ddz);
-            const float lim = cut + m.w;
-            if (ddx * ddx + ddy * ddy + ddz * ddz <= lim * lim) {
-                const int slot = atomicAdd(&ncand, 1);
-                if (slot < ER_PAIR_CAP) cand[slot] = j;
-                else  // list full: evaluate in place (same arithmetic, same single owner)
-                    U[urow + (size_t)j * pa.NI] += molecule_pair_sum(pa, fc, n, j, molbeg, posn, ljtype, insposn,
-                                                                     sltlj, slt_a0, soln, ljene, ljlensq);
+        const unsigned int p0 = cstart[key], p1 = cstart[key + 1];
+        // the cell's molecules four at a time: indices, then reference sites
+        for (unsigned int pb = p0; pb < p1; pb += 4) {
+            int jk[4];
+            float4 mk[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) jk[k] = (pb + k < p1) ? perm[pb + k] : -1;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) mk[k] = (jk[k] >= 0) ? molref[jk[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const int j = jk[k];
+                if (j < 0 || j == skipmol) continue;
+                const float4 m = mk[k];
+                float ddx = m.x - ref.x, ddy = m.y - ref.y, ddz = m.z - ref.z;
+                min_image(fc, ddx, ddy, ddz);
+                const float lim = cut + m.w;
+                if (ddx * ddx + ddy * ddy + ddz * ddz <= lim * lim) {
+                    const int slot = atomicAdd(&ncand, 1);
+                    if (slot < ER_PAIR_CAP) cand[slot] = j;
+                    else  // list full: evaluate in place (same arithmetic, same single owner)
+                        U[urow + (size_t)j * pa.NI] += pair_sum(j);
+                }
             }
         }
     }
@@ -1534,8 +1601,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     const int nc = min(ncand, ER_PAIR_CAP);
     for (int c = tid; c < nc; c += NT) {
         const int jm = cand[c];
-        U[urow + (size_t)jm * pa.NI] += molecule_pair_sum(pa, fc, n, jm, molbeg, posn, ljtype, insposn, sltlj,
-                                                          slt_a0, soln, ljene, ljlensq);
+        U[urow + (size_t)jm * pa.NI] += pair_sum(jm);
     }
 }
 
