@@ -181,6 +181,7 @@ struct er_ctx {
 
     // energies / histograms
     DBuf<float> d_meshf, d_resself;
+    DBuf<double> d_wpart;  // k_weights block partials
     DBuf<unsigned short> d_blut;  // bin brackets per species (k_hist)
     DBuf<double> d_flpart;
     int nslice = 1;
@@ -1165,6 +1166,7 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     HIPCHK(c, c->S().d_u0.ensure(nins));
     HIPCHK(c, c->d_C.ensure((size_t)nins * c->ermax));
     HIPCHK(c, c->d_w.ensure(nins));
+    HIPCHK(c, c->d_wpart.ensure((size_t)4 * std::max(1, (nins + 255) / 256)));
     HIPCHK(c, c->d_perm.ensure(c->nummol));
     HIPCHK(c, c->d_keyrank.ensure(2 * (size_t)c->nummol));
     HIPCHK(c, c->d_ccnt.ensure(c->ncell));
@@ -1260,11 +1262,18 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         wa.screen = c->prm.screen;
         wa.qtot = c->qtot;
         wa.temp = c->prm.temp;
-        hipLaunchKernelGGL(k_weights, dim3(1), dim3(256), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
-                           c->d_resself.p, c->S().wslt_given ? c->S().d_wslt.p : (const float *)nullptr, c->S().d_state.p,
-                           c->S().d_flags.p, c->S().d_fstate.p, c->S().d_u0.p, c->d_w.p, c->d_escrd.p, c->S().d_eself.p, c->S().d_minmax.p,
+        const int nblk = std::max(1, (nch + 255) / 256);
+        hipLaunchKernelGGL(k_weights_init, dim3(1), dim3(64), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
+                           c->d_resself.p, c->S().d_state.p, c->S().d_flags.p, c->S().d_fstate.p);
+        hipLaunchKernelGGL(k_weights, dim3(nblk), dim3(256), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
+                           c->d_resself.p, c->S().wslt_given ? c->S().d_wslt.p : (const float *)nullptr,
+                           c->S().d_state.p, c->S().d_fstate.p, c->S().d_u0.p, c->d_w.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
-                           c->S().d_slnuv.p, c->S().d_errkey.p, c->nummol);
+                           c->d_wpart.p);
+        hipLaunchKernelGGL(k_weights_fin, dim3(1), dim3(256), 0, c->stream, wa, nblk, c->d_wpart.p, c->S().d_state.p,
+                           c->S().d_u0.p, c->d_w.p, c->d_escrd.p, c->S().d_eself.p, c->S().d_minmax.p,
+                           soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
+                           c->S().d_slnuv.p, c->S().d_errkey.p);
     }
     // insertion slices of at most kper (a multiple of 4) for the split-K sums
     const int kper = std::max(4, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);

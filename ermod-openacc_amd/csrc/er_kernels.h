@@ -1786,49 +1786,59 @@ struct WeightArgs {
     double temp;
 };
 
+// uvengy0 of insertion n: uvrecp (FP64 half-plane partials) + pairep +
+// residual, in the reference's real arithmetic
+__device__ __forceinline__ float weights_uv0(const WeightArgs &wa, const double *__restrict__ selfpart,
+                                             const float *__restrict__ resself, int n, float real)
+{
+    double rec = 0.0;
+    for (int m = 0; m < wa.H; ++m) rec += selfpart[(size_t)n * wa.H + m];
+    return (float)rec + real + resself[n];
+}
+
+// first run of a block: the rigid solute's real self term and voffset
+__global__ void k_weights_init(WeightArgs wa, const double *__restrict__ selfpart,
+                               const float *__restrict__ selfreal, const float *__restrict__ resself,
+                               double *__restrict__ state, int *__restrict__ flags, float *__restrict__ fstate)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (wa.rigid && !flags[1]) {
+        fstate[0] = selfreal[wa.n_off];
+        flags[1] = 1;
+    }
+    if (wa.wgtslf && !flags[0]) {
+        const float real0 = wa.rigid ? fstate[0] : selfreal[wa.n_off];
+        state[3] = (double)weights_uv0(wa, selfpart, resself, wa.n_off, real0);
+        flags[0] = 1;
+    }
+}
+
+// per insertion (one thread each): uvengy0, the weight engnmfc
+// (engproc.F90:858-912), flceng; per block: fixed-order tree partials of
+// sum w, sum w_tot and the min/max of uvengy0 into wpart[block][4]
 __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__restrict__ selfpart,
                                                  const float *__restrict__ selfreal,
                                                  const float *__restrict__ resself, const float *__restrict__ wslt,
-                                                 double *__restrict__ state, int *__restrict__ flags,
-                                                 float *__restrict__ fstate, float *__restrict__ u0,
-                                                 double *__restrict__ wout, const double *__restrict__ escrd,
-                                                 double *__restrict__ eself, unsigned int *__restrict__ minmax,
+                                                 const double *__restrict__ state, const float *__restrict__ fstate,
+                                                 float *__restrict__ u0, double *__restrict__ wout,
                                                  const double *__restrict__ flpart, double *__restrict__ flceng,
-                                                 double *__restrict__ slnuv,
-                                                 unsigned long long *__restrict__ errkey, int nmol)
+                                                 double *__restrict__ wpart)
 {
     __shared__ double r_w[256], r_t[256];
     __shared__ float r_mn[256], r_mx[256];
     const int tid = threadIdx.x, NT = blockDim.x;
-    auto uv0_of = [&](int n, float real) -> float {
-        double rec = 0.0;
-        for (int m = 0; m < wa.H; ++m) rec += selfpart[(size_t)n * wa.H + m];
-        // uvrecp + pairep + residual in the reference's real arithmetic
-        return (float)rec + real + resself[n];
-    };
-    if (tid == 0) {
-        if (wa.rigid && !flags[1]) {
-            fstate[0] = selfreal[wa.n_off];
-            flags[1] = 1;
-        }
-        if (wa.wgtslf && !flags[0]) {
-            const float real0 = wa.rigid ? fstate[0] : selfreal[wa.n_off];
-            state[3] = (double)uv0_of(wa.n_off, real0);
-            flags[0] = 1;
-        }
-    }
-    __syncthreads();
     const double voffset = state[3];
     const float usreal = fstate[0];
     double sw = 0.0, st = 0.0;
     float mn = 3.402823466e38f, mx = -3.402823466e38f;
-    for (int nl = tid; nl < wa.nchunk; nl += NT) {
+    const int nl = blockIdx.x * NT + tid;
+    if (nl < wa.nchunk) {
         const int n = wa.n_off + nl;
         const float real = wa.rigid ? usreal : selfreal[n];
-        const float u = uv0_of(n, real);
+        const float u = weights_uv0(wa, selfpart, resself, n, real);
         u0[n] = u;
-        mn = fminf(mn, u);
-        mx = fmaxf(mx, u);
+        mn = u;
+        mx = u;
         double engnmfc = 1.0;
         if (wa.wgtslf) {
             const float factor = (float)((double)u - voffset);
@@ -1842,8 +1852,8 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
         }
         engnmfc *= (double)tw;
         wout[n] = engnmfc;
-        sw += engnmfc;
-        st += (double)tw;
+        sw = engnmfc;
+        st = (double)tw;
         if (flpart) {  // flceng(sp, n): slices summed in a fixed order
             for (int sp = 0; sp < wa.numslv; ++sp) {
                 double s = 0.0;
@@ -1852,7 +1862,6 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
             }
         }
     }
-    // ordered tree reduction (fixed order -> deterministic)
     r_w[tid] = sw;
     r_t[tid] = st;
     r_mn[tid] = mn;
@@ -1868,13 +1877,41 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
         __syncthreads();
     }
     if (tid == 0) {
-        state[0] += r_w[0];
+        wpart[4 * blockIdx.x + 0] = r_w[0];
+        wpart[4 * blockIdx.x + 1] = r_t[0];
+        wpart[4 * blockIdx.x + 2] = (double)r_mn[0];
+        wpart[4 * blockIdx.x + 3] = (double)r_mx[0];
+    }
+}
+
+// the block partials in block order, then the sequential parts: the
+// self-energy distribution and slnuv, in n as the reference adds them
+__global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, const double *__restrict__ wpart,
+                                                     double *__restrict__ state, const float *__restrict__ u0,
+                                                     const double *__restrict__ wout,
+                                                     const double *__restrict__ escrd, double *__restrict__ eself,
+                                                     unsigned int *__restrict__ minmax,
+                                                     const double *__restrict__ flpart,
+                                                     const double *__restrict__ flceng, double *__restrict__ slnuv,
+                                                     unsigned long long *__restrict__ errkey)
+{
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        double sw = 0.0, st = 0.0;
+        float mn = 3.402823466e38f, mx = -3.402823466e38f;
+        for (int b = 0; b < nblk; ++b) {
+            sw += wpart[4 * b + 0];
+            st += wpart[4 * b + 1];
+            mn = fminf(mn, (float)wpart[4 * b + 2]);
+            mx = fmaxf(mx, (float)wpart[4 * b + 3]);
+        }
+        state[0] += sw;
         state[1] += (double)wa.nchunk;
-        state[2] += r_t[0];
+        state[2] += st;
         // solute self-energy extremes (engproc.F90:914-915)
         if (wa.nchunk > 0) {
-            atomicMin(&minmax[0], f2ord(r_mn[0]));
-            atomicMax(&minmax[1], f2ord(r_mx[0]));
+            atomicMin(&minmax[0], f2ord(mn));
+            atomicMax(&minmax[1], f2ord(mx));
         }
         // self-energy distribution, sequential in n as the reference adds it
         if (wa.selfcal) {
@@ -1892,7 +1929,6 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
             }
         }
     }
-    __syncthreads();
     // slnuv += flceng * engnmfc (engproc.F90:939-941), sequential in n
     if (flpart && tid < wa.numslv) {
         double s = slnuv[tid];
