@@ -1599,9 +1599,43 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     }
     __syncthreads();
     const int nc = min(ncand, ER_PAIR_CAP);
-    for (int c = tid; c < nc; c += NT) {
-        const int jm = cand[c];
-        U[urow + (size_t)jm * pa.NI] += pair_sum(jm);
+    if (pa.nslt <= 8) {  // few sites: one thread per candidate molecule
+        for (int c = tid; c < nc; c += NT) {
+            const int jm = cand[c];
+            U[urow + (size_t)jm * pa.NI] += pair_sum(jm);
+        }
+        return;
+    }
+    // many sites: a group of GS lanes (16, 32 or 64) per candidate molecule;
+    // lane g sums the sites g, g + GS, ... over the molecule's atoms (atom
+    // order), then a fixed xor tree adds the lanes (deterministic)
+    const int GS = pa.nslt <= 16 ? 16 : (pa.nslt <= 32 ? 32 : 64);
+    const int cpw = 64 / GS, lane = tid & 63, wave = tid >> 6, nwave = NT / 64;
+    const int grp = lane / GS, gl = lane % GS;
+    auto site_sum = [&](int s, int jm) {
+        const float4 xa = in_lds ? sx[s] : insposn[(size_t)n * pa.nslt + s];
+        const int ti = in_lds ? stj[s] : (soln ? ljtype[slt_a0 + s] : sltlj[s]);
+        float e = 0.0f;
+        for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
+            const float4 xb = posn[b];
+            const int tj = ljtype[b];
+            float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
+            min_image(fc, dx, dy, dz);
+            const float le = in_lds ? te[ti + pa.ntype * tj] : ljene[ti + pa.ntype * tj];
+            const float ls = in_lds ? ts[ti + pa.ntype * tj] : ljlensq[ti + pa.ntype * tj];
+            e += pair_energy(pa.pp, dx * dx + dy * dy + dz * dz, xa.w, xb.w, le, ls);
+        }
+        return e;
+    };
+    for (int c0 = wave * cpw; c0 < nc; c0 += nwave * cpw) {  // wave-uniform
+        const int c = c0 + grp;
+        const bool live = c < nc;
+        const int jm = live ? cand[c] : 0;
+        float e = 0.0f;
+        if (live)
+            for (int st = gl; st < pa.nslt; st += GS) e += site_sum(st, jm);
+        for (int off = GS / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
+        if (live && gl == 0) U[urow + (size_t)jm * pa.NI] += e;
     }
 }
 
