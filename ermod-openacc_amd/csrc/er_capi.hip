@@ -1213,8 +1213,10 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
     {
         ProfScope ps(c, ER_K_PAIRS);
         PairArgs pa{};
-        // enough workgroups per insertion to fill the chip (at least 4 per CU)
+        // enough workgroups per insertion to fill the chip (at least 4 per CU);
+        // a many-site solute reaches far, so its candidates are spread over 16
         pa.nsplit = std::max(1, std::min(16, (4 * 256 + nch - 1) / nch));
+        if (c->S().nslt > 8) pa.nsplit = 16;
         pa.nmol = c->nummol;
         pa.nchunk = nch;
         pa.n_off = n_off;

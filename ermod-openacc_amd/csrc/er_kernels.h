@@ -1387,6 +1387,16 @@ __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const i
 // cells within reach; each (insertion, molecule) sum is owned by one thread
 // and added once to U (deterministic).
 // ---------------------------------------------------------------------------
+// squared distance beyond which pair_energy is exactly zero: both cutoffs
+// test sqrtf(dis2) against the cutoff, so pairs with dis2 above the larger
+// cutoff squared (with a relative margin for the rounding of sqrtf and of
+// the square) contribute +0 and are skipped
+__device__ __forceinline__ float pair_skip2(const PairParams &p)
+{
+    const float c = fmaxf(p.upljcut, p.elecut) * 1.0001f;
+    return c * c;
+}
+
 __device__ __forceinline__ float pair_energy(const PairParams &p, float dis2, float qa, float qb, float ljeps,
                                              float ljsgm2)
 {
@@ -1467,9 +1477,10 @@ __device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const Fra
             const float4 xb = posn[b];
             float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
             min_image(fc, dx, dy, dz);
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > pair_skip2(pa.pp)) continue;
             const int tj = ljtype[b];
-            e += pair_energy(pa.pp, dx * dx + dy * dy + dz * dz, xa.w, xb.w, ljene[ti + pa.ntype * tj],
-                             ljlensq[ti + pa.ntype * tj]);
+            e += pair_energy(pa.pp, d2, xa.w, xb.w, ljene[ti + pa.ntype * tj], ljlensq[ti + pa.ntype * tj]);
         }
     }
     return e;
@@ -1496,11 +1507,13 @@ __device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const
         tb4[k] = (b0 + k < b1) ? ljtype[b0 + k] : 0;
     }
     float e = 0.0f;
+    const float skip2 = pair_skip2(pa.pp);
     auto term = [&](const float4 &xa, int ti, const float4 &xb, int tj) {
         float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
         min_image(fc, dx, dy, dz);
-        e += pair_energy(pa.pp, dx * dx + dy * dy + dz * dz, xa.w, xb.w, te[ti + pa.ntype * tj],
-                         ts[ti + pa.ntype * tj]);
+        const float d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > skip2) return;
+        e += pair_energy(pa.pp, d2, xa.w, xb.w, te[ti + pa.ntype * tj], ts[ti + pa.ntype * tj]);
     };
     for (int s = 0; s < pa.nslt; ++s) {
         const float4 xa = sx[s];
@@ -1612,18 +1625,30 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     const int GS = pa.nslt <= 16 ? 16 : (pa.nslt <= 32 ? 32 : 64);
     const int cpw = 64 / GS, lane = tid & 63, wave = tid >> 6, nwave = NT / 64;
     const int grp = lane / GS, gl = lane % GS;
+    const float skip2 = pair_skip2(pa.pp);
+    const float skipc = fmaxf(pa.pp.upljcut, pa.pp.elecut) * 1.0001f + 1e-3f;
     auto site_sum = [&](int s, int jm) {
         const float4 xa = in_lds ? sx[s] : insposn[(size_t)n * pa.nslt + s];
         const int ti = in_lds ? stj[s] : (soln ? ljtype[slt_a0 + s] : sltlj[s]);
         float e = 0.0f;
+        // the molecule's reference site and reach radius: beyond cutoff +
+        // reach, every site pair is beyond both cutoffs (triangle inequality
+        // on minimum images), so the site's sum is +0
+        const float4 m = molref[jm];
+        float rx = xa.x - m.x, ry = xa.y - m.y, rz = xa.z - m.z;
+        min_image(fc, rx, ry, rz);
+        const float lim = skipc + m.w;
+        if (rx * rx + ry * ry + rz * rz > lim * lim) return e;
         for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
             const float4 xb = posn[b];
             const int tj = ljtype[b];
             float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
             min_image(fc, dx, dy, dz);
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 > skip2) continue;
             const float le = in_lds ? te[ti + pa.ntype * tj] : ljene[ti + pa.ntype * tj];
             const float ls = in_lds ? ts[ti + pa.ntype * tj] : ljlensq[ti + pa.ntype * tj];
-            e += pair_energy(pa.pp, dx * dx + dy * dy + dz * dz, xa.w, xb.w, le, ls);
+            e += pair_energy(pa.pp, d2, xa.w, xb.w, le, ls);
         }
         return e;
     };
