@@ -471,6 +471,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
     for (int k = tid; k < K2; k += NT) t2[k] = tw2[k];
     for (int k = tid; k < K3; k += NT) t3[k] = tw3[k];
     const int per = H + K2 + K3;
+    double sp = 0.0;  // this thread's share of the FP64 half-plane self energy
 
     if constexpr (!SPARSE) {
         // ---- separable generation, sites in LDS chunks ----
@@ -501,10 +502,19 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
             }
             __syncthreads();
         }
+        // engfac multiply and FP64 half-plane self energy on the registers,
+        // then one store of the plane
+        const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
+        const float *ef = engfac + (size_t)m1 * NE;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
             const int e = tid + k * NT;
-            if (e < NE) plane[(e % K2) * PS + e / K2] = acc[k];
+            if (e < NE) {
+                const float f = ef[e];  // [m1][m3][m2]
+                const float2 a = acc[k];
+                sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+                plane[(e % K2) * PS + e / K2] = erfft::cscale(a, f);
+            }
         }
     } else {
         // ---- sparse spread of the plane, then forward DFT along axes 3, 2 ----
@@ -528,10 +538,9 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
         __syncthreads();
         plane_dft<K2, K3, R1, R2, C1, C2, NT, -1>(plane, t2, t3, tws, tid);
     }
-    __syncthreads();
-    // ---- engfac multiply + FP64 half-plane self energy ----
-    double sp = 0.0;
-    {
+    if constexpr (SPARSE) {
+        __syncthreads();
+        // ---- engfac multiply + FP64 half-plane self energy ----
         const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
         const float *ef = engfac + (size_t)m1 * NE;
 #pragma unroll
