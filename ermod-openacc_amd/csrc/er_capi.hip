@@ -974,18 +974,18 @@ template <int K1, int NI> static hipError_t launch_axis1_k(er_ctx *c, int nch)
                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
                 attr = true;
             }
-            const size_t nblk = (size_t)(c->K2 / RL) * c->K3 * ntile;  // one flat, XCD-remapped range
+            const size_t nblk = (size_t)(c->K3 / RL) * c->K2 * ntile;  // one flat, XCD-remapped range
             hipLaunchKernelGGL((k_pme_axis1_t<K1, RL>), dim3((unsigned)(8 * ((nblk + 7) / 8))), dim3(RL * 16), lds,
                                c->stream, nch, c->K2, c->K3, c->d_T.p, c->d_phi.p, c->d_tw1.p, c->d_twm.p);
             return hipGetLastError();
         };
-        if (c->K2 % 8 == 0) return go(std::integral_constant<int, 8>{});
-        if (c->K2 % 4 == 0) return go(std::integral_constant<int, 4>{});
+        if (c->K3 % 8 == 0) return go(std::integral_constant<int, 8>{});
+        if (c->K3 % 4 == 0) return go(std::integral_constant<int, 4>{});
     }
-    const int R = divisor_le(c->K2, 256 / NI);  // x2 lines per block
+    const int R = divisor_le(c->K3, 256 / NI);  // x3 lines per block
     const int nt = ((R * NI + 63) / 64) * 64;
     const int ntile = (nch + NI - 1) / NI;
-    hipLaunchKernelGGL((k_pme_axis1<K1, NI>), dim3(c->K2 / R, c->K3, ntile), dim3(nt), 0, c->stream, nch, c->K2,
+    hipLaunchKernelGGL((k_pme_axis1<K1, NI>), dim3(c->K3 / R, c->K2, ntile), dim3(nt), 0, c->stream, nch, c->K2,
                        c->K3, R, c->d_T.p, c->d_phi.p, c->d_tw1.p, c->d_twm.p);
     return hipGetLastError();
 }

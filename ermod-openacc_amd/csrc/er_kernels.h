@@ -272,7 +272,7 @@ __global__ void k_engfac(int K1, int K2, int K3, int ord, const double *__restri
 // split N = R1*R2 of each line: pass A runs R2-strided R1-point DFTs and the
 // W_N^{n2 k1} twiddle, pass B contiguous R2-point DFTs; every pass keeps all
 // 256 threads busy on small register DFTs.
-// T layout: [chunk][H][K3][K2] (x2 fastest).
+// T layout: [chunk][H][K2][K3] (x3 fastest).
 // ---------------------------------------------------------------------------
 #define ER_SCH 8  // solute sites per LDS table chunk
 
@@ -320,7 +320,8 @@ __global__ void k_ins_factors(int nrows, int ord, int K1, int K2, int K3, int H,
 // (e^{+i}) transform, -1 the forward one; both unnormalised.
 template <int K2, int K3, int R1, int R2, int C1, int C2, int NT, int SIGN>
 __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const float2 *t3,
-                                          const float2 *__restrict__ tws, int tid)
+                                          const float2 *__restrict__ tws, int tid,
+                                          float2 *__restrict__ gout = nullptr)
 {
     constexpr int PS = K3 + 1;
     // ---- rows (axis 3), pass A in place: each thread's items are loaded
@@ -418,6 +419,20 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
                 erfft::fft<C2, SIGN>(hold[j], tws + tws_off(C2));
             }
         }
+        if (gout) {
+            // results straight to global memory as [x2][x3] (x3 fastest:
+            // consecutive lanes write consecutive x3)
+#pragma unroll
+            for (int j = 0; j < IC; ++j) {
+                const int it = tid + j * NT;
+                if (it < C1 * K3) {
+                    const int k1 = it / K3, x3 = it % K3;
+#pragma unroll
+                    for (int k2 = 0; k2 < C2; ++k2) gout[(size_t)(k1 + C1 * k2) * K3 + x3] = hold[j][k2];
+                }
+            }
+            return;
+        }
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < IC; ++j) {
@@ -441,7 +456,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
 //      along axes 3 and 2 (the same transform as the spread + forward FFT
 //      of recpcal.F90:446-478, done per plane).
 //   2. FP64 half-plane self-energy partial and the engfac multiply.
-//   3. inverse DFT along axes 3 and 2; T[nl][m1][x3][x2] for stage B.
+//   3. inverse DFT along axes 3 and 2; T[nl][m1][x2][x3] for stage B.
 template <int K2, int K3, int R1, int R2, int C1, int C2, int NT, bool SPARSE>
 __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
                                                     const float2 *__restrict__ Ftab,
@@ -557,10 +572,8 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
         }
     }
     __syncthreads();
-    plane_dft<K2, K3, R1, R2, C1, C2, NT, +1>(plane, t2, t3, tws, tid);
-    // ---- out: T[nl][m1][x3][x2] ----
-    float2 *out = T + ((size_t)nl * H + m1) * (size_t)NE;
-    for (int f = tid; f < NE; f += NT) out[f] = plane[(f % K2) * PS + (f / K2)];
+    // ---- inverse DFT along axes 3 and 2, written to T[nl][m1][x2][x3] ----
+    plane_dft<K2, K3, R1, R2, C1, C2, NT, +1>(plane, t2, t3, tws, tid, T + ((size_t)nl * H + m1) * (size_t)NE);
 
     for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off, 64);
     if ((tid & 63) == 0) red[tid >> 6] = sp;
@@ -587,15 +600,17 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
 {
     constexpr int M = K1 / 2;
     constexpr int H = M + 1;
-    const int tile = blockIdx.z, x3 = blockIdx.y, x2_0 = blockIdx.x * R;
+    // lines (x2 = blockIdx.y, x3 = x3_0 + r): consecutive x3 are adjacent in T
+    const int tile = blockIdx.z, x2 = blockIdx.y, x3_0 = blockIdx.x * R;
     const int tid = threadIdx.x;
     const int i = tid % NI, r = tid / NI;
     if (r >= R) return;
+    const int x3 = x3_0 + r;
     const int nl = tile * NI + i;
     const size_t plane = (size_t)K2 * K3;
     float2 z[M];
     if (nl < nchunk) {
-        const float2 *src = T + (size_t)nl * H * plane + (size_t)x3 * K2 + x2_0 + r;
+        const float2 *src = T + (size_t)nl * H * plane + (size_t)x2 * K3 + x3;
         float2 X[H];
 #pragma unroll
         for (int m = 0; m < H; ++m) X[m] = src[(size_t)m * plane];
@@ -613,7 +628,7 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
         for (int k = 0; k < M; ++k) z[k] = make_float2(0.f, 0.f);
     }
     const size_t G = plane * K1;
-    float *dst = phi + (size_t)tile * G * NI + (((size_t)x3 * K2 + x2_0 + r) * K1) * NI + i;
+    float *dst = phi + (size_t)tile * G * NI + (((size_t)x3 * K2 + x2) * K1) * NI + i;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         dst[(2 * j) * NI] = z[j].x;
@@ -621,15 +636,15 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
     }
 }
 
-// Batched form of stage B: RL lines x 16 insertions per workgroup.  Lanes
-// run along x2 first, so a wave reads RL * 8-byte runs of T; the lines are
-// transposed through LDS ([line][x1][16], rows padded by 8 floats: the
-// transposed writes hit distinct banks) and leave as one contiguous
-// RL * K1 * 16-float block of phi written with 16-byte stores.  Per-line
-// arithmetic as in k_pme_axis1.  XCD-aware order: workgroup b runs on XCD
-// b % 8, and each XCD sweeps one contiguous eighth of the (x2 group, x3,
-// tile) blocks with x2 groups fastest, so the groups sharing a 128-byte
-// line of T are fetched back to back through the same L2.
+// Batched form of stage B: RL lines (consecutive x3, one x2) x 16
+// insertions per workgroup.  Lanes run along x3 first, so a wave reads
+// RL * 8-byte runs of T; the lines are transposed through LDS ([line][x1][16],
+// rows padded by 8 floats: the transposed writes hit distinct banks) and each
+// leaves as one contiguous K1 * 16-float run of phi written with 16-byte
+// stores.  Per-line arithmetic as in k_pme_axis1.  XCD-aware order: workgroup
+// b runs on XCD b % 8, and each XCD sweeps one contiguous eighth of the
+// (x3 group, x2, tile) blocks with x3 groups fastest, so the groups sharing a
+// 128-byte line of T are fetched back to back through the same L2.
 template <int K1, int RL>
 __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int K3,
                                                           const float2 *__restrict__ T, float *__restrict__ phi,
@@ -638,19 +653,19 @@ __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int
 {
     constexpr int NI = 16, M = K1 / 2, H = M + 1, LS = K1 * NI + 8;
     extern __shared__ __align__(16) float sml[];  // [RL][LS]
-    const int ngx = K2 / RL;
-    const int total = ngx * K3 * ((nchunk + NI - 1) / NI);
+    const int ngx = K3 / RL;
+    const int total = ngx * K2 * ((nchunk + NI - 1) / NI);
     const int per = (total + 7) / 8;
     const int blk = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     if (blk >= total) return;
-    const int tile = blk / (ngx * K3), x3 = (blk / ngx) % K3, x2_0 = (blk % ngx) * RL;
+    const int tile = blk / (ngx * K2), x2 = (blk / ngx) % K2, x3_0 = (blk % ngx) * RL;
     const int tid = threadIdx.x;
     const int r = tid % RL, i = tid / RL;
     const int nl = tile * NI + i;
     const size_t plane = (size_t)K2 * K3;
     float2 z[M];
     if (nl < nchunk) {
-        const float2 *src = T + (size_t)nl * H * plane + (size_t)x3 * K2 + x2_0 + r;
+        const float2 *src = T + (size_t)nl * H * plane + (size_t)x2 * K3 + x3_0 + r;
         float2 X[H];
 #pragma unroll
         for (int m = 0; m < H; ++m) X[m] = src[(size_t)m * plane];
@@ -675,11 +690,12 @@ __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int
     }
     __syncthreads();
     const size_t G = plane * K1;
-    float4 *dst = reinterpret_cast<float4 *>(phi + (size_t)tile * G * NI + ((size_t)x3 * K2 + x2_0) * K1 * NI);
+    float4 *dst = reinterpret_cast<float4 *>(phi + (size_t)tile * G * NI);
     constexpr int PERLINE = K1 * NI / 4;
     for (int f = tid; f < RL * PERLINE; f += RL * 16) {
         const int line = f / PERLINE, rem = f % PERLINE;
-        dst[f] = *reinterpret_cast<const float4 *>(sml + line * LS + rem * 4);
+        // line (x2, x3_0 + line) is the contiguous block [x3][x2][x1][16]
+        dst[((size_t)(x3_0 + line) * K2 + x2) * PERLINE + rem] = *reinterpret_cast<const float4 *>(sml + line * LS + rem * 4);
     }
 }
 
