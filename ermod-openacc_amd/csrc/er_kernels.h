@@ -628,7 +628,7 @@ __global__ void __launch_bounds__(256) k_pme_axis1(int nchunk, int K2, int K3, i
         for (int k = 0; k < M; ++k) z[k] = make_float2(0.f, 0.f);
     }
     const size_t G = plane * K1;
-    float *dst = phi + (size_t)tile * G * NI + (((size_t)x3 * K2 + x2) * K1) * NI + i;
+    float *dst = phi + (size_t)tile * G * NI + (((size_t)x2 * K3 + x3) * K1) * NI + i;
 #pragma unroll
     for (int j = 0; j < M; ++j) {
         dst[(2 * j) * NI] = z[j].x;
@@ -690,12 +690,13 @@ __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int
     }
     __syncthreads();
     const size_t G = plane * K1;
-    float4 *dst = reinterpret_cast<float4 *>(phi + (size_t)tile * G * NI);
     constexpr int PERLINE = K1 * NI / 4;
+    // the RL lines (x2, x3_0 .. x3_0 + RL - 1) are one contiguous block of
+    // phi [x2][x3][x1][16]
+    float4 *dst = reinterpret_cast<float4 *>(phi + (size_t)tile * G * NI) + ((size_t)x2 * K3 + x3_0) * PERLINE;
     for (int f = tid; f < RL * PERLINE; f += RL * 16) {
         const int line = f / PERLINE, rem = f % PERLINE;
-        // line (x2, x3_0 + line) is the contiguous block [x3][x2][x1][16]
-        dst[((size_t)(x3_0 + line) * K2 + x2) * PERLINE + rem] = *reinterpret_cast<const float4 *>(sml + line * LS + rem * 4);
+        dst[f] = *reinterpret_cast<const float4 *>(sml + line * LS + rem * 4);
     }
 }
 
@@ -856,7 +857,7 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
         for (int c3 = 0; c3 < ORD; ++c3) {
 #pragma unroll
             for (int c2 = 0; c2 < ORD; ++c2) {
-                const size_t row = ((size_t)i3[c3] * K2 + i2[c2]) * K1;
+                const size_t row = ((size_t)i2[c2] * K3 + i3[c3]) * K1;
                 float s = 0.0f;
 #pragma unroll
                 for (int c1 = 0; c1 < ORD; ++c1) s += w1[c1] * ph[(row + i1[c1]) * NI];
@@ -1113,7 +1114,7 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
             if (r < NROW) {
                 const int rz = r / RS, ry = r % RS;
                 const int gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
-                const char *row = reinterpret_cast<const char *>(ph4 + ((size_t)gz * K2 + gy) * K1 * NV);
+                const char *row = reinterpret_cast<const char *>(ph4 + ((size_t)gy * K3 + gz) * K1 * NV);
                 if (lane < RSU)
                     __builtin_amdgcn_global_load_lds(
                         (const __attribute__((address_space(1))) void *)(row + lbyte),
@@ -1215,7 +1216,7 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
             if (r < NROW) {
                 const int rz = r / RS, ry = r % RS;
                 const int gy = wrap1(oy + ry, K2), gz = wrap1(oz + rz, K3);
-                const char *row = reinterpret_cast<const char *>(ph4 + ((size_t)gz * K2 + gy) * K1 * NV);
+                const char *row = reinterpret_cast<const char *>(ph4 + ((size_t)gy * K3 + gz) * K1 * NV);
                 if (lane < RSU)
                     __builtin_amdgcn_global_load_lds(
                         (const __attribute__((address_space(1))) void *)(row + lbyte),
