@@ -489,10 +489,17 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
     double sp = 0.0;  // this thread's share of the FP64 half-plane self energy
 
     if constexpr (!SPARSE) {
-        // ---- separable generation, sites in LDS chunks ----
-        float2 acc[EPT];
+        // ---- separable generation, sites in LDS chunks: thread t owns row
+        // m2 = t % K2 and the m3 = g, g + GR, ... of group g = t / K2, so the
+        // row factor is read once per site and the column factors are shared
+        // by the wave's lanes ----
+        constexpr int GR = NT / K2;                  // thread groups per plane (>= 1)
+        constexpr int EPG = (K3 + GR - 1) / GR;      // elements per thread
+        const int m2 = tid % K2, g = tid / K2;
+        const bool owner = g < GR;
+        float2 acc[EPG];
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) acc[k] = make_float2(0.f, 0.f);
+        for (int k = 0; k < EPG; ++k) acc[k] = make_float2(0.f, 0.f);
         for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
             const int ns = min(ER_SCH, nslt - s0);
             for (int f = tid; f < ns * (K2 + K3); f += NT) {
@@ -502,16 +509,17 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
                 else F3[s * K3 + (r - K2)] = Fs[H + r];
             }
             __syncthreads();
+            if (owner) {
+                for (int s = 0; s < ns; ++s) {
+                    const float2 B = F2[s * K2 + m2];
 #pragma unroll
-            for (int k = 0; k < EPT; ++k) {
-                const int e = tid + k * NT;
-                if (e < NE) {
-                    const int m2 = e % K2, m3 = e / K2;
-                    for (int s = 0; s < ns; ++s) {
-                        const float2 B = F2[s * K2 + m2];
-                        const float2 f3 = F3[s * K3 + m3];
-                        acc[k].x += B.x * f3.x - B.y * f3.y;
-                        acc[k].y += B.x * f3.y + B.y * f3.x;
+                    for (int k = 0; k < EPG; ++k) {
+                        const int m3 = g + k * GR;
+                        if (m3 < K3) {
+                            const float2 f3 = F3[s * K3 + m3];
+                            acc[k].x += B.x * f3.x - B.y * f3.y;
+                            acc[k].y += B.x * f3.y + B.y * f3.x;
+                        }
                     }
                 }
             }
@@ -521,14 +529,16 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
         // then one store of the plane
         const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
         const float *ef = engfac + (size_t)m1 * NE;
+        if (owner) {
 #pragma unroll
-        for (int k = 0; k < EPT; ++k) {
-            const int e = tid + k * NT;
-            if (e < NE) {
-                const float f = ef[e];  // [m1][m3][m2]
-                const float2 a = acc[k];
-                sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
-                plane[(e % K2) * PS + e / K2] = erfft::cscale(a, f);
+            for (int k = 0; k < EPG; ++k) {
+                const int m3 = g + k * GR;
+                if (m3 < K3) {
+                    const float f = ef[m3 * K2 + m2];  // [m1][m3][m2]
+                    const float2 a = acc[k];
+                    sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+                    plane[m2 * PS + m3] = erfft::cscale(a, f);
+                }
             }
         }
     } else {
