@@ -164,6 +164,18 @@ def test_noncubic_grid_and_order6():
     assert U.energy_mismatch(u0, u0o)[0] == 0
 
 
+def test_dense_bricks_read_overflow_records():
+    """8000 waters in a 62.1 A cube on a 32^3 grid: about 250 atoms per 8x8x6
+    brick, so the batched gather runs atoms beyond its 128 LDS-staged records
+    with records read from global memory; energies to the oracle bar."""
+    w = synth.refs_water(9, 8000, 62.1, 24, 1, 32)
+    p = U.workload_params(w)
+    u, u0 = _run_refs(_ctx(w, p), w, 0, capi.ER_BATCHED)
+    uo, u0o = U.oracle_refs(w, 0, p)
+    assert U.energy_mismatch(u, uo)[0] == 0
+    assert U.energy_mismatch(u0, u0o)[0] == 0
+
+
 def test_soln_config1_two_frames():
     w = synth.config1()
     p = U.workload_params(w, corrcal=1)
