@@ -360,7 +360,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         for (int j = 0; j < IB; ++j) {
             const int it = tid + j * NT;
             if (it < K2 * R1) {
-                const int m2 = it / R1, k1 = it % R1;
+                const int m2 = it % K2, k1 = it / K2;  // rows fastest: lanes one row apart, no bank conflicts
 #pragma unroll
                 for (int b = 0; b < R2; ++b) hold[j][b] = plane[m2 * PS + R2 * k1 + b];
                 erfft::fft<R2, SIGN>(hold[j], tws + tws_off(R2));
@@ -371,7 +371,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         for (int j = 0; j < IB; ++j) {
             const int it = tid + j * NT;
             if (it < K2 * R1) {
-                const int m2 = it / R1, k1 = it % R1;
+                const int m2 = it % K2, k1 = it / K2;
 #pragma unroll
                 for (int k2 = 0; k2 < R2; ++k2) plane[m2 * PS + k1 + R1 * k2] = hold[j][k2];
             }
