@@ -6,8 +6,9 @@
 ! &ene_param values, the energy-coordinate mesh and, per frame, the frame's
 ! coordinates and the insertion coordinates (layout: read_workload below).
 ! Output (current directory): engref.XX / corref.XX (engsln / corsln for
-! soln runs), weight_refs|weight_soln, aveuv.tt (soln), uvrange.tt, as engstore
-! writes them.
+! soln runs), slfeng.XX (selfcal), weight_refs|weight_soln, aveuv.tt (soln),
+! uvrange.tt, as engstore writes them, and per frame flcuv.tt (soln) or
+! progress.tt (refs), as engconst writes them (engproc.F90:308-315, 485-509).
 program erdst_host
   use iso_c_binding
   use er_capi
@@ -16,16 +17,16 @@ program erdst_host
   type(c_ptr) :: ctx
   type(er_topology) :: topo
   type(er_params) :: prm
-  integer(c_int32_t) :: numatm, nummol, ntype, numslv, nslt, nins, nframes, ermax, engdiv, mode
+  integer(c_int32_t) :: numatm, nummol, ntype, numslv, nslt, nins, nframes, ermax, engdiv, mode, esmax
   integer(c_int32_t), allocatable, target :: mol_begin(:), ljtype(:), uvspec(:), slt_lj(:), tagslt(:)
   integer(c_int32_t), allocatable :: uvmax(:), uvsoft(:)
   real(c_float), allocatable, target :: charge(:), ljene(:), ljlensq(:), slt_q(:)
   real(c_float), allocatable :: sitepos(:,:), sltpos(:,:), wslt(:), minuv(:), maxuv(:)
   real(c_float) :: cell(3,3)
   real(c_double), allocatable :: uvcrd(:), edens(:), ecorr(:), slnuv(:), avediv(:,:), escrd(:), eself(:)
-  real(c_double), allocatable :: aveuv(:,:)
+  real(c_double), allocatable :: aveuv(:,:), flceng(:,:)
   real(c_double) :: scal(4), dummy(1)
-  integer :: iu, rc, f, division, frames_per_div, k
+  integer :: iu, rc, f, division, frames_per_div, k, io_flcuv
   integer, parameter :: stdout = 6
 
   call get_command_argument(1, path)
@@ -46,7 +47,11 @@ program erdst_host
   topo%charge = c_loc(charge); topo%ljene_mat = c_loc(ljene); topo%ljlensq_mat = c_loc(ljlensq)
   rc = er_create(topo, prm, ctx)
   call check(rc, "er_create")
-  rc = er_set_mesh(ctx, uvmax, uvsoft, uvcrd, 0_c_int32_t, dummy)
+  if (esmax > 0) then
+     rc = er_set_mesh(ctx, uvmax, uvsoft, uvcrd, esmax, escrd)
+  else
+     rc = er_set_mesh(ctx, uvmax, uvsoft, uvcrd, 0_c_int32_t, dummy)
+  end if
   call check(rc, "er_set_mesh")
   if (prm%slttype == 1) then
      rc = er_set_solute_molecules(ctx, nins, tagslt)
@@ -56,7 +61,14 @@ program erdst_host
   call check(rc, "er_set_solute")
 
   allocate(edens(ermax), ecorr(int(ermax, 8) * ermax), slnuv(numslv), minuv(0:numslv), maxuv(0:numslv))
-  allocate(avediv(engdiv, 2), aveuv(engdiv, numslv))
+  allocate(avediv(engdiv, 2), aveuv(engdiv, numslv), eself(max(esmax, 1)))
+  ! engproc.F90:308-315: flcuv.tt (soln) or progress.tt (refs), status='new'
+  if (prm%slttype == 1) then
+     allocate(flceng(numslv, nins))
+     open(newunit=io_flcuv, file='flcuv.tt', status='new', action='write')
+  else
+     open(newunit=io_flcuv, file='progress.tt', status='new', action='write')
+  end if
   frames_per_div = nframes / engdiv
   do f = 1, nframes
      read(iu) sitepos
@@ -69,6 +81,7 @@ program erdst_host
      end if
      rc = er_run(ctx, mode)                                   ! engconst
      call check(rc, "er_run")
+     call frame_output(f)
      if (mod(f, frames_per_div) == 0) then
         division = f / frames_per_div
         call engstore(division)
@@ -77,6 +90,7 @@ program erdst_host
      end if
   end do
   close(iu)
+  close(io_flcuv)
   call er_destroy(ctx)
 
 contains
@@ -111,13 +125,17 @@ contains
     nins = hdr(6); nframes = hdr(7); ermax = hdr(8)
     prm%slttype = hdr(9); prm%ms = hdr(10:12); prm%splodr = hdr(13); prm%ljswitch = hdr(14)
     prm%estype = hdr(15); prm%corrcal = hdr(16); prm%wgtslf = hdr(17); prm%selfcal = hdr(18)
-    engdiv = hdr(19)
+    engdiv = hdr(19); esmax = hdr(20)
+    if (esmax > 0 .and. prm%selfcal /= 1) stop "workload: a self-energy mesh needs selfcal = 1"
+    if (esmax <= 0 .and. prm%selfcal == 1) stop "workload: selfcal = 1 needs a self-energy mesh"
     prm%cltype = 2; prm%boxshp = 1
     read(iu) rp, t
     prm%elecut = rp(1); prm%lwljcut = rp(2); prm%upljcut = rp(3); prm%screen = rp(4); prm%temp = t
     allocate(mol_begin(nummol + 1), ljtype(numatm), uvspec(nummol), charge(numatm))
     allocate(ljene(ntype * ntype), ljlensq(ntype * ntype), uvmax(numslv), uvsoft(numslv), uvcrd(ermax))
     read(iu) mol_begin, ljtype, uvspec, charge, ljene, ljlensq, uvmax, uvsoft, uvcrd
+    allocate(escrd(max(esmax, 1)))
+    if (esmax > 0) read(iu) escrd(1:esmax)
     if (prm%slttype == 1) then
        allocate(tagslt(nins))
        read(iu) tagslt
@@ -158,6 +176,39 @@ contains
     end if
   end subroutine bin_info
 
+  ! engconst's per-frame output (engproc.F90:461-509, single process): per
+  ! solute molecule of a soln frame its summed pair energy with each solvent
+  ! species, written as real like the reference's flceng; the frame number for
+  ! refs.  The workload's frames are the ones kept, so skpcnf = 1.
+  subroutine frame_output(stnum)
+    integer, intent(in) :: stnum
+    integer :: cnt
+    if (prm%slttype == 1) then
+       rc = er_get_flceng(ctx, flceng)
+       call check(rc, "er_get_flceng")
+       do cnt = 1, nins
+          if (nins == 1) then
+             write(io_flcuv, '(i9, 9999f15.5)') stnum, real(flceng(1:numslv, cnt))
+          else
+             write(io_flcuv, '(2i9, 9999f15.5)') cnt, stnum, real(flceng(1:numslv, cnt))
+          end if
+       end do
+    else
+       write(io_flcuv, *) stnum
+    end if
+    flush(io_flcuv)
+  end subroutine frame_output
+
+  ! representative_bin_selfenergy (engproc.F90:1344-1353)
+  real function selfenergy_coord(iduv)
+    integer, intent(in) :: iduv
+    if (iduv < esmax) then
+       selfenergy_coord = (escrd(iduv) + escrd(iduv + 1)) / 2.0
+    else
+       selfenergy_coord = escrd(esmax)
+    end if
+  end function selfenergy_coord
+
   ! engstore for one block (engproc.F90:520-725, single process)
   subroutine engstore(division)
     integer, intent(in) :: division
@@ -165,13 +216,14 @@ contains
     character(len=10), parameter :: numbers = '0123456789'
     real(c_double) :: engnorm, engsmpl, avslf, voffset, leftbin, middlebin, invwt
     integer :: iduv, pti, io, j, kk
-    rc = er_fetch(ctx, edens, ecorr, dummy, scal, minuv, maxuv, slnuv)
+    rc = er_fetch(ctx, edens, ecorr, eself, scal, minuv, maxuv, slnuv)
     call check(rc, "er_fetch")
     rc = er_sync(ctx)
     call check(rc, "er_sync")
     engnorm = scal(1); engsmpl = scal(2); avslf = scal(3); voffset = scal(4)
     edens(:) = edens(:) / engnorm
     if (prm%corrcal == 1) ecorr(:) = ecorr(:) / engnorm
+    if (esmax > 0) eself(1:esmax) = eself(1:esmax) / engnorm
     avslf = avslf / engnorm
     avediv(division, 1) = engnorm / engsmpl
     if (prm%slttype == 1) then
@@ -184,7 +236,7 @@ contains
        if (prm%slttype == 1) then
           open(newunit=io, file='aveuv.tt', action='write')
           do kk = 1, engdiv
-             write(io, '(i5,9999f15.5)') kk, aveuv(kk, :)
+             write(io, '(i5,9999f15.5)') kk, real(aveuv(kk, :))   ! aveuv is real (engmain.F90:339)
           end do
           close(io)
        end if
@@ -237,6 +289,13 @@ contains
           open(newunit=io, file='corref' // suffeng, form="UNFORMATTED", action='write')
        end if
        write(io) ecorr
+       close(io)
+    end if
+    if (esmax > 0) then
+       open(newunit=io, file='slfeng' // suffeng, form="FORMATTED", action='write')
+       do iduv = 1, esmax
+          write(io, '(g15.7,g25.15)') selfenergy_coord(iduv), eself(iduv)
+       end do
        close(io)
     end if
   end subroutine engstore

@@ -65,13 +65,19 @@ def oracle_soln(w, f, params, nthreads=0):
                            topo.tagslt, nthreads=nthreads)
 
 
-def oracle_distribution(w, params) -> O.Distribution:
+def oracle_distribution(w, params, escrd=None) -> O.Distribution:
     uvmax, uvsoft, uvcrd = w.uvmax_uvsoft_uvcrd()
     invcl, vol = O.inverted_cell(w.cell.astype(np.float64))
     qtot = float(np.float64(w.topo.charge32.astype(np.float64).sum()))
     return O.Distribution(params["slttype"], uvmax, uvsoft, uvcrd, params["temp"], vol, params["screen"],
                           qtot, wgtslf=params["wgtslf"], estype=params["estype"], selfcal=params["selfcal"],
-                          corrcal=params["corrcal"])
+                          corrcal=params["corrcal"], escrd=escrd)
+
+
+# Self-energy mesh for the small test solutes: their self energies (solute
+# intramolecular + Ewald self terms) lie within a few 1e-4 kcal/mol of zero,
+# so 2e-5-wide bins spread 100 insertions over tens of bins.
+SELF_MESH = np.linspace(-2.0e-3, 2.0e-3, 201)
 
 
 def species_of(topo, nins, soln_tag=None):

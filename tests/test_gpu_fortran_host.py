@@ -64,6 +64,68 @@ def test_fortran_host_two_blocks(tmp_path):
 
 
 @pytest.mark.gpu
+def test_fortran_host_selfcal_and_progress(tmp_path):
+    """selfcal = 1: slfeng.XX per block equals eself/engnorm of the same run
+    through the Python binding, written as the Python writer writes it;
+    progress.tt holds one frame number per frame."""
+    from erdst_gpu import engstore
+    w = synth.config0()
+    p = U.workload_params(w, selfcal=1)
+    frames = [0, 1, 2, 3]
+    wl = tmp_path / "workload.bin"
+    write_workload(wl, w, p, frames, engdiv=2, nins=40, escrd=U.SELF_MESH)
+    r = subprocess.run([HOST, str(wl)], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd(), escrd=U.SELF_MESH)
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    for blk in (1, 2):
+        for f in frames[2 * (blk - 1):2 * blk]:
+            ctx.set_frame(w.frame(f), w.cell)
+            ctx.set_insertions(w.insertions(f)[:40])
+            ctx.run(capi.ER_BATCHED)
+        g = ctx.fetch()
+        ctx.clear()
+        engstore.write_slfeng(tmp_path / f"py_slfeng.0{blk}", g["eself"] / g["engnorm"], U.SELF_MESH)
+        assert open(tmp_path / f"slfeng.0{blk}").read() == open(tmp_path / f"py_slfeng.0{blk}").read()
+        rows = open(tmp_path / f"slfeng.0{blk}").read().splitlines()
+        vals = np.array([float(ln[15:40]) for ln in rows])
+        assert len(rows) == U.SELF_MESH.size and abs(vals.sum() - 1.0) < 1e-12
+    assert open(tmp_path / "progress.tt").read() == "".join(engstore.progress_line(f) + "\n" for f in (1, 2, 3, 4))
+
+
+@pytest.mark.gpu
+def test_fortran_host_soln_flcuv(tmp_path):
+    """A soln run (one flexible solute in 8000 waters): flcuv.tt holds, per
+    frame, the solute's summed pair energy with the solvent, equal to the
+    Python binding's flceng written by the Python writer and to the FP64
+    oracle's sums within the summed energy tolerance."""
+    from erdst_gpu import engstore
+    w = synth.config1()
+    p = U.workload_params(w)
+    write_workload(tmp_path / "workload.bin", w, p, [0, 1])
+    r = subprocess.run([HOST, str(tmp_path / "workload.bin")], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_solute_molecules(w.topo.tagslt)
+    want = []
+    sp = U.species_of(w.topo, 1, w.topo.tagslt)[0]
+    for f in (0, 1):
+        ctx.set_frame(w.frame(f), w.cell)
+        ctx.run(capi.ER_BATCHED)
+        fl = ctx.flceng(1)
+        want += engstore.flcuv_lines(f + 1, fl)
+        uo, _ = U.oracle_soln(w, f, p)
+        ref = np.array([uo[0][sp == s + 1].sum() for s in range(w.topo.numslv)])
+        tol = U.energy_tolerance(uo[0][sp > 0]).sum()
+        assert np.all(np.abs(fl[0] - ref) <= tol), (fl[0], ref, tol)
+    got = open(tmp_path / "flcuv.tt").read().splitlines()
+    assert got == want and len(got) == 2
+
+
+@pytest.mark.gpu
 def test_fortran_host_files_feed_slvfe(tmp_path):
     """erdst (Fortran host, soln and refs runs) -> files -> slvfe merge: the
     solvation free energy equals the one from the Python binding's blocks."""

@@ -18,14 +18,14 @@ from erdst_gpu import capi, synth
 pytestmark = pytest.mark.gpu
 
 
-def _ctx(w, params, chunk=None):
+def _ctx(w, params, chunk=None, escrd=None):
     if chunk is not None:
         os.environ["ER_CHUNK"] = str(chunk)
     try:
         ctx = capi.Context(w.topo, params)
     finally:
         os.environ.pop("ER_CHUNK", None)
-    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd(), escrd=escrd)
     if params["slttype"] == 1:
         ctx.set_solute_molecules(w.topo.tagslt)
     else:
@@ -198,6 +198,32 @@ def test_soln_config1_two_frames():
     g = ctx.fetch()
     _assert_same_distribution(g, d_gpu_e)
     np.testing.assert_allclose(g["slnuv"], d_gpu_e.slnuv, rtol=1e-6)
+
+
+def test_selfcal_self_energy_distribution(cfg0):
+    """selfcal = 1: eself, binned on the GPU from its own self energies, equals
+    the oracle's binning of those energies; every sample lands in one bin
+    (sum eself = engnorm); the GPU's self energies agree with the FP64
+    oracle's, and the solvent distributions are those of a selfcal = 0 run."""
+    p = U.workload_params(cfg0, selfcal=1)
+    ctx = _ctx(cfg0, p, escrd=U.SELF_MESH)
+    ctx0 = _ctx(cfg0, U.workload_params(cfg0))
+    d = U.oracle_distribution(cfg0, p, escrd=U.SELF_MESH)
+    for f in (0, 1):
+        u, u0 = _run_refs(ctx, cfg0, f, capi.ER_BATCHED)
+        _run_refs(ctx0, cfg0, f, capi.ER_BATCHED)
+        uo, u0o = U.oracle_refs(cfg0, f, p)
+        assert U.energy_mismatch(u0, u0o)[0] == 0
+        for n in range(u.shape[0]):
+            d.update(cfg0.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+    g, g0 = ctx.fetch(), ctx0.fetch()
+    assert g["eself"].shape == (U.SELF_MESH.size,)
+    assert np.count_nonzero(g["eself"]) > 10        # the mesh resolves the self energies
+    np.testing.assert_allclose(g["eself"] / g["engnorm"], d.eself / d.d.engnorm, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["eself"].sum(), g["engnorm"], rtol=1e-12)
+    assert g["minuv"][0] == d.minuv[0] and g["maxuv"][0] == d.maxuv[0]
+    np.testing.assert_array_equal(g["edens"], g0["edens"])
+    np.testing.assert_array_equal(g["ecorr"], g0["ecorr"])
 
 
 def test_eng_min_halt_is_reported(cfg0):

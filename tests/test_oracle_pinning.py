@@ -345,3 +345,33 @@ def test_energy_below_mesh_halts_eng_min():
     d, mesh = _dist(rng)
     with pytest.raises(RuntimeError, match="eng_min"):
         d.update(np.ones(3, np.int32), np.array([0.0, -5.0, 1.0]), 0.0)   # v == mesh[0]
+
+
+def test_self_energy_histogram():
+    """selfcal: every update adds its weight to exactly one self-energy bin,
+    the one binsearch gives on escrd (erdst/engproc.F90:895-899, 1130-1140);
+    values above the top edge land in the top bin, values at or below the
+    bottom edge halt with eng_min; the solvent histograms do not change."""
+    rng = np.random.default_rng(4)
+    escrd = np.linspace(-1.0, 1.0, 21)
+    mesh = np.linspace(-5, 5, 41)
+    mk = lambda selfcal: O.Distribution(3, [41], [41], mesh, temp=0.6, volume=1000.0, screen=0.35,
+                                        total_charge=0.0, wgtslf=0, estype=2, selfcal=selfcal,
+                                        escrd=escrd if selfcal else None)
+    d, d0 = mk(1), mk(0)
+    u0s = np.concatenate([rng.uniform(-0.99, 0.99, 40), [1.0, 1.7, 25.0]])
+    for u0 in u0s:
+        row = rng.uniform(-4.9, 6, 20)
+        before, n0 = d.eself.copy(), d.d.engnorm
+        d.update(np.ones(20, np.int32), row, u0)
+        d0.update(np.ones(20, np.int32), row, u0)
+        changed = np.flatnonzero(d.eself != before)
+        assert changed.size == 1
+        assert changed[0] == min(O.binsearch(escrd, u0), escrd.size) - 1
+        np.testing.assert_allclose(d.eself[changed[0]] - before[changed[0]], d.d.engnorm - n0, rtol=1e-13)
+    np.testing.assert_allclose(d.eself.sum(), d.d.engnorm, rtol=1e-13)
+    assert d.eself[-1] > 0                      # 1.7 and 25 above the top edge
+    np.testing.assert_array_equal(d.edens, d0.edens)
+    assert d.minuv[0] == u0s.min() and d.maxuv[0] == 25.0
+    with pytest.raises(RuntimeError, match="eng_min"):
+        d.update(np.ones(3, np.int32), np.zeros(3), -1.0)   # at the bottom edge
