@@ -1,4 +1,5 @@
-"""ctypes binding of the C ABI in include/er_capi.h (lib/liberdst_hip.so).
+"""ctypes binding of the C ABI in include/er_capi.h and include/er_traj.h
+(lib/liberdst_hip.so).
 
 The library is loaded from this package's lib/ directory; if it is missing
 the import of this module fails loudly -- there is no CPU fallback.
@@ -21,6 +22,9 @@ EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_s
             "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_sync",
             "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
             "er_destroy", "er_profile", "er_profile_read"]
+EXPORTED_TRAJ = ["er_traj_open", "er_traj_read", "er_traj_info", "er_traj_close"]
+ER_TRAJ_EOF, ER_TRAJ_IO, ER_TRAJ_FORMAT = 100, 101, 102
+TRAJ_FLAGS = {"charmm": 1, "cell": 2, "4d": 4, "rec64": 8, "swapped": 16}
 KERNELS = ["prep", "spec2d", "axis1", "gather", "pairs", "hist", "weights", "ecorr"]
 
 
@@ -82,8 +86,79 @@ def load() -> C.CDLL:
     for name in EXPORTED:
         if name not in ("er_error_string", "er_destroy"):
             getattr(L, name).restype = C.c_int
+    L.er_traj_open.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(i32)]
+    L.er_traj_read.argtypes = [vp, i32, vp, vp]
+    L.er_traj_info.argtypes = [vp] + [C.POINTER(i32)] * 3 + [C.POINTER(C.c_double)] + [C.POINTER(i32)] * 2
+    L.er_traj_close.argtypes = [vp]
+    L.er_traj_close.restype = None
+    for name in ("er_traj_open", "er_traj_read", "er_traj_info"):
+        getattr(L, name).restype = C.c_int
     _lib = L
     return L
+
+
+class TrajError(RuntimeError):
+    def __init__(self, status, what):
+        kind = {ER_E_ARG: "argument", ER_TRAJ_IO: "read", ER_TRAJ_FORMAT: "format"}.get(status, "status")
+        super().__init__(f"{what}: {kind} error [{status}]")
+        self.status = status
+
+
+class Trajectory:
+    """A DCD trajectory (HISTORY or SltConf) through er_traj_* (include/er_traj.h),
+    erdst's open_trajectory / read_trajectory / close_trajectory
+    (erdst/trajectory.F90:41-96).  read() returns (coordinates [natoms, 3]
+    float32, cell [3, 3] float32 with the cell vectors a, b, c as columns), or
+    None after the last frame."""
+
+    def __init__(self, path):
+        L = load()
+        h, n = C.c_void_p(), C.c_int32()
+        rc = L.er_traj_open(os.fsencode(path), C.byref(h), C.byref(n))
+        if rc:
+            raise TrajError(rc, f"er_traj_open({path})")
+        self.h, self.natoms, self.path = h, int(n.value), path
+
+    def info(self) -> dict:
+        v = [C.c_int32() for _ in range(3)] + [C.c_double()] + [C.c_int32() for _ in range(2)]
+        rc = load().er_traj_info(self.h, *[C.byref(x) for x in v])
+        if rc:
+            raise TrajError(rc, "er_traj_info")
+        flags = v[5].value
+        return dict(nframes=v[0].value, istart=v[1].value, nsavc=v[2].value, delta=v[3].value,
+                    nfixed=v[4].value, **{k: bool(flags & b) for k, b in TRAJ_FLAGS.items()})
+
+    def read(self, natoms=None):
+        n = self.natoms if natoms is None else int(natoms)
+        xyz = np.empty((max(n, 0), 3), np.float32)
+        cell = np.empty(9, np.float32)
+        rc = load().er_traj_read(self.h, n, xyz.ctypes.data, cell.ctypes.data)
+        if rc == ER_TRAJ_EOF:
+            return None
+        if rc:
+            raise TrajError(rc, f"er_traj_read({self.path})")
+        return xyz, cell.reshape((3, 3), order="F")
+
+    def __iter__(self):
+        while (fr := self.read()) is not None:
+            yield fr
+
+    def close(self):
+        if self.h:
+            load().er_traj_close(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _ptr(a):

@@ -12,13 +12,16 @@ Layout (little-endian, no record markers):
   soln: int32 tagslt[nins] (1-based)
   float32 cell[9] (column-major)
   per frame: float32 sitepos[3*numatm] ; refs: float32 sltpos[3*nslt*nins]
+  (sitepos is left out when the host reads frames from a HISTORY DCD, sltpos
+  when it reads insertions from a SltConf DCD)
 """
 from __future__ import annotations
 
 import numpy as np
 
 
-def write_workload(path, w, params: dict, frames, engdiv: int = 1, nins: int | None = None, escrd=None):
+def write_workload(path, w, params: dict, frames, engdiv: int = 1, nins: int | None = None, escrd=None,
+                   inline_system: bool = True, inline_insertions: bool = True):
     t = w.topo
     if (escrd is not None) != (params["selfcal"] == 1):
         raise ValueError("a self-energy mesh is written exactly when selfcal = 1")
@@ -53,7 +56,8 @@ def write_workload(path, w, params: dict, frames, engdiv: int = 1, nins: int | N
             (t.tagslt + 1).astype(np.int32).tofile(fh)
         w.cell.astype(np.float32).ravel(order="F").tofile(fh)
         for f in frames:
-            w.frame(f).astype(np.float32).tofile(fh)
-            if refs:
+            if inline_system:
+                w.frame(f).astype(np.float32).tofile(fh)
+            if refs and inline_insertions:
                 w.insertions(f)[:nins].astype(np.float32).tofile(fh)
     return nins

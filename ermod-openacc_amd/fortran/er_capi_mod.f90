@@ -1,5 +1,5 @@
-! er_capi_mod.f90 -- iso_c_binding interface of include/er_capi.h for a
-! Fortran erdst host.  engconst/engstore bind the GPU hot path through this
+! er_capi_mod.f90 -- iso_c_binding interface of include/er_capi.h and
+! include/er_traj.h for a Fortran erdst host.  engconst/engstore bind the GPU hot path through this
 ! module; every argument is an engmain array or scalar passed as is
 ! (1-based indices, column-major arrays).
 module er_capi
@@ -10,6 +10,7 @@ module er_capi
   integer(c_int), parameter, public :: ER_OK = 0, ER_E_ARG = 1, ER_E_UNSUPPORTED = 2, &
        ER_E_HIP = 3, ER_E_HALT = 4, ER_E_STATE = 5
   integer(c_int), parameter, public :: ER_PLAIN = 0, ER_BATCHED = 1
+  integer(c_int), parameter, public :: ER_TRAJ_EOF = 100, ER_TRAJ_IO = 101, ER_TRAJ_FORMAT = 102
 
   type, bind(C), public :: er_topology
      integer(c_int32_t) :: numatm, nummol, ljtype_max, numslv
@@ -26,7 +27,7 @@ module er_capi
 
   public :: er_create, er_set_mesh, er_set_solute, er_set_solutes, er_select_solute, er_set_solute_molecules, &
        er_set_frame, er_set_insertions, er_run, er_sync, er_get_uvengy, er_fetch, &
-       er_get_flceng, er_clear, er_last_error, er_destroy
+       er_get_flceng, er_clear, er_last_error, er_destroy, er_traj_open, er_traj_read, er_traj_close
 
   interface
      integer(c_int) function er_create(topo, prm, ctx) bind(C, name="er_create")
@@ -141,5 +142,26 @@ module er_capi
        import :: c_ptr
        type(c_ptr), value :: ctx
      end subroutine er_destroy
+
+     ! open_trajectory / read_trajectory / close_trajectory (trajectory.F90:41-96)
+     ! for DCD files; path is NUL-terminated (trim(name) // c_null_char)
+     integer(c_int) function er_traj_open(path, traj, natoms) bind(C, name="er_traj_open")
+       import :: c_int, c_int32_t, c_char, c_ptr
+       character(kind=c_char), intent(in) :: path(*)
+       type(c_ptr), intent(out) :: traj
+       integer(c_int32_t), intent(out) :: natoms
+     end function er_traj_open
+
+     integer(c_int) function er_traj_read(traj, natoms, xyz, cell) bind(C, name="er_traj_read")
+       import :: c_int, c_int32_t, c_float, c_ptr
+       type(c_ptr), value :: traj
+       integer(c_int32_t), value :: natoms
+       real(c_float), intent(out) :: xyz(*), cell(*)
+     end function er_traj_read
+
+     subroutine er_traj_close(traj) bind(C, name="er_traj_close")
+       import :: c_ptr
+       type(c_ptr), value :: traj
+     end subroutine er_traj_close
   end interface
 end module er_capi

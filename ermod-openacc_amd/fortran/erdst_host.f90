@@ -5,6 +5,12 @@
 ! Input: one stream file (argument 1) holding the engmain topology, the
 ! &ene_param values, the energy-coordinate mesh and, per frame, the frame's
 ! coordinates and the insertion coordinates (layout: read_workload below).
+! Optional arguments: "plain" (one insertion per launch sequence);
+! "history=FILE" takes each frame's coordinates and cell from a DCD file
+! instead, as erdst reads HISTORY (setconf.F90:103-105: a failed read halts
+! with set_trj); "sltconf=FILE" takes the refs insertions from a DCD file of
+! one solute configuration per frame, read in turn and reopened at its end
+! as erdst reads SltConf (setconf.F90:106-116).
 ! Output (current directory): engref.XX / corref.XX (engsln / corsln for
 ! soln runs), slfeng.XX (selfcal), weight_refs|weight_soln, aveuv.tt (soln),
 ! uvrange.tt, as engstore writes them, and per frame flcuv.tt (soln) or
@@ -13,8 +19,8 @@ program erdst_host
   use iso_c_binding
   use er_capi
   implicit none
-  character(len=512) :: path
-  type(c_ptr) :: ctx
+  character(len=512) :: path, history, sltconf
+  type(c_ptr) :: ctx, htraj, hslt
   type(er_topology) :: topo
   type(er_params) :: prm
   integer(c_int32_t) :: numatm, nummol, ntype, numslv, nslt, nins, nframes, ermax, engdiv, mode, esmax
@@ -26,21 +32,35 @@ program erdst_host
   real(c_double), allocatable :: uvcrd(:), edens(:), ecorr(:), slnuv(:), avediv(:,:), escrd(:), eself(:)
   real(c_double), allocatable :: aveuv(:,:), flceng(:,:)
   real(c_double) :: scal(4), dummy(1)
-  integer :: iu, rc, f, division, frames_per_div, k, io_flcuv
+  integer :: iu, rc, f, division, frames_per_div, k, io_flcuv, narg
   integer, parameter :: stdout = 6
 
   call get_command_argument(1, path)
   if (len_trim(path) == 0) stop "usage: erdst_host workload.bin [plain]"
   mode = ER_BATCHED
-  if (command_argument_count() >= 2) then
+  history = ""; sltconf = ""
+  do narg = 2, command_argument_count()
      block
-       character(len=16) :: arg
-       call get_command_argument(2, arg)
-       if (trim(arg) == "plain") mode = ER_PLAIN
+       character(len=600) :: arg
+       call get_command_argument(narg, arg)
+       if (trim(arg) == "plain") then
+          mode = ER_PLAIN
+       else if (arg(1:8) == "history=") then
+          history = arg(9:)
+       else if (arg(1:8) == "sltconf=") then
+          sltconf = arg(9:)
+       else
+          stop "usage: erdst_host workload.bin [plain] [history=FILE] [sltconf=FILE]"
+       end if
      end block
-  end if
+  end do
   open(newunit=iu, file=trim(path), access="stream", form="unformatted", status="old", action="read")
   call read_workload()
+  if (len_trim(history) > 0) call open_dcd(history, numatm, htraj)
+  if (len_trim(sltconf) > 0) then
+     if (prm%slttype == 1) stop "sltconf= applies to refs runs"
+     call open_dcd(sltconf, nslt, hslt)
+  end if
 
   topo%numatm = numatm; topo%nummol = nummol; topo%ljtype_max = ntype; topo%numslv = numslv
   topo%mol_begin_index = c_loc(mol_begin); topo%ljtype = c_loc(ljtype); topo%uvspec = c_loc(uvspec)
@@ -71,11 +91,20 @@ program erdst_host
   end if
   frames_per_div = nframes / engdiv
   do f = 1, nframes
-     read(iu) sitepos
+     if (len_trim(history) > 0) then
+        rc = er_traj_read(htraj, numatm, sitepos, cell)
+        if (rc /= ER_OK) stop "set_trj: HISTORY has no further frame"
+     else
+        read(iu) sitepos
+     end if
      rc = er_set_frame(ctx, sitepos, cell, 1.0_c_float)
      call check(rc, "er_set_frame")
      if (prm%slttype /= 1) then
-        read(iu) sltpos
+        if (len_trim(sltconf) > 0) then
+           call read_sltconf()
+        else
+           read(iu) sltpos
+        end if
         rc = er_set_insertions(ctx, nins, sltpos, wslt)
         call check(rc, "er_set_insertions")
      end if
@@ -91,9 +120,36 @@ program erdst_host
   end do
   close(iu)
   close(io_flcuv)
+  if (len_trim(history) > 0) call er_traj_close(htraj)
+  if (len_trim(sltconf) > 0) call er_traj_close(hslt)
   call er_destroy(ctx)
 
 contains
+
+  subroutine open_dcd(name, natoms_expected, h)
+    character(len=*), intent(in) :: name
+    integer(c_int32_t), intent(in) :: natoms_expected
+    type(c_ptr), intent(out) :: h
+    integer(c_int32_t) :: n
+    if (er_traj_open(trim(name) // c_null_char, h, n) /= ER_OK) &
+         stop "unable to open trajectory"
+    if (n /= natoms_expected) stop "# of atoms in trajectory does not match with # of atoms in configurations"
+  end subroutine open_dcd
+
+  ! one solute configuration per insertion, reopening SltConf at its end
+  subroutine read_sltconf()
+    integer :: n
+    real(c_float) :: scell(9)
+    do n = 0, nins - 1
+       rc = er_traj_read(hslt, nslt, sltpos(:, n * nslt + 1:), scell)
+       if (rc /= ER_OK) then
+          call er_traj_close(hslt)
+          call open_dcd(sltconf, nslt, hslt)
+          rc = er_traj_read(hslt, nslt, sltpos(:, n * nslt + 1:), scell)
+          if (rc /= ER_OK) stop "Failed to reload solute trajectory"
+       end if
+    end do
+  end subroutine read_sltconf
 
   subroutine check(rc, what)
     integer, intent(in) :: rc

@@ -126,6 +126,58 @@ def test_fortran_host_soln_flcuv(tmp_path):
 
 
 @pytest.mark.gpu
+def test_fortran_host_reads_history_and_sltconf(tmp_path):
+    """Frames from a HISTORY DCD and insertions from a SltConf DCD (one solute
+    configuration per DCD frame, 3 frames' worth, so the fourth frame's
+    insertions wrap around to the file's start as erdst's OUTconfig does):
+    the engref/corref/weight files equal those of the same inputs through the
+    Python binding, and the DCD coordinates are the workload's float32
+    coordinates exactly."""
+    from dcdfiles import write_dcd
+    w = synth.config0()
+    p = U.workload_params(w)
+    frames, nins = [0, 1, 2, 3], 40
+    L = np.diag(w.cell).astype(float)
+    write_dcd(tmp_path / "HISTORY", [w.frame(f) for f in frames], [(L[0], 90.0, L[1], 90.0, 90.0, L[2])] * 4)
+    confs = [c for f in frames[:3] for c in w.insertions(f)[:nins]]
+    write_dcd(tmp_path / "SltConf", confs, None, charmm=False)
+    write_workload(tmp_path / "workload.bin", w, p, frames, engdiv=2, nins=nins, inline_system=False,
+                   inline_insertions=False)
+    r = subprocess.run([HOST, "workload.bin", "history=HISTORY", "sltconf=SltConf"], cwd=tmp_path,
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    hist = capi.Trajectory(str(tmp_path / "HISTORY"))
+    slt = capi.Trajectory(str(tmp_path / "SltConf"))
+    for blk in (1, 2):
+        for f in frames[2 * (blk - 1):2 * blk]:
+            xyz, cell = hist.read()
+            np.testing.assert_array_equal(xyz, w.frame(f))
+            ins = []
+            for _ in range(nins):
+                fr = slt.read()
+                if fr is None:                      # wrap around
+                    slt.close()
+                    slt = capi.Trajectory(str(tmp_path / "SltConf"))
+                    fr = slt.read()
+                ins.append(fr[0])
+            ins = np.stack(ins)
+            np.testing.assert_array_equal(ins, w.insertions(f % 3)[:nins])
+            ctx.set_frame(xyz, cell)
+            ctx.set_insertions(ins)
+            ctx.run(capi.ER_BATCHED)
+        g = ctx.fetch()
+        ctx.clear()
+        eng = read_engfile(tmp_path / f"engref.0{blk}")
+        np.testing.assert_allclose(eng[:, 3], g["edens"] / g["engnorm"], rtol=1e-13, atol=1e-300)
+        cor = read_fortran_record(tmp_path / f"corref.0{blk}").reshape(g["ecorr"].shape)
+        np.testing.assert_allclose(cor, g["ecorr"] / g["engnorm"], rtol=1e-13, atol=1e-300)
+    assert open(tmp_path / "progress.tt").read() == " 1\n 2\n 3\n 4\n"
+
+
+@pytest.mark.gpu
 def test_fortran_host_files_feed_slvfe(tmp_path):
     """erdst (Fortran host, soln and refs runs) -> files -> slvfe merge: the
     solvation free energy equals the one from the Python binding's blocks."""

@@ -14,19 +14,22 @@ from erdst_gpu.params import hot_path_params, temperature_kt
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    text = open(os.path.join(ROOT, "include", "er_capi.h")).read()
+def header_functions(name):
+    text = open(os.path.join(ROOT, "include", name)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(er_[a-z_]+)\s*\(", text)))
 
 
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(capi.LIB_PATH)          # loads without touching a device
-    names = header_functions()
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert headers == ["er_capi.h", "er_traj.h"]
+    names = header_functions("er_capi.h")
     assert "er_run" in names and "er_create" in names
-    for n in names:
+    for n in names + header_functions("er_traj.h"):
         assert hasattr(lib, n), n
     assert sorted(capi.EXPORTED) == names
+    assert sorted(capi.EXPORTED_TRAJ) == header_functions("er_traj.h")
 
 
 def test_water_and_solute_geometry():
