@@ -18,11 +18,13 @@ def temperature_kt(inptemp: float = 300.0, double_build: bool = False) -> float:
 def hot_path_params(slttype: int, ms, *, elecut=12.0, upljcut=None, lwljcut=None, screen=0.35,
                     splodr=4, ljswitch=0, estype=2, corrcal=None, wgtslf=None, selfcal=0,
                     inptemp=300.0) -> dict:
-    """Parameters after iniparam's defaulting: upljcut = elecut,
-    lwljcut = upljcut - 2 (setconf.F90:189-192); corrcal and wgtslf default
-    to YES for refs with PME, corrcal NO for soln (setconf.F90:219-230)."""
-    up = elecut if upljcut is None else upljcut
-    lw = up - 2.0 if lwljcut is None else lwljcut
+    """Parameters after iniparam's defaulting: upljcut and lwljcut default to
+    12 and 10 A, set from the default elecut before the namelist is read
+    (setconf.F90:180-188), so a user's elecut does not move them; corrcal
+    and wgtslf default to YES for refs with PME, corrcal NO for soln
+    (setconf.F90:219-230)."""
+    up = 12.0 if upljcut is None else upljcut
+    lw = 10.0 if lwljcut is None else lwljcut
     refs = slttype != SLT_SOLN
     return dict(elecut=float(elecut), lwljcut=float(lw), upljcut=float(up), screen=float(screen),
                 ljswitch=int(ljswitch), cltype=2, splodr=int(splodr), ms=tuple(int(m) for m in ms),

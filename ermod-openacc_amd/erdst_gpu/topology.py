@@ -70,13 +70,17 @@ class Topology:
 
 def build_topology(species: list[Species], solute_index: int, refs: bool,
                    ljformat: int = LJFMT_EPS_CAL_SGM_A, cmbrule: int = LJCMB_ARITH,
-                   extra_lj_species: list[Species] | None = None) -> Topology:
+                   extra_lj_species: list[Species] | None = None, lj_table=None) -> Topology:
     """setparam for species listed in MDinfo order.  For refs runs the solute
     (test particle) must be the last species (setconf.F90:598-600).
 
     extra_lj_species: further species whose LJ types are appended to the type
     table after those of `species` (not part of the system), so that several
-    test particles analysed in one context share one table."""
+    test particles analysed in one context share one table.
+
+    lj_table: ljformat 5 (setconf.F90:751-753, 794-808): (per-species 1-based
+    type numbers, sigma^2 and epsilon matrices in float32 and float64); the
+    types and matrices are taken as given."""
     if refs and solute_index != len(species) - 1:
         raise ValueError("the test particle must be the last species")
     t = Topology(species=species, solute_index=solute_index, refs=refs)
@@ -92,6 +96,19 @@ def build_topology(species: list[Species], solute_index: int, refs: bool,
             sig = (np.float32(SGMCNV) * sig).astype(np.float32)
         return list(zip(sig.tolist(), eps.tolist()))
 
+    if lj_table is not None:
+        types, lensq32, ene32, lensq64, ene64 = lj_table
+        ntype = int(np.asarray(lensq32).shape[0])
+        site_types = [np.asarray(ty, np.int32) - 1 for ty in types]
+        for ty in site_types:
+            if ty.size and (ty.min() < 0 or ty.max() >= ntype):
+                raise ValueError("LJ type outside the LJTable")
+        t.ntype = ntype
+        t.ljene32 = np.asarray(ene32, np.float32).ravel(order="F")
+        t.ljlensq32 = np.asarray(lensq32, np.float32).ravel(order="F")
+        t.ljene64 = np.asarray(ene64, np.float64).ravel(order="F")
+        t.ljlensq64 = np.asarray(lensq64, np.float64).ravel(order="F")
+        return _assemble(t, species, solute_index, refs, site_types)
     order = list(species)
     if extra_lj_species:
         order = [sp for i, sp in enumerate(species) if not (refs and i == solute_index)] + list(extra_lj_species)
@@ -118,7 +135,12 @@ def build_topology(species: list[Species], solute_index: int, refs: bool,
     # column-major [ntype, ntype]; symmetric, so ravel order is immaterial
     t.ljene32, t.ljlensq32 = ene32.ravel(order="F"), lensq32.ravel(order="F")
     t.ljene64, t.ljlensq64 = ene64.ravel(order="F"), lensq64.ravel(order="F")
+    return _assemble(t, species, solute_index, refs, site_types)
 
+
+def _assemble(t: Topology, species, solute_index, refs, site_types) -> Topology:
+    """Per-atom and per-molecule arrays of the physical system (setparam's
+    fill loop, setconf.F90:780-787, and engproc.F90:112-118 for uvspec)."""
     phys = [k for k in range(len(species)) if not (refs and k == solute_index)]
     mol_begin = [0]
     ljt, q32, q64, uvspec, moltype = [], [], [], [], []
