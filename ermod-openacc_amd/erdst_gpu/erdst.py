@@ -1,0 +1,157 @@
+"""erdst over an ERmod run directory, with the per-frame work on the GPU.
+
+    python -m erdst_gpu.erdst [DIRECTORY] [--plain] [--out OUTDIR]
+
+Reads what erdst reads (erdst/setconf.F90, erdst/engproc.F90:36-318):
+parameters_er (&ene_param, &hist), MDinfo, MolPrm<n>, SltInfo, LJTable,
+EcdInfo / EcdMesh, PermIndex, SysWght, SltWght, and the DCD trajectories
+HISTORY and SltConf; runs the engdiv x frame loop of trjmain
+(erdst/enganal.F90:82-95) through the C ABI; writes engsln|engref.XX,
+corsln|corref.XX, slfeng.XX, weight_soln|weight_refs, aveuv.tt, uvrange.tt
+and flcuv.tt|progress.tt.
+
+Supported: PME (cltype 2) in a periodic cell; soln runs; refs runs whose
+solute configurations are used as they are (insposition 1 and insorient 1):
+flexible solutes (slttype 3) from SltConf, one DCD frame per insertion,
+reopened at its end, and rigid solutes (slttype 2) at their SltInfo
+coordinates.  Random insertion (the reference's compiler-specific
+random_number stream) and region restraints raise InputError.
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import sys
+
+import numpy as np
+
+from . import capi
+from .engstore import BlockWriter
+from .inputs import (SLT_REFS_FLEX, SLT_REFS_RIGID, SLT_SOLN, InputError, WeightFile, apply_permutation,
+                     iniparam, load_system, read_namelist, read_permindex)
+from .mesh import species_meshes
+
+
+def _hot_params(p: dict) -> dict:
+    return dict(elecut=float(p["elecut"]), lwljcut=float(p["lwljcut"]), upljcut=float(p["upljcut"]),
+                screen=float(p["screen"]), ljswitch=int(p["ljswitch"]), cltype=int(p["cltype"]),
+                splodr=int(p["splodr"]), ms=(int(p["ms1max"]), int(p["ms2max"]), int(p["ms3max"])),
+                boxshp=int(p["boxshp"]), slttype=int(p["slttype"]), wgtslf=int(p["wgtslf"]),
+                estype=int(p["estype"]), corrcal=int(p["corrcal"]), selfcal=int(p["selfcal"]),
+                temp=float(p["temp"]))
+
+
+def _check_supported(p: dict):
+    if p["cltype"] != 2:
+        raise InputError("only PME (cltype = 2) runs on the GPU path")
+    if p["boxshp"] != 1:
+        raise InputError("only periodic systems (boxshp = 1) run on the GPU path")
+    if p["slttype"] != SLT_SOLN and not (p["insposition"] == 1 and p["insorient"] == 1):
+        raise InputError("refs runs need the solute configurations as they are (insposition = 1, insorient = 1)")
+    if p["slttype"] != SLT_SOLN and p["insstructure"] != 0:
+        raise InputError("structure rejection (insstructure) is not supported")
+
+
+def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log=print) -> dict:
+    """One erdst run; returns the last block's fetched accumulators."""
+    d = directory
+    out = out or d
+    path = lambda name: os.path.join(d, name)
+    p = iniparam(read_namelist(path("parameters_er"), "ene_param"))
+    _check_supported(p)
+    sysin = load_system(d, p)
+    topo = sysin.topology
+    soln = p["slttype"] == SLT_SOLN
+    hist = read_namelist(path("parameters_er"), "hist")
+    uvmax, uvsoft, uvcrd, escrd = species_meshes(
+        path("parameters_er"), topo.numslv,
+        path("EcdInfo") if os.path.exists(path("EcdInfo")) else None,
+        path("EcdMesh") if int(hist.get("meshread", 0)) == 1 else None)
+    ctx = capi.Context(topo, _hot_params(p))
+    ctx.set_mesh(uvmax, uvsoft, uvcrd, escrd if p["selfcal"] == 1 else None)
+    if soln:
+        ctx.set_solute_molecules(topo.tagslt)
+    else:
+        ctx.set_solute(topo.slt_ljtype, topo.slt_charge32)
+    writer = BlockWriter(out, uvmax, uvsoft, uvcrd, p["temp"], soln=soln, engdiv=p["engdiv"],
+                         wgtslf=bool(p["wgtslf"]), corrcal=bool(p["corrcal"]),
+                         escrd=escrd if p["selfcal"] == 1 else None)
+    maxcnf, skpcnf, engdiv = sysin.mdinfo.nrun, p["skpcnf"], p["engdiv"]
+    frames_per_div = maxcnf // skpcnf // engdiv
+    if frames_per_div <= 0:
+        raise InputError("fewer frames than blocks", "eng_par")
+    perm = read_permindex(path("PermIndex"), topo.natom) if os.path.exists(path("PermIndex")) else None
+    syswgt = WeightFile(path("SysWght"), rewind=False) if p["wgtsys"] == 1 else None
+    history = capi.Trajectory(path("HISTORY"))
+    if history.natoms != topo.natom:
+        raise InputError("# of atoms in HISTORY does not match the configurations")
+    nslt = 0 if soln else len(topo.slt_charge32)
+    nins = 1 if soln else p["maxins"]
+    sltconf = None
+    sltwgt = None
+    if p["slttype"] == SLT_REFS_FLEX:
+        sltconf = capi.Trajectory(path("SltConf"))
+        if sltconf.natoms != nslt:
+            raise InputError("# of atoms in SltConf does not match the solute")
+        if p["wgtins"] == 1:
+            sltwgt = WeightFile(path("SltWght"), rewind=True)
+    g = None
+    for division in range(1, engdiv + 1):
+        for _ in range(frames_per_div):
+            w_sys = np.float32(1.0)
+            for _ in range(skpcnf):
+                fr = history.read()
+                if fr is None:
+                    raise InputError("HISTORY has no further frame", "set_trj")
+                if syswgt is not None:
+                    w_sys = syswgt.next()
+            xyz, cell = fr
+            if perm is not None:
+                xyz = apply_permutation(xyz, perm)
+            ctx.set_frame(xyz, cell, float(w_sys))
+            if not soln:
+                ins = np.empty((nins, nslt, 3), np.float32)
+                w_slt = np.ones(nins, np.float32)
+                for n in range(nins):
+                    if p["slttype"] == SLT_REFS_RIGID:
+                        ins[n] = sysin.bfcoord
+                        continue
+                    c = sltconf.read()
+                    if c is None:                       # reopen at the end (setconf.F90:106-116)
+                        sltconf.close()
+                        sltconf = capi.Trajectory(path("SltConf"))
+                        c = sltconf.read()
+                        if c is None:
+                            raise InputError("Failed to reload solute trajectory")
+                    ins[n] = c[0]
+                    if sltwgt is not None:
+                        w_slt[n] = sltwgt.next()
+                ctx.set_insertions(ins, w_slt if sltwgt is not None else None)
+            ctx.run(mode)
+            writer.frame(ctx.flceng(len(topo.tagslt)) if soln else None, skpcnf)
+        g = ctx.fetch()
+        writer.store(g)
+        ctx.clear()
+        log(f"erdst: block {division} of {engdiv} stored")
+    history.close()
+    if sltconf is not None:
+        sltconf.close()
+    return g
+
+
+def main(argv=None) -> int:
+    ap = argparse.ArgumentParser(prog="python -m erdst_gpu.erdst", description=__doc__.split("\n\n")[0])
+    ap.add_argument("directory", nargs="?", default=".")
+    ap.add_argument("--out", default=None, help="directory for the output files (default: the run directory)")
+    ap.add_argument("--plain", action="store_true", help="one insertion per launch sequence")
+    a = ap.parse_args(argv)
+    try:
+        run(a.directory, a.out, capi.ER_PLAIN if a.plain else capi.ER_BATCHED)
+    except (InputError, capi.TrajError, capi.ErError) as e:
+        print(f"erdst: {e}", file=sys.stderr)
+        return 1
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,88 @@
+"""erdst over a complete ERmod run directory (erdst_gpu/erdst.py): parameters_er,
+MDinfo, MolPrm1, SltInfo, EcdMesh, HISTORY and SltConf, as the reference run
+of the 1000-water refs configuration is laid out (slttype 3, insertions from
+SltConf as they are).  Every output file equals the one written from the same
+inputs fed programmatically through the Python binding."""
+import os
+
+import numpy as np
+import pytest
+
+import erutil as U
+from dcdfiles import write_dcd
+from erdst_gpu import capi, erdst, synth
+from erdst_gpu.engstore import BlockWriter
+
+pytestmark = pytest.mark.gpu
+
+NINS, FRAMES = 40, 4
+
+
+def fmt(x):
+    return repr(float(np.float32(x)))
+
+
+def write_run_directory(d, w):
+    topo = w.topo
+    water, solute = topo.species[0], topo.species[topo.solute_index]
+    mesh, pecore = w.mesh[0]
+    (d / "parameters_er").write_text(f"""&ene_param
+  slttype = 3, insposition = 1, insorient = 1, maxins = {NINS},
+  ljformat = 0, inptemp = 300.0,
+  elecut = 12.0, cltype = 2, screen = 0.35, splodr = 4, ms1max = {w.params['ms'][0]},
+  selfcal = 1, engdiv = 2
+/
+&hist
+  eclbin = 5.0e-2, ecfbin = 2.0e-3, ec0bin = 2.0e-4, finfac = 10.0,
+  ecdmin = -20.0, ecfmns = -0.20, ecdcen = 0.0, eccore = 5.0,
+  ecdmax = 1.0e4, pecore = {pecore}, meshread = 1
+/
+""")
+    lines = [f"0 {U.SELF_MESH.size}"] + [f"0 {i + 1} {float(v)!r}" for i, v in enumerate(U.SELF_MESH)]
+    lines += [f"1 {len(mesh)} {pecore}"] + [f"1 {i + 1} {float(v)!r}" for i, v in enumerate(mesh)]
+    (d / "EcdMesh").write_text("\n".join(lines) + "\n")
+    (d / "MDinfo").write_text(f"{FRAMES} 1\n{water.nmol}\n{water.nsite}\n")
+    with open(d / "MolPrm1", "w") as fh:
+        for i, typ in enumerate(["OW", "HW", "HW"]):
+            fh.write(f"{i + 1} 15.9994 {typ} {typ}{i + 1} {fmt(water.charge[i])} {fmt(water.eps[i])} "
+                     f"{fmt(water.sigma[i])}\n")
+    with open(d / "SltInfo", "w") as fh:
+        for i in range(solute.nsite):
+            typ = "C" if i == 0 else "H"
+            fh.write(f"{i + 1:4d}  {typ}  {fmt(solute.charge[i])}  {fmt(solute.eps[i])}  {fmt(solute.sigma[i])}\n")
+    L = np.diag(w.cell).astype(float)
+    write_dcd(d / "HISTORY", [w.frame(f) for f in range(FRAMES)], [(L[0], 90.0, L[1], 90.0, 90.0, L[2])] * FRAMES)
+    confs = [c for f in range(3) for c in w.insertions(f)[:NINS]]
+    write_dcd(d / "SltConf", confs, None, charmm=False)
+
+
+def test_erdst_driver_over_a_run_directory(tmp_path):
+    w = synth.config0()
+    run_dir = tmp_path / "run"
+    run_dir.mkdir()
+    write_run_directory(run_dir, w)
+    erdst.run(str(run_dir), log=lambda *a: None)
+    # the same inputs through the binding, written by the same writers
+    ref_dir = tmp_path / "ref"
+    ref_dir.mkdir()
+    p = U.workload_params(w, selfcal=1)
+    ctx = capi.Context(w.topo, p)
+    uvmax, uvsoft, uvcrd = w.uvmax_uvsoft_uvcrd()
+    ctx.set_mesh(uvmax, uvsoft, uvcrd, escrd=U.SELF_MESH)
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    wr = BlockWriter(str(ref_dir), uvmax, uvsoft, uvcrd, p["temp"], soln=False, engdiv=2, escrd=U.SELF_MESH)
+    hist = capi.Trajectory(str(run_dir / "HISTORY"))
+    for blk in range(2):
+        for f in range(2 * blk, 2 * blk + 2):
+            xyz, cell = hist.read()
+            ctx.set_frame(xyz, cell)
+            ctx.set_insertions(w.insertions(f % 3)[:NINS])
+            ctx.run(capi.ER_BATCHED)
+            wr.frame()
+        wr.store(ctx.fetch())
+        ctx.clear()
+    names = sorted(os.listdir(ref_dir))
+    assert names == ["corref.01", "corref.02", "engref.01", "engref.02", "progress.tt", "slfeng.01", "slfeng.02",
+                     "uvrange.tt", "weight_refs"]
+    for n in names:
+        assert (run_dir / n).read_bytes() == (ref_dir / n).read_bytes(), n
