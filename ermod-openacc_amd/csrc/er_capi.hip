@@ -398,8 +398,10 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
         !topo->mol_begin_index || !topo->ljtype || !topo->uvspec || !topo->charge || !topo->ljene_mat ||
         !topo->ljlensq_mat)
         return bad(fail(c, ER_E_ARG, "er_create: incomplete topology"));
-    if (prm->cltype != 2)
-        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: only cltype=2 (PME) is implemented"));
+    if (prm->cltype != 2 && prm->cltype != 3)
+        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: cltype 2 (PME) and 3 (PPPM) are implemented"));
+    if (prm->cltype == 3 && prm->splodr % 2 == 1)
+        return bad(fail(c, ER_E_ARG, "er_create: interpolation order for PPPM must be even"));
     if (prm->boxshp != 1)
         return bad(fail(c, ER_E_UNSUPPORTED, "er_create: only boxshp=1 (periodic) is implemented"));
     if (prm->slttype < 1 || prm->slttype > 3) return bad(fail(c, ER_E_ARG, "er_create: slttype"));
@@ -1100,9 +1102,47 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
     return hipGetLastError();
 }
 
+// PPPM coefficients: gf_b of calc_gfb_pppm (recpcal.F90:179-199; the last
+// update of gf_b(1) sees the inner loop's exit value l = 0) and the alias
+// range of recpcal.F90:228-233, formed in single precision as there
+static PPPMCoef pppm_coef(const er_ctx *c)
+{
+    PPPMCoef pc{};
+    const int P = c->ord;
+    pc.order = P;
+    double b[8] = {1.0, 0, 0, 0, 0, 0, 0, 0};
+    for (int m = 1; m <= P - 1; ++m) {
+        for (int l = m; l >= 1; --l)
+            b[l] = 4.0 * (b[l] * (l - m) * (l - m - 0.5) - b[l - 1] * (l - m - 1) * (l - m - 1));
+        b[0] = 4.0 * (b[0] * (0 - m) * (0 - m - 0.5));
+    }
+    double fact = 1.0;
+    for (int n = 2; n <= 2 * P - 1; ++n) fact *= n;
+    for (int m = 0; m < P; ++m) pc.b[m] = b[m] / fact;
+    const float root = std::pow(-std::log(1.0e-7f), 0.25f), pi = 3.14159265358979f;
+    const float mm[3] = {c->prm.screen / (pi * c->K1) * root, c->prm.screen / (pi * c->K2) * root,
+                         c->prm.screen / (pi * c->K3) * root};
+    int a[3];
+    for (int d = 0; d < 3; ++d)
+        a[d] = (int)(c->cell[0 + 3 * d] * mm[0] + c->cell[1 + 3 * d] * mm[1] + c->cell[2 + 3 * d] * mm[2]);
+    pc.a1 = std::max(a[0], 0);
+    pc.a2 = std::max(a[1], 0);
+    pc.a3 = std::max(a[2], 0);
+    return pc;
+}
+
 static hipError_t prep_frame(er_ctx *c)
 {
     const int B = 256;
+    if (c->cell_changed && c->prm.cltype == 3) {
+        const double *i = c->invcl_d;
+        hipLaunchKernelGGL(k_engfac_pppm, dim3((unsigned)((c->Gc + B - 1) / B)), dim3(B), 0, c->stream, c->K1,
+                           c->K2, c->K3, pppm_coef(c), i[0], i[1], i[2], i[3], i[4], i[5], i[6], i[7], i[8],
+                           c->volume_d, (double)c->prm.screen, c->d_engfac.p);
+        std::memcpy(c->prev_cell, c->cell, sizeof c->cell);
+        c->have_prev_cell = true;
+        c->cell_changed = false;
+    }
     if (c->cell_changed) {
         const double *i = c->invcl_d;
         hipLaunchKernelGGL(k_engfac, dim3((unsigned)((c->Gc + B - 1) / B)), dim3(B), 0, c->stream, c->K1, c->K2, c->K3,

@@ -260,6 +260,76 @@ __global__ void k_engfac(int K1, int K2, int K3, int ord, const double *__restri
     engfac[idx] = (float)f;
 }
 
+// PPPM optimal influence function (recpcal.F90:217-337), same layout as
+// k_engfac.  For each wave vector k = 2 pi invcl^T m:
+//   4 pi / V * sum_alias (k.km / |km|^2) exp(-|km|^2 / 4 a^2) U(km)^2
+//   / (|k|^2 (s1 s2 s3)^2),  s_d = sum_l b_l sin^2(pi m_d / K_d)^l,
+// aliases km = k + 2 pi invcl^T (mx K1, my K2, mz K2) over |m.| <= amax
+// (the third alias component scales with K2, as the reference writes it).
+// Computed in FP64, stored FP32; runs only when the cell changes.
+struct PPPMCoef {
+    double b[8];      // gf_b(1:order)
+    int order, a1, a2, a3;
+};
+
+__global__ void k_engfac_pppm(int K1, int K2, int K3, PPPMCoef pc, double i00, double i10, double i20, double i01,
+                              double i11, double i21, double i02, double i12, double i22, double volume,
+                              double screen, float *__restrict__ engfac)
+{
+    const int H = K1 / 2 + 1;
+    const size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= (size_t)H * K2 * K3) return;
+    const int m2 = idx % K2, m3 = (idx / K2) % K3, m1 = idx / ((size_t)K2 * K3);
+    const double pi = 3.14159265358979323846;
+    const int rc[3] = {m1, m2, m3}, Km[3] = {K1, K2, K3};
+    double inm[3], s2[3];
+    for (int d = 0; d < 3; ++d) {
+        inm[d] = (rc[d] <= Km[d] / 2) ? (double)rc[d] : (double)(rc[d] - Km[d]);
+        const double sv = sin(pi * inm[d] / Km[d]);
+        s2[d] = sv * sv;
+    }
+    const double k0 = 2.0 * pi * (i00 * inm[0] + i10 * inm[1] + i20 * inm[2]);
+    const double k1 = 2.0 * pi * (i01 * inm[0] + i11 * inm[1] + i21 * inm[2]);
+    const double k2c = 2.0 * pi * (i02 * inm[0] + i12 * inm[1] + i22 * inm[2]);
+    const double kk = k0 * k0 + k1 * k1 + k2c * k2c;
+    double f = 0.0;
+    if (!(m1 == 0 && m2 == 0 && m3 == 0) && kk > 0.0) {
+        auto u2p = [&](double t) {  // (sin t / t)^(2 order)
+            if (t == 0.0) return 1.0;
+            const double r = sin(t) / t, r2 = r * r;
+            double u = 1.0;
+            for (int l = 0; l < pc.order; ++l) u *= r2;
+            return u;
+        };
+        const double inv4a2 = 0.25 / (screen * screen);
+        double sum = 0.0;
+        for (int mx = -pc.a1; mx <= pc.a1; ++mx) {
+            const double u1 = u2p(pi * (inm[0] / K1 + mx));
+            for (int my = -pc.a2; my <= pc.a2; ++my) {
+                const double u2 = u2p(pi * (inm[1] / K2 + my));
+                for (int mz = -pc.a3; mz <= pc.a3; ++mz) {
+                    const double u3 = u2p(pi * (inm[2] / K3 + mz));
+                    const double n0 = (double)mx * K1, n1 = (double)my * K2, n2 = (double)mz * K2;
+                    const double q0 = k0 + 2.0 * pi * (i00 * n0 + i10 * n1 + i20 * n2);
+                    const double q1 = k1 + 2.0 * pi * (i01 * n0 + i11 * n1 + i21 * n2);
+                    const double q2 = k2c + 2.0 * pi * (i02 * n0 + i12 * n1 + i22 * n2);
+                    const double g = exp(-q0 * q0 * inv4a2) * exp(-q1 * q1 * inv4a2) * exp(-q2 * q2 * inv4a2);
+                    sum += (k0 * q0 + k1 * q1 + k2c * q2) * g * (u1 * u2 * u3) / (q0 * q0 + q1 * q1 + q2 * q2);
+                }
+            }
+        }
+        double s1 = 0.0, sy = 0.0, sz = 0.0;
+        for (int l = pc.order - 1; l >= 0; --l) {
+            s1 = pc.b[l] + s1 * s2[0];
+            sy = pc.b[l] + sy * s2[1];
+            sz = pc.b[l] + sz * s2[2];
+        }
+        const double den = (s1 * sy * sz) * (s1 * sy * sz);
+        f = sum * 4.0 * pi / (kk * den) / volume;
+    }
+    engfac[idx] = (float)f;
+}
+
 // ---------------------------------------------------------------------------
 // PME, stage A: per (insertion, m1) plane.  The solute's structure factor is
 // generated directly from its separable B-spline factors,

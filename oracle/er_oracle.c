@@ -325,6 +325,112 @@ void orc_engfac(const orc_params *p, const double *invcl, double volume, double 
     free(s1); free(s2); free(s3);
 }
 
+/* calc_gfb_pppm, erdst/recpcal.F90:179-199: coefficients of the aliasing-sum
+ * polynomial in sin^2; the final update of gf_b(1) uses the inner loop's
+ * exit value of l, which is 0 */
+void orc_pppm_gfb(int splodr, double *gf_b)
+{
+    for (int l = 1; l < splodr; l++) gf_b[l] = 0.0;
+    gf_b[0] = 1.0;
+    for (int m = 1; m <= splodr - 1; m++) {
+        for (int l = m; l >= 1; l--)
+            gf_b[l] = 4.0 * (gf_b[l] * (l - m) * (l - m - 0.5) - gf_b[l - 1] * (l - m - 1) * (l - m - 1));
+        gf_b[0] = 4.0 * (gf_b[0] * (0 - m) * (0 - m - 0.5));
+    }
+    double fact = 1.0;
+    for (int n = 2; n <= 2 * splodr - 1; n++) fact *= n;
+    for (int m = 0; m < splodr; m++) gf_b[m] /= fact;
+}
+
+/* gf_denom, erdst/recpcal.F90:201-214 */
+double orc_pppm_denom(int splodr, const double *gf_b, double x, double y, double z)
+{
+    double s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    for (int l = splodr - 1; l >= 0; l--) {
+        s1 = gf_b[l] + s1 * x;
+        s2 = gf_b[l] + s2 * y;
+        s3 = gf_b[l] + s3 * z;
+    }
+    double pr = s1 * s2 * s3;
+    return pr * pr;
+}
+
+/* recpcal_pppm_greenfunc, erdst/recpcal.F90:217-337: for each wave vector k,
+ * 4 pi / V * sum over aliases km = k + 2 pi invcl^T (mx K1, my K2, mz K2) of
+ * (k.km / km^2) exp(-km^2 / 4 screen^2) U(km)^2, over k^2 gf_denom; the
+ * alias range from LAMMPS' bound with EPS_HOC = 1e-7.  The third alias
+ * component is mz*K2 as in the reference (equal to mz*K3 on the grids where
+ * K2 = K3). */
+int orc_pppm_engfac(const orc_params *p, const double *cell, const double *invcl, double volume,
+                    double *engfac)
+{
+    const int K1 = p->ms[0], K2 = p->ms[1], K3 = p->ms[2], H = K1 / 2 + 1, P = p->splodr;
+    if (P % 2 == 1) return -1;
+    double gf_b[16];
+    orc_pppm_gfb(P, gf_b);
+    const double root = pow(-log(1.0e-7), 0.25);
+    const double mmax[3] = {p->screen / (ORC_PI * K1) * root, p->screen / (ORC_PI * K2) * root,
+                            p->screen / (ORC_PI * K3) * root};
+    int amax[3];
+    for (int d = 0; d < 3; d++)
+        amax[d] = (int)(C(cell,0,d) * mmax[0] + C(cell,1,d) * mmax[1] + C(cell,2,d) * mmax[2]);
+    if (amax[0] < 0 || amax[1] < 0 || amax[2] < 0) return -1;
+    const int Km[3] = {K1, K2, K3};
+    for (int r3 = 0; r3 < K3; r3++)
+        for (int r2 = 0; r2 < K2; r2++)
+            for (int r1 = 0; r1 < H; r1++) {
+                const int rc[3] = {r1, r2, r3};
+                double inm[3], s2h[3], k[3], k2 = 0.0, factor = 0.0;
+                for (int d = 0; d < 3; d++) {
+                    inm[d] = (rc[d] <= Km[d] / 2) ? rc[d] : rc[d] - Km[d];
+                    double sv = sin(ORC_PI * inm[d] / Km[d]);
+                    s2h[d] = sv * sv;
+                }
+                for (int d = 0; d < 3; d++) {
+                    k[d] = 2.0 * ORC_PI * (C(invcl,0,d) * inm[0] + C(invcl,1,d) * inm[1] + C(invcl,2,d) * inm[2]);
+                    k2 += k[d] * k[d];
+                }
+                if (!(r1 == 0 && r2 == 0 && r3 == 0) && k2 > 0.0) {
+                    double sum = 0.0;
+                    for (int mx = -amax[0]; mx <= amax[0]; mx++) {
+                        double t1 = ORC_PI * (inm[0] / K1 + mx);
+                        double u1 = (t1 == 0.0) ? 1.0 : pow(sin(t1) / t1, 2 * P);
+                        for (int my = -amax[1]; my <= amax[1]; my++) {
+                            double t2 = ORC_PI * (inm[1] / K2 + my);
+                            double u2 = (t2 == 0.0) ? 1.0 : pow(sin(t2) / t2, 2 * P);
+                            for (int mz = -amax[2]; mz <= amax[2]; mz++) {
+                                double t3 = ORC_PI * (inm[2] / K3 + mz);
+                                double u3 = (t3 == 0.0) ? 1.0 : pow(sin(t3) / t3, 2 * P);
+                                const double mN[3] = {(double)mx * K1, (double)my * K2, (double)mz * K2};
+                                double km[3], kdk = 0.0, km2 = 0.0, g = 1.0;
+                                for (int d = 0; d < 3; d++) {
+                                    km[d] = k[d] + 2.0 * ORC_PI * (C(invcl,0,d) * mN[0] + C(invcl,1,d) * mN[1] +
+                                                                   C(invcl,2,d) * mN[2]);
+                                    g *= exp(-0.25 * (km[d] / p->screen) * (km[d] / p->screen));
+                                    kdk += k[d] * km[d];
+                                    km2 += km[d] * km[d];
+                                }
+                                sum += kdk * g * (u1 * u2 * u3) / km2;
+                            }
+                        }
+                    }
+                    factor = sum * 4.0 * ORC_PI / (k2 * orc_pppm_denom(P, gf_b, s2h[0], s2h[1], s2h[2])) / volume;
+                }
+                engfac[r1 + H * (r2 + K2 * r3)] = factor;
+            }
+    engfac[0] = 0.0;
+    return 0;
+}
+
+/* the influence function of the run's cltype (engproc.F90:425-432); the
+ * callers have checked a PPPM order is even */
+static void influence(const orc_params *p, const double *cell, const double *invcl, double volume,
+                      double *engfac)
+{
+    if (p->cltype == 3) (void)orc_pppm_engfac(p, cell, invcl, volume, engfac);
+    else orc_engfac(p, invcl, volume, engfac);
+}
+
 /* erdst/recpcal.F90:504-538 */
 void orc_spline_atom(const orc_params *p, const double *invcl, const double *x,
                      double *w, int *grid)
@@ -743,6 +849,7 @@ int orc_energies_refs(const orc_topo *t, const orc_params *p, const double *cell
                       const int *slt_ljtype, int nins, const double *slt_pos,
                       int rigid, double *uv, double *uv0, int nthreads)
 {
+    if (p->cltype == 3 && p->splodr % 2 == 1) return 2;   /* "interpolation order for PPPM must be even" */
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -763,7 +870,7 @@ int orc_energies_refs(const orc_topo *t, const orc_params *p, const double *cell
     int K1 = p->ms[0], K2 = p->ms[1], K3 = p->ms[2], H = K1 / 2 + 1;
     size_t G = (size_t)K1 * K2 * K3, Gc = (size_t)H * K2 * K3;
     double *engfac = malloc(sizeof(double) * Gc);
-    orc_engfac(p, invcl, volume, engfac);
+    influence(p, cell, invcl, volume, engfac);
     double *selfrec = malloc(sizeof(double) * nins);
     double *splw = malloc(sizeof(double) * 3 * p->splodr * (size_t)t->natom);
     int *splg = malloc(sizeof(int) * 3 * (size_t)t->natom);
@@ -811,6 +918,7 @@ int orc_energies_soln(const orc_topo *t, const orc_params *p, const double *cell
                       const double *pos, int nslt_mol, const int *tagslt,
                       double *uv, double *uv0, int nthreads)
 {
+    if (p->cltype == 3 && p->splodr % 2 == 1) return 2;   /* "interpolation order for PPPM must be even" */
 #ifdef _OPENMP
     if (nthreads > 0) omp_set_num_threads(nthreads);
 #else
@@ -827,7 +935,7 @@ int orc_energies_soln(const orc_topo *t, const orc_params *p, const double *cell
     size_t G = (size_t)K1 * K2 * K3, Gc = (size_t)H * K2 * K3;
     double *engfac = malloc(sizeof(double) * Gc), *phi = malloc(sizeof(double) * G),
            *rho = malloc(sizeof(double) * G), *spec = malloc(sizeof(double) * 2 * Gc);
-    orc_engfac(p, invcl, volume, engfac);
+    influence(p, cell, invcl, volume, engfac);
     double *splw = malloc(sizeof(double) * 3 * p->splodr * (size_t)t->natom);
     int *splg = malloc(sizeof(int) * 3 * (size_t)t->natom);
     orc_spline_all(p, invcl, t->natom, pos, splw, splg);

@@ -42,6 +42,7 @@ typedef struct {
     int ljswitch;
     int splodr;
     int ms[3];
+    int cltype;                /* 2 = PME, 3 = PPPM (erdst/engmain.F90:359)      */
 } orc_params;
 
 /* cell normalisation result (erdst/realcal.F90:22-26) */
@@ -80,6 +81,13 @@ double orc_spline_value(int order, double x);
 void orc_init_spline_axis(int order, int K, double *splfc);
 /* erdst/recpcal.F90:121-163 ; engfac[(K1/2+1)*K2*K3] */
 void orc_engfac(const orc_params *p, const double *invcl, double volume, double *engfac);
+/* PPPM optimal influence function, erdst/recpcal.F90:179-337 (same layout);
+ * returns 0, or -1 for an odd order or a negative alias range */
+int orc_pppm_engfac(const orc_params *p, const double *cell, const double *invcl, double volume,
+                    double *engfac);
+/* gf_b coefficients (recpcal.F90:179-199) and gf_denom (:201-214) */
+void orc_pppm_gfb(int splodr, double *gf_b);
+double orc_pppm_denom(int splodr, const double *gf_b, double x, double y, double z);
 /* erdst/recpcal.F90:504-538 for one atom */
 void orc_spline_atom(const orc_params *p, const double *invcl, const double *x,
                      double *w /*[3][order]*/, int *grid /*[3]*/);

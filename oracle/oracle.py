@@ -38,7 +38,7 @@ class Topo(C.Structure):
 class Params(C.Structure):
     _fields_ = [("elecut", C.c_double), ("lwljcut", C.c_double), ("upljcut", C.c_double),
                 ("screen", C.c_double), ("ljswitch", C.c_int), ("splodr", C.c_int),
-                ("ms", C.c_int * 3)]
+                ("ms", C.c_int * 3), ("cltype", C.c_int)]
 
 
 class Box(C.Structure):
@@ -97,6 +97,11 @@ def lib():
         L.orc_energies_soln.argtypes = [C.POINTER(Topo), C.POINTER(Params), _dp, _dp, C.c_int,
                                         _ip, _dp, _dp, C.c_int]
         L.orc_engclear.argtypes = [C.POINTER(Dist)]
+        L.orc_pppm_denom.restype = C.c_double
+        L.orc_pppm_denom.argtypes = [C.c_int, _dp, C.c_double, C.c_double, C.c_double]
+        L.orc_pppm_gfb.argtypes = [C.c_int, _dp]
+        L.orc_pppm_engfac.argtypes = [C.POINTER(Params), _dp, _dp, C.c_double, _dp]
+        L.orc_pppm_engfac.restype = C.c_int
         L.orc_update_histogram.restype = C.c_int
         L.orc_update_histogram.argtypes = [C.POINTER(Dist), C.c_int, _ip, _dp, C.c_double,
                                            C.c_double, C.c_double]
@@ -113,12 +118,24 @@ def i32(a):
 
 
 def make_params(elecut=12.0, lwljcut=10.0, upljcut=12.0, screen=0.35, ljswitch=0, splodr=4,
-                ms=(32, 32, 32)) -> Params:
+                ms=(32, 32, 32), cltype=2) -> Params:
     p = Params()
     p.elecut, p.lwljcut, p.upljcut, p.screen = elecut, lwljcut, upljcut, screen
     p.ljswitch, p.splodr = ljswitch, splodr
     p.ms[0], p.ms[1], p.ms[2] = ms
+    p.cltype = cltype
     return p
+
+
+def pppm_gfb(splodr):
+    g = np.zeros(16)
+    lib().orc_pppm_gfb(int(splodr), g)
+    return g[:splodr].copy()
+
+
+def pppm_denom(splodr, gfb, x, y, z):
+    g = f64(np.concatenate([gfb, np.zeros(16 - len(gfb))]))
+    return lib().orc_pppm_denom(int(splodr), g, float(x), float(y), float(z))
 
 
 class OracleSystem:
@@ -161,10 +178,17 @@ def inverted_cell(cell):
 
 
 def engfac(params: Params, cell):
+    """The influence function of params.cltype (2 PME, 3 PPPM) on the half
+    spectrum, [K3][K2][K1/2+1]."""
     invcl, vol = inverted_cell(cell)
     K1, K2, K3 = params.ms
     out = np.zeros((K1 // 2 + 1) * K2 * K3)
-    lib().orc_engfac(C.byref(params), f64(invcl.ravel(order="F")), vol, out)
+    if params.cltype == 3:
+        if lib().orc_pppm_engfac(C.byref(params), f64(np.asarray(cell).ravel(order="F")),
+                                 f64(invcl.ravel(order="F")), vol, out):
+            raise ValueError("PPPM needs an even order")
+    else:
+        lib().orc_engfac(C.byref(params), f64(invcl.ravel(order="F")), vol, out)
     return out.reshape((K3, K2, K1 // 2 + 1))
 
 
@@ -235,7 +259,7 @@ def normalize_periodic(cell, pos, elecut=12.0, upljcut=12.0):
     rc = lib().orc_normalize_periodic(f64(np.asarray(cell).ravel(order="F")), elecut, upljcut,
                                       pos.shape[0], pos.ravel(), out.ravel(), C.byref(box))
     if rc:
-        raise ValueError("cutoff exceeds half the box")
+        raise ValueError("PPPM needs an even order" if rc == 2 else "cutoff exceeds half the box")
     return out, box
 
 
@@ -259,7 +283,7 @@ def energies_refs(sysm: OracleSystem, params: Params, cell, pos, slt_charge, slt
                                  f64(slt_charge), i32(slt_ljtype), nins, f64(slt_pos).ravel(),
                                  int(rigid), uv, uv0, int(nthreads))
     if rc:
-        raise ValueError("cutoff exceeds half the box")
+        raise ValueError("PPPM needs an even order" if rc == 2 else "cutoff exceeds half the box")
     return uv.reshape(nins, nmol), uv0
 
 
@@ -272,7 +296,7 @@ def energies_soln(sysm: OracleSystem, params: Params, cell, pos, tagslt, nthread
                                  f64(np.asarray(cell).ravel(order="F")), f64(pos).ravel(), tag.size,
                                  tag, uv, uv0, int(nthreads))
     if rc:
-        raise ValueError("cutoff exceeds half the box")
+        raise ValueError("PPPM needs an even order" if rc == 2 else "cutoff exceeds half the box")
     return uv.reshape(tag.size, nmol), uv0
 
 

@@ -375,3 +375,51 @@ def test_self_energy_histogram():
     assert d.minuv[0] == u0s.min() and d.maxuv[0] == 25.0
     with pytest.raises(RuntimeError, match="eng_min"):
         d.update(np.ones(3, np.int32), np.zeros(3), -1.0)   # at the bottom edge
+
+
+@pytest.mark.parametrize("order", [2, 4, 6, 8])
+def test_pppm_denominator_is_the_alias_sum(order):
+    """calc_gfb_pppm / gf_denom (recpcal.F90:179-214): sum_l gf_b(l) x^(l-1)
+    at x = sin^2(theta) is the aliased assignment-function sum
+    sum_n (sin(theta) / (theta + pi n))^(2 order), checked by brute force."""
+    gfb = O.pppm_gfb(order)
+    for theta in np.linspace(0.05, np.pi / 2, 9):
+        x = np.sin(theta) ** 2
+        poly = sum(gfb[l] * x ** l for l in range(order))
+        n = np.arange(-4000, 4001)
+        brute = np.sum((np.sin(theta) / (theta + np.pi * n)) ** (2 * order))
+        assert abs(poly - brute) <= 1e-10 * brute
+        assert O.pppm_denom(order, gfb, x, x, x) == pytest.approx(poly ** 6, rel=1e-13)
+
+
+@pytest.mark.parametrize("order,K,rtol", [(4, 32, 2e-4), (6, 48, 2e-6), (8, 64, 1e-8)])
+def test_pppm_pair_energy_converges_to_ewald(ewald_system, order, K, rtol):
+    """PPPM (cltype 3) differs from PME only in the influence function; its
+    energies must converge to the Ewald sum like PME's."""
+    L, qa, xa, qb, xb = ewald_system
+    ref = ewald_pair(qa, xa, qb, xb, L, 0.35)
+    p = O.make_params(screen=0.35, splodr=order, ms=(K, K, K), cltype=3)
+    sysm = O.OracleSystem([0, 2], qb, [0, 0], np.array([[0.0]]), np.array([[1.0]]))
+    uv, uv0 = O.energies_refs(sysm, p, np.diag([L] * 3), xb, qa, np.zeros(3, np.int32), xa[None], nthreads=2)
+    assert abs(uv[0, 0] - ref) <= rtol * abs(ref)
+    refself = ewald_self(qa, xa, L, 0.35)
+    assert abs(uv0[0] - refself) <= max(rtol * 30, 1e-9) * abs(refself) + 1e-9
+
+
+def test_pppm_influence_function_properties():
+    """Zero at k = 0; the alias range is empty beyond m = 0 here, so each
+    value is exp(-k^2/4a^2) U(k)^2 4 pi / (V k^2 denom); odd orders refused."""
+    L, K, P = 25.0, 16, 4
+    p = O.make_params(screen=0.35, splodr=P, ms=(K, K, K), cltype=3)
+    g = O.engfac(p, np.diag([L] * 3))
+    assert g[0, 0, 0] == 0.0 and np.all(g >= 0.0)
+    gfb = O.pppm_gfb(P)
+    m = np.array([3, -2, 5])                       # (rc1, rc2, rc3) = (3, 14, 5)
+    k = 2 * np.pi * m / L
+    th = np.pi * m / K
+    U2 = np.prod((np.sin(th) / th) ** (2 * P))
+    den = O.pppm_denom(P, gfb, *(np.sin(th) ** 2))
+    want = np.exp(-(k @ k) / (4 * 0.35 ** 2)) * U2 * 4 * np.pi / (L ** 3 * (k @ k) * den)
+    assert g[5, 14, 3] == pytest.approx(want, rel=1e-12)
+    with pytest.raises(ValueError):
+        O.engfac(O.make_params(splodr=5, ms=(K, K, K), cltype=3), np.diag([L] * 3))
