@@ -398,8 +398,8 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
         !topo->mol_begin_index || !topo->ljtype || !topo->uvspec || !topo->charge || !topo->ljene_mat ||
         !topo->ljlensq_mat)
         return bad(fail(c, ER_E_ARG, "er_create: incomplete topology"));
-    if (prm->cltype != 2 && prm->cltype != 3)
-        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: cltype 2 (PME) and 3 (PPPM) are implemented"));
+    if (prm->cltype != 0 && prm->cltype != 2 && prm->cltype != 3)
+        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: cltype 0 (bare Coulomb), 2 (PME) and 3 (PPPM) are implemented"));
     if (prm->cltype == 3 && prm->splodr % 2 == 1)
         return bad(fail(c, ER_E_ARG, "er_create: interpolation order for PPPM must be even"));
     if (prm->boxshp != 1)
@@ -494,6 +494,7 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     pp.lwljcut = prm->lwljcut;
     pp.upljcut = prm->upljcut;
     pp.screen = prm->screen;
+    pp.bare = prm->cltype == 0;
     pp.ljswitch = prm->ljswitch;
     pp.lw3 = prm->lwljcut * prm->lwljcut * prm->lwljcut;
     pp.up3 = prm->upljcut * prm->upljcut * prm->upljcut;
@@ -1134,6 +1135,11 @@ static PPPMCoef pppm_coef(const er_ctx *c)
 static hipError_t prep_frame(er_ctx *c)
 {
     const int B = 256;
+    if (c->cell_changed && c->prm.cltype == 0) {  // bare Coulomb: no influence function
+        std::memcpy(c->prev_cell, c->cell, sizeof c->cell);
+        c->have_prev_cell = true;
+        c->cell_changed = false;
+    }
     if (c->cell_changed && c->prm.cltype == 3) {
         const double *i = c->invcl_d;
         hipLaunchKernelGGL(k_engfac_pppm, dim3((unsigned)((c->Gc + B - 1) / B)), dim3(B), 0, c->stream, c->K1,
@@ -1226,11 +1232,23 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     return ER_OK;
 }
 
+template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, int nch);
+
 // energies and histograms of insertions [n_off, n_off+nch); NI = 16 for the
 // batched path, 1 for the plain path
 template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
 {
     hipError_t e;
+    if (c->prm.cltype == 0) {
+        // bare Coulomb (engproc.F90:761-769, 819-826): no reciprocal part; the
+        // chunk's rows of U start at zero for the pair sums
+        const size_t t0 = (size_t)n_off / NI, t1 = ((size_t)n_off + nch + NI - 1) / NI;
+        const size_t n = (t1 - t0) * NI * c->nummol;
+        hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                           reinterpret_cast<uint32_t *>(c->S().d_U.p + t0 * NI * c->nummol));
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        return run_chunk_pairs_hist<NI>(c, n_off, nch);
+    }
     {
         ProfScope ps(c, ER_K_SPEC2D);
         if ((e = launch_spec2d(c, n_off, nch)) != hipSuccess) return e;
@@ -1249,6 +1267,13 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
         }
     }
     if (e != hipSuccess) return e;
+    return run_chunk_pairs_hist<NI>(c, n_off, nch);
+}
+
+// the real-space pairs and the histograms of insertions [n_off, n_off+nch)
+template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, int nch)
+{
+    hipError_t e;
     const bool soln = c->prm.slttype == 1;
     {
         ProfScope ps(c, ER_K_PAIRS);
@@ -1289,7 +1314,8 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         WeightArgs wa{};
         wa.nchunk = nch;
         wa.n_off = n_off;
-        wa.H = c->H;
+        wa.H = (c->prm.cltype == 0) ? 0 : c->H;   // no reciprocal self part for bare Coulomb
+        wa.bare = c->prm.cltype == 0;
         wa.slttype = c->prm.slttype;
         wa.wgtslf = c->prm.wgtslf;
         wa.estype = c->prm.estype;
@@ -1366,7 +1392,8 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
                            c->fc, c->K1, c->K2, c->K3, c->ord, c->d_insposn.p, c->d_insw.p, c->d_insg.p);
         HIPCHK(c, hipGetLastError());
         hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)((nins + 3) / 4)), dim3(256), 0, c->stream, nins, c->S().nslt,
-                           c->fc, c->prm.screen, c->d_insposn.p, c->d_insref.p, c->d_selfreal.p, c->d_resself.p);
+                           c->fc, c->prm.screen, (int)(c->prm.cltype == 0), c->d_insposn.p, c->d_insref.p,
+                           c->d_selfreal.p, c->d_resself.p);
         HIPCHK(c, hipGetLastError());
         const size_t nf = (size_t)nins * c->S().nslt * (c->H + c->K2 + c->K3);
         hipLaunchKernelGGL(k_ins_factors, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, c->stream,

@@ -31,6 +31,7 @@ struct FrameConst {
 struct PairParams {
     float elecut, lwljcut, upljcut, screen;
     int ljswitch;
+    int bare;  // cltype 0: plain q q / r within elecut (realcal.F90:502-506)
     // force-switch constants (realcal.F90:82-93, 899-915)
     float lw3, up3, lw6, up6;
     float repA, repB, repC, attA, attB, attC;
@@ -182,7 +183,7 @@ __global__ void k_prep_sites(int nins, int nslt, const float *__restrict__ sltpo
 
 // One wave per insertion (after k_prep_sites): reference site and reach
 // radius, real-space self energy and residual self term.
-__global__ void __launch_bounds__(256) k_prep_ins(int nins, int nslt, FrameConst fc, float screen,
+__global__ void __launch_bounds__(256) k_prep_ins(int nins, int nslt, FrameConst fc, float screen, int bare,
                                                   const float4 *__restrict__ insposn, float4 *__restrict__ insref,
                                                   float *__restrict__ selfreal, float *__restrict__ resself)
 {
@@ -198,6 +199,14 @@ __global__ void __launch_bounds__(256) k_prep_ins(int nins, int nslt, FrameConst
         R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
     }
     for (int off = 32; off > 0; off >>= 1) R = fmaxf(R, __shfl_xor(R, off, 64));
+    if (bare) {  // cltype 0: realcal_self and residual_self_ene return at once
+        if (lane == 0) {
+            insref[n] = make_float4(c0.x, c0.y, c0.z, R);
+            selfreal[n] = 0.0f;
+            resself[n] = 0.0f;
+        }
+        return;
+    }
     // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r;
     // FP32 terms as in the reference's single build, summed in FP64 (a
     // 150-site solute has 11 175 terms that largely cancel): lane l takes
@@ -1558,7 +1567,7 @@ __device__ __forceinline__ float pair_energy(const PairParams &p, float dis2, fl
         }
         }
     }
-    if (!(rst >= p.elecut)) epcl = qa * qb * erfcf(p.screen * rst) / rst;
+    if (!(rst >= p.elecut)) epcl = p.bare ? qa * qb / rst : qa * qb * erfcf(p.screen * rst) / rst;
     return eplj + epcl;
 }
 
@@ -1946,6 +1955,7 @@ __global__ void k_halt_capture(unsigned int seq, int NI, int nmol, const unsigne
 // ---------------------------------------------------------------------------
 struct WeightArgs {
     int nchunk, n_off, H, slttype, wgtslf, estype, selfcal, esmax, rigid, numslv, nslice;
+    int bare;  // cltype 0: no reciprocal self part, no Ewald factor in volcorrect
     unsigned int seq;
     float w_sys, volume, screen, qtot;
     double temp;
@@ -2010,10 +2020,12 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
             engnmfc = (wa.slttype == 1) ? exp((double)factor / wa.temp) : exp(-(double)factor / wa.temp);
         }
         float tw = wa.w_sys * (wslt ? wslt[n] : 1.0f);
-        if (wa.estype == 2) {
+        if (wa.estype == 2) {  // volcorrect (engproc.F90:1022-1038)
             tw = tw * wa.volume;
-            const float f2 = ER_PI_F * (wa.qtot * wa.qtot) / (wa.screen * wa.screen) / wa.volume / 2.0f;
-            tw = (float)((double)tw * exp((double)f2 / wa.temp));
+            if (!wa.bare) {
+                const float f2 = ER_PI_F * (wa.qtot * wa.qtot) / (wa.screen * wa.screen) / wa.volume / 2.0f;
+                tw = (float)((double)tw * exp((double)f2 / wa.temp));
+            }
         }
         engnmfc *= (double)tw;
         wout[n] = engnmfc;

@@ -10,7 +10,7 @@ HISTORY and SltConf; runs the engdiv x frame loop of trjmain
 corsln|corref.XX, slfeng.XX, weight_soln|weight_refs, aveuv.tt, uvrange.tt
 and flcuv.tt|progress.tt.
 
-Supported: PME (cltype 2) or PPPM (cltype 3) in a periodic cell; soln
+Supported: bare Coulomb, PME or PPPM (cltype 0, 2, 3) in a periodic cell; soln
 runs; refs runs whose solute configurations are used as they are
 (insposition 1 and insorient 1):
 flexible solutes (slttype 3) from SltConf, one DCD frame per insertion,
@@ -43,8 +43,8 @@ def _hot_params(p: dict) -> dict:
 
 
 def _check_supported(p: dict):
-    if p["cltype"] not in (2, 3):
-        raise InputError("only PME (cltype = 2) and PPPM (cltype = 3) run on the GPU path")
+    if p["cltype"] not in (0, 2, 3):
+        raise InputError("bare Coulomb (cltype = 0), PME (2) and PPPM (3) run on the GPU path, not Ewald (1)")
     if p["boxshp"] != 1:
         raise InputError("only periodic systems (boxshp = 1) run on the GPU path")
     if p["slttype"] != SLT_SOLN and not (p["insposition"] == 1 and p["insorient"] == 1):

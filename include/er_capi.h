@@ -69,7 +69,7 @@ typedef struct {
 typedef struct {
     float elecut, lwljcut, upljcut, screen;
     int32_t ljswitch;           /* 0..3, erdst/engmain.F90:358              */
-    int32_t cltype;             /* 2 PME or 3 PPPM (even splodr)            */
+    int32_t cltype;             /* 0 bare Coulomb, 2 PME, 3 PPPM (even splodr) */
     int32_t splodr;             /* B-spline order, 2..12                    */
     int32_t ms[3];              /* ms1max, ms2max, ms3max                    */
     int32_t boxshp;             /* only 1 (periodic) is implemented         */

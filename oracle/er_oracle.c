@@ -174,8 +174,8 @@ static double pair_from_dis2(const orc_params *p, double dis2, double qa, double
             eplj = NAN;
         }
     }
-    if (!(rst >= p->elecut))
-        epcl = qa * qb * erfc(p->screen * rst) / rst;
+    if (!(rst >= p->elecut))   /* bare Coulomb for cltype 0 (realcal.F90:502-506) */
+        epcl = (p->cltype == 0) ? qa * qb / rst : qa * qb * erfc(p->screen * rst) / rst;
     return eplj + epcl;
 }
 
@@ -752,6 +752,7 @@ void orc_engclear(orc_dist *d)
 double orc_volcorrect(const orc_dist *d, double weight)
 {
     weight = weight * d->volume;
+    if (d->cltype == 0) return weight;   /* bare Coulomb: no Ewald factor (engproc.F90:1029) */
     double factor = ORC_PI * d->total_charge * d->total_charge / (d->screen * d->screen) / d->volume / 2.0;
     return weight * exp(factor / d->temp);
 }
@@ -867,6 +868,17 @@ int orc_energies_refs(const orc_topo *t, const orc_params *p, const double *cell
     }
     const double *sltn = alln + 3 * (size_t)t->natom;
     memset(uv, 0, sizeof(double) * (size_t)nins * t->nmol);
+    if (p->cltype == 0) {
+        /* bare Coulomb (engproc.F90:819-840): pairs only; the self energy
+         * terms return at once (realcal.F90:730, engproc.F90:973), so
+         * uvengy0 = 0.  Each molecule's sum goes to its own row: the
+         * reference's realcal_bare_refs adds into uvengy(k, cnt) with k never
+         * set (realcal.F90:660), an undefined element. */
+        orc_realcal_refs(t, p, &box, alln, nslt, slt_charge, slt_ljtype, nins, sltn, uv);
+        for (int cnt = 0; cnt < nins; cnt++) uv0[cnt] = 0.0;
+        free(all); free(alln);
+        return 0;
+    }
     int K1 = p->ms[0], K2 = p->ms[1], K3 = p->ms[2], H = K1 / 2 + 1;
     size_t G = (size_t)K1 * K2 * K3, Gc = (size_t)H * K2 * K3;
     double *engfac = malloc(sizeof(double) * Gc);
@@ -930,6 +942,16 @@ int orc_energies_soln(const orc_topo *t, const orc_params *p, const double *cell
     orc_box box;
     if (orc_normalize_periodic(cell, p->elecut, p->upljcut, t->natom, pos, posn, &box)) {
         free(posn); return 1;
+    }
+    if (p->cltype == 0) {
+        /* bare Coulomb (engproc.F90:761-769): pairs only, uvengy0 = 0 */
+        memset(uv, 0, sizeof(double) * (size_t)nslt_mol * t->nmol);
+        for (int c = 0; c < nslt_mol; c++) {
+            orc_realcal_soln(t, p, &box, posn, tagslt[c], uv + (size_t)c * t->nmol);
+            uv0[c] = 0.0;
+        }
+        free(posn);
+        return 0;
     }
     int K1 = p->ms[0], K2 = p->ms[1], K3 = p->ms[2], H = K1 / 2 + 1;
     size_t G = (size_t)K1 * K2 * K3, Gc = (size_t)H * K2 * K3;

@@ -42,7 +42,7 @@ typedef struct {
     int ljswitch;
     int splodr;
     int ms[3];
-    int cltype;                /* 2 = PME, 3 = PPPM (erdst/engmain.F90:359)      */
+    int cltype;                /* 0 = bare Coulomb, 2 = PME, 3 = PPPM (erdst/engmain.F90:359) */
 } orc_params;
 
 /* cell normalisation result (erdst/realcal.F90:22-26) */
@@ -141,6 +141,7 @@ typedef struct {
     int err_species;
     double err_energy;
     int warn_mbin;
+    int cltype;                  /* volcorrect's Ewald factor for cltype 1-3 */
 } orc_dist;
 
 /* erdst/engproc.F90:321-340 */

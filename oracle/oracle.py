@@ -58,7 +58,7 @@ class Dist(C.Structure):
                 ("engsmpl", C.c_double), ("avslf", C.c_double), ("voffset", C.c_double),
                 ("voffset_initialized", C.c_int), ("insdst", C.c_void_p),
                 ("errcode", C.c_char * 8), ("err_species", C.c_int), ("err_energy", C.c_double),
-                ("warn_mbin", C.c_int)]
+                ("warn_mbin", C.c_int), ("cltype", C.c_int)]
 
 
 def lib():
@@ -304,7 +304,7 @@ class Distribution:
     """orc_dist plus the numpy arrays it points at (engproc.F90:858-964 state)."""
 
     def __init__(self, slttype, uvmax, uvsoft, uvcrd, temp, volume, screen, total_charge,
-                 wgtslf=1, estype=2, selfcal=0, corrcal=1, escrd=None):
+                 wgtslf=1, estype=2, selfcal=0, corrcal=1, escrd=None, cltype=2):
         self.uvmax, self.uvsoft, self.uvcrd = i32(uvmax), i32(uvsoft), f64(uvcrd)
         self.numslv = self.uvmax.size
         self.ermax = int(self.uvmax.sum())
@@ -328,6 +328,7 @@ class Distribution:
             setattr(d, name, getattr(self, name).ctypes.data)
         d.voffset = -np.inf
         d.voffset_initialized = 0
+        d.cltype = cltype
         self.d = d
 
     def clear(self):

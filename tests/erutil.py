@@ -72,7 +72,7 @@ def oracle_distribution(w, params, escrd=None) -> O.Distribution:
     qtot = float(np.float64(w.topo.charge32.astype(np.float64).sum()))
     return O.Distribution(params["slttype"], uvmax, uvsoft, uvcrd, params["temp"], vol, params["screen"],
                           qtot, wgtslf=params["wgtslf"], estype=params["estype"], selfcal=params["selfcal"],
-                          corrcal=params["corrcal"], escrd=escrd)
+                          corrcal=params["corrcal"], escrd=escrd, cltype=params.get("cltype", 2))
 
 
 # Self-energy mesh for the small test solutes: their self energies (solute

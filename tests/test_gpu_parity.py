@@ -258,6 +258,28 @@ def test_pppm_energies_cfg0(cfg0, order):
         assert rms(ug - uo) < 0.3 * rms(ug - uop)
 
 
+def test_bare_coulomb_cfg0(cfg0):
+    """cltype 0: real-space pairs with plain q q / r within elecut, no
+    reciprocal or residual part, uvengy0 = 0, volcorrect by the volume only;
+    energies against the oracle and the distribution against the oracle's
+    binning of the GPU's energies."""
+    p = U.workload_params(cfg0, cltype=0)
+    ctx = _ctx(cfg0, p)
+    d = U.oracle_distribution(cfg0, p)
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        u, u0 = _run_refs(ctx, cfg0, 0, mode, nins=32)
+        uo, u0o = U.oracle_refs(cfg0, 0, p, nins=32)
+        nbad, worst, at = U.energy_mismatch(u, uo)
+        assert nbad == 0, (mode, worst, at, u[at], uo[at])
+        assert np.all(u0 == 0.0) and np.all(u0o == 0.0)
+        for n in range(u.shape[0]):
+            d.update(cfg0.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+    g = ctx.fetch()
+    _assert_same_distribution(g, d)
+    assert g["engsmpl"] == 64
+    np.testing.assert_allclose(g["engnorm"], 64 * np.float32(np.prod(np.diag(cfg0.cell))), rtol=1e-6)
+
+
 def test_pppm_rejects_odd_order(cfg0):
     with pytest.raises(capi.ErError):
         capi.Context(cfg0.topo, U.workload_params(cfg0, cltype=3, splodr=4) | {"splodr": 5})

@@ -423,3 +423,37 @@ def test_pppm_influence_function_properties():
     assert g[5, 14, 3] == pytest.approx(want, rel=1e-12)
     with pytest.raises(ValueError):
         O.engfac(O.make_params(splodr=5, ms=(K, K, K), cltype=3), np.diag([L] * 3))
+
+
+def test_bare_coulomb_is_the_minimum_image_sum():
+    """cltype 0 (realcal.F90:367-666, engproc.F90:819-840): each pair adds
+    q_i q_j / r for r < elecut under the minimum image, no reciprocal or
+    residual term; uvengy0 is 0 (the self terms return at once); volcorrect
+    multiplies by the volume only."""
+    rng = np.random.default_rng(11)
+    L, n = 30.0, 40
+    pos = rng.uniform(0, L, (2 * n, 3))
+    pos[1::2] = pos[::2] + rng.normal(0, 0.9, (n, 3))       # diatomic molecules
+    q = np.tile([0.4, -0.4], n) * CH
+    slt = rng.uniform(0, L, (3, 3))
+    slt[1:] = slt[0] + rng.normal(0, 0.9, (2, 3))
+    qs = np.array([0.3, -0.1, -0.2]) * CH
+    p = O.make_params(screen=0.35, splodr=4, ms=(32, 32, 32), cltype=0)
+    sysm = O.OracleSystem(np.arange(0, 2 * n + 1, 2), q, np.zeros(2 * n, np.int32), np.array([[0.0]]),
+                          np.array([[1.0]]))
+    uv, uv0 = O.energies_refs(sysm, p, np.diag([L] * 3), pos, qs, np.zeros(3, np.int32), slt[None], nthreads=2)
+    want = np.zeros(n)
+    for m in range(n):
+        for a in (2 * m, 2 * m + 1):
+            for s in range(3):
+                d = slt[s] - pos[a]
+                d -= L * np.round(d / L)
+                r = np.linalg.norm(d)
+                if r < 12.0:
+                    want[m] += qs[s] * q[a] / r
+    np.testing.assert_allclose(uv[0], want, rtol=1e-12, atol=1e-12)
+    assert uv0[0] == 0.0
+    d = O.Distribution(3, [41], [41], np.linspace(-50, 50, 41), temp=0.6, volume=L ** 3, screen=0.35,
+                       total_charge=1.0, wgtslf=1, estype=2, cltype=0)
+    d.update(np.ones(n, np.int32), np.clip(uv[0], -49, 49), 0.0)
+    assert d.d.engnorm == pytest.approx(L ** 3, rel=1e-15)
