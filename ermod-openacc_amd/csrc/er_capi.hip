@@ -764,10 +764,33 @@ extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *t
 static int frame_cell(er_ctx *c, const float *cell)
 {
     // column-major: cell(i,j) = cell[i + 3j]; sub-diagonal (2,1),(3,1),(3,2)
-    if (cell[1] != 0.f || cell[2] != 0.f || cell[5] != 0.f)
-        return fail(c, ER_E_UNSUPPORTED, "cell is not upper triangular (make_cell_uppertriangular path)");
     float cn[9];
     std::memcpy(cn, cell, sizeof cn);
+    FrameConst &fc = c->fc;
+    fc.rotate = 0;
+    if (cell[1] != 0.f || cell[2] != 0.f || cell[5] != 0.f) {
+        // make_cell_uppertriangular (realcal.F90:776-817): cell = QR by
+        // modified Gram-Schmidt, cell_normal = R, pair positions Q^T x.  Each
+        // column is normalised by its own length (the reference divides all
+        // three by the length of column 1, line 799, which is no rotation).
+        double t[9], q[9], r[9] = {0};
+        for (int k = 0; k < 9; ++k) t[k] = cell[k];
+        for (int i = 0; i < 3; ++i) {
+            const double len = std::sqrt(t[3 * i] * t[3 * i] + t[3 * i + 1] * t[3 * i + 1] + t[3 * i + 2] * t[3 * i + 2]);
+            if (!(len > 0.0)) return fail(c, ER_E_ARG, "cell has a zero or dependent axis");
+            for (int k = 0; k < 3; ++k) q[k + 3 * i] = t[k + 3 * i] / len;
+            r[i + 3 * i] = len;
+            for (int j = i + 1; j < 3; ++j) {
+                const double d = q[3 * i] * t[3 * j] + q[3 * i + 1] * t[3 * j + 1] + q[3 * i + 2] * t[3 * j + 2];
+                r[i + 3 * j] = d;
+                for (int k = 0; k < 3; ++k) t[k + 3 * j] -= d * q[k + 3 * i];
+            }
+        }
+        for (int k = 0; k < 9; ++k) cn[k] = (float)r[k];
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) fc.rotT[3 * i + k] = (float)q[k + 3 * i];
+        fc.rotate = 1;
+    }
     // normalize_cell_vector (realcal.F90:826-830): only column 1 can be flipped
     for (int i = 0; i < 3; ++i)
         if (cn[0] < 0.f)
@@ -778,7 +801,6 @@ static int frame_cell(er_ctx *c, const float *cell)
             return fail(c, ER_E_ARG,
                         "One of axis in periodic cell is too small. This is either the box is too small "
                         "compared to the cell, or the periodic cell is too skewed.");
-    FrameConst &fc = c->fc;
     std::memcpy(fc.cellN, cn, sizeof cn);
     for (int i = 0; i < 3; ++i) {
         fc.lenN[i] = std::fabs(cn[i + 3 * i]);
@@ -805,6 +827,12 @@ static int frame_cell(er_ctx *c, const float *cell)
     for (int k = 0; k < 9; ++k) {
         iv[k] /= v;
         fc.invcl[k] = (float)iv[k];
+        fc.invP[k] = fc.invcl[k];
+    }
+    if (fc.rotate) {  // inverse of the upper-triangular cell_normal
+        const double a = cn[0], b = cn[3], d = cn[6], e = cn[4], f = cn[7], g = cn[8];
+        double ip[9] = {1.0 / a, 0, 0, -b / (a * e), 1.0 / e, 0, (b * f - d * e) / (a * e * g), -f / (e * g), 1.0 / g};
+        for (int k = 0; k < 9; ++k) fc.invP[k] = (float)ip[k];
     }
     c->volume_d = v;
     fc.volume = (float)v;

@@ -26,6 +26,10 @@ struct FrameConst {
     float invcl[9];  // inverse of the raw cell (engproc.F90:1049-1070)
     int is_cuboid;
     float volume;
+    int rotate;      // cell not upper triangular: positions x -> Q^T x first
+    float rotT[9];   // Q^T, row-major (realcal.F90:776-817)
+    float invP[9];   // inverse of the cell the pair positions live in (cell_normal
+                     // when rotated, else the raw cell: equal to invcl)
 };
 
 struct PairParams {
@@ -61,9 +65,17 @@ __device__ __forceinline__ void min_image(const FrameConst &fc, float &dx, float
     }
 }
 
-// wrap into the cell with Z, then Y, then X shifts (realcal.F90:885-895)
+// rotate into the upper-triangular frame when the cell needs it
+// (realcal.F90:814-816), then wrap into the cell with Z, then Y, then X
+// shifts (realcal.F90:885-895)
 __device__ __forceinline__ void wrap_site(const FrameConst &fc, float &x, float &y, float &z)
 {
+    if (fc.rotate) {
+        const float rx = fc.rotT[0] * x + fc.rotT[1] * y + fc.rotT[2] * z;
+        const float ry = fc.rotT[3] * x + fc.rotT[4] * y + fc.rotT[5] * z;
+        const float rz = fc.rotT[6] * x + fc.rotT[7] * y + fc.rotT[8] * z;
+        x = rx; y = ry; z = rz;
+    }
     float f = floorf(fc.invN[2] * z);
     x -= fc.cellN[6] * f; y -= fc.cellN[7] * f; z -= fc.cellN[8] * f;
     f = floorf(fc.invN[1] * y);
@@ -816,7 +828,7 @@ __device__ __forceinline__ void site_cell(const FrameConst &fc, float x, float y
     float u[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float f = fc.invcl[k] * x + fc.invcl[k + 3] * y + fc.invcl[k + 6] * z;
+        float f = fc.invP[k] * x + fc.invP[k + 3] * y + fc.invP[k + 6] * z;
         u[k] = f - floorf(f);
     }
     cx = min(nc - 1, (int)(u[0] * nc));

@@ -56,6 +56,29 @@ int orc_normalize_periodic(const double *cell, double elecut, double upljcut,
     memcpy(box->cell_normal, cell, 9 * sizeof(double));
     memset(box->rot, 0, sizeof(box->rot));
     box->rot[0] = box->rot[4] = box->rot[8] = 1.0;
+    const int rotate = C(cell,1,0) != 0.0 || C(cell,2,0) != 0.0 || C(cell,2,1) != 0.0;
+    if (rotate) {
+        /* make_cell_uppertriangular (realcal.F90:776-817): cell = QR by
+         * modified Gram-Schmidt, cell_normal = R, positions Q^T x.  Each
+         * column is normalised by its own length; the reference takes the
+         * length of column 1 for all three (line 799), which makes Q no
+         * rotation. */
+        double t[9], q[9], r[9] = {0};
+        memcpy(t, cell, sizeof t);
+        for (int i = 0; i < 3; i++) {
+            double len = sqrt(C(t,0,i) * C(t,0,i) + C(t,1,i) * C(t,1,i) + C(t,2,i) * C(t,2,i));
+            for (int k = 0; k < 3; k++) C(q,k,i) = C(t,k,i) / len;
+            C(r,i,i) = len;
+            for (int j = i + 1; j < 3; j++) {
+                double d = C(q,0,i) * C(t,0,j) + C(q,1,i) * C(t,1,j) + C(q,2,i) * C(t,2,j);
+                C(r,i,j) = d;
+                for (int k = 0; k < 3; k++) C(t,k,j) -= d * C(q,k,i);
+            }
+        }
+        memcpy(box->cell_normal, r, sizeof r);
+        for (int i = 0; i < 3; i++)
+            for (int k = 0; k < 3; k++) C(box->rot,i,k) = C(q,k,i);   /* rot = Q^T */
+    }
     /* normalize_cell_vector, realcal.F90:826-830: the test re-reads (1,1)
      * after each flip, so only the first column is ever negated */
     for (int i = 0; i < 3; i++)
@@ -74,6 +97,12 @@ int orc_normalize_periodic(const double *cell, double elecut, double upljcut,
     /* realcal.F90:885-895: shift Z, then Y, then X */
     for (int a = 0; a < n; a++) {
         double x[3] = {pos[3*a], pos[3*a+1], pos[3*a+2]};
+        if (rotate) {
+            double y[3];
+            for (int i = 0; i < 3; i++)
+                y[i] = C(box->rot,i,0) * x[0] + C(box->rot,i,1) * x[1] + C(box->rot,i,2) * x[2];
+            x[0] = y[0]; x[1] = y[1]; x[2] = y[2];
+        }
         for (int ax = 2; ax >= 0; ax--) {
             double f = floor(box->invcell[ax] * x[ax]);
             for (int r = 0; r < 3; r++) x[r] -= C(box->cell_normal, r, ax) * f;

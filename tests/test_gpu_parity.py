@@ -280,6 +280,41 @@ def test_bare_coulomb_cfg0(cfg0):
     np.testing.assert_allclose(g["engnorm"], 64 * np.float32(np.prod(np.diag(cfg0.cell))), rtol=1e-6)
 
 
+def test_rotated_cell_cfg0(cfg0):
+    """The whole configuration-0 system rotated, so the cell is not upper
+    triangular (make_cell_uppertriangular, realcal.F90:776-817): energies
+    against the oracle on the same inputs, and against the unrotated
+    system's oracle energies (rotation only changes float32 rounding)."""
+    rng = np.random.default_rng(21)
+    qm, rm = np.linalg.qr(rng.normal(size=(3, 3)))
+    R = qm * np.sign(np.diag(rm))
+    if np.linalg.det(R) < 0:
+        R = -R
+    p = U.workload_params(cfg0)
+    pos = (cfg0.frame(0).astype(np.float64) @ R.T).astype(np.float32)
+    cell = (R @ cfg0.cell.astype(np.float64)).astype(np.float32)
+    ins = (cfg0.insertions(0)[:32].astype(np.float64) @ R.T).astype(np.float32)
+    assert abs(cell[1, 0]) + abs(cell[2, 0]) + abs(cell[2, 1]) > 1.0
+    ctx = _ctx(cfg0, p)
+    topo = cfg0.topo
+    orc = lambda ps, c, x: U.O.energies_refs(U.oracle_system(topo), U.oracle_params(p), c.astype(np.float64),
+                                             ps.astype(np.float64), topo.slt_charge64, topo.slt_ljtype,
+                                             x.astype(np.float64), rigid=(p["slttype"] == 2))
+    uo, u0o = orc(pos, cell, ins)
+    uo_plain, u0o_plain = orc(cfg0.frame(0), cfg0.cell, cfg0.insertions(0)[:32])
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx.set_frame(pos, cell)
+        ctx.set_insertions(ins)
+        ctx.run(mode)
+        ctx.sync()
+        u, u0 = ctx.uvengy(32)
+        nbad, worst, at = U.energy_mismatch(u, uo)
+        assert nbad == 0, (mode, worst, at, u[at], uo[at])
+        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.energy_mismatch(u, uo_plain)[0] == 0
+        assert U.energy_mismatch(u0, u0o_plain)[0] == 0
+
+
 def test_pppm_rejects_odd_order(cfg0):
     with pytest.raises(capi.ErError):
         capi.Context(cfg0.topo, U.workload_params(cfg0, cltype=3, splodr=4) | {"splodr": 5})

@@ -457,3 +457,37 @@ def test_bare_coulomb_is_the_minimum_image_sum():
                        total_charge=1.0, wgtslf=1, estype=2, cltype=0)
     d.update(np.ones(n, np.int32), np.clip(uv[0], -49, 49), 0.0)
     assert d.d.engnorm == pytest.approx(L ** 3, rel=1e-15)
+
+
+def _rotation(rng):
+    q, r = np.linalg.qr(rng.normal(size=(3, 3)))
+    q = q * np.sign(np.diag(r))
+    return q if np.linalg.det(q) > 0 else -q
+
+
+def test_rotated_system_has_the_same_energies():
+    """A cell that is not upper triangular is brought to upper-triangular
+    form by a rotation (make_cell_uppertriangular, realcal.F90:776-817):
+    rotating frame, cell and insertions together leaves every energy and
+    self energy unchanged."""
+    rng = np.random.default_rng(5)
+    L, n = 26.0, 60
+    pos = rng.uniform(0, L, (2 * n, 3))
+    pos[1::2] = pos[::2] + rng.normal(0, 0.9, (n, 3))
+    q = np.tile([0.4, -0.4], n) * CH
+    slt = rng.uniform(0, L, (4, 3, 3))
+    slt[:, 1:] = slt[:, :1] + rng.normal(0, 0.9, (4, 2, 3))
+    qs = np.array([0.3, -0.1, -0.2]) * CH
+    sysm = O.OracleSystem(np.arange(0, 2 * n + 1, 2), q, np.tile([0, 1], n), np.array([[0.2, 0.1], [0.1, 0.05]]),
+                          np.array([[9.0, 7.0], [7.0, 5.0]]))
+    R = _rotation(rng)
+    cell = np.diag([L, L * 1.1, L * 0.95])
+    for cltype in (0, 2):
+        p = O.make_params(screen=0.35, splodr=6, ms=(48, 48, 48), cltype=cltype)
+        u, u0 = O.energies_refs(sysm, p, cell, pos, qs, np.array([0, 1, 1], np.int32), slt, nthreads=2)
+        rc = R @ cell
+        assert abs(rc[1, 0]) + abs(rc[2, 0]) + abs(rc[2, 1]) > 1.0
+        ur, u0r = O.energies_refs(sysm, p, rc, pos @ R.T, qs, np.array([0, 1, 1], np.int32), slt @ R.T,
+                                  nthreads=2)
+        np.testing.assert_allclose(ur, u, rtol=1e-9, atol=1e-9)
+        np.testing.assert_allclose(u0r, u0, rtol=1e-9, atol=1e-9)
