@@ -200,6 +200,12 @@ struct er_ctx {
     int64_t prof_n[ER_K_COUNT] = {0};
 
     hipStream_t stream = nullptr;
+    // the frame's atom preparation (splines, cell list, brick records) runs
+    // on a second stream, default priority, beside the insertions' plane
+    // DFTs; the main stream waits on ev_atoms before the gather or the pairs
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_in = nullptr, ev_atoms = nullptr;
+    bool atoms_pending = false;
     std::string errmsg;
     char halt_code[8] = {0};
     int halt_species = 0;
@@ -504,6 +510,9 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     gmx_params(6, prm->lwljcut, prm->upljcut, pp.attA, pp.attB, pp.attC);
 
     HIPCHK(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(c, hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_in, hipEventDisableTiming));
+    HIPCHK(c, hipEventCreateWithFlags(&c->ev_atoms, hipEventDisableTiming));
     {
         int dev = 0;
         HIPCHK(c, hipGetDevice(&dev));
@@ -1160,6 +1169,20 @@ static PPPMCoef pppm_coef(const er_ctx *c)
     return pc;
 }
 
+#define HIPCHK_E(x)                              \
+    do {                                         \
+        const hipError_t e_ = (x);               \
+        if (e_ != hipSuccess) return e_;         \
+    } while (0)
+
+// the main stream waits for the frame's atom preparation (once per frame)
+static hipError_t join_atoms(er_ctx *c)
+{
+    if (!c->atoms_pending) return hipSuccess;
+    c->atoms_pending = false;
+    return hipStreamWaitEvent(c->stream, c->ev_atoms, 0);
+}
+
 static hipError_t prep_frame(er_ctx *c)
 {
     const int B = 256;
@@ -1187,35 +1210,41 @@ static hipError_t prep_frame(er_ctx *c)
         c->cell_changed = false;
     }
     if (c->frame_dirty) {
-        hipLaunchKernelGGL(k_prep_atoms, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm, c->d_sitepos,
+        // after the frame's upload on the main stream
+        hipStream_t s2 = c->stream2;
+        HIPCHK_E(hipEventRecord(c->ev_in, c->stream));
+        HIPCHK_E(hipStreamWaitEvent(s2, c->ev_in, 0));
+        hipLaunchKernelGGL(k_prep_atoms, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm, c->d_sitepos,
                            c->d_charge.p, c->fc, c->K1, c->K2, c->K3, c->ord, c->d_posn.p, c->d_splw.p, c->d_splg.p);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, c->stream, (size_t)1,
+        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s2, (size_t)1,
                            reinterpret_cast<uint32_t *>(c->d_rmolmax.p));
-        hipLaunchKernelGGL(k_prep_mols, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
+        hipLaunchKernelGGL(k_prep_mols, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
                            c->d_molbeg.p, c->d_posn.p, c->fc, c->d_molref.p, c->d_rmolmax.p);
         // spatial order of the molecules and the cell list
-        hipLaunchKernelGGL(k_zero_u32, dim3((c->ncell + B - 1) / B), dim3(B), 0, c->stream, (size_t)c->ncell,
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->ncell + B - 1) / B), dim3(B), 0, s2, (size_t)c->ncell,
                            c->d_ccnt.p);
-        hipLaunchKernelGGL(k_cell_count, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
+        hipLaunchKernelGGL(k_cell_count, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
                            c->d_molref.p, c->fc, c->nc, c->d_ccnt.p, c->d_keyrank.p);
-        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->ncell, c->d_ccnt.p, c->d_cstart.p);
-        hipLaunchKernelGGL(k_cell_scatter, dim3((c->nummol + B - 1) / B), dim3(B), 0, c->stream, c->nummol,
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s2, c->ncell, c->d_ccnt.p, c->d_cstart.p);
+        hipLaunchKernelGGL(k_cell_scatter, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
                            c->d_keyrank.p, c->d_cstart.p, c->d_perm.p);
         // atoms by PME brick for the brick gather
-        hipLaunchKernelGGL(k_zero_u32, dim3((c->nbrick + B - 1) / B), dim3(B), 0, c->stream, (size_t)c->nbrick,
+        hipLaunchKernelGGL(k_zero_u32, dim3((c->nbrick + B - 1) / B), dim3(B), 0, s2, (size_t)c->nbrick,
                            c->d_bcnt.p);
-        hipLaunchKernelGGL(k_atom_brick_count, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+        hipLaunchKernelGGL(k_atom_brick_count, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
                            c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, brick_z(c->ord), c->d_bcnt.p,
                            c->d_akeyrank.p);
-        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, c->stream, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
+        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s2, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
         if (c->ord == 4)
-            hipLaunchKernelGGL(k_brick_records<4>, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+            hipLaunchKernelGGL(k_brick_records<4>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
                                c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
                                c->K3, c->nbx, c->nby, c->d_brec.p);
         else if (c->ord == 6)
-            hipLaunchKernelGGL(k_brick_records<6>, dim3((c->numatm + B - 1) / B), dim3(B), 0, c->stream, c->numatm,
+            hipLaunchKernelGGL(k_brick_records<6>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
                                c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
                                c->K3, c->nbx, c->nby, c->d_brec.p);
+        HIPCHK_E(hipEventRecord(c->ev_atoms, s2));
+        c->atoms_pending = true;
         c->frame_dirty = false;
     }
     return hipGetLastError();
@@ -1285,6 +1314,7 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
         ProfScope ps(c, ER_K_AXIS1);
         if ((e = launch_axis1<NI>(c, nch)) != hipSuccess) return e;
     }
+    if ((e = join_atoms(c)) != hipSuccess) return e;
     {
         ProfScope ps(c, ER_K_GATHER);
         if constexpr (NI == 1) e = launch_gather<1, 1>(c, n_off, nch);
@@ -1302,6 +1332,7 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
 template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, int nch)
 {
     hipError_t e;
+    if ((e = join_atoms(c)) != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
     {
         ProfScope ps(c, ER_K_PAIRS);
@@ -1444,6 +1475,7 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
             HIPCHK(c, hipStreamSynchronize(c->stream));
         }
     }
+    HIPCHK(c, join_atoms(c));  // a run without chunks still orders the next upload after the atom work
     hipLaunchKernelGGL(k_halt_capture, dim3(1), dim3(1), 0, c->stream, c->S().seq, NI, c->nummol,
                        c->S().d_errkey.p, c->S().d_U.p, c->S().d_u0.p, c->S().d_halt.p);
     HIPCHK(c, hipGetLastError());
@@ -1650,6 +1682,10 @@ extern "C" void er_destroy(er_ctx *c)
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_Ftab.release(); c->d_T.release(); c->d_phi.release();
     c->d_C.release(); c->d_w.release(); c->d_border.release(); c->d_flceng.release(); c->d_part.release(); c->d_epart.release();
+    if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream2) (void)hipStreamDestroy(c->stream2);
+    if (c->ev_in) (void)hipEventDestroy(c->ev_in);
+    if (c->ev_atoms) (void)hipEventDestroy(c->ev_atoms);
     delete c;
 }
