@@ -408,8 +408,12 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
         return bad(fail(c, ER_E_UNSUPPORTED, "er_create: cltype 0 (bare Coulomb), 2 (PME) and 3 (PPPM) are implemented"));
     if (prm->cltype == 3 && prm->splodr % 2 == 1)
         return bad(fail(c, ER_E_ARG, "er_create: interpolation order for PPPM must be even"));
-    if (prm->boxshp != 1)
-        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: only boxshp=1 (periodic) is implemented"));
+    if (prm->boxshp == 0) {  // an open system: bare Coulomb at constant volume (setconf.F90:361-364, set_prs)
+        if (prm->cltype != 0 || prm->estype == 2)
+            return bad(fail(c, ER_E_ARG, "er_create: a non-periodic system needs cltype=0 and estype=1 (set_prs)"));
+    } else if (prm->boxshp != 1) {
+        return bad(fail(c, ER_E_ARG, "er_create: boxshp must be 0 or 1"));
+    }
     if (prm->slttype < 1 || prm->slttype > 3) return bad(fail(c, ER_E_ARG, "er_create: slttype"));
     if (prm->ljswitch < 0 || prm->ljswitch > 3) return bad(fail(c, ER_E_ARG, "er_create: ljswitch"));
     if (!(prm->splodr == 4 || prm->splodr == 6 || prm->splodr == 8))
@@ -772,6 +776,22 @@ extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *t
 // realcal_prepare's cell handling (realcal.F90:776-896) + get_inverted_cell
 static int frame_cell(er_ctx *c, const float *cell)
 {
+    if (c->prm.boxshp == 0) {
+        // open system (realcal_prepare without normalize_periodic): no cell.
+        // Zero cell constants leave wrap_site an identity and min_image a
+        // sign change that keeps every norm; splines and the cell list see
+        // fractional coordinates 0 (they serve no kernel of this path).
+        FrameConst &fo = c->fc;
+        std::memset(&fo, 0, sizeof fo);
+        fo.is_cuboid = 1;
+        c->volume_d = 0.0;
+        for (int k = 0; k < 9; ++k) c->invcl_d[k] = 0.0;
+        c->nc = 1;
+        c->cell_w = 1.0e30f;
+        c->cell_changed = false;
+        std::memcpy(c->cell, cell, sizeof c->cell);
+        return ER_OK;
+    }
     // column-major: cell(i,j) = cell[i + 3j]; sub-diagonal (2,1),(3,1),(3,2)
     float cn[9];
     std::memcpy(cn, cell, sizeof cn);
@@ -1334,7 +1354,22 @@ template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, i
     hipError_t e;
     if ((e = join_atoms(c)) != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
-    {
+    if (c->prm.boxshp == 0) {  // open system: every molecule, no cell list
+        ProfScope ps(c, ER_K_PAIRS);
+        PairArgs pa{};
+        pa.nmol = c->nummol;
+        pa.nchunk = nch;
+        pa.n_off = n_off;
+        pa.nslt = c->S().nslt;
+        pa.NI = NI;
+        pa.ntype = c->ntype;
+        pa.pp = c->pp;
+        hipLaunchKernelGGL(k_pairs_open, dim3((unsigned)((c->nummol + 255) / 256), (unsigned)nch), dim3(256), 0,
+                           c->stream, pa, c->d_molbeg.p, c->d_posn.p, c->d_ljtype.p, c->d_insposn.p,
+                           c->S().d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
+                           c->d_ljlensq.p, c->S().d_U.p);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    } else {
         ProfScope ps(c, ER_K_PAIRS);
         PairArgs pa{};
         // enough workgroups per insertion to fill the chip (at least 4 per CU);

@@ -1619,6 +1619,43 @@ __device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const Fra
 
 // molecule_pair_sum with the insertion's sites and the LJ tables in LDS and
 // the molecule's first four atoms loaded together; same terms, same order
+// Open (non-periodic) systems, bare Coulomb (realcal.F90:367-666 with
+// boxshp 0, reelcut infinite): no periodic image, Coulomb between every site
+// pair, LJ within upljcut.  One thread per (insertion, molecule) of the
+// chunk, solute sites then molecule atoms in order; the sum is stored in U
+// (every molecule is written; a soln solute's own entry is 0).
+__global__ void __launch_bounds__(256) k_pairs_open(PairArgs pa, const int *__restrict__ molbeg,
+                                                    const float4 *__restrict__ posn, const int *__restrict__ ljtype,
+                                                    const float4 *__restrict__ insposn,
+                                                    const int *__restrict__ sltlj, const int *__restrict__ tag,
+                                                    const float *__restrict__ ljene,
+                                                    const float *__restrict__ ljlensq, float *__restrict__ U)
+{
+    const int jm = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nl = blockIdx.y;
+    if (jm >= pa.nmol || nl >= pa.nchunk) return;
+    const int n = pa.n_off + nl;
+    PairParams p = pa.pp;
+    p.elecut = __builtin_huge_valf();
+    float e = 0.0f;
+    const bool soln = tag != nullptr;
+    if (!(soln && jm == tag[n])) {
+        const int slt_a0 = soln ? molbeg[tag[n]] : 0;
+        for (int s = 0; s < pa.nslt; ++s) {
+            const float4 xa = insposn[(size_t)n * pa.nslt + s];
+            const int ti = soln ? ljtype[slt_a0 + s] : sltlj[s];
+            for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
+                const float4 xb = posn[b];
+                const float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
+                const int tj = ljtype[b];
+                e += pair_energy(p, dx * dx + dy * dy + dz * dz, xa.w, xb.w, ljene[ti + pa.ntype * tj],
+                                 ljlensq[ti + pa.ntype * tj]);
+            }
+        }
+    }
+    U[((size_t)(n / pa.NI) * pa.nmol + jm) * pa.NI + (n % pa.NI)] = e;
+}
+
 __device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const FrameConst &fc, int jm,
                                                        const int *__restrict__ molbeg,
                                                        const float4 *__restrict__ posn,

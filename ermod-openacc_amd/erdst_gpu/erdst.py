@@ -10,7 +10,8 @@ HISTORY and SltConf; runs the engdiv x frame loop of trjmain
 corsln|corref.XX, slfeng.XX, weight_soln|weight_refs, aveuv.tt, uvrange.tt
 and flcuv.tt|progress.tt.
 
-Supported: bare Coulomb, PME or PPPM (cltype 0, 2, 3) in a periodic cell; soln
+Supported: bare Coulomb, PME or PPPM (cltype 0, 2, 3) in a periodic cell,
+and bare Coulomb in an open system (boxshp 0); soln
 runs; refs runs whose solute configurations are used as they are
 (insposition 1 and insorient 1):
 flexible solutes (slttype 3) from SltConf, one DCD frame per insertion,
@@ -45,8 +46,8 @@ def _hot_params(p: dict) -> dict:
 def _check_supported(p: dict):
     if p["cltype"] not in (0, 2, 3):
         raise InputError("bare Coulomb (cltype = 0), PME (2) and PPPM (3) run on the GPU path, not Ewald (1)")
-    if p["boxshp"] != 1:
-        raise InputError("only periodic systems (boxshp = 1) run on the GPU path")
+    if p["boxshp"] not in (0, 1):
+        raise InputError("boxshp must be 0 (open, bare Coulomb) or 1 (periodic)")
     if p["slttype"] != SLT_SOLN and not (p["insposition"] == 1 and p["insorient"] == 1):
         raise InputError("refs runs need the solute configurations as they are (insposition = 1, insorient = 1)")
     if p["slttype"] != SLT_SOLN and p["insstructure"] != 0:

@@ -72,7 +72,7 @@ typedef struct {
     int32_t cltype;             /* 0 bare Coulomb, 2 PME, 3 PPPM (even splodr) */
     int32_t splodr;             /* B-spline order, 2..12                    */
     int32_t ms[3];              /* ms1max, ms2max, ms3max                    */
-    int32_t boxshp;             /* only 1 (periodic) is implemented         */
+    int32_t boxshp;             /* 1 periodic; 0 open (cltype 0, estype 1)  */
     int32_t slttype;            /* 1 soln, 2 refs rigid, 3 refs flexible    */
     int32_t wgtslf, estype, corrcal, selfcal;
     double temp;                /* kT in kcal/mol (setconf.F90:304)         */

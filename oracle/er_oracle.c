@@ -54,6 +54,7 @@ int orc_normalize_periodic(const double *cell, double elecut, double upljcut,
                            int n, const double *pos, double *pos_normal, orc_box *box)
 {
     memcpy(box->cell_normal, cell, 9 * sizeof(double));
+    box->open = 0;
     memset(box->rot, 0, sizeof(box->rot));
     box->rot[0] = box->rot[4] = box->rot[8] = 1.0;
     const int rotate = C(cell,1,0) != 0.0 || C(cell,2,0) != 0.0 || C(cell,2,1) != 0.0;
@@ -115,6 +116,7 @@ int orc_normalize_periodic(const double *cell, double elecut, double upljcut,
 /* minimum image, realcal.F90:271-284 */
 static void min_image(const orc_box *box, double *d)
 {
+    if (box->open) return;   /* boxshp 0: no periodic images */
     if (box->is_cuboid) {
         for (int k = 0; k < 3; k++) {
             double h = 0.5 * box->cell_len[k];
@@ -215,6 +217,11 @@ double orc_pair_energy(const orc_params *p, const orc_box *box,
     double d[3] = {xa[0] - xb[0], xa[1] - xb[1], xa[2] - xb[2]};
     min_image(box, d);
     double dis2 = d[0] * d[0] + d[1] * d[1] + d[2] * d[2];
+    if (box->open) {          /* reelcut = infinity for an open system (realcal.F90:388) */
+        orc_params q = *p;
+        q.elecut = INFINITY;
+        return pair_from_dis2(&q, dis2, qa, qb, eps, sgm2);
+    }
     return pair_from_dis2(p, dis2, qa, qb, eps, sgm2);
 }
 
@@ -892,7 +899,12 @@ int orc_energies_refs(const orc_topo *t, const orc_params *p, const double *cell
     memcpy(all, pos, sizeof(double) * 3 * t->natom);
     memcpy(all + 3 * (size_t)t->natom, slt_pos, sizeof(double) * 3 * (size_t)nins * nslt);
     orc_box box;
-    if (orc_normalize_periodic(cell, p->elecut, p->upljcut, (int)nall, all, alln, &box)) {
+    memset(&box, 0, sizeof box);
+    if (p->boxshp == 0) {     /* realcal_prepare without normalize_periodic */
+        if (p->cltype != 0) { free(all); free(alln); return 3; }   /* set_prs */
+        memcpy(alln, all, sizeof(double) * 3 * nall);
+        box.open = 1;
+    } else if (orc_normalize_periodic(cell, p->elecut, p->upljcut, (int)nall, all, alln, &box)) {
         free(all); free(alln); return 1;
     }
     const double *sltn = alln + 3 * (size_t)t->natom;
@@ -969,7 +981,12 @@ int orc_energies_soln(const orc_topo *t, const orc_params *p, const double *cell
     orc_inverted_cell(cell, invcl, &volume);
     double *posn = malloc(sizeof(double) * 3 * t->natom);
     orc_box box;
-    if (orc_normalize_periodic(cell, p->elecut, p->upljcut, t->natom, pos, posn, &box)) {
+    memset(&box, 0, sizeof box);
+    if (p->boxshp == 0) {
+        if (p->cltype != 0) { free(posn); return 3; }
+        memcpy(posn, pos, sizeof(double) * 3 * t->natom);
+        box.open = 1;
+    } else if (orc_normalize_periodic(cell, p->elecut, p->upljcut, t->natom, pos, posn, &box)) {
         free(posn); return 1;
     }
     if (p->cltype == 0) {

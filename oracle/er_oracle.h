@@ -43,6 +43,7 @@ typedef struct {
     int splodr;
     int ms[3];
     int cltype;                /* 0 = bare Coulomb, 2 = PME, 3 = PPPM (erdst/engmain.F90:359) */
+    int boxshp;                /* 1 = periodic, 0 = open (bare Coulomb only)      */
 } orc_params;
 
 /* cell normalisation result (erdst/realcal.F90:22-26) */
@@ -52,6 +53,7 @@ typedef struct {
     double invcell[3];
     int is_cuboid;
     double rot[9];             /* Q^T applied to coordinates, (i,j) at i+3*j */
+    int open;                  /* boxshp 0: no image, Coulomb without cutoff */
 } orc_box;
 
 /* erdst/engproc.F90:1049-1070 */

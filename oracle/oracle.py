@@ -38,12 +38,13 @@ class Topo(C.Structure):
 class Params(C.Structure):
     _fields_ = [("elecut", C.c_double), ("lwljcut", C.c_double), ("upljcut", C.c_double),
                 ("screen", C.c_double), ("ljswitch", C.c_int), ("splodr", C.c_int),
-                ("ms", C.c_int * 3), ("cltype", C.c_int)]
+                ("ms", C.c_int * 3), ("cltype", C.c_int), ("boxshp", C.c_int)]
 
 
 class Box(C.Structure):
     _fields_ = [("cell_normal", C.c_double * 9), ("cell_len", C.c_double * 3),
-                ("invcell", C.c_double * 3), ("is_cuboid", C.c_int), ("rot", C.c_double * 9)]
+                ("invcell", C.c_double * 3), ("is_cuboid", C.c_int), ("rot", C.c_double * 9),
+                ("open", C.c_int)]
 
 
 class Dist(C.Structure):
@@ -118,12 +119,13 @@ def i32(a):
 
 
 def make_params(elecut=12.0, lwljcut=10.0, upljcut=12.0, screen=0.35, ljswitch=0, splodr=4,
-                ms=(32, 32, 32), cltype=2) -> Params:
+                ms=(32, 32, 32), cltype=2, boxshp=1) -> Params:
     p = Params()
     p.elecut, p.lwljcut, p.upljcut, p.screen = elecut, lwljcut, upljcut, screen
     p.ljswitch, p.splodr = ljswitch, splodr
     p.ms[0], p.ms[1], p.ms[2] = ms
     p.cltype = cltype
+    p.boxshp = boxshp
     return p
 
 
@@ -259,7 +261,8 @@ def normalize_periodic(cell, pos, elecut=12.0, upljcut=12.0):
     rc = lib().orc_normalize_periodic(f64(np.asarray(cell).ravel(order="F")), elecut, upljcut,
                                       pos.shape[0], pos.ravel(), out.ravel(), C.byref(box))
     if rc:
-        raise ValueError("PPPM needs an even order" if rc == 2 else "cutoff exceeds half the box")
+        raise ValueError({2: "PPPM needs an even order", 3: "an open system needs bare Coulomb"}.get(
+            rc, "cutoff exceeds half the box"))
     return out, box
 
 
@@ -283,7 +286,8 @@ def energies_refs(sysm: OracleSystem, params: Params, cell, pos, slt_charge, slt
                                  f64(slt_charge), i32(slt_ljtype), nins, f64(slt_pos).ravel(),
                                  int(rigid), uv, uv0, int(nthreads))
     if rc:
-        raise ValueError("PPPM needs an even order" if rc == 2 else "cutoff exceeds half the box")
+        raise ValueError({2: "PPPM needs an even order", 3: "an open system needs bare Coulomb"}.get(
+            rc, "cutoff exceeds half the box"))
     return uv.reshape(nins, nmol), uv0
 
 
@@ -296,7 +300,8 @@ def energies_soln(sysm: OracleSystem, params: Params, cell, pos, tagslt, nthread
                                  f64(np.asarray(cell).ravel(order="F")), f64(pos).ravel(), tag.size,
                                  tag, uv, uv0, int(nthreads))
     if rc:
-        raise ValueError("PPPM needs an even order" if rc == 2 else "cutoff exceeds half the box")
+        raise ValueError({2: "PPPM needs an even order", 3: "an open system needs bare Coulomb"}.get(
+            rc, "cutoff exceeds half the box"))
     return uv.reshape(tag.size, nmol), uv0
 
 

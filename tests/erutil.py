@@ -39,7 +39,7 @@ def oracle_system(topo) -> O.OracleSystem:
 def oracle_params(p: dict) -> O.Params:
     return O.make_params(elecut=p["elecut"], lwljcut=p["lwljcut"], upljcut=p["upljcut"],
                          screen=p["screen"], ljswitch=p["ljswitch"], splodr=p["splodr"], ms=p["ms"],
-                         cltype=p.get("cltype", 2))
+                         cltype=p.get("cltype", 2), boxshp=p.get("boxshp", 1))
 
 
 def workload_params(w: synth.Workload, **over) -> dict:

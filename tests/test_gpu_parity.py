@@ -280,6 +280,34 @@ def test_bare_coulomb_cfg0(cfg0):
     np.testing.assert_allclose(g["engnorm"], 64 * np.float32(np.prod(np.diag(cfg0.cell))), rtol=1e-6)
 
 
+@pytest.mark.parametrize("mode", [capi.ER_BATCHED, capi.ER_PLAIN])
+def test_open_system_bare_coulomb_cfg0(cfg0, mode):
+    """boxshp 0 (an open cluster: the configuration-0 coordinates taken as
+    they are), bare Coulomb between every pair, constant volume: energies
+    against the oracle; uvengy0 = 0; the distribution against the oracle's
+    binning of the GPU's energies with no volume factor."""
+    p = U.workload_params(cfg0, cltype=0, boxshp=0, estype=1)
+    ctx = _ctx(cfg0, p)
+    d = U.oracle_distribution(cfg0, p)
+    u, u0 = _run_refs(ctx, cfg0, 0, mode, nins=24)
+    uo, u0o = U.oracle_refs(cfg0, 0, p, nins=24)
+    nbad, worst, at = U.energy_mismatch(u, uo)
+    assert nbad == 0, (worst, at, u[at], uo[at])
+    assert np.all(u0 == 0.0)
+    # far molecules still interact (no Coulomb cutoff in an open system)
+    assert np.count_nonzero(u) > 0.9 * u.size
+    for n in range(u.shape[0]):
+        d.update(cfg0.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+    g = ctx.fetch()
+    _assert_same_distribution(g, d)
+    np.testing.assert_allclose(g["engnorm"], 24.0, rtol=1e-6)
+
+
+def test_open_system_needs_bare_coulomb(cfg0):
+    with pytest.raises(capi.ErError):
+        capi.Context(cfg0.topo, U.workload_params(cfg0, boxshp=0, estype=1))
+
+
 def test_rotated_cell_cfg0(cfg0):
     """The whole configuration-0 system rotated, so the cell is not upper
     triangular (make_cell_uppertriangular, realcal.F90:776-817): energies

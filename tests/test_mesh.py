@@ -85,7 +85,7 @@ def test_ecdinfo_overrides_hist(tmp_path):
     assert uvsoft[0] == soft
 
 
-@pytest.mark.parametrize("over,msg", [(dict(cltype=1), "Ewald"), (dict(boxshp=0, estype=1, cltype=0), "periodic"),
+@pytest.mark.parametrize("over,msg", [(dict(cltype=1), "Ewald"), (dict(boxshp=2), "boxshp"), (dict(insstructure=1), "structure"),
                                       (dict(insposition=0), "as they are")])
 def test_driver_rejects_what_the_gpu_path_does_not_run(over, msg):
     p = dict(slttype=3, cltype=2, boxshp=1, insposition=1, insorient=1, insstructure=0)

@@ -491,3 +491,44 @@ def test_rotated_system_has_the_same_energies():
                                   nthreads=2)
         np.testing.assert_allclose(ur, u, rtol=1e-9, atol=1e-9)
         np.testing.assert_allclose(u0r, u0, rtol=1e-9, atol=1e-9)
+
+
+def test_open_system_bare_coulomb_is_the_all_pairs_sum():
+    """boxshp 0 (realcal.F90:387-395, 367-666): no periodic images, Coulomb
+    between every pair (reelcut infinite), LJ within upljcut; a second
+    configuration shifted by a box length changes the energies, which a
+    periodic treatment would not."""
+    rng = np.random.default_rng(13)
+    n = 30
+    pos = rng.uniform(0, 20.0, (2 * n, 3))
+    pos[1::2] = pos[::2] + rng.normal(0, 0.9, (n, 3))
+    q = np.tile([0.4, -0.4], n) * CH
+    slt = rng.uniform(0, 20.0, (1, 3, 3))
+    slt[0, 1:] = slt[0, 0] + rng.normal(0, 0.9, (2, 3))
+    qs = np.array([0.3, -0.1, -0.2]) * CH
+    eps, sig2 = np.array([[0.15]]), np.array([[9.0]])
+    sysm = O.OracleSystem(np.arange(0, 2 * n + 1, 2), q, np.zeros(2 * n, np.int32), eps, sig2)
+    p = O.make_params(cltype=0, boxshp=0, upljcut=12.0, lwljcut=10.0, ljswitch=0)
+    uv, uv0 = O.energies_refs(sysm, p, np.zeros((3, 3)), pos, qs, np.zeros(3, np.int32), slt, nthreads=2)
+    want = np.zeros(n)
+    for m in range(n):
+        for a in (2 * m, 2 * m + 1):
+            for s in range(3):
+                r = np.linalg.norm(slt[0, s] - pos[a])
+                e = qs[s] * q[a] / r
+                if r <= 12.0:
+                    x6 = (sig2[0, 0] / r ** 2) ** 3
+                    lj = 4 * eps[0, 0] * x6 * (x6 - 1)
+                    if r > 10.0:
+                        lw2, up2 = 100.0, 144.0
+                        lj *= (2 * r * r + up2 - 3 * lw2) * (r * r - up2) ** 2 / (up2 - lw2) ** 3
+                    e += lj
+                want[m] += e
+    np.testing.assert_allclose(uv[0], want, rtol=1e-12, atol=1e-12)
+    assert uv0[0] == 0.0
+    far = slt + np.array([60.0, 0.0, 0.0])
+    uvf, _ = O.energies_refs(sysm, p, np.zeros((3, 3)), pos, qs, np.zeros(3, np.int32), far, nthreads=2)
+    assert np.max(np.abs(uvf - uv)) > 1e-3
+    with pytest.raises(ValueError, match="bare Coulomb"):
+        O.energies_refs(sysm, O.make_params(cltype=2, boxshp=0), np.zeros((3, 3)), pos, qs,
+                        np.zeros(3, np.int32), slt, nthreads=2)
