@@ -1,8 +1,9 @@
 """Stream file read by the Fortran host driver (fortran/erdst_host.f90).
 
 Layout (little-endian, no record markers):
-  int32[20]  numatm nummol ntype numslv nslt nins nframes ermax slttype
+  int32[22]  numatm nummol ntype numslv nslt nins nframes ermax slttype
              ms1 ms2 ms3 splodr ljswitch estype corrcal wgtslf selfcal engdiv esmax
+             cltype boxshp
   float32[4] elecut lwljcut upljcut screen ; float64 temp
   int32 mol_begin[nummol+1] (1-based), ljtype[numatm] (1-based), uvspec[nummol]
   float32 charge[numatm], ljene[ntype^2], ljlensq[ntype^2]
@@ -33,7 +34,8 @@ def write_workload(path, w, params: dict, frames, engdiv: int = 1, nins: int | N
     frames = list(frames)
     hdr = np.array([t.natom, t.nmol, t.ntype, t.numslv, nslt, nins, len(frames), int(um.sum()),
                     params["slttype"], *params["ms"], params["splodr"], params["ljswitch"], params["estype"],
-                    params["corrcal"], params["wgtslf"], params["selfcal"], engdiv, esmax], np.int32)
+                    params["corrcal"], params["wgtslf"], params["selfcal"], engdiv, esmax,
+                    params.get("cltype", 2), params.get("boxshp", 1)], np.int32)
     with open(path, "wb") as fh:
         hdr.tofile(fh)
         np.array([params["elecut"], params["lwljcut"], params["upljcut"], params["screen"]], np.float32).tofile(fh)

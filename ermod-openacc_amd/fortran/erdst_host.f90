@@ -173,7 +173,7 @@ contains
   end subroutine check
 
   subroutine read_workload()
-    integer(c_int32_t) :: hdr(20)
+    integer(c_int32_t) :: hdr(22)
     real(c_float) :: rp(4)
     real(c_double) :: t
     read(iu) hdr
@@ -184,7 +184,7 @@ contains
     engdiv = hdr(19); esmax = hdr(20)
     if (esmax > 0 .and. prm%selfcal /= 1) stop "workload: a self-energy mesh needs selfcal = 1"
     if (esmax <= 0 .and. prm%selfcal == 1) stop "workload: selfcal = 1 needs a self-energy mesh"
-    prm%cltype = 2; prm%boxshp = 1
+    prm%cltype = hdr(21); prm%boxshp = hdr(22)
     read(iu) rp, t
     prm%elecut = rp(1); prm%lwljcut = rp(2); prm%upljcut = rp(3); prm%screen = rp(4); prm%temp = t
     allocate(mol_begin(nummol + 1), ljtype(numatm), uvspec(nummol), charge(numatm))

@@ -178,6 +178,28 @@ def test_fortran_host_reads_history_and_sltconf(tmp_path):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("variant", [dict(cltype=3), dict(cltype=0), dict(cltype=0, boxshp=0, estype=1)])
+def test_fortran_host_interaction_variants(tmp_path, variant):
+    """PPPM, bare Coulomb and an open system through the Fortran host: its
+    engref equals the Python binding's on the same inputs."""
+    w = synth.config0()
+    p = U.workload_params(w, **variant)
+    write_workload(tmp_path / "workload.bin", w, p, [0, 1], nins=24)
+    r = subprocess.run([HOST, "workload.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    for f in (0, 1):
+        ctx.set_frame(w.frame(f), w.cell)
+        ctx.set_insertions(w.insertions(f)[:24])
+        ctx.run(capi.ER_BATCHED)
+    g = ctx.fetch()
+    eng = read_engfile(tmp_path / "engref.tt")
+    np.testing.assert_allclose(eng[:, 3], g["edens"] / g["engnorm"], rtol=1e-13, atol=1e-300)
+
+
+@pytest.mark.gpu
 def test_fortran_host_files_feed_slvfe(tmp_path):
     """erdst (Fortran host, soln and refs runs) -> files -> slvfe merge: the
     solvation free energy equals the one from the Python binding's blocks."""
