@@ -86,13 +86,17 @@ def bin_info(uvmax, uvsoft, uvcrd):
             g = off + i
             idpt = i + 1
             spec[g] = p + 1
-            if idpt <= uvsoft[p]:
+            if idpt <= uvsoft[p] and g + 1 < ermax:
+                # (for the last bin of a species without a core region this is
+                # the next species' first point, as the reference reads it)
                 mid[g] = (uvcrd[g] + uvcrd[g + 1]) / 2.0
                 inv[g] = 1.0 / (uvcrd[g + 1] - uvcrd[g])
             elif idpt < n:
                 mid[g] = math.sqrt(uvcrd[g] * uvcrd[g + 1])
                 inv[g] = 1.0 / (uvcrd[g + 1] - uvcrd[g])
             else:
+                # the open top bin; also the last bin overall when its species
+                # has no core region, where the reference reads past the mesh
                 mid[g] = uvcrd[g]
                 inv[g] = 0.0
         off += n

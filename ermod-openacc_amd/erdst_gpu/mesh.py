@@ -7,7 +7,8 @@ ecfbin, ec0bin, ecfbin, eclbin, then pecore log-spaced bins up to ecdmax.
 With meshread an EcdMesh block replaces the built mesh.  As in the reference,
 &hist values the namelist does not set keep the values the previous species
 left in them (the boundaries are shifted in place), and the EcdMesh line
-counter is not reset for a species without a block.
+counter is the mesh build's loop index, so a species without a block keeps
+its &hist mesh only when that mesh has a core region.
 """
 from __future__ import annotations
 
@@ -30,6 +31,13 @@ def _nint(x: float) -> int:
 def build_mesh(h: dict):
     """Mesh of one species from the &hist values in h (modified in place as
     the reference modifies them).  Returns (mesh [pemax], pemax, pesoft)."""
+    return _build_mesh(h)[:3]
+
+
+def _build_mesh(h: dict):
+    """build_mesh, plus the DO index the build's last region loop ends with
+    (engproc.F90:222): maxrg of that region, i.e. pemax with a core region
+    and pesoft + 1 without."""
     ectmvl = h["finfac"] * h["ecfbin"]
     h["ecdmin"] = h["ecdmin"] - ectmvl
     h["ecfmns"] = h["ecfmns"] - ectmvl
@@ -65,7 +73,7 @@ def build_mesh(h: dict):
         for iduv in range(minrg, maxrg):
             inc = factor * float(np.float32(iduv - minrg))
             ercrd[iduv] = ercrd[minrg] + inc if regn <= RGLMAX else ercrd[minrg] * math.exp(inc)
-    return ercrd[1:pemax + 1].copy(), pemax, pesoft
+    return ercrd[1:pemax + 1].copy(), pemax, pesoft, uprgcd[nreg]
 
 
 def _ecdinfo_line(path, pti):
@@ -89,7 +97,6 @@ def species_meshes(parameters_path, numslv: int, ecdinfo_path=None, ecdmesh_path
     h: dict = {}
     uvmax, uvsoft, crd = [], [], []
     escrd = None
-    mesh_iduv = 0
     for pti in range(numslv + 1):
         nml = read_namelist(parameters_path, "hist")
         for k in _HIST_REAL:
@@ -122,7 +129,10 @@ def species_meshes(parameters_path, numslv: int, ecdinfo_path=None, ecdmesh_path
                     h["pecore"] = 0
                 if h["pecore"] == 1:
                     raise InputError("EcdInfo pecore", "eng_pcr")
-        mesh, pemax, pesoft = build_mesh(h)
+        # the reference reuses the build loop's index as the EcdMesh line
+        # counter: a species without a block passes the count check only when
+        # that index ended at pemax (a core region), as erdst's does
+        mesh, pemax, pesoft, mesh_iduv = _build_mesh(h)
         if h["meshread"] == 1:
             if ecdmesh_path is None:
                 raise InputError("EcdMesh requested but absent", "eng_ecm")

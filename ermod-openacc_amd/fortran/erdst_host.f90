@@ -220,7 +220,7 @@ contains
     spec = cnt
     idpt = iduv - idmin
     engleft = uvcrd(iduv)
-    if (idpt <= uvsoft(spec)) then
+    if (idpt <= uvsoft(spec) .and. iduv < ermax) then
        engmiddle = (uvcrd(iduv) + uvcrd(iduv + 1)) / 2.0
        enginvwidth = 1. / (uvcrd(iduv + 1) - uvcrd(iduv))
     else if (idpt < uvmax(spec)) then

@@ -92,3 +92,23 @@ def test_driver_rejects_what_the_gpu_path_does_not_run(over, msg):
     p.update(over)
     with pytest.raises(I.InputError, match=msg):
         erdst._check_supported(p)
+
+
+@pytest.mark.parametrize("pecore,ok", [(200, True), (0, False)])
+def test_ecdmesh_species_without_a_block(tmp_path, pecore, ok):
+    """With meshread, a species with no EcdMesh block keeps its &hist mesh
+    when that mesh has a core region: the line counter is the build loop's
+    index, which then ends at pemax (engproc.F90:222-228, 238-261); without
+    a core region the count check fails (eng_emf)."""
+    (tmp_path / "parameters_er").write_text(hist_text(dict(HIST, pecore=pecore), ",\n  meshread = 1"))
+    own = np.linspace(-10, 5, 301)
+    lines = [f"1 {own.size} 0"] + [f"1 {i + 1} {float(v)!r}" for i, v in enumerate(own)]
+    (tmp_path / "EcdMesh").write_text("\n".join(lines) + "\n")
+    if ok:
+        _, _, uvcrd, escrd = M.species_meshes(tmp_path / "parameters_er", 1, ecdmesh_path=tmp_path / "EcdMesh")
+        ref, soft = O.build_mesh(**dict(HIST, pecore=pecore))
+        np.testing.assert_allclose(escrd, ref[:soft], rtol=1e-14, atol=1e-13)
+        np.testing.assert_array_equal(uvcrd, own)
+    else:
+        with pytest.raises(I.InputError, match="eng_emf"):
+            M.species_meshes(tmp_path / "parameters_er", 1, ecdmesh_path=tmp_path / "EcdMesh")

@@ -184,3 +184,16 @@ def test_selfenergy_bin_coords():
     c = engstore.selfenergy_bin_coords(escrd)
     assert c.dtype == np.float32
     np.testing.assert_array_equal(c, np.float32([-0.75, -0.2, 1.05, 2.0]))
+
+
+def test_bin_info_without_core_region():
+    """A species without a core region: its last soft bin averages with the
+    next species' first point (as the reference reads uvcrd(iduv + 1)); the
+    last bin of the last species has no next point and is the open top bin."""
+    uvmax, uvsoft = np.array([3, 3]), np.array([3, 3])
+    uvcrd = np.array([-1.0, 0.0, 1.0, -2.0, -1.5, -1.0])
+    left, mid, inv, spec = engstore.bin_info(uvmax, uvsoft, uvcrd)
+    np.testing.assert_array_equal(spec, [1, 1, 1, 2, 2, 2])
+    assert mid[2] == (1.0 + -2.0) / 2 and inv[2] == 1.0 / (-2.0 - 1.0)
+    assert mid[5] == -1.0 and inv[5] == 0.0
+    np.testing.assert_array_equal(mid[:2], [-0.5, 0.5])
