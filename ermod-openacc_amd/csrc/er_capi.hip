@@ -768,6 +768,7 @@ extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *t
     }
     c->nsltmol = nslt;
     c->S().nslt = ns;
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // a run in flight may still read the list
     HIPCHK(c, c->d_tag.ensure(nslt));
     HIPCHK(c, hipMemcpy(c->d_tag.p, c->tagslt.data(), nslt * sizeof(int), hipMemcpyHostToDevice));
     return ER_OK;

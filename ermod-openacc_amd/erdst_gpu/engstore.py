@@ -190,17 +190,21 @@ def write_slfeng(path, eself_norm, escrd):
             fh.write(fortran_g(coord[i], 15, 7) + fortran_g(v, 25, 15) + "\n")
 
 
-def flcuv_lines(frame_label: int, flceng) -> list[str]:
+def flcuv_lines(frame_label: int, flceng, indices=None, maxdst: int | None = None) -> list[str]:
     """flcuv.tt lines of one frame of a soln run (erdst/engproc.F90:485-499):
     per solute molecule the sum of its pair energies with each solvent
-    species; the solute index is written only when the frame has more than
-    one.  flceng [nsolute, numslv] is stored as real (float32) like the
-    reference's accumulator."""
+    species; the solute index is written only when the run has more than
+    one solute molecule.  flceng [rows, numslv] is stored as real (float32)
+    like the reference's accumulator; indices (0-based, default all) name
+    the solute molecule of each row when some were rejected this frame, and
+    maxdst is the run's solute-molecule count (default: the rows)."""
     fl = np.atleast_2d(np.asarray(flceng, np.float64)).astype(np.float32).astype(np.float64)
+    idx = list(range(fl.shape[0])) if indices is None else list(indices)
+    maxdst = fl.shape[0] if maxdst is None else maxdst
     vals = lambda row: "".join(fortran_f(v, 15, 5) for v in row)
-    if fl.shape[0] == 1:
-        return [f"{frame_label:9d}" + vals(fl[0])]
-    return [f"{k:9d}{frame_label:9d}" + vals(fl[k - 1]) for k in range(1, fl.shape[0] + 1)]
+    if maxdst == 1:
+        return [f"{frame_label:9d}" + vals(fl[r]) for r in range(len(idx))]
+    return [f"{k + 1:9d}{frame_label:9d}" + vals(fl[r]) for r, k in enumerate(idx)]
 
 
 def progress_line(frame_label: int) -> str:
@@ -227,7 +231,7 @@ class BlockWriter:
         self.avediv1, self.avediv2, self.aveuv = [], [], []
         self.nframe = 0
 
-    def frame(self, flceng=None, skpcnf: int = 1):
+    def frame(self, flceng=None, skpcnf: int = 1, indices=None, maxdst=None):
         """After each frame's er_run, as engconst does (erdst/engproc.F90:
         308-315, 485-509): soln runs append flceng [nsolute, numslv] to
         flcuv.tt, refs runs the frame number to progress.tt.  The reference
@@ -238,7 +242,7 @@ class BlockWriter:
         name = os.path.join(self.dir, "flcuv.tt" if self.soln else "progress.tt")
         if self.nframe == 1 and os.path.exists(name):
             raise FileExistsError(name)
-        lines = flcuv_lines(label, flceng) if self.soln else [progress_line(label)]
+        lines = flcuv_lines(label, flceng, indices, maxdst) if self.soln else [progress_line(label)]
         with open(name, "a") as fh:
             fh.write("".join(ln + "\n" for ln in lines))
 

@@ -16,8 +16,9 @@ runs; refs runs whose solute configurations are used as they are
 (insposition 1 and insorient 1):
 flexible solutes (slttype 3) from SltConf, one DCD frame per insertion,
 reopened at its end, and rigid solutes (slttype 2) at their SltInfo
-coordinates.  Random insertion (the reference's compiler-specific
-random_number stream) and region restraints raise InputError.
+coordinates.  Soln runs apply the sphere and slab region rejection of
+check_mol_configuration.  Random insertion (the reference's compiler-specific
+random_number stream) and RMSD-based restraints raise InputError.
 """
 from __future__ import annotations
 
@@ -43,6 +44,32 @@ def _hot_params(p: dict) -> dict:
                 temp=float(p["temp"]))
 
 
+def region_distance(p: dict, sysin, xyz, cell, mol: int) -> float:
+    """check_mol_configuration's distance for a soln solute molecule
+    (erdst/engproc.F90:1197-1245): its centre of mass relative to the
+    aggregate of the hostspec species (erdst/insertion.F90:587-614,
+    erdst/bestfit.F90:252-266), both mass-weighted means of the raw
+    coordinates; the norm for a sphere, the component along c for a slab
+    (its magnitude for a symmetric slab)."""
+    topo = sysin.topology
+    mb, mass = topo.mol_begin, sysin.mass.astype(np.float64)
+    x = np.asarray(xyz, np.float64)
+
+    def com(sel):
+        m = mass[sel]
+        if m.sum() == 0.0:
+            raise InputError("zero total mass", "bst_zrw")
+        return (x[sel] * m[:, None]).sum(0) / m.sum()
+    hosts = {int(v) for v in p["hostspec"].values()} if isinstance(p["hostspec"], dict) else set()
+    agg = np.concatenate([np.arange(mb[i], mb[i + 1]) for i in range(topo.nmol) if topo.moltype[i] in hosts])
+    dx = com(np.arange(mb[mol], mb[mol + 1])) - com(agg)
+    if p["insposition"] == 2:
+        return float(np.sqrt(dx @ dx))
+    c = np.asarray(cell, np.float64)
+    d = float(np.linalg.inv(c)[2] @ dx) * float(np.linalg.norm(c[:, 2]))
+    return abs(d) if p["insposition"] == 4 else d
+
+
 def _check_supported(p: dict):
     if p["cltype"] not in (0, 2, 3):
         raise InputError("bare Coulomb (cltype = 0), PME (2) and PPPM (3) run on the GPU path, not Ewald (1)")
@@ -52,6 +79,14 @@ def _check_supported(p: dict):
         raise InputError("refs runs need the solute configurations as they are (insposition = 1, insorient = 1)")
     if p["slttype"] != SLT_SOLN and p["insstructure"] != 0:
         raise InputError("structure rejection (insstructure) is not supported")
+    if p["slttype"] == SLT_SOLN and p["insposition"] in (5, 6):
+        raise InputError("RMSD-based region rejection (insposition 5, 6) is not supported")
+    if p["slttype"] == SLT_SOLN and p["insposition"] in (2, 3, 4):
+        hosts = [v for v in p["hostspec"].values()] if isinstance(p["hostspec"], dict) else []
+        if not any(int(v) >= 1 for v in hosts):
+            raise InputError("a region needs a hostspec species", "set_ins")
+        if p["lwreg"] > p["upreg"] or (p["insposition"] != 3 and min(p["lwreg"], p["upreg"]) < 0.0):
+            raise InputError("region bounds", "set_reg")
 
 
 def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log=print) -> dict:
@@ -98,6 +133,7 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
         if p["wgtins"] == 1:
             sltwgt = WeightFile(path("SltWght"), rewind=True)
     g = None
+    active = list(range(len(topo.tagslt)))
     for division in range(1, engdiv + 1):
         for _ in range(frames_per_div):
             w_sys = np.float32(1.0)
@@ -129,6 +165,20 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
                     if sltwgt is not None:
                         w_slt[n] = sltwgt.next()
                 ctx.set_insertions(ins, w_slt if sltwgt is not None else None)
+            if soln and p["insposition"] in (2, 3, 4):
+                # check_mol_configuration per solute molecule (engproc.F90:
+                # 750-756): a rejected one adds nothing this frame
+                keep = [k for k, m in enumerate(topo.tagslt)
+                        if p["lwreg"] <= region_distance(p, sysin, fr[0], fr[1], int(m)) <= p["upreg"]]
+                if not keep:
+                    writer.frame(np.zeros((0, topo.numslv)), skpcnf, [], len(topo.tagslt))
+                    continue
+                if keep != active:
+                    ctx.set_solute_molecules(topo.tagslt[keep])
+                    active = keep
+                ctx.run(mode)
+                writer.frame(ctx.flceng(len(keep)), skpcnf, keep, len(topo.tagslt))
+                continue
             ctx.run(mode)
             writer.frame(ctx.flceng(len(topo.tagslt)) if soln else None, skpcnf)
         g = ctx.fetch()

@@ -142,3 +142,69 @@ def test_erdst_driver_soln_run_directory(tmp_path):
     assert names == ["aveuv.tt", "engsln.tt", "flcuv.tt", "uvrange.tt", "weight_soln"]
     for n in names:
         assert (run_dir / n).read_bytes() == (ref_dir / n).read_bytes(), n
+
+
+def test_erdst_driver_soln_sphere_region(tmp_path):
+    """Soln with a sphere region around the water aggregate (insposition 2,
+    hostspec = species 2): the bounds accept the solute in one of two frames
+    only (distances from the mass-weighted centres); the outputs equal the
+    binding's run over the accepted frame, and flcuv.tt has that frame's
+    line alone."""
+    from erdst_gpu import inputs as I
+    w = synth.config1()
+    topo = w.topo
+    chain, water = topo.species[0], topo.species[1]
+    mesh, pecore = w.mesh[0]
+    run_dir = tmp_path / "run"
+    run_dir.mkdir()
+    mw = np.tile(np.float32([15.9994, 1.008, 1.008]), water.nmol).astype(np.float64)
+    mc = np.full(chain.nsite, 12.0107)
+    dist = []
+    for f in (0, 1):
+        x = w.frame(f).astype(np.float64)
+        c_slt = (x[:chain.nsite] * mc[:, None]).sum(0) / mc.sum()
+        c_agg = (x[chain.nsite:] * mw[:, None]).sum(0) / mw.sum()
+        dist.append(float(np.linalg.norm(c_slt - c_agg)))
+    keep = int(np.argmin(dist))
+    up = 0.5 * (dist[0] + dist[1])
+    (run_dir / "parameters_er").write_text(f"""&ene_param
+  slttype = 1, ljformat = 0, inptemp = 300.0,
+  elecut = 12.0, cltype = 2, screen = 0.35, splodr = 4, ms1max = {w.params['ms'][0]}, engdiv = 1,
+  insposition = 2, lwreg = 0.0, upreg = {up!r}, hostspec(1) = 2
+/
+&hist
+  eclbin = 5.0e-2, ecfbin = 2.0e-3, ec0bin = 2.0e-4, finfac = 10.0,
+  ecdmin = -20.0, ecfmns = -0.20, ecdcen = 0.0, eccore = 5.0,
+  ecdmax = 1.0e4, pecore = 100, meshread = 1
+/
+""")
+    lines = [f"1 {len(mesh)} {pecore}"] + [f"1 {i + 1} {float(v)!r}" for i, v in enumerate(mesh)]
+    (run_dir / "EcdMesh").write_text("\n".join(lines) + "\n")
+    (run_dir / "MDinfo").write_text(f"2 2\n1 {water.nmol}\n{chain.nsite} {water.nsite}\n")
+    with open(run_dir / "SltInfo", "w") as fh:
+        for i in range(chain.nsite):
+            fh.write(f"{i + 1} 12.0107 C C{i + 1} {fmt(chain.charge[i])} {fmt(chain.eps[i])} {fmt(chain.sigma[i])}\n")
+    with open(run_dir / "MolPrm1", "w") as fh:
+        for i, (typ, m) in enumerate([("OW", "15.9994"), ("HW", "1.008"), ("HW", "1.008")]):
+            fh.write(f"{i + 1} {m} {typ} {typ}{i + 1} {fmt(water.charge[i])} {fmt(water.eps[i])} "
+                     f"{fmt(water.sigma[i])}\n")
+    L = np.diag(w.cell).astype(float)
+    write_dcd(run_dir / "HISTORY", [w.frame(f) for f in (0, 1)], [(L[0], 90.0, L[1], 90.0, 90.0, L[2])] * 2)
+    erdst.run(str(run_dir), log=lambda *a: None)
+    ref_dir = tmp_path / "ref"
+    ref_dir.mkdir()
+    p = U.workload_params(w)
+    ctx = capi.Context(topo, p)
+    uvmax, uvsoft, uvcrd = w.uvmax_uvsoft_uvcrd()
+    ctx.set_mesh(uvmax, uvsoft, uvcrd)
+    ctx.set_solute_molecules(topo.tagslt)
+    xyz, cell = list(capi.Trajectory(str(run_dir / "HISTORY")))[keep]
+    ctx.set_frame(xyz, cell)
+    ctx.run(capi.ER_BATCHED)
+    wr = BlockWriter(str(ref_dir), uvmax, uvsoft, uvcrd, p["temp"], soln=True, engdiv=1, wgtslf=False,
+                     corrcal=False)
+    wr.store(ctx.fetch())
+    for n in ("aveuv.tt", "engsln.tt", "uvrange.tt", "weight_soln"):
+        assert (run_dir / n).read_bytes() == (ref_dir / n).read_bytes(), n
+    fl = open(run_dir / "flcuv.tt").read().splitlines()
+    assert len(fl) == 1 and int(fl[0][:9]) == keep + 1

@@ -112,3 +112,26 @@ def test_ecdmesh_species_without_a_block(tmp_path, pecore, ok):
     else:
         with pytest.raises(I.InputError, match="eng_emf"):
             M.species_meshes(tmp_path / "parameters_er", 1, ecdmesh_path=tmp_path / "EcdMesh")
+
+
+def test_region_distance_sphere_and_slab():
+    """check_mol_configuration's distances (engproc.F90:1228-1249): the solute
+    centre of mass relative to the hostspec aggregate's, as a norm (sphere),
+    as the component along c (slab) and its magnitude (symmetric slab)."""
+    from types import SimpleNamespace
+    topo = SimpleNamespace(mol_begin=np.array([0, 2, 4, 6]), nmol=3, moltype=np.array([1, 2, 2]))
+    mass = np.array([1.0, 3.0, 2.0, 2.0, 1.0, 1.0], np.float32)
+    xyz = np.array([[0, 0, 0], [4, 0, 0], [1, 1, 1], [3, 1, 1], [0, 0, 5], [0, 0, 7]], np.float32)
+    sysin = SimpleNamespace(topology=topo, mass=mass)
+    slt = np.array([3.0, 0.0, 0.0])                     # (0*1 + 4*3) / 4
+    agg = (2 * np.array([1, 1, 1]) + 2 * np.array([3, 1, 1]) + np.array([0, 0, 5]) + np.array([0, 0, 7])) / 6
+    dx = slt - agg
+    cell = np.diag([20.0, 20.0, 20.0])
+    p = dict(insposition=2, hostspec={1: 2})
+    assert erdst.region_distance(p, sysin, xyz, cell, 0) == pytest.approx(np.linalg.norm(dx), rel=1e-12)
+    assert erdst.region_distance(dict(p, insposition=3), sysin, xyz, cell, 0) == pytest.approx(dx[2], rel=1e-12)
+    assert erdst.region_distance(dict(p, insposition=4), sysin, xyz, cell, 0) == pytest.approx(abs(dx[2]))
+    sheared = np.array([[20.0, 0.0, 5.0], [0.0, 20.0, 0.0], [0.0, 0.0, 20.0]])   # c = (5, 0, 20)
+    s3 = np.linalg.solve(sheared, dx)[2]
+    assert erdst.region_distance(dict(p, insposition=3), sysin, xyz, sheared, 0) == pytest.approx(
+        s3 * np.hypot(5.0, 20.0), rel=1e-12)
