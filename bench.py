@@ -228,7 +228,13 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--plain-insertions", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--builtin-shapes", action="store_true",
+                    help="skip er_autotune, which otherwise times launch shapes on the last warm-up frame "
+                         "before the timed steps")
     args = ap.parse_args()
+    args.autotune = not args.builtin_shapes
+    if args.autotune and args.warmup < 1:
+        ap.error("er_autotune tunes on a warm-up frame: give --warmup 1 or more, or --builtin-shapes")
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is a design note, not an implemented path.", file=sys.stderr)
@@ -247,6 +253,11 @@ def main():
     # warm-up (compiles nothing; first-touch of buffers, caches, clocks)
     job.run_frames(ctx, pos[:W], ins[:W], capi.ER_BATCHED)
     ctx.sync()
+    tuning, tune_s = None, None
+    if args.autotune:
+        ta = time.perf_counter()
+        tuning = ctx.autotune(reps=3)
+        tune_s = round(time.perf_counter() - ta, 3)
     ctx.profile(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -321,6 +332,8 @@ def main():
             "plain_insertions_per_s": round(plain, 1),
             "batched_over_plain": round(value / plain, 2),
             "precision": "pair/PME f32; self energy, weights, edens, ecorr f64",
+            "tuning": tuning if tuning is not None else "built-in launch shapes",
+            "tuning_s": tune_s,
         },
         "kernels": kernels,
         "roofline": {"kernel": dom, "bound": bound, "achieved": round(achieved, 2), "peak": peak,

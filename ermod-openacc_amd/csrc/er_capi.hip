@@ -87,6 +87,7 @@ struct Slot {
     DBuf<unsigned int> d_minmax;
     DBuf<unsigned long long> d_errkey, d_halt;  // halt key; captured key + energy
     unsigned int seq = 0;                         // runs of this slot
+    int pair_split = 0;            // pair-search workgroups per insertion (0: built-in choice)
     void release()
     {
         d_sltq.release(); d_sltlj.release(); d_sltpos_own.release(); d_wslt.release(); d_U.release();
@@ -1349,6 +1350,18 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
     return run_chunk_pairs_hist<NI>(c, n_off, nch);
 }
 
+// workgroups per insertion in the pair search: the slot's tuned value in a
+// batched run (the shape it was tuned for), else enough to fill the chip (at
+// least 4 per CU); a many-site solute reaches far, so its candidates are
+// spread over 16.  Every (insertion, molecule) sum has one owner whatever
+// the split, so the split never changes a result.
+static int pair_split(er_ctx *c, int nch, bool batched)
+{
+    if (batched && c->S().pair_split > 0) return c->S().pair_split;
+    if (c->S().nslt > 8) return 16;
+    return std::max(1, std::min(16, (4 * 256 + nch - 1) / nch));
+}
+
 // the real-space pairs and the histograms of insertions [n_off, n_off+nch)
 template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, int nch)
 {
@@ -1373,10 +1386,7 @@ template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, i
     } else {
         ProfScope ps(c, ER_K_PAIRS);
         PairArgs pa{};
-        // enough workgroups per insertion to fill the chip (at least 4 per CU);
-        // a many-site solute reaches far, so its candidates are spread over 16
-        pa.nsplit = std::max(1, std::min(16, (4 * 256 + nch - 1) / nch));
-        if (c->S().nslt > 8) pa.nsplit = 16;
+        pa.nsplit = pair_split(c, nch, NI > 1);
         pa.nmol = c->nummol;
         pa.nchunk = nch;
         pa.n_off = n_off;
@@ -1520,22 +1530,27 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
     return ER_OK;
 }
 
-extern "C" int er_run(er_ctx *c, int32_t mode)
+// the slots a run processes (every solute with insertions; the one soln
+// slot), after the state checks of er_run
+static int run_plan(er_ctx *c, const char *who, std::vector<int> &todo)
 {
-    if (!c) return ER_E_ARG;
-    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_run: er_set_mesh first");
-    if (!c->have_frame) return fail(c, ER_E_STATE, "er_run: no frame set");
-    if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run: mode");
-    std::vector<int> todo;
+    if (!c->have_mesh) return fail(c, ER_E_STATE, "%s: er_set_mesh first", who);
+    if (!c->have_frame) return fail(c, ER_E_STATE, "%s: no frame set", who);
+    todo.clear();
     if (c->prm.slttype == 1) {
-        if (c->nsltmol <= 0) return fail(c, ER_E_STATE, "er_run: er_set_solute_molecules first");
+        if (c->nsltmol <= 0) return fail(c, ER_E_STATE, "%s: er_set_solute_molecules first", who);
         todo.push_back(0);
     } else {
         for (int k = 0; k < (int)c->slots.size(); ++k)
             if (c->slots[k].have_ins) todo.push_back(k);
-        if (todo.empty()) return fail(c, ER_E_STATE, "er_run: no insertions set");
+        if (todo.empty()) return fail(c, ER_E_STATE, "%s: no insertions set", who);
     }
-    // every solute of the frame in turn; the frame's preparation is shared
+    return ER_OK;
+}
+
+// every solute of the frame in turn; the frame's preparation is shared
+static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
+{
     const int keep = c->cur;
     int rc = ER_OK;
     for (size_t t = 0; t < todo.size() && rc == ER_OK; ++t) {
@@ -1544,6 +1559,16 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
     }
     c->cur = keep;
     return rc;
+}
+
+extern "C" int er_run(er_ctx *c, int32_t mode)
+{
+    if (!c) return ER_E_ARG;
+    if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run: mode");
+    std::vector<int> todo;
+    const int rc = run_plan(c, "er_run", todo);
+    if (rc) return rc;
+    return run_slots(c, todo, mode);
 }
 
 // the halt of slot c->S(): (insertion, molecule) from the key, the energy as
@@ -1697,6 +1722,179 @@ extern "C" int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int6
     if (launches) *launches = c->prof_n[kernel];
     c->prof_ms[kernel] = 0.0;
     c->prof_n[kernel] = 0;
+    return ER_OK;
+}
+
+// ---- launch-shape tuning on the context's own frame and insertions ----
+
+// the per-slot device state a tuning run changes (block accumulators,
+// voffset, flags, minuv/maxuv, halt keys), kept aside while tuning
+struct SlotSnap {
+    DBuf<double> edens, ecorr, eself, state, slnuv;
+    DBuf<int> flags, warn;
+    DBuf<float> fstate;
+    DBuf<unsigned int> minmax;
+    DBuf<unsigned long long> errkey, halt;
+};
+
+template <class T> static hipError_t snap_take(DBuf<T> &dst, const DBuf<T> &src, hipStream_t s)
+{
+    if (!src.p || src.n == 0) return hipSuccess;
+    hipError_t e = dst.ensure(src.n);
+    if (e != hipSuccess) return e;
+    return hipMemcpyAsync(dst.p, src.p, src.n * sizeof(T), hipMemcpyDeviceToDevice, s);
+}
+
+template <class T> static hipError_t snap_give(DBuf<T> &dst, const DBuf<T> &src, hipStream_t s)
+{
+    if (!src.p || src.n == 0) return hipSuccess;
+    return hipMemcpyAsync(dst.p, src.p, src.n * sizeof(T), hipMemcpyDeviceToDevice, s);
+}
+
+static hipError_t slot_snap(Slot &sl, SlotSnap &sn, hipStream_t s, bool take)
+{
+    auto f = [&](auto &live, auto &kept) { return take ? snap_take(kept, live, s) : snap_give(live, kept, s); };
+    hipError_t e;
+    if ((e = f(sl.d_edens, sn.edens)) != hipSuccess || (e = f(sl.d_ecorr, sn.ecorr)) != hipSuccess ||
+        (e = f(sl.d_eself, sn.eself)) != hipSuccess || (e = f(sl.d_state, sn.state)) != hipSuccess ||
+        (e = f(sl.d_slnuv, sn.slnuv)) != hipSuccess || (e = f(sl.d_flags, sn.flags)) != hipSuccess ||
+        (e = f(sl.d_warn, sn.warn)) != hipSuccess || (e = f(sl.d_fstate, sn.fstate)) != hipSuccess ||
+        (e = f(sl.d_minmax, sn.minmax)) != hipSuccess || (e = f(sl.d_errkey, sn.errkey)) != hipSuccess ||
+        (e = f(sl.d_halt, sn.halt)) != hipSuccess)
+        return e;
+    return hipSuccess;
+}
+
+struct EventPair {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~EventPair()
+    {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+
+// a candidate replaces the current shape only when it is faster by more
+// than this fraction, so that timing noise keeps the current one
+constexpr double TUNE_MARGIN = 0.02;
+
+// the tuning runs proper: chunk over whole batched runs of every slot, then
+// each slot's pair split from the pair kernel's own event times.  The
+// current shapes are always among the candidates.
+static int autotune_runs(er_ctx *c, const std::vector<int> &todo, int reps)
+{
+    int maxins = 1;
+    for (int k : todo) maxins = std::max(maxins, c->prm.slttype == 1 ? c->nsltmol : c->slots[k].nins);
+    // plane spectra and potentials of a chunk; candidates beyond 6 GiB of
+    // them are skipped unless the current chunk already takes more.  The
+    // histogram's slicing follows the chunk, so a chunk beyond the
+    // insertions still differs from a smaller one; past the larger of the
+    // current chunk and the insertions nothing changes.
+    const int cur_chunk = c->chunk;
+    const size_t per_ins = c->Gc * sizeof(float2) + c->G * sizeof(float);
+    const size_t cap = std::max<size_t>((size_t)6 << 30, (size_t)cur_chunk * per_ins);
+    const int reach = std::max(cur_chunk, (maxins + NI_B - 1) / NI_B * NI_B);
+    std::vector<int> chunks = {cur_chunk};
+    for (int ch : {256, 512, 1024, 2048, 4096})
+        if (ch != cur_chunk && ch <= reach && (size_t)ch * per_ins <= cap) chunks.push_back(ch);
+    EventPair ev;
+    HIPCHK(c, hipEventCreate(&ev.a));
+    HIPCHK(c, hipEventCreate(&ev.b));
+    int rc;
+    if (chunks.size() > 1) {
+        int best = cur_chunk;
+        float best_ms = 0.f;
+        for (size_t i = 0; i < chunks.size(); ++i) {
+            c->chunk = chunks[i];
+            if ((rc = run_slots(c, todo, ER_BATCHED))) return rc;  // sizes the buffers
+            HIPCHK(c, hipEventRecord(ev.a, c->stream));
+            for (int r = 0; r < reps; ++r)
+                if ((rc = run_slots(c, todo, ER_BATCHED))) return rc;
+            HIPCHK(c, hipEventRecord(ev.b, c->stream));
+            HIPCHK(c, hipEventSynchronize(ev.b));
+            float ms = 0.f;
+            HIPCHK(c, hipEventElapsedTime(&ms, ev.a, ev.b));
+            if (i == 0 || ms < best_ms * (1.0 - TUNE_MARGIN)) {
+                best_ms = ms;
+                best = chunks[i];
+            }
+        }
+        c->chunk = best;
+    }
+    if (c->prm.boxshp == 0) return ER_OK;  // the open-system pair kernel has no split
+    c->prof = true;
+    for (int k : todo) {
+        c->cur = k;
+        int best = c->S().pair_split;
+        double best_ms = 0.0;
+        std::vector<int> splits = {c->S().pair_split};  // 0: the built-in choice
+        for (int sp : {1, 2, 4, 8, 16, 32, 64})
+            if (sp != c->S().pair_split) splits.push_back(sp);
+        for (size_t i = 0; i < splits.size(); ++i) {
+            c->S().pair_split = splits[i];
+            if ((rc = run_slot(c, ER_BATCHED, true))) return rc;
+            prof_collect(c);
+            c->prof_ms[ER_K_PAIRS] = 0.0;
+            for (int r = 0; r < reps; ++r)
+                if ((rc = run_slot(c, ER_BATCHED, true))) return rc;
+            prof_collect(c);
+            if (i == 0 || c->prof_ms[ER_K_PAIRS] < best_ms * (1.0 - TUNE_MARGIN)) {
+                best_ms = c->prof_ms[ER_K_PAIRS];
+                best = splits[i];
+            }
+        }
+        c->S().pair_split = best;
+    }
+    return ER_OK;
+}
+
+extern "C" int er_autotune(er_ctx *c, int32_t reps)
+{
+    if (!c) return ER_E_ARG;
+    if (reps < 0 || reps > 100) return fail(c, ER_E_ARG, "er_autotune: reps must lie in 0..100");
+    if (reps == 0) reps = 3;
+    std::vector<int> todo;
+    int rc = run_plan(c, "er_autotune", todo);
+    if (rc) return rc;
+    // the caller's profile totals and every slot's accumulators are kept
+    prof_collect(c);
+    const bool prof_keep = c->prof;
+    double ms_keep[ER_K_COUNT];
+    int64_t n_keep[ER_K_COUNT];
+    std::memcpy(ms_keep, c->prof_ms, sizeof ms_keep);
+    std::memcpy(n_keep, c->prof_n, sizeof n_keep);
+    std::vector<SlotSnap> snap(c->slots.size());
+    for (size_t k = 0; k < c->slots.size(); ++k) HIPCHK(c, slot_snap(c->slots[k], snap[k], c->stream, true));
+    const int keep = c->cur;
+    rc = autotune_runs(c, todo, reps);
+    c->cur = keep;
+    if (rc == ER_OK) rc = er_sync(c);  // a halt hit by a tuning run is reported here
+    (void)hipStreamSynchronize(c->stream);
+    prof_collect(c);
+    std::memcpy(c->prof_ms, ms_keep, sizeof ms_keep);
+    std::memcpy(c->prof_n, n_keep, sizeof n_keep);
+    c->prof = prof_keep;
+    for (size_t k = 0; k < c->slots.size(); ++k) HIPCHK(c, slot_snap(c->slots[k], snap[k], c->stream, false));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return rc;
+}
+
+extern "C" int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split)
+{
+    if (!c) return ER_E_ARG;
+    if (chunk != 0 && (chunk < NI_B || chunk > 65536 || chunk % NI_B != 0))
+        return fail(c, ER_E_ARG, "er_set_tuning: chunk must be 0 or a multiple of %d in %d..65536", NI_B, NI_B);
+    if (pair_split < 0 || pair_split > 64) return fail(c, ER_E_ARG, "er_set_tuning: pair_split must lie in 0..64");
+    if (chunk != 0) c->chunk = chunk;
+    c->S().pair_split = pair_split;
+    return ER_OK;
+}
+
+extern "C" int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split)
+{
+    if (!c) return ER_E_ARG;
+    if (chunk) *chunk = c->chunk;
+    if (pair_split) *pair_split = c->S().pair_split;
     return ER_OK;
 }
 

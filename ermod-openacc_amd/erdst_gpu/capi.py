@@ -21,7 +21,7 @@ EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_s
             "er_set_solute_molecules", "er_set_frame",
             "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_sync",
             "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
-            "er_destroy", "er_profile", "er_profile_read"]
+            "er_destroy", "er_profile", "er_profile_read", "er_autotune", "er_set_tuning", "er_get_tuning"]
 EXPORTED_TRAJ = ["er_traj_open", "er_traj_read", "er_traj_info", "er_traj_close"]
 ER_TRAJ_EOF, ER_TRAJ_IO, ER_TRAJ_FORMAT = 100, 101, 102
 TRAJ_FLAGS = {"charmm": 1, "cell": 2, "4d": 4, "rec64": 8, "swapped": 16}
@@ -83,6 +83,9 @@ def load() -> C.CDLL:
     L.er_destroy.restype = None
     L.er_profile.argtypes = [vp, i32]
     L.er_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
+    L.er_autotune.argtypes = [vp, i32]
+    L.er_set_tuning.argtypes = [vp, i32, i32]
+    L.er_get_tuning.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
     for name in EXPORTED:
         if name not in ("er_error_string", "er_destroy"):
             getattr(L, name).restype = C.c_int
@@ -171,6 +174,7 @@ class Context:
     def __init__(self, topo, params: dict):
         L = load()
         self._keep = []
+        self.nsolute = 1
         t = ErTopology()
         t.numatm, t.nummol = topo.natom, topo.nmol
         t.ljtype_max, t.numslv = topo.ntype, topo.numslv
@@ -232,6 +236,7 @@ class Context:
         lj = np.ascontiguousarray(np.concatenate([np.asarray(t) + 1 for t, _ in solutes]), np.int32)
         q = np.ascontiguousarray(np.concatenate([np.asarray(c) for _, c in solutes]), np.float32)
         self._check(self.L.er_set_solutes(self.h, ns.size, _ptr(ns), _ptr(lj), _ptr(q)))
+        self.nsolute = int(ns.size)
 
     def select_solute(self, k):
         """Select solute k (0-based here, 1-based in the C ABI)."""
@@ -309,6 +314,37 @@ class Context:
             self._check(self.L.er_profile_read(self.h, k, C.byref(ms), C.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+    def autotune(self, reps=0):
+        """Times launch shapes on the current frame and insertions and keeps
+        the fastest (er_autotune); returns tuning()."""
+        self._check(self.L.er_autotune(self.h, int(reps)))
+        return self.tuning()
+
+    def tuning(self):
+        """{"chunk": insertions per launch sequence, "pair_split": [per
+        solute; 0 = built-in choice]}; leaves solute 0 selected."""
+        ch, sp = C.c_int32(), C.c_int32()
+        splits = []
+        for k in range(self.nsolute):
+            if self.nsolute > 1:
+                self.select_solute(k)
+            self._check(self.L.er_get_tuning(self.h, C.byref(ch), C.byref(sp)))
+            splits.append(sp.value)
+        if self.nsolute > 1:
+            self.select_solute(0)
+        return {"chunk": ch.value, "pair_split": splits}
+
+    def set_tuning(self, tuning):
+        """Applies a tuning() dictionary (for instance one kept in a cache
+        file); leaves solute 0 selected."""
+        splits = list(tuning["pair_split"])
+        for k, sp in enumerate(splits):
+            if len(splits) > 1:
+                self.select_solute(k)
+            self._check(self.L.er_set_tuning(self.h, int(tuning["chunk"]), int(sp)))
+        if len(splits) > 1:
+            self.select_solute(0)
 
     def close(self):
         if self.h is not None and self.h.value:

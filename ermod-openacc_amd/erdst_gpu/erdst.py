@@ -1,6 +1,6 @@
 """erdst over an ERmod run directory, with the per-frame work on the GPU.
 
-    python -m erdst_gpu.erdst [DIRECTORY] [--plain] [--out OUTDIR]
+    python -m erdst_gpu.erdst [DIRECTORY] [--plain] [--autotune] [--out OUTDIR]
 
 Reads what erdst reads (erdst/setconf.F90, erdst/engproc.F90:36-318):
 parameters_er (&ene_param, &hist), MDinfo, MolPrm<n>, SltInfo, LJTable,
@@ -89,8 +89,11 @@ def _check_supported(p: dict):
             raise InputError("region bounds", "set_reg")
 
 
-def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log=print) -> dict:
-    """One erdst run; returns the last block's fetched accumulators."""
+def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log=print,
+        autotune: bool = False) -> dict:
+    """One erdst run; returns the last block's fetched accumulators.  With
+    autotune the launch shapes are timed on the first frame (er_autotune),
+    which changes no output."""
     d = directory
     out = out or d
     path = lambda name: os.path.join(d, name)
@@ -133,6 +136,7 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
         if p["wgtins"] == 1:
             sltwgt = WeightFile(path("SltWght"), rewind=True)
     g = None
+    tune = autotune and mode == capi.ER_BATCHED
     active = list(range(len(topo.tagslt)))
     for division in range(1, engdiv + 1):
         for _ in range(frames_per_div):
@@ -176,9 +180,15 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
                 if keep != active:
                     ctx.set_solute_molecules(topo.tagslt[keep])
                     active = keep
+                if tune:
+                    log(f"erdst: launch shapes {ctx.autotune()}")
+                    tune = False
                 ctx.run(mode)
                 writer.frame(ctx.flceng(len(keep)), skpcnf, keep, len(topo.tagslt))
                 continue
+            if tune:
+                log(f"erdst: launch shapes {ctx.autotune()}")
+                tune = False
             ctx.run(mode)
             writer.frame(ctx.flceng(len(topo.tagslt)) if soln else None, skpcnf)
         g = ctx.fetch()
@@ -196,9 +206,10 @@ def main(argv=None) -> int:
     ap.add_argument("directory", nargs="?", default=".")
     ap.add_argument("--out", default=None, help="directory for the output files (default: the run directory)")
     ap.add_argument("--plain", action="store_true", help="one insertion per launch sequence")
+    ap.add_argument("--autotune", action="store_true", help="time the launch shapes on the first frame")
     a = ap.parse_args(argv)
     try:
-        run(a.directory, a.out, capi.ER_PLAIN if a.plain else capi.ER_BATCHED)
+        run(a.directory, a.out, capi.ER_PLAIN if a.plain else capi.ER_BATCHED, autotune=a.autotune)
     except (InputError, capi.TrajError, capi.ErError) as e:
         print(f"erdst: {e}", file=sys.stderr)
         return 1

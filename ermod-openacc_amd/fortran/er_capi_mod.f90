@@ -27,7 +27,8 @@ module er_capi
 
   public :: er_create, er_set_mesh, er_set_solute, er_set_solutes, er_select_solute, er_set_solute_molecules, &
        er_set_frame, er_set_insertions, er_run, er_sync, er_get_uvengy, er_fetch, &
-       er_get_flceng, er_clear, er_last_error, er_destroy, er_traj_open, er_traj_read, er_traj_close
+       er_get_flceng, er_clear, er_last_error, er_destroy, er_traj_open, er_traj_read, er_traj_close, &
+       er_autotune, er_set_tuning, er_get_tuning
 
   interface
      integer(c_int) function er_create(topo, prm, ctx) bind(C, name="er_create")
@@ -124,6 +125,24 @@ module er_capi
        type(c_ptr), value :: ctx
        real(c_double), intent(out) :: flceng(*)
      end function er_get_flceng
+
+     integer(c_int) function er_autotune(ctx, reps) bind(C, name="er_autotune")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: reps
+     end function er_autotune
+
+     integer(c_int) function er_set_tuning(ctx, chunk, pair_split) bind(C, name="er_set_tuning")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: chunk, pair_split
+     end function er_set_tuning
+
+     integer(c_int) function er_get_tuning(ctx, chunk, pair_split) bind(C, name="er_get_tuning")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: chunk, pair_split
+     end function er_get_tuning
 
      integer(c_int) function er_clear(ctx) bind(C, name="er_clear")
        import :: c_int, c_ptr

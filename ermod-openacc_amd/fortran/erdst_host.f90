@@ -10,7 +10,9 @@
 ! instead, as erdst reads HISTORY (setconf.F90:103-105: a failed read halts
 ! with set_trj); "sltconf=FILE" takes the refs insertions from a DCD file of
 ! one solute configuration per frame, read in turn and reopened at its end
-! as erdst reads SltConf (setconf.F90:106-116).
+! as erdst reads SltConf (setconf.F90:106-116); "autotune" times the launch
+! shapes on the first frame (er_autotune) before its er_run, which changes
+! no output.
 ! Output (current directory): engref.XX / corref.XX (engsln / corsln for
 ! soln runs), slfeng.XX (selfcal), weight_refs|weight_soln, aveuv.tt (soln),
 ! uvrange.tt, as engstore writes them, and per frame flcuv.tt (soln) or
@@ -33,24 +35,28 @@ program erdst_host
   real(c_double), allocatable :: aveuv(:,:), flceng(:,:)
   real(c_double) :: scal(4), dummy(1)
   integer :: iu, rc, f, division, frames_per_div, k, io_flcuv, narg
+  integer(c_int32_t) :: tune_chunk, tune_split
+  logical :: autotune
   integer, parameter :: stdout = 6
 
   call get_command_argument(1, path)
   if (len_trim(path) == 0) stop "usage: erdst_host workload.bin [plain]"
   mode = ER_BATCHED
-  history = ""; sltconf = ""
+  history = ""; sltconf = ""; autotune = .false.
   do narg = 2, command_argument_count()
      block
        character(len=600) :: arg
        call get_command_argument(narg, arg)
        if (trim(arg) == "plain") then
           mode = ER_PLAIN
+       else if (trim(arg) == "autotune") then
+          autotune = .true.
        else if (arg(1:8) == "history=") then
           history = arg(9:)
        else if (arg(1:8) == "sltconf=") then
           sltconf = arg(9:)
        else
-          stop "usage: erdst_host workload.bin [plain] [history=FILE] [sltconf=FILE]"
+          stop "usage: erdst_host workload.bin [plain] [autotune] [history=FILE] [sltconf=FILE]"
        end if
      end block
   end do
@@ -107,6 +113,12 @@ program erdst_host
         end if
         rc = er_set_insertions(ctx, nins, sltpos, wslt)
         call check(rc, "er_set_insertions")
+     end if
+     if (autotune .and. f == 1) then
+        rc = er_autotune(ctx, 0_c_int32_t)
+        call check(rc, "er_autotune")
+        rc = er_get_tuning(ctx, tune_chunk, tune_split)
+        write(stdout, '(a,i0,a,i0)') "erdst_host: launch shapes: chunk ", tune_chunk, ", pair split ", tune_split
      end if
      rc = er_run(ctx, mode)                                   ! engconst
      call check(rc, "er_run")

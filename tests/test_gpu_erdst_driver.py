@@ -56,12 +56,13 @@ def write_run_directory(d, w):
     write_dcd(d / "SltConf", confs, None, charmm=False)
 
 
-def test_erdst_driver_over_a_run_directory(tmp_path):
+@pytest.mark.parametrize("autotune", [False, True])
+def test_erdst_driver_over_a_run_directory(tmp_path, autotune):
     w = synth.config0()
     run_dir = tmp_path / "run"
     run_dir.mkdir()
     write_run_directory(run_dir, w)
-    erdst.run(str(run_dir), log=lambda *a: None)
+    erdst.run(str(run_dir), log=lambda *a: None, autotune=autotune)
     # the same inputs through the binding, written by the same writers
     ref_dir = tmp_path / "ref"
     ref_dir.mkdir()

@@ -64,6 +64,27 @@ def test_fortran_host_two_blocks(tmp_path):
 
 
 @pytest.mark.gpu
+def test_fortran_host_autotune_changes_no_output(tmp_path):
+    """"autotune" tunes the launch shapes on the first frame; every output
+    file is byte-identical to the untuned run's."""
+    w = synth.config0()
+    p = U.workload_params(w)
+    runs = {}
+    for name, extra in (("builtin", []), ("tuned", ["autotune"])):
+        d = tmp_path / name
+        d.mkdir()
+        write_workload(d / "workload.bin", w, p, [0, 1, 2, 3], engdiv=2, nins=40)
+        r = subprocess.run([HOST, "workload.bin"] + extra, cwd=d, capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stdout + r.stderr
+        runs[name] = {f.name: f.read_bytes() for f in sorted(d.iterdir()) if f.name != "workload.bin"}
+        if extra:
+            assert "launch shapes: chunk" in r.stdout
+    assert sorted(runs["tuned"]) == sorted(runs["builtin"]) and "engref.02" in runs["tuned"]
+    for name, data in runs["builtin"].items():
+        assert runs["tuned"][name] == data, name
+
+
+@pytest.mark.gpu
 def test_fortran_host_selfcal_and_progress(tmp_path):
     """selfcal = 1: slfeng.XX per block equals eself/engnorm of the same run
     through the Python binding, written as the Python writer writes it;
