@@ -186,6 +186,7 @@ struct er_ctx {
     DBuf<unsigned short> d_blut;  // bin brackets per species (k_hist)
     DBuf<double> d_flpart;
     int nslice = 1;
+    int hist_slices = 0;           // tuned histogram slices of a refs run (0: built-in choice)
     DBuf<uint32_t> d_C;
     DBuf<double> d_w, d_flceng;
 
@@ -1304,6 +1305,8 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     // enough histogram workgroups to fill the chip
     const int ntile_chunk = (chunk + NI - 1) / NI;
     c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
+    // soln runs keep the built-in slicing: their flceng partial sums follow it
+    if (c->hist_slices > 0 && c->prm.slttype != 1) c->nslice = c->hist_slices;
     if (c->prm.slttype == 1) {
         HIPCHK(c, c->d_flceng.ensure((size_t)nins * c->numslv));
         HIPCHK(c, c->d_flpart.ensure((size_t)nins * c->nslice * c->numslv));
@@ -1821,8 +1824,31 @@ static int autotune_runs(er_ctx *c, const std::vector<int> &todo, int reps)
         }
         c->chunk = best;
     }
-    if (c->prm.boxshp == 0) return ER_OK;  // the open-system pair kernel has no split
     c->prof = true;
+    // histogram slices (refs runs): the histogram kernel's own time over
+    // whole runs; slices change only integer counts and minima/maxima
+    if (c->prm.slttype != 1) {
+        int best = c->hist_slices;
+        double best_ms = 0.0;
+        std::vector<int> nsl = {c->hist_slices};
+        for (int v : {1, 2, 4, 8, 16, 32, 64})
+            if (v != c->hist_slices) nsl.push_back(v);
+        for (size_t i = 0; i < nsl.size(); ++i) {
+            c->hist_slices = nsl[i];
+            if ((rc = run_slots(c, todo, ER_BATCHED))) return rc;
+            prof_collect(c);
+            c->prof_ms[ER_K_HIST] = 0.0;
+            for (int r = 0; r < reps; ++r)
+                if ((rc = run_slots(c, todo, ER_BATCHED))) return rc;
+            prof_collect(c);
+            if (i == 0 || c->prof_ms[ER_K_HIST] < best_ms * (1.0 - TUNE_MARGIN)) {
+                best_ms = c->prof_ms[ER_K_HIST];
+                best = nsl[i];
+            }
+        }
+        c->hist_slices = best;
+    }
+    if (c->prm.boxshp == 0) return ER_OK;  // the open-system pair kernel has no split
     for (int k : todo) {
         c->cur = k;
         int best = c->S().pair_split;
@@ -1879,22 +1905,25 @@ extern "C" int er_autotune(er_ctx *c, int32_t reps)
     return rc;
 }
 
-extern "C" int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split)
+extern "C" int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split, int32_t hist_slices)
 {
     if (!c) return ER_E_ARG;
     if (chunk != 0 && (chunk < NI_B || chunk > 65536 || chunk % NI_B != 0))
         return fail(c, ER_E_ARG, "er_set_tuning: chunk must be 0 or a multiple of %d in %d..65536", NI_B, NI_B);
     if (pair_split < 0 || pair_split > 64) return fail(c, ER_E_ARG, "er_set_tuning: pair_split must lie in 0..64");
+    if (hist_slices < 0 || hist_slices > 64) return fail(c, ER_E_ARG, "er_set_tuning: hist_slices must lie in 0..64");
     if (chunk != 0) c->chunk = chunk;
     c->S().pair_split = pair_split;
+    c->hist_slices = hist_slices;
     return ER_OK;
 }
 
-extern "C" int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split)
+extern "C" int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split, int32_t *hist_slices)
 {
     if (!c) return ER_E_ARG;
     if (chunk) *chunk = c->chunk;
     if (pair_split) *pair_split = c->S().pair_split;
+    if (hist_slices) *hist_slices = c->hist_slices;
     return ER_OK;
 }
 

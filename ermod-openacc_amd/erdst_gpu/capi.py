@@ -84,8 +84,8 @@ def load() -> C.CDLL:
     L.er_profile.argtypes = [vp, i32]
     L.er_profile_read.argtypes = [vp, i32, C.POINTER(C.c_double), C.POINTER(C.c_int64)]
     L.er_autotune.argtypes = [vp, i32]
-    L.er_set_tuning.argtypes = [vp, i32, i32]
-    L.er_get_tuning.argtypes = [vp, C.POINTER(i32), C.POINTER(i32)]
+    L.er_set_tuning.argtypes = [vp, i32, i32, i32]
+    L.er_get_tuning.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     for name in EXPORTED:
         if name not in ("er_error_string", "er_destroy"):
             getattr(L, name).restype = C.c_int
@@ -322,18 +322,19 @@ class Context:
         return self.tuning()
 
     def tuning(self):
-        """{"chunk": insertions per launch sequence, "pair_split": [per
-        solute; 0 = built-in choice]}; leaves solute 0 selected."""
-        ch, sp = C.c_int32(), C.c_int32()
+        """{"chunk": insertions per launch sequence, "hist_slices": histogram
+        workgroups per 16 insertions, "pair_split": [per solute]} (0: the
+        built-in choice); leaves solute 0 selected."""
+        ch, sp, hs = C.c_int32(), C.c_int32(), C.c_int32()
         splits = []
         for k in range(self.nsolute):
             if self.nsolute > 1:
                 self.select_solute(k)
-            self._check(self.L.er_get_tuning(self.h, C.byref(ch), C.byref(sp)))
+            self._check(self.L.er_get_tuning(self.h, C.byref(ch), C.byref(sp), C.byref(hs)))
             splits.append(sp.value)
         if self.nsolute > 1:
             self.select_solute(0)
-        return {"chunk": ch.value, "pair_split": splits}
+        return {"chunk": ch.value, "hist_slices": hs.value, "pair_split": splits}
 
     def set_tuning(self, tuning):
         """Applies a tuning() dictionary (for instance one kept in a cache
@@ -342,7 +343,8 @@ class Context:
         for k, sp in enumerate(splits):
             if len(splits) > 1:
                 self.select_solute(k)
-            self._check(self.L.er_set_tuning(self.h, int(tuning["chunk"]), int(sp)))
+            self._check(self.L.er_set_tuning(self.h, int(tuning["chunk"]), int(sp),
+                                             int(tuning.get("hist_slices", 0))))
         if len(splits) > 1:
             self.select_solute(0)
 

@@ -132,16 +132,16 @@ module er_capi
        integer(c_int32_t), value :: reps
      end function er_autotune
 
-     integer(c_int) function er_set_tuning(ctx, chunk, pair_split) bind(C, name="er_set_tuning")
+     integer(c_int) function er_set_tuning(ctx, chunk, pair_split, hist_slices) bind(C, name="er_set_tuning")
        import :: c_int, c_int32_t, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int32_t), value :: chunk, pair_split
+       integer(c_int32_t), value :: chunk, pair_split, hist_slices
      end function er_set_tuning
 
-     integer(c_int) function er_get_tuning(ctx, chunk, pair_split) bind(C, name="er_get_tuning")
+     integer(c_int) function er_get_tuning(ctx, chunk, pair_split, hist_slices) bind(C, name="er_get_tuning")
        import :: c_int, c_int32_t, c_ptr
        type(c_ptr), value :: ctx
-       integer(c_int32_t), intent(out) :: chunk, pair_split
+       integer(c_int32_t), intent(out) :: chunk, pair_split, hist_slices
      end function er_get_tuning
 
      integer(c_int) function er_clear(ctx) bind(C, name="er_clear")

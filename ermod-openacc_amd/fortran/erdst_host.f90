@@ -35,7 +35,7 @@ program erdst_host
   real(c_double), allocatable :: aveuv(:,:), flceng(:,:)
   real(c_double) :: scal(4), dummy(1)
   integer :: iu, rc, f, division, frames_per_div, k, io_flcuv, narg
-  integer(c_int32_t) :: tune_chunk, tune_split
+  integer(c_int32_t) :: tune_chunk, tune_split, tune_slices
   logical :: autotune
   integer, parameter :: stdout = 6
 
@@ -117,8 +117,9 @@ program erdst_host
      if (autotune .and. f == 1) then
         rc = er_autotune(ctx, 0_c_int32_t)
         call check(rc, "er_autotune")
-        rc = er_get_tuning(ctx, tune_chunk, tune_split)
-        write(stdout, '(a,i0,a,i0)') "erdst_host: launch shapes: chunk ", tune_chunk, ", pair split ", tune_split
+        rc = er_get_tuning(ctx, tune_chunk, tune_split, tune_slices)
+        write(stdout, '(a,i0,a,i0,a,i0)') "erdst_host: launch shapes: chunk ", tune_chunk, ", pair split ", &
+             tune_split, ", histogram slices ", tune_slices
      end if
      rc = er_run(ctx, mode)                                   ! engconst
      call check(rc, "er_run")

@@ -182,23 +182,28 @@ int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int64_t *launch
 /* Launch-shape tuning, no counterpart in the reference (its launch shapes
  * are fixed by the OpenACC compiler).  er_autotune times candidate shapes on
  * the frame and insertions already set, with HIP events on the context's
- * stream, over reps runs per candidate (0: 3): first the insertions per
- * launch sequence (chunk: the current one, 256 .. 4096; whole batched runs
- * of every solute), then each solute's pair-search workgroups per insertion
- * (pair_split: the current one, 1 .. 64; the pair kernel's own time).  A
- * candidate replaces the current shape only when it is more than 2% faster,
- * so the result is never worse than the current shapes by more than timing
- * noise and can be 0 (the built-in choice).  It restores every
+ * stream, over reps runs per candidate (0: 3):
+ *   chunk       insertions per launch sequence: the current one, 256 ..
+ *               4096 (whole batched runs of every solute);
+ *   hist_slices histogram workgroups per 16 insertions of a refs run: the
+ *               current one, 1 .. 64 (the histogram kernel's own time);
+ *   pair_split  pair-search workgroups per insertion, per solute: the
+ *               current one, 1 .. 64 (the pair kernel's own time).
+ * A candidate replaces the current shape only when it is more than 2%
+ * faster, so the result is never worse than the current shapes by more than
+ * timing noise; 0 stands for the built-in choice.  It restores every
  * solute's block accumulators, minuv/maxuv and halt state as they were, and
  * leaves the per-insertion energies of the current frame for
- * er_get_uvengy.  No shape changes any result, bit for bit.
+ * er_get_uvengy.  No shape changes any result, bit for bit (soln runs keep
+ * the built-in histogram slices, which their flceng sums follow).
  * The shapes apply to ER_BATCHED runs; ER_PLAIN keeps its built-in ones.
  * er_set_tuning applies shapes kept from an earlier tuning (chunk 0 keeps
- * the current one; pair_split 0 restores the built-in choice of the selected
- * solute); er_get_tuning reads them (pair_split of the selected solute). */
+ * the current one; pair_split 0 and hist_slices 0 restore the built-in
+ * choice; pair_split applies to the selected solute); er_get_tuning reads
+ * them (pair_split of the selected solute). */
 int er_autotune(er_ctx *c, int32_t reps);
-int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split);
-int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split);
+int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split, int32_t hist_slices);
+int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split, int32_t *hist_slices);
 
 #ifdef __cplusplus
 }

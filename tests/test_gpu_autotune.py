@@ -54,6 +54,7 @@ def test_autotune_keeps_accumulators_and_results(cfg0):
     t = tuned.autotune(reps=1)
     print("tuning:", t, flush=True)
     assert t["chunk"] in (256, 512, 1024, 2048, 4096)
+    assert t["hist_slices"] in (0, 1, 2, 4, 8, 16, 32, 64)
     assert len(t["pair_split"]) == 1 and t["pair_split"][0] in (0, 1, 2, 4, 8, 16, 32, 64)
     _same_block(tuned.fetch(), before)
     # the energies tuning leaves behind are the current frame's
@@ -67,13 +68,13 @@ def test_autotune_keeps_accumulators_and_results(cfg0):
     _same_block(tuned.fetch(), plain.fetch())
 
 
-@pytest.mark.parametrize("chunk,split", [(16, 1), (48, 3), (4096, 64)])
-def test_explicit_tunings_equal_builtin(cfg0, chunk, split):
+@pytest.mark.parametrize("chunk,split,slices", [(16, 1, 1), (48, 3, 7), (4096, 64, 64)])
+def test_explicit_tunings_equal_builtin(cfg0, chunk, split, slices):
     w, nins = cfg0, 100
     p = U.workload_params(w)
     ref, alt = _ctx(w, p), _ctx(w, p)
-    alt.set_tuning({"chunk": chunk, "pair_split": [split]})
-    assert alt.tuning() == {"chunk": chunk, "pair_split": [split]}
+    alt.set_tuning({"chunk": chunk, "hist_slices": slices, "pair_split": [split]})
+    assert alt.tuning() == {"chunk": chunk, "hist_slices": slices, "pair_split": [split]}
     for f in (0, 1):
         for c in (ref, alt):
             _frame(c, w, f, nins)
@@ -82,14 +83,15 @@ def test_explicit_tunings_equal_builtin(cfg0, chunk, split):
         for a, b in zip(alt.uvengy(nins), ref.uvengy(nins)):
             np.testing.assert_array_equal(a, b)
     _same_block(alt.fetch(), ref.fetch())
-    alt.set_tuning({"chunk": 0, "pair_split": [0]})
-    assert alt.tuning() == {"chunk": chunk, "pair_split": [0]}
+    alt.set_tuning({"chunk": 0, "hist_slices": 0, "pair_split": [0]})
+    assert alt.tuning() == {"chunk": chunk, "hist_slices": 0, "pair_split": [0]}
 
 
 def test_tuning_arguments_rejected(cfg0):
     ctx = _ctx(cfg0, U.workload_params(cfg0))
     for bad in ({"chunk": 17, "pair_split": [0]}, {"chunk": 8, "pair_split": [0]},
-                {"chunk": 0, "pair_split": [65]}, {"chunk": 0, "pair_split": [-1]}):
+                {"chunk": 0, "pair_split": [65]}, {"chunk": 0, "pair_split": [-1]},
+                {"chunk": 0, "pair_split": [0], "hist_slices": 65}):
         with pytest.raises(capi.ErError):
             ctx.set_tuning(bad)
     with pytest.raises(capi.ErError):
@@ -138,9 +140,12 @@ def test_autotune_several_solutes():
 
 
 def test_autotune_soln():
+    """A soln run: tuned shapes, and histogram slices set by hand (which soln
+    runs do not follow), give the untuned run's results bit for bit."""
     w = synth.config1()
     p = U.workload_params(w, corrcal=1)
     ref, tuned = _ctx(w, p), _ctx(w, p)
+    tuned.set_tuning({"chunk": 0, "hist_slices": 5, "pair_split": [0]})
     for f in (0, 1):
         for c in (ref, tuned):
             _frame(c, w, f)
@@ -152,4 +157,5 @@ def test_autotune_soln():
     n = len(w.topo.tagslt)
     for a, b in zip(tuned.uvengy(n), ref.uvengy(n)):
         np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(tuned.flceng(n), ref.flceng(n))
     _same_block(tuned.fetch(), ref.fetch())
