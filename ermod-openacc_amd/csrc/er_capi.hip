@@ -141,6 +141,7 @@ struct er_ctx {
     DBuf<float> d_sitepos_own;
     const float *d_sitepos = nullptr;
     DBuf<float4> d_posn, d_molref;
+    DBuf<float4> d_segref;         // segment reference + radius at each segment's first atom
     DBuf<float> d_splw;
     DBuf<int4> d_splg;
     std::vector<float> h_pin;      // host staging (pageable)
@@ -572,6 +573,7 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     HIPCHK(c, c->d_splw.ensure((size_t)c->numatm * 3 * c->ord));
     HIPCHK(c, c->d_splg.ensure(c->numatm));
     HIPCHK(c, c->d_molref.ensure(c->nummol));
+    HIPCHK(c, c->d_segref.ensure(c->numatm));
     HIPCHK(c, c->d_sitepos_own.ensure((size_t)3 * c->numatm));
 
     {
@@ -1242,7 +1244,7 @@ static hipError_t prep_frame(er_ctx *c)
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s2, (size_t)1,
                            reinterpret_cast<uint32_t *>(c->d_rmolmax.p));
         hipLaunchKernelGGL(k_prep_mols, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
-                           c->d_molbeg.p, c->d_posn.p, c->fc, c->d_molref.p, c->d_rmolmax.p);
+                           c->d_molbeg.p, c->d_posn.p, c->fc, c->d_molref.p, c->d_rmolmax.p, c->d_segref.p);
         // spatial order of the molecules and the cell list
         hipLaunchKernelGGL(k_zero_u32, dim3((c->ncell + B - 1) / B), dim3(B), 0, s2, (size_t)c->ncell,
                            c->d_ccnt.p);
@@ -1400,7 +1402,8 @@ template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, i
         pa.pp = c->pp;
         pa.cell_w = c->cell_w;
         hipLaunchKernelGGL(k_pairs, dim3(nch * pa.nsplit), dim3(256), 0, c->stream, pa, c->fc, c->d_molbeg.p, c->d_posn.p,
-                           c->d_ljtype.p, c->d_molref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p,
+                           c->d_ljtype.p, c->d_molref.p, c->d_segref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p,
+                           c->d_insposn.p,
                            c->d_insref.p, c->S().d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
                            c->d_ljlensq.p, c->S().d_U.p);
         if ((e = hipGetLastError()) != hipSuccess) return e;
@@ -1939,7 +1942,7 @@ extern "C" void er_destroy(er_ctx *c)
     for (Slot &sl : c->slots) sl.release();
     c->d_tag.release();
     c->d_uvcrd.release(); c->d_escrd.release(); c->d_uvmax.release(); c->d_uvsoft.release(); c->d_idmin.release();
-    c->d_sitepos_own.release(); c->d_posn.release(); c->d_molref.release(); c->d_splw.release(); c->d_splg.release();
+    c->d_sitepos_own.release(); c->d_posn.release(); c->d_molref.release(); c->d_segref.release(); c->d_splw.release(); c->d_splg.release();
     c->d_insposn.release(); c->d_insref.release(); c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();

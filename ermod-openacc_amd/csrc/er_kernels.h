@@ -141,8 +141,16 @@ __global__ void k_prep_atoms(int natom, const float *__restrict__ pos, const flo
 }
 
 // reference site and reach radius of each molecule (for cutoff pruning)
+#ifndef ER_SEG
+#define ER_SEG 8  // atoms per molecule segment of the pair search's pruning
+#endif
+
+// Per molecule: its reference site (first atom) and reach radius, and for
+// every segment of ER_SEG consecutive atoms (molecule-relative) the same pair
+// for the segment, stored at the segment's first atom in segref
 __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const float4 *__restrict__ posn,
-                            FrameConst fc, float4 *__restrict__ molref, float *__restrict__ rmolmax)
+                            FrameConst fc, float4 *__restrict__ molref, float *__restrict__ rmolmax,
+                            float4 *__restrict__ segref)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nmol) return;
@@ -156,6 +164,17 @@ __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const floa
         R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
     }
     molref[j] = make_float4(r.x, r.y, r.z, R);
+    for (int s0 = a0; s0 < a1; s0 += ER_SEG) {
+        const float4 q = posn[s0];
+        float Rs = 0.0f;
+        for (int a = s0 + 1; a < min(s0 + ER_SEG, a1); ++a) {
+            const float4 p = posn[a];
+            float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+            min_image(fc, dx, dy, dz);
+            Rs = fmaxf(Rs, sqrtf(dx * dx + dy * dy + dz * dz));
+        }
+        segref[s0] = make_float4(q.x, q.y, q.z, Rs);
+    }
     // frame maximum (non-negative floats order like their bit patterns)
     atomicMax(reinterpret_cast<unsigned int *>(rmolmax), __float_as_uint(R));
 }
@@ -1693,6 +1712,7 @@ __device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const
 __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const int *__restrict__ molbeg,
                                                const float4 *__restrict__ posn, const int *__restrict__ ljtype,
                                                const float4 *__restrict__ molref,
+                                               const float4 *__restrict__ segref,
                                                const float *__restrict__ rmolmax,
                                                const unsigned int *__restrict__ cstart,
                                                const int *__restrict__ perm, const float4 *__restrict__ insposn,
@@ -1803,16 +1823,30 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
         min_image(fc, rx, ry, rz);
         const float lim = skipc + m.w;
         if (rx * rx + ry * ry + rz * rz > lim * lim) return e;
-        for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
-            const float4 xb = posn[b];
-            const int tj = ljtype[b];
-            float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
-            min_image(fc, dx, dy, dz);
-            const float d2 = dx * dx + dy * dy + dz * dz;
-            if (d2 > skip2) continue;
-            const float le = in_lds ? te[ti + pa.ntype * tj] : ljene[ti + pa.ntype * tj];
-            const float ls = in_lds ? ts[ti + pa.ntype * tj] : ljlensq[ti + pa.ntype * tj];
-            e += pair_energy(pa.pp, d2, xa.w, xb.w, le, ls);
+        // a molecule of three or more segments: the same test per segment,
+        // whose skipped pairs all fail the per-pair test below (same sum,
+        // same order)
+        const int b0 = molbeg[jm], b1 = molbeg[jm + 1];
+        const bool segmented = b1 - b0 > 2 * ER_SEG;
+        for (int s0 = b0; s0 < b1; s0 += ER_SEG) {
+            if (segmented) {
+                const float4 q = segref[s0];
+                float qx = xa.x - q.x, qy = xa.y - q.y, qz = xa.z - q.z;
+                min_image(fc, qx, qy, qz);
+                const float ql = skipc + q.w;
+                if (qx * qx + qy * qy + qz * qz > ql * ql) continue;
+            }
+            for (int b = s0; b < min(s0 + ER_SEG, b1); ++b) {
+                const float4 xb = posn[b];
+                const int tj = ljtype[b];
+                float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
+                min_image(fc, dx, dy, dz);
+                const float d2 = dx * dx + dy * dy + dz * dz;
+                if (d2 > skip2) continue;
+                const float le = in_lds ? te[ti + pa.ntype * tj] : ljene[ti + pa.ntype * tj];
+                const float ls = in_lds ? ts[ti + pa.ntype * tj] : ljlensq[ti + pa.ntype * tj];
+                e += pair_energy(pa.pp, d2, xa.w, xb.w, le, ls);
+            }
         }
         return e;
     };
