@@ -87,6 +87,29 @@ def test_explicit_tunings_equal_builtin(cfg0, chunk, split, slices):
     assert alt.tuning() == {"chunk": chunk, "hist_slices": 0, "pair_split": [0]}
 
 
+@pytest.mark.parametrize("over", [dict(cltype=0, boxshp=0, estype=1), dict(cltype=0), dict(cltype=3)],
+                         ids=["open-bare", "bare", "pppm"])
+def test_autotune_other_interactions(cfg0, over):
+    """Open systems (no pair split to tune), bare Coulomb and PPPM: tuned
+    runs give the untuned runs' results bit for bit."""
+    w, nins = cfg0, 40
+    p = U.workload_params(w, **over)
+    ref, tuned = _ctx(w, p), _ctx(w, p)
+    for f in (0, 1):
+        for c in (ref, tuned):
+            _frame(c, w, f, nins)
+        if f == 0:
+            t = tuned.autotune(reps=1)
+            if over.get("boxshp") == 0:
+                assert t["pair_split"] == [0]
+        for c in (ref, tuned):
+            c.run(capi.ER_BATCHED)
+            c.sync()
+        for a, b in zip(tuned.uvengy(nins), ref.uvengy(nins)):
+            np.testing.assert_array_equal(a, b)
+    _same_block(tuned.fetch(), ref.fetch())
+
+
 def test_tuning_arguments_rejected(cfg0):
     ctx = _ctx(cfg0, U.workload_params(cfg0))
     for bad in ({"chunk": 17, "pair_split": [0]}, {"chunk": 8, "pair_split": [0]},
