@@ -1,6 +1,6 @@
 """erdst over an ERmod run directory, with the per-frame work on the GPU.
 
-    python -m erdst_gpu.erdst [DIRECTORY] [--plain] [--autotune] [--out OUTDIR]
+    python -m erdst_gpu.erdst [DIRECTORY] [--plain] [--autotune [--tuning-cache FILE]] [--out OUTDIR]
 
 Reads what erdst reads (erdst/setconf.F90, erdst/engproc.F90:36-318):
 parameters_er (&ene_param, &hist), MDinfo, MolPrm<n>, SltInfo, LJTable,
@@ -33,6 +33,7 @@ from .engstore import BlockWriter
 from .inputs import (SLT_REFS_FLEX, SLT_REFS_RIGID, SLT_SOLN, InputError, WeightFile, apply_permutation,
                      iniparam, load_system, read_namelist, read_permindex)
 from .mesh import species_meshes
+from .tuning import tune_cached, tuning_key
 
 
 def _hot_params(p: dict) -> dict:
@@ -90,10 +91,11 @@ def _check_supported(p: dict):
 
 
 def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log=print,
-        autotune: bool = False) -> dict:
+        autotune: bool = False, tuning_cache: str | None = None) -> dict:
     """One erdst run; returns the last block's fetched accumulators.  With
     autotune the launch shapes are timed on the first frame (er_autotune),
-    which changes no output."""
+    or taken from tuning_cache when it holds this system's; either way no
+    output changes."""
     d = directory
     out = out or d
     path = lambda name: os.path.join(d, name)
@@ -137,6 +139,16 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
             sltwgt = WeightFile(path("SltWght"), rewind=True)
     g = None
     tune = autotune and mode == capi.ER_BATCHED
+
+    def apply_tuning():
+        if tuning_cache is None:
+            log(f"erdst: launch shapes {ctx.autotune()}")
+            return
+        mb = topo.mol_begin
+        sites = [int(mb[m + 1] - mb[m]) for m in topo.tagslt] if soln else [nslt]
+        key = tuning_key(topo, _hot_params(p), sites, len(topo.tagslt) if soln else nins)
+        shapes, found = tune_cached(ctx, key, tuning_cache)
+        log(f"erdst: launch shapes {shapes} ({'from' if found else 'added to'} {tuning_cache})")
     active = list(range(len(topo.tagslt)))
     for division in range(1, engdiv + 1):
         for _ in range(frames_per_div):
@@ -181,13 +193,13 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
                     ctx.set_solute_molecules(topo.tagslt[keep])
                     active = keep
                 if tune:
-                    log(f"erdst: launch shapes {ctx.autotune()}")
+                    apply_tuning()
                     tune = False
                 ctx.run(mode)
                 writer.frame(ctx.flceng(len(keep)), skpcnf, keep, len(topo.tagslt))
                 continue
             if tune:
-                log(f"erdst: launch shapes {ctx.autotune()}")
+                apply_tuning()
                 tune = False
             ctx.run(mode)
             writer.frame(ctx.flceng(len(topo.tagslt)) if soln else None, skpcnf)
@@ -207,9 +219,14 @@ def main(argv=None) -> int:
     ap.add_argument("--out", default=None, help="directory for the output files (default: the run directory)")
     ap.add_argument("--plain", action="store_true", help="one insertion per launch sequence")
     ap.add_argument("--autotune", action="store_true", help="time the launch shapes on the first frame")
+    ap.add_argument("--tuning-cache", default=None,
+                    help="with --autotune: take the shapes from this JSON file, or add them to it")
     a = ap.parse_args(argv)
+    if a.tuning_cache and not a.autotune:
+        ap.error("--tuning-cache needs --autotune")
     try:
-        run(a.directory, a.out, capi.ER_PLAIN if a.plain else capi.ER_BATCHED, autotune=a.autotune)
+        run(a.directory, a.out, capi.ER_PLAIN if a.plain else capi.ER_BATCHED, autotune=a.autotune,
+            tuning_cache=a.tuning_cache)
     except (InputError, capi.TrajError, capi.ErError) as e:
         print(f"erdst: {e}", file=sys.stderr)
         return 1

@@ -3,6 +3,7 @@ MDinfo, MolPrm1, SltInfo, EcdMesh, HISTORY and SltConf, as the reference run
 of the 1000-water refs configuration is laid out (slttype 3, insertions from
 SltConf as they are).  Every output file equals the one written from the same
 inputs fed programmatically through the Python binding."""
+import json
 import os
 
 import numpy as np
@@ -56,13 +57,22 @@ def write_run_directory(d, w):
     write_dcd(d / "SltConf", confs, None, charmm=False)
 
 
-@pytest.mark.parametrize("autotune", [False, True])
+@pytest.mark.parametrize("autotune", [False, True, "cache"])
 def test_erdst_driver_over_a_run_directory(tmp_path, autotune):
+    """Every output byte-identical to the same inputs through the binding;
+    also with tuned launch shapes, and with shapes stored in a tuning cache
+    by a first run and read back by a second."""
     w = synth.config0()
     run_dir = tmp_path / "run"
     run_dir.mkdir()
     write_run_directory(run_dir, w)
-    erdst.run(str(run_dir), log=lambda *a: None, autotune=autotune)
+    cache = str(tmp_path / "shapes.json") if autotune == "cache" else None
+    if cache:   # a first run tunes and stores the shapes (its outputs elsewhere)
+        (tmp_path / "first").mkdir()
+        erdst.run(str(run_dir), out=str(tmp_path / "first"), log=lambda *a: None, autotune=True,
+                  tuning_cache=cache)
+        assert len(json.load(open(cache))) == 1
+    erdst.run(str(run_dir), log=lambda *a: None, autotune=bool(autotune), tuning_cache=cache)
     # the same inputs through the binding, written by the same writers
     ref_dir = tmp_path / "ref"
     ref_dir.mkdir()
