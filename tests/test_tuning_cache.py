@@ -50,3 +50,11 @@ def test_cache_file_must_hold_an_object(tmp_path):
     path.write_text("[1, 2]")
     with pytest.raises(ValueError):
         tuning.tune_cached(FakeContext({}), "k", str(path))
+
+
+def test_driver_tuning_cache_needs_autotune(tmp_path, capsys):
+    from erdst_gpu import erdst
+    with pytest.raises(SystemExit) as e:
+        erdst.main([str(tmp_path), "--tuning-cache", str(tmp_path / "shapes.json")])
+    assert e.value.code == 2
+    assert "--tuning-cache needs --autotune" in capsys.readouterr().err
