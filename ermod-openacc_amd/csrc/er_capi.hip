@@ -161,6 +161,7 @@ struct er_ctx {
     DBuf<float2> d_T;
     DBuf<float> d_phi;
     int chunk = 1024;              // insertions per launch sequence (ER_CHUNK overrides)
+    int ecorr_kmin = 64;           // least insertions per split-K slice of edens/ecorr (ER_ECORR_KMIN overrides)
 
     // spatial order / cell list (per frame)
     int nc = 1, ncell = 1;
@@ -463,6 +464,7 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
         HIPCHK(c, hipMemcpy(c->d_border.p, order.data(), c->nbrick * sizeof(int), hipMemcpyHostToDevice));
     }
     if (const char *e = getenv("ER_CHUNK")) c->chunk = std::max(NI_B, atoi(e) / NI_B * NI_B);
+    if (const char *e = getenv("ER_ECORR_KMIN")) c->ecorr_kmin = std::max(4, std::min(65536, atoi(e) / 4 * 4));
 
     c->molbeg.resize(c->nummol + 1);
     for (int j = 0; j <= c->nummol; ++j) c->molbeg[j] = topo->mol_begin_index[j] - 1;
@@ -1455,7 +1457,7 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
                            c->S().d_slnuv.p, c->S().d_errkey.p);
     }
     // insertion slices of at most kper (a multiple of 4) for the split-K sums
-    const int kper = std::max(4, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
+    const int kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
     const int nks = (nch + kper - 1) / kper;
     const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
     {
