@@ -82,17 +82,18 @@ struct Slot {
     int last_NI = 16, last_nins = 0;
     // accumulators
     DBuf<double> d_edens, d_ecorr, d_eself, d_state, d_slnuv;
-    DBuf<int> d_flags, d_warn;
+    DBuf<int> d_flags;
     DBuf<float> d_fstate;
     DBuf<unsigned int> d_minmax;
-    DBuf<unsigned long long> d_errkey, d_halt;  // halt key; captured key + energy
+    // halt key, 'mbin' warning key; captured (key, energy) of each
+    DBuf<unsigned long long> d_errkey, d_warnkey, d_halt;
     unsigned int seq = 0;                         // runs of this slot
     int pair_split = 0;            // pair-search workgroups per insertion (0: built-in choice)
     void release()
     {
         d_sltq.release(); d_sltlj.release(); d_sltpos_own.release(); d_wslt.release(); d_U.release();
         d_u0.release(); d_edens.release(); d_ecorr.release(); d_eself.release(); d_state.release();
-        d_slnuv.release(); d_flags.release(); d_warn.release(); d_fstate.release(); d_minmax.release();
+        d_slnuv.release(); d_flags.release(); d_warnkey.release(); d_fstate.release(); d_minmax.release();
         d_errkey.release(); d_halt.release();
     }
 };
@@ -350,17 +351,17 @@ static int slot_state_init(er_ctx *c, Slot &sl)
     HIPCHK(c, sl.d_state.ensure(4));
     HIPCHK(c, sl.d_flags.ensure(4));
     HIPCHK(c, sl.d_fstate.ensure(4));
-    HIPCHK(c, sl.d_warn.ensure(1));
+    HIPCHK(c, sl.d_warnkey.ensure(1));
     HIPCHK(c, sl.d_errkey.ensure(1));
-    HIPCHK(c, sl.d_halt.ensure(2));
+    HIPCHK(c, sl.d_halt.ensure(4));
     HIPCHK(c, sl.d_minmax.ensure(2 * (size_t)(c->numslv + 1)));
     HIPCHK(c, sl.d_slnuv.ensure(c->numslv));
     HIPCHK(c, hipMemsetAsync(sl.d_state.p, 0, 4 * sizeof(double), s));
     HIPCHK(c, hipMemsetAsync(sl.d_flags.p, 0, 4 * sizeof(int), s));
     HIPCHK(c, hipMemsetAsync(sl.d_fstate.p, 0, 4 * sizeof(float), s));
-    HIPCHK(c, hipMemsetAsync(sl.d_warn.p, 0, sizeof(int), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_warnkey.p, 0xff, sizeof(unsigned long long), s));
     HIPCHK(c, hipMemsetAsync(sl.d_errkey.p, 0xff, sizeof(unsigned long long), s));
-    HIPCHK(c, hipMemsetAsync(sl.d_halt.p, 0xff, 2 * sizeof(unsigned long long), s));
+    HIPCHK(c, hipMemsetAsync(sl.d_halt.p, 0xff, 4 * sizeof(unsigned long long), s));
     sl.seq = 0;
     HIPCHK(c, hipMemsetAsync(sl.d_slnuv.p, 0, c->numslv * sizeof(double), s));
     std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
@@ -1163,7 +1164,7 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
                        soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_blut.p, c->d_spb.p, c->d_spe.p,
                        c->d_idmin.p,
                        c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p, c->S().d_minmax.p, soln ? c->d_flpart.p : (double *)nullptr,
-                       c->S().d_errkey.p, c->S().d_warn.p);
+                       c->S().d_errkey.p, c->S().d_warnkey.p);
     return hipGetLastError();
 }
 
@@ -1454,7 +1455,7 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         hipLaunchKernelGGL(k_weights_fin, dim3(1), dim3(256), 0, c->stream, wa, nblk, c->d_wpart.p, c->S().d_state.p,
                            c->S().d_u0.p, c->d_w.p, c->d_escrd.p, c->S().d_eself.p, c->S().d_minmax.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
-                           c->S().d_slnuv.p, c->S().d_errkey.p);
+                           c->S().d_slnuv.p, c->S().d_errkey.p, c->S().d_warnkey.p);
     }
     // insertion slices of at most kper (a multiple of 4) for the split-K sums
     const int kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
@@ -1530,8 +1531,8 @@ static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
         }
     }
     HIPCHK(c, join_atoms(c));  // a run without chunks still orders the next upload after the atom work
-    hipLaunchKernelGGL(k_halt_capture, dim3(1), dim3(1), 0, c->stream, c->S().seq, NI, c->nummol,
-                       c->S().d_errkey.p, c->S().d_U.p, c->S().d_u0.p, c->S().d_halt.p);
+    hipLaunchKernelGGL(k_halt_capture, dim3(1), dim3(64), 0, c->stream, c->S().seq, NI, c->nummol,
+                       c->S().d_errkey.p, c->S().d_warnkey.p, c->S().d_U.p, c->S().d_u0.p, c->S().d_halt.p);
     HIPCHK(c, hipGetLastError());
     c->S().last_NI = NI;
     c->S().last_nins = nins;
@@ -1579,21 +1580,28 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
     return run_slots(c, todo, mode);
 }
 
-// the halt of slot c->S(): (insertion, molecule) from the key, the energy as
-// captured at the end of the run that produced it
-static void decode_halt(er_ctx *c, unsigned long long key)
+// the halt (or, warn = true, the 'mbin' warning) of slot c->S():
+// (insertion, molecule) from the key, the energy as captured at the end of
+// the run that produced it
+static void decode_halt(er_ctx *c, unsigned long long key, bool warn)
 {
     const int jp1 = (int)(key & 0x1fffffu);
-    unsigned long long cap[2] = {~0ull, 0ull};
+    unsigned long long cap[4] = {~0ull, 0ull, ~0ull, 0ull};
     (void)hipMemcpy(cap, c->S().d_halt.p, sizeof cap, hipMemcpyDeviceToHost);
+    const int o = warn ? 2 : 0;
     float v = 0.f;
-    if (cap[0] == key) {
-        const unsigned int bits = (unsigned int)cap[1];
+    if (cap[o] == key) {
+        const unsigned int bits = (unsigned int)cap[o + 1];
         std::memcpy(&v, &bits, 4);
     } else {
         v = NAN;  // not captured (cannot happen after a completed run)
     }
     c->halt_energy = v;
+    if (warn) {
+        std::strcpy(c->halt_code, "mbin");
+        c->halt_species = jp1 == 0 ? 0 : c->uvspec[jp1 - 1];
+        return;
+    }
     if (jp1 == 0) {
         std::strcpy(c->halt_code, "eng_min");
         c->halt_species = 0;
@@ -1617,13 +1625,23 @@ extern "C" int er_sync(er_ctx *c)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const int keep = c->cur;
     for (int k = 0; k < (int)c->slots.size(); ++k) {
-        unsigned long long key = ~0ull;
+        unsigned long long key = ~0ull, wkey = ~0ull;
         HIPCHK(c, hipMemcpy(&key, c->slots[k].d_errkey.p, sizeof key, hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(&wkey, c->slots[k].d_warnkey.p, sizeof wkey, hipMemcpyDeviceToHost));
+        // getiduv tests the halt conditions before the warning, so at equal
+        // keys (one energy) the halt is met first
+        const bool warn = !c->prm.force_calculation && wkey < key;
+        if (warn) key = wkey;
         if (key != ~0ull) {
             c->cur = k;
-            decode_halt(c, key);
+            decode_halt(c, key, warn);
             c->cur = keep;
             c->halt_slot = k;
+            if (warn)
+                return fail(c, ER_E_HALT,
+                            "warning mbin: the maximum binning energy is too low for this species (solute %d, "
+                            "species %d, energy %g); force_calculation lets the run go on",
+                            k + 1, c->halt_species, (double)c->halt_energy);
             return fail(c, ER_E_HALT, "halt condition %s (solute %d, species %d, energy %g)", c->halt_code, k + 1,
                         c->halt_species, (double)c->halt_energy);
         }
@@ -1739,10 +1757,10 @@ extern "C" int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int6
 // voffset, flags, minuv/maxuv, halt keys), kept aside while tuning
 struct SlotSnap {
     DBuf<double> edens, ecorr, eself, state, slnuv;
-    DBuf<int> flags, warn;
+    DBuf<int> flags;
     DBuf<float> fstate;
     DBuf<unsigned int> minmax;
-    DBuf<unsigned long long> errkey, halt;
+    DBuf<unsigned long long> errkey, warnkey, halt;
 };
 
 template <class T> static hipError_t snap_take(DBuf<T> &dst, const DBuf<T> &src, hipStream_t s)
@@ -1766,7 +1784,7 @@ static hipError_t slot_snap(Slot &sl, SlotSnap &sn, hipStream_t s, bool take)
     if ((e = f(sl.d_edens, sn.edens)) != hipSuccess || (e = f(sl.d_ecorr, sn.ecorr)) != hipSuccess ||
         (e = f(sl.d_eself, sn.eself)) != hipSuccess || (e = f(sl.d_state, sn.state)) != hipSuccess ||
         (e = f(sl.d_slnuv, sn.slnuv)) != hipSuccess || (e = f(sl.d_flags, sn.flags)) != hipSuccess ||
-        (e = f(sl.d_warn, sn.warn)) != hipSuccess || (e = f(sl.d_fstate, sn.fstate)) != hipSuccess ||
+        (e = f(sl.d_warnkey, sn.warnkey)) != hipSuccess || (e = f(sl.d_fstate, sn.fstate)) != hipSuccess ||
         (e = f(sl.d_minmax, sn.minmax)) != hipSuccess || (e = f(sl.d_errkey, sn.errkey)) != hipSuccess ||
         (e = f(sl.d_halt, sn.halt)) != hipSuccess)
         return e;
@@ -1897,9 +1915,18 @@ extern "C" int er_autotune(er_ctx *c, int32_t reps)
     std::vector<SlotSnap> snap(c->slots.size());
     for (size_t k = 0; k < c->slots.size(); ++k) HIPCHK(c, slot_snap(c->slots[k], snap[k], c->stream, true));
     const int keep = c->cur;
+    // the shapes in force, restored if tuning fails part way
+    const int chunk_keep = c->chunk, slices_keep = c->hist_slices;
+    std::vector<int> split_keep(c->slots.size());
+    for (size_t k = 0; k < c->slots.size(); ++k) split_keep[k] = c->slots[k].pair_split;
     rc = autotune_runs(c, todo, reps);
     c->cur = keep;
     if (rc == ER_OK) rc = er_sync(c);  // a halt hit by a tuning run is reported here
+    if (rc != ER_OK) {
+        c->chunk = chunk_keep;
+        c->hist_slices = slices_keep;
+        for (size_t k = 0; k < c->slots.size(); ++k) c->slots[k].pair_split = split_keep[k];
+    }
     (void)hipStreamSynchronize(c->stream);
     prof_collect(c);
     std::memcpy(c->prof_ms, ms_keep, sizeof ms_keep);
@@ -1907,6 +1934,14 @@ extern "C" int er_autotune(er_ctx *c, int32_t reps)
     c->prof = prof_keep;
     for (size_t k = 0; k < c->slots.size(); ++k) HIPCHK(c, slot_snap(c->slots[k], snap[k], c->stream, false));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    // spectra and potentials sized for a larger candidate chunk than the one
+    // kept are given back (the next run sizes them for the kept chunk)
+    {
+        const size_t ntile = (size_t)(c->chunk + NI_B - 1) / NI_B;
+        if (c->d_T.n > (size_t)c->chunk * c->Gc) c->d_T.release();
+        if (c->d_phi.n > ntile * NI_B * c->G) c->d_phi.release();
+        if (c->d_Eat.n > ntile * NI_B * (size_t)c->numatm) c->d_Eat.release();
+    }
     return rc;
 }
 

@@ -1911,7 +1911,8 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
                                               const int *__restrict__ uvmax, const int *__restrict__ uvsoft,
                                               uint32_t *__restrict__ Cnt, unsigned int *__restrict__ minmax,
                                               double *__restrict__ flpart,
-                                              unsigned long long *__restrict__ errkey, int *__restrict__ warn)
+                                              unsigned long long *__restrict__ errkey,
+                                              unsigned long long *__restrict__ warnkey)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     double *rsum = reinterpret_cast<double *>(smem);                 // [NT]
@@ -1970,7 +1971,9 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
                 }
                 const int idpti = lo;
                 if (idpti <= 0 || (ha.slttype == 1 && idpti > soft)) atomicMin(errkey, halt_key(ha.seq, n, j + 1));
-                if (idpti == num && v < 1.0e3f) atomicOr(warn, 1);
+                // the 'mbin' warning (engproc.F90:1158-1167): an energy below
+                // 1e3 in the open top bin; the earliest one is kept
+                if (idpti == num && v < 1.0e3f) atomicMin(warnkey, halt_key(ha.seq, n, j + 1));
                 atomicAdd(&hist[i * ha.ermax + base + max(idpti, 1) - 1], 1u);
                 lmin = fminf(lmin, v);
                 lmax = fmaxf(lmax, v);
@@ -2014,19 +2017,22 @@ __global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
     if (i < n) p[i] = 0u;
 }
 
-// At the end of a run: if the halt key belongs to this run and has not been
-// captured yet, record the offending energy while U / uvengy0 still hold it
-// (cap[0] = captured key, cap[1] = the energy's bits).
+// At the end of a run: if the halt key (thread 0) or the warning key (thread
+// 1) belongs to this run and has not been captured yet, record the
+// offending energy while U / uvengy0 still hold it (cap[2t] = captured key,
+// cap[2t + 1] = the energy's bits).
 __global__ void k_halt_capture(unsigned int seq, int NI, int nmol, const unsigned long long *__restrict__ errkey,
-                               const float *__restrict__ U, const float *__restrict__ u0,
-                               unsigned long long *__restrict__ cap)
+                               const unsigned long long *__restrict__ warnkey, const float *__restrict__ U,
+                               const float *__restrict__ u0, unsigned long long *__restrict__ cap)
 {
-    const unsigned long long key = *errkey;
-    if (key == ~0ull || (unsigned int)(key >> 42) != (seq & 0x3fffffu) || cap[0] == key) return;
+    const int t = threadIdx.x;
+    if (t > 1) return;
+    const unsigned long long key = t == 0 ? *errkey : *warnkey;
+    if (key == ~0ull || (unsigned int)(key >> 42) != (seq & 0x3fffffu) || cap[2 * t] == key) return;
     const int n = (int)((key >> 21) & 0x1fffffu), jp1 = (int)(key & 0x1fffffu);
     const float v = jp1 == 0 ? u0[n] : U[((size_t)(n / NI) * nmol + (jp1 - 1)) * NI + (n % NI)];
-    cap[0] = key;
-    cap[1] = (unsigned long long)__float_as_uint(v);
+    cap[2 * t] = key;
+    cap[2 * t + 1] = (unsigned long long)__float_as_uint(v);
 }
 
 // ---------------------------------------------------------------------------
@@ -2153,7 +2159,8 @@ __global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, co
                                                      unsigned int *__restrict__ minmax,
                                                      const double *__restrict__ flpart,
                                                      const double *__restrict__ flceng, double *__restrict__ slnuv,
-                                                     unsigned long long *__restrict__ errkey)
+                                                     unsigned long long *__restrict__ errkey,
+                                                     unsigned long long *__restrict__ warnkey)
 {
     const int tid = threadIdx.x;
     if (tid == 0) {
@@ -2185,6 +2192,7 @@ __global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, co
                     else hi = mid;
                 }
                 if (lo <= 0) atomicMin(errkey, halt_key(wa.seq, n, 0));
+                if (lo == wa.esmax && u < 1.0e3f) atomicMin(warnkey, halt_key(wa.seq, n, 0));
                 eself[max(lo, 1) - 1] += wout[n];
             }
         }

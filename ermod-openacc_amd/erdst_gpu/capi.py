@@ -40,7 +40,8 @@ class ErParams(C.Structure):
                 ("screen", C.c_float), ("ljswitch", C.c_int32), ("cltype", C.c_int32),
                 ("splodr", C.c_int32), ("ms", C.c_int32 * 3), ("boxshp", C.c_int32),
                 ("slttype", C.c_int32), ("wgtslf", C.c_int32), ("estype", C.c_int32),
-                ("corrcal", C.c_int32), ("selfcal", C.c_int32), ("temp", C.c_double)]
+                ("corrcal", C.c_int32), ("selfcal", C.c_int32), ("temp", C.c_double),
+                ("force_calculation", C.c_int32)]
 
 
 class ErError(RuntimeError):
@@ -196,6 +197,7 @@ class Context:
         p.boxshp, p.slttype, p.wgtslf = params["boxshp"], params["slttype"], params["wgtslf"]
         p.estype, p.corrcal, p.selfcal = params["estype"], params["corrcal"], params["selfcal"]
         p.temp = params["temp"]
+        p.force_calculation = int(bool(params.get("force_calculation", 0)))
         h = C.c_void_p()
         rc = L.er_create(C.byref(t), C.byref(p), C.byref(h))
         self.h = h

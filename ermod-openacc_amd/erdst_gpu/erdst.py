@@ -42,7 +42,7 @@ def _hot_params(p: dict) -> dict:
                 splodr=int(p["splodr"]), ms=(int(p["ms1max"]), int(p["ms2max"]), int(p["ms3max"])),
                 boxshp=int(p["boxshp"]), slttype=int(p["slttype"]), wgtslf=int(p["wgtslf"]),
                 estype=int(p["estype"]), corrcal=int(p["corrcal"]), selfcal=int(p["selfcal"]),
-                temp=float(p["temp"]))
+                temp=float(p["temp"]), force_calculation=int(bool(p["force_calculation"])))
 
 
 def region_distance(p: dict, sysin, xyz, cell, mol: int) -> float:
@@ -149,6 +149,19 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
         key = tuning_key(topo, _hot_params(p), sites, len(topo.tagslt) if soln else nins)
         shapes, found = tune_cached(ctx, key, tuning_cache)
         log(f"erdst: launch shapes {shapes} ({'from' if found else 'added to'} {tuning_cache})")
+    def run_frame(m):
+        # engconst halts inside the frame on eng_min / eng_sft, and on the
+        # 'mbin' warning unless force_calculation (engproc.F90:1142-1167,
+        # mpiproc.F90:238-252): the frame's output lines and the block's
+        # files are not written
+        ctx.run(m)
+        try:
+            ctx.sync()
+        except capi.ErError as e:
+            if e.status != capi.ER_E_HALT:
+                raise
+            code, sp, en = ctx.last_error()
+            raise InputError(f"energy of {en:.4g} for {sp}-th species: {e}", code) from None
     active = list(range(len(topo.tagslt)))
     for division in range(1, engdiv + 1):
         for _ in range(frames_per_div):
@@ -185,7 +198,7 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
                 # check_mol_configuration per solute molecule (engproc.F90:
                 # 750-756): a rejected one adds nothing this frame
                 keep = [k for k, m in enumerate(topo.tagslt)
-                        if p["lwreg"] <= region_distance(p, sysin, fr[0], fr[1], int(m)) <= p["upreg"]]
+                        if p["lwreg"] <= region_distance(p, sysin, xyz, cell, int(m)) <= p["upreg"]]
                 if not keep:
                     writer.frame(np.zeros((0, topo.numslv)), skpcnf, [], len(topo.tagslt))
                     continue
@@ -195,13 +208,13 @@ def run(directory: str, out: str | None = None, mode: int = capi.ER_BATCHED, log
                 if tune:
                     apply_tuning()
                     tune = False
-                ctx.run(mode)
+                run_frame(mode)
                 writer.frame(ctx.flceng(len(keep)), skpcnf, keep, len(topo.tagslt))
                 continue
             if tune:
                 apply_tuning()
                 tune = False
-            ctx.run(mode)
+            run_frame(mode)
             writer.frame(ctx.flceng(len(topo.tagslt)) if soln else None, skpcnf)
         g = ctx.fetch()
         writer.store(g)

@@ -17,7 +17,7 @@ def temperature_kt(inptemp: float = 300.0, double_build: bool = False) -> float:
 
 def hot_path_params(slttype: int, ms, *, elecut=12.0, upljcut=None, lwljcut=None, screen=0.35,
                     splodr=4, ljswitch=0, estype=2, corrcal=None, wgtslf=None, selfcal=0,
-                    inptemp=300.0, cltype=2, boxshp=1) -> dict:
+                    inptemp=300.0, cltype=2, boxshp=1, force_calculation=False) -> dict:
     """Parameters after iniparam's defaulting: upljcut and lwljcut default to
     12 and 10 A, set from the default elecut before the namelist is read
     (setconf.F90:180-188), so a user's elecut does not move them; corrcal
@@ -30,4 +30,4 @@ def hot_path_params(slttype: int, ms, *, elecut=12.0, upljcut=None, lwljcut=None
                 ljswitch=int(ljswitch), cltype=int(cltype), splodr=int(splodr), ms=tuple(int(m) for m in ms),
                 boxshp=int(boxshp), slttype=int(slttype), wgtslf=int((1 if refs else 0) if wgtslf is None else wgtslf),
                 estype=int(estype), corrcal=int((1 if refs else 0) if corrcal is None else corrcal),
-                selfcal=int(selfcal), temp=temperature_kt(inptemp))
+                selfcal=int(selfcal), temp=temperature_kt(inptemp), force_calculation=int(bool(force_calculation)))

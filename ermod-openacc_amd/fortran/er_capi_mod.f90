@@ -23,6 +23,7 @@ module er_capi
      integer(c_int32_t) :: ms(3)
      integer(c_int32_t) :: boxshp, slttype, wgtslf, estype, corrcal, selfcal
      real(c_double) :: temp
+     integer(c_int32_t) :: force_calculation
   end type er_params
 
   public :: er_create, er_set_mesh, er_set_solute, er_set_solutes, er_select_solute, er_set_solute_molecules, &

@@ -123,6 +123,8 @@ program erdst_host
      end if
      rc = er_run(ctx, mode)                                   ! engconst
      call check(rc, "er_run")
+     rc = er_sync(ctx)                 ! engconst's halts and the mbin warning stop the run here
+     call check(rc, "er_run")
      call frame_output(f)
      if (mod(f, frames_per_div) == 0) then
         division = f / frames_per_div
@@ -200,6 +202,7 @@ contains
     prm%cltype = hdr(21); prm%boxshp = hdr(22)
     read(iu) rp, t
     prm%elecut = rp(1); prm%lwljcut = rp(2); prm%upljcut = rp(3); prm%screen = rp(4); prm%temp = t
+    prm%force_calculation = 0                   ! force_calculation = .false. (setconf.F90 default)
     allocate(mol_begin(nummol + 1), ljtype(numatm), uvspec(nummol), charge(numatm))
     allocate(ljene(ntype * ntype), ljlensq(ntype * ntype), uvmax(numslv), uvsoft(numslv), uvcrd(ermax))
     read(iu) mol_begin, ljtype, uvspec, charge, ljene, ljlensq, uvmax, uvsoft, uvcrd

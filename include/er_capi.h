@@ -76,6 +76,8 @@ typedef struct {
     int32_t slttype;            /* 1 soln, 2 refs rigid, 3 refs flexible    */
     int32_t wgtslf, estype, corrcal, selfcal;
     double temp;                /* kT in kcal/mol (setconf.F90:304)         */
+    int32_t force_calculation;  /* 1: the 'mbin' warning does not stop the run
+                                   (engmain force_calculation, mpiproc.F90:238-252) */
 } er_params;
 
 /* Replaces: setparam's device uploads (setconf.F90:656-841), recpcal_init
@@ -138,7 +140,13 @@ int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sltpos_dev,
 int er_run(er_ctx *c, int32_t mode);
 
 /* Waits for the context's stream; returns ER_E_HALT if a run hit a halt
- * condition. */
+ * condition (getiduv's 'eng_min' / 'eng_sft', engproc.F90:1142-1156) or,
+ * unless force_calculation is set, the 'mbin' warning: an energy below
+ * 1e3 kcal/mol in a species' open top bin (engproc.F90:1158-1167), on which
+ * the reference stops (warning, mpiproc.F90:238-252).  The earliest event in
+ * (run, insertion, molecule) order is reported, as the reference meets
+ * them; er_last_error gives its code ("mbin" for the warning), species and
+ * energy. */
 int er_sync(er_ctx *c);
 
 /* Per-insertion energies of the last er_run, for parity checks.

@@ -105,6 +105,61 @@ def energy_mismatch(e_gpu, e_ref, rtol=RTOL, atol=ATOL):
     return int((excess > 0).sum()), float(excess.max()), worst
 
 
+REL_FLOOR = 1.0  # kcal/mol: relative errors are reported for |E| at or above it
+
+
+def energy_report(e_gpu, e_ref) -> dict:
+    """Error summary of FP32 energies against the FP64 oracle: the largest
+    absolute error below E_MID (the regime ATOL bounds), the largest relative
+    error of energies of at least REL_FLOOR in magnitude below E_MID, and the
+    largest relative error on the LJ wall (above E_MID)."""
+    e_gpu = np.asarray(e_gpu, np.float64)
+    e_ref = np.asarray(e_ref, np.float64)
+    d = np.abs(e_gpu - e_ref)
+    a = np.abs(e_ref)
+    low = a < E_MID
+    rel = low & (a >= REL_FLOOR)
+    wall = ~low
+    return dict(max_abs=float(d[low].max()) if low.any() else 0.0,
+                max_rel=float((d[rel] / a[rel]).max()) if rel.any() else 0.0,
+                max_rel_wall=float((d[wall] / a[wall]).max()) if wall.any() else 0.0,
+                n=int(e_ref.size))
+
+
+def flip_report(e_gpu, e_ref, species, uvmax, uvcrd) -> dict:
+    """Molecules whose bin differs between the GPU's FP32 energies and the
+    oracle's FP64 energies.  e_*: [nins, nmol]; species: [nmol] (0 = the
+    solute itself, skipped).  Every flip must have a bin edge between the two
+    energies within the energy tolerance; each is reported with the FP32
+    ulps between the FP64 energy and that edge."""
+    e_gpu = np.asarray(e_gpu, np.float64)
+    e_ref = np.asarray(e_ref, np.float64)
+    idmin = np.concatenate([[0], np.cumsum(uvmax)[:-1]])
+    nflip, max_ulp, unexplained = 0, 0.0, 0
+    for sp in range(1, len(uvmax) + 1):
+        cols = np.nonzero(np.asarray(species) == sp)[0]
+        if cols.size == 0:
+            continue
+        mesh = np.asarray(uvcrd[idmin[sp - 1]:idmin[sp - 1] + uvmax[sp - 1]], np.float64)
+        g, r = e_gpu[:, cols], e_ref[:, cols]
+        bg = np.searchsorted(mesh, g, side="left")
+        br = np.searchsorted(mesh, r, side="left")
+        diff = bg != br
+        if not diff.any():
+            continue
+        nflip += int(diff.sum())
+        lo = np.minimum(g, r)[diff]
+        hi = np.maximum(g, r)[diff]
+        tol = energy_tolerance(r[diff])
+        ilo = np.searchsorted(mesh, lo, side="left")
+        edge = mesh[np.minimum(ilo, mesh.size - 1)]
+        ok = (np.searchsorted(mesh, hi, side="left") > ilo) & (hi - lo <= tol)
+        unexplained += int((~ok).sum())
+        ulp = np.spacing(np.abs(edge).astype(np.float32)).astype(np.float64)
+        max_ulp = max(max_ulp, float((np.abs(r[diff] - edge) / ulp).max()))
+    return dict(flips=nflip, max_ulp_to_edge=max_ulp, unexplained=unexplained, n=int(e_ref.size))
+
+
 def oracle_block(d) -> dict:
     """An oracle Distribution in er_fetch's dict layout."""
     g = dict(edens=d.edens.copy(), ecorr=d.ecorr.reshape(d.ermax, d.ermax).copy(), slnuv=d.slnuv.copy())

@@ -27,8 +27,10 @@ def cfg3():
     return synth.config3()
 
 
-@pytest.mark.parametrize("k", [6, 5, 1, 0])
+@pytest.mark.parametrize("k", range(16))
 def test_config3_solute_energies(cfg3, k):
+    """Every one of configuration 3's 16 solutes (1 to 45 sites) against the
+    oracle: 8 insertions of frame 0."""
     w = cfg3.workload(k)
     p = U.workload_params(w)
     ctx = _ctx(w, p)
@@ -40,10 +42,60 @@ def test_config3_solute_energies(cfg3, k):
     u, u0 = ctx.uvengy(8)
     uo, u0o = U.oracle_refs(w, 0, p, nins=8)
     nbad, worst, at = U.energy_mismatch(u, uo)
-    print(f"solute {k} ({w.topo.slt_charge32.size} sites): max |dE| {np.abs(u - uo).max():.3e}, "
-          f"worst excess {worst:.3e}")
+    print(f"ENERGY config3 solute {k} ({w.topo.slt_charge32.size} sites): {U.energy_report(u, uo)}, "
+          f"uvengy0 {U.energy_report(u0, u0o)}, worst excess {worst:.3e}")
     assert nbad == 0, (worst, at, u[at], uo[at])
     assert U.energy_mismatch(u0, u0o)[0] == 0
+
+
+@pytest.fixture(scope="module")
+def cfg2():
+    return synth.config2()
+
+
+def test_config2_energies_and_distributions(cfg2):
+    """Configuration 2 at full size (30 000 waters, 67 A cube, PME 72^3,
+    600 bins, ecorr on): frames 0 and 1, 64 insertions each, batched.
+    Energies and uvengy0 at the bar against the oracle; edens, ecorr,
+    engnorm, avslf, voffset and minuv/maxuv exactly equal to the oracle's
+    binning of the GPU's own energies; every bin that differs from the
+    binning of the FP64 energies has an edge within the energy tolerance."""
+    w = cfg2
+    p = U.workload_params(w)
+    ctx = _ctx(w, p)
+    nins = 64
+    d_gpu = U.oracle_distribution(w, p)
+    uvmax, _, uvcrd = w.uvmax_uvsoft_uvcrd()
+    for f in (0, 1):
+        ctx.set_frame(w.frame(f), w.cell)
+        ctx.set_insertions(w.insertions(f)[:nins])
+        ctx.run(capi.ER_BATCHED)
+        ctx.sync()
+        u, u0 = ctx.uvengy(nins)
+        uo, u0o = U.oracle_refs(w, f, p, nins=nins)
+        nbad, worst, at = U.energy_mismatch(u, uo)
+        fl = U.flip_report(u, uo, w.topo.uvspec, uvmax, uvcrd)
+        print(f"ENERGY config2 frame {f}: {U.energy_report(u, uo)}, uvengy0 {U.energy_report(u0, u0o)}, "
+              f"worst excess {worst:.3e}; flips {fl}")
+        assert nbad == 0, (worst, at, u[at], uo[at])
+        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert fl["unexplained"] == 0
+        for n in range(nins):
+            d_gpu.update(w.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
+    g = ctx.fetch()
+    assert g["edens"].size == 600 and g["engsmpl"] == 2 * nins
+    np.testing.assert_allclose(g["edens"] / g["engnorm"], d_gpu.edens / d_gpu.d.engnorm, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["ecorr"] / g["engnorm"], d_gpu.ecorr.reshape(600, 600) / d_gpu.d.engnorm,
+                               rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["engnorm"], d_gpu.d.engnorm, rtol=1e-7)
+    np.testing.assert_allclose(g["avslf"], d_gpu.d.avslf, rtol=1e-7)
+    assert g["voffset"] == d_gpu.d.voffset
+    np.testing.assert_array_equal(g["minuv"].astype(np.float64), d_gpu.minuv)
+    np.testing.assert_array_equal(g["maxuv"].astype(np.float64), d_gpu.maxuv)
+    # invariants: symmetric ecorr, rows sum to N * edens, edens sums to N
+    np.testing.assert_array_equal(g["ecorr"], g["ecorr"].T)
+    np.testing.assert_allclose(g["ecorr"].sum(1), w.topo.nmol * g["edens"], rtol=1e-12)
+    np.testing.assert_allclose(g["edens"].sum() / g["engnorm"], w.topo.nmol, rtol=1e-12)
 
 
 def test_config3_two_species_distribution(cfg3):

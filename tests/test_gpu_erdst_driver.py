@@ -23,10 +23,11 @@ def fmt(x):
     return repr(float(np.float32(x)))
 
 
-def write_run_directory(d, w):
+def write_run_directory(d, w, mesh_shift=0.0):
     topo = w.topo
     water, solute = topo.species[0], topo.species[topo.solute_index]
     mesh, pecore = w.mesh[0]
+    mesh = np.asarray(mesh, np.float64) + mesh_shift
     (d / "parameters_er").write_text(f"""&ene_param
   slttype = 3, insposition = 1, insorient = 1, maxins = {NINS},
   ljformat = 0, inptemp = 300.0,
@@ -155,12 +156,15 @@ def test_erdst_driver_soln_run_directory(tmp_path):
         assert (run_dir / n).read_bytes() == (ref_dir / n).read_bytes(), n
 
 
-def test_erdst_driver_soln_sphere_region(tmp_path):
+@pytest.mark.parametrize("permuted", [False, True])
+def test_erdst_driver_soln_sphere_region(tmp_path, permuted):
     """Soln with a sphere region around the water aggregate (insposition 2,
     hostspec = species 2): the bounds accept the solute in one of two frames
     only (distances from the mass-weighted centres); the outputs equal the
     binding's run over the accepted frame, and flcuv.tt has that frame's
-    line alone."""
+    line alone.  Permuted: HISTORY holds the atoms in reverse order and a
+    PermIndex file maps them back (setconf.F90:877-919, 971-975), so the
+    region test must read the permuted coordinates."""
     from erdst_gpu import inputs as I
     w = synth.config1()
     topo = w.topo
@@ -200,7 +204,12 @@ def test_erdst_driver_soln_sphere_region(tmp_path):
             fh.write(f"{i + 1} {m} {typ} {typ}{i + 1} {fmt(water.charge[i])} {fmt(water.eps[i])} "
                      f"{fmt(water.sigma[i])}\n")
     L = np.diag(w.cell).astype(float)
-    write_dcd(run_dir / "HISTORY", [w.frame(f) for f in (0, 1)], [(L[0], 90.0, L[1], 90.0, 90.0, L[2])] * 2)
+    frames = [w.frame(f) for f in (0, 1)]
+    if permuted:
+        n = topo.natom
+        (run_dir / "PermIndex").write_text("".join(f"{i + 1} {n - i}\n" for i in range(n)))
+        frames = [x[::-1].copy() for x in frames]
+    write_dcd(run_dir / "HISTORY", frames, [(L[0], 90.0, L[1], 90.0, 90.0, L[2])] * 2)
     erdst.run(str(run_dir), log=lambda *a: None)
     ref_dir = tmp_path / "ref"
     ref_dir.mkdir()
@@ -210,6 +219,8 @@ def test_erdst_driver_soln_sphere_region(tmp_path):
     ctx.set_mesh(uvmax, uvsoft, uvcrd)
     ctx.set_solute_molecules(topo.tagslt)
     xyz, cell = list(capi.Trajectory(str(run_dir / "HISTORY")))[keep]
+    if permuted:
+        xyz = xyz[::-1].copy()
     ctx.set_frame(xyz, cell)
     ctx.run(capi.ER_BATCHED)
     wr = BlockWriter(str(ref_dir), uvmax, uvsoft, uvcrd, p["temp"], soln=True, engdiv=1, wgtslf=False,
@@ -219,3 +230,18 @@ def test_erdst_driver_soln_sphere_region(tmp_path):
         assert (run_dir / n).read_bytes() == (ref_dir / n).read_bytes(), n
     fl = open(run_dir / "flcuv.tt").read().splitlines()
     assert len(fl) == 1 and int(fl[0][:9]) == keep + 1
+
+
+def test_erdst_driver_stops_on_a_halt(tmp_path, capsys):
+    """A mesh whose lowest point lies above the lowest pair energy: erdst
+    halts with eng_min in the first frame (engproc.F90:1142-1147) and writes
+    no block file; the command exits with status 1 and names the code."""
+    w = synth.config0()
+    run_dir = tmp_path / "run"
+    run_dir.mkdir()
+    write_run_directory(run_dir, w, mesh_shift=19.95)   # ecdmin -0.05 kcal/mol
+    rc = erdst.main([str(run_dir)])
+    err = capsys.readouterr().err
+    assert rc == 1 and "eng_min" in err, err
+    names = os.listdir(run_dir)
+    assert not any(n.startswith(("engref", "corref", "slfeng", "weight")) for n in names), names

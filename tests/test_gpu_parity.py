@@ -401,3 +401,89 @@ def test_halt_reports_the_first_offending_energy_across_runs(cfg0):
     print("first offending (frame, insertion, molecule, energy):", first, "reported", code, species, energy)
     assert code == "eng_min" and species == 1
     assert np.float32(energy) == np.float32(first[3])
+
+
+def test_eng_sft_halt_in_soln():
+    """Solution run with a soft region ending below the frame's highest
+    solute-solvent energy: getiduv halts with eng_sft at the first molecule
+    (in molecule order) beyond it (engproc.F90:1150-1156)."""
+    w = synth.config1()
+    p = U.workload_params(w, corrcal=1)
+    uvmax, uvsoft, uvcrd = w.uvmax_uvsoft_uvcrd()
+    ref = _ctx(w, p)
+    ref.set_frame(w.frame(0), w.cell)
+    ref.run(capi.ER_BATCHED)
+    ref.sync()
+    u, _ = ref.uvengy(1)
+    row = u[0].astype(np.float64)
+    sp = U.species_of(w.topo, 1, w.topo.tagslt)[0]
+    live = np.nonzero(sp > 0)[0]
+    top = np.sort(row[live])[-3]                 # three energies lie at or above it
+    soft = int(np.searchsorted(uvcrd, top, side="left"))   # mesh points below `top`
+    assert 0 < soft < uvmax[0]
+    beyond = [j for j in live if np.searchsorted(uvcrd, row[j], side="left") > soft]
+    assert beyond
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(uvmax, np.array([soft], np.int32), uvcrd)
+    ctx.set_solute_molecules(w.topo.tagslt)
+    ctx.set_frame(w.frame(0), w.cell)
+    ctx.run(capi.ER_BATCHED)
+    with pytest.raises(capi.ErError) as ei:
+        ctx.sync()
+    assert ei.value.status == capi.ER_E_HALT
+    code, species, energy = ctx.last_error()
+    print("eng_sft: first molecule beyond the soft region", beyond[0], row[beyond[0]], "reported", code, energy)
+    assert code == "eng_sft" and species == 1
+    assert np.float32(energy) == np.float32(row[beyond[0]])
+
+
+def _top_bin_mesh(w):
+    """The linear part of the mesh only (no log-spaced core): energies above
+    its last point fall in the open top bin."""
+    mesh, pecore = w.mesh[0]
+    lin = np.asarray(mesh[:len(mesh) - pecore], np.float64)
+    return np.array([lin.size], np.int32), np.array([lin.size], np.int32), lin
+
+
+def test_mbin_warning_stops_the_run_unless_forced(cfg0):
+    """An energy below 1e3 kcal/mol in the open top bin raises the 'mbin'
+    warning (engproc.F90:1158-1167), on which the reference stops unless
+    force_calculation is set (mpiproc.F90:238-252).  Without it er_sync
+    reports the first such (insertion, molecule) with its energy; with it
+    the run goes on and its distributions equal the oracle's binning of the
+    GPU's energies on the same mesh."""
+    nins = 40
+    uvmax, uvsoft, lin = _top_bin_mesh(cfg0)
+    ref = _ctx(cfg0, U.workload_params(cfg0))
+    u, u0 = _run_refs(ref, cfg0, 0, capi.ER_BATCHED, nins=nins)
+    u64 = u.astype(np.float64)
+    hit = np.argwhere((u64 > lin[-1]) & (u64 < 1.0e3))
+    assert hit.size, "no energy in the top bin"
+    n, j = hit[0]
+    for force in (0, 1):
+        p = U.workload_params(cfg0) | {"force_calculation": force}
+        ctx = capi.Context(cfg0.topo, p)
+        ctx.set_mesh(uvmax, uvsoft, lin)
+        ctx.set_solute(cfg0.topo.slt_ljtype, cfg0.topo.slt_charge32)
+        ctx.set_frame(cfg0.frame(0), cfg0.cell)
+        ctx.set_insertions(cfg0.insertions(0)[:nins])
+        ctx.run(capi.ER_BATCHED)
+        if not force:
+            with pytest.raises(capi.ErError) as ei:
+                ctx.sync()
+            assert ei.value.status == capi.ER_E_HALT
+            code, species, energy = ctx.last_error()
+            print("mbin: first (insertion, molecule)", (int(n), int(j)), u[n, j], "reported", code, energy)
+            assert code == "mbin" and species == 1
+            assert np.float32(energy) == u[n, j]
+            continue
+        ctx.sync()
+        g = ctx.fetch()
+        import oracle as O
+        _, vol = O.inverted_cell(cfg0.cell.astype(np.float64))
+        qtot = float(cfg0.topo.charge32.astype(np.float64).sum())
+        d = O.Distribution(p["slttype"], uvmax, uvsoft, lin, p["temp"], vol, p["screen"], qtot)
+        for k in range(nins):
+            d.update(cfg0.topo.uvspec, u64[k], float(u0[k]))
+        assert d.d.warn_mbin == 1
+        _assert_same_distribution(g, d)

@@ -155,3 +155,19 @@ def test_multi_solute_topologies_share_one_lj_table():
         sp = w3.solutes[k]
         sig = np.sqrt(t.ljlensq32.reshape(t.ntype, t.ntype)[t.slt_ljtype, t.slt_ljtype])
         np.testing.assert_allclose(sig, sp.sigma.astype(np.float32), rtol=1e-6)
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors of er_topology and er_params have the C header's
+    size and field offsets (force_calculation included)."""
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "er_capi.h"\n'
+                   "int main(void) { printf(\"%zu %zu %zu %zu %zu\\n\", sizeof(er_topology), sizeof(er_params),"
+                   " offsetof(er_params, temp), offsetof(er_params, force_calculation),"
+                   " offsetof(er_topology, ljlensq_mat)); return 0; }\n")
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    want = [ctypes.sizeof(capi.ErTopology), ctypes.sizeof(capi.ErParams), capi.ErParams.temp.offset,
+            capi.ErParams.force_calculation.offset, capi.ErTopology.ljlensq_mat.offset]
+    assert got == want

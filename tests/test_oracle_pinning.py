@@ -340,6 +340,100 @@ def test_normalised_distribution_is_voffset_invariant():
     np.testing.assert_allclose(out[0][1], out[1][1], rtol=1e-12)
 
 
+def _weighted_log_mean_exp(x, w):
+    """ln( sum w e^x / sum w ), shifted by max(x) (a different route from the
+    oracle's voffset shift)."""
+    m = float(np.max(x))
+    return m + float(np.log(np.sum(w * np.exp(x - m)) / np.sum(w)))
+
+
+@pytest.mark.parametrize("slttype,estype,qtot", [(2, 1, 0.0), (3, 1, 0.0), (3, 2, 7.3), (1, 1, 0.0), (1, 2, -4.1)])
+def test_insertion_weight_gives_the_self_free_energy(slttype, estype, qtot):
+    """The weight update_histogram gives each sample (engproc.F90:883-903,
+    with volcorrect :1022-1038) and engstore's normalisation (:609-628) must
+    yield the solute's self free energy that statistical mechanics fixes:
+    refs runs  avediv2 = voffset + kT ln(avslf/engnorm) = -kT ln<exp(-u0/kT)>_w,
+    soln runs  avediv2 = voffset - kT ln(avslf/engnorm) = +kT ln<exp(+u0/kT)>_w,
+    where u0 is the solute self energy and w = w_sys w_slt the statistical
+    weights, whatever sample sets voffset.  A flipped exponent sign, a
+    missing voffset shift or weights applied on one side only fail it; the
+    volume and net-charge factor of volcorrect (NPT) multiplies every sample
+    alike and must drop out.  The written weight file (the product's writer)
+    carries the same value in its third column."""
+    import tempfile
+    from erdst_gpu.engstore import BlockWriter
+    rng = np.random.default_rng(11 + slttype + 7 * estype)
+    kT = 0.5961612775922495
+    mesh = np.linspace(-5, 5, 41)
+    d = O.Distribution(slttype, [41], [41], mesh, temp=kT, volume=2.7e4, screen=0.35, total_charge=qtot,
+                       wgtslf=1, estype=estype)
+    nmol, nsample = 6, 400
+    u0 = rng.uniform(-2.5, 2.5, nsample) + rng.choice([0.0, 1.5], nsample)   # spread over several kT
+    w_sys = rng.uniform(0.5, 2.0, nsample)
+    w_slt = rng.uniform(0.2, 3.0, nsample)
+    for n in range(nsample):
+        d.update(np.ones(nmol, np.int32), rng.uniform(-4.9, 4.9, nmol), u0[n], w_sys[n], w_slt[n])
+    sc = d.scalars
+    w = w_sys * w_slt
+    sign = +1.0 if slttype == 1 else -1.0
+    exact = sign * kT * _weighted_log_mean_exp(sign * u0 / kT, w)
+    got = sc["voffset"] - sign * kT * np.log(sc["avslf"] / sc["engnorm"])
+    assert abs(got - exact) <= 1e-12 * max(1.0, abs(exact)), (got, exact)
+    assert sc["voffset"] == u0[0]
+    # the same through the product's writer (weight_refs / weight_soln, e20.7)
+    with tempfile.TemporaryDirectory() as td:
+        wr = BlockWriter(td, np.array([41]), np.array([41]), mesh, kT, soln=slttype == 1, engdiv=1)
+        wr.store(dict(edens=d.edens, ecorr=d.ecorr.reshape(41, 41), eself=np.zeros(0), slnuv=d.slnuv,
+                      minuv=None, maxuv=None, **sc))
+        name = "weight_soln" if slttype == 1 else "weight_refs"
+        col3 = float(open(f"{td}/{name}").read().split()[2])
+    assert abs(col3 - exact) <= 1e-6 * max(1.0, abs(exact))
+
+
+def madelung_cubic_point_charge(Q, L):
+    """Ewald energy of one point charge with its periodic images and a
+    uniform neutralising background in a cubic box: Q^2 xi / (2 L) with the
+    simple-cubic Ewald Madelung constant xi = -2.837297479 (Nijboer and De
+    Wette 1957)."""
+    return Q * Q * -2.837297479 / (2.0 * L)
+
+
+def test_volcorrect_factor_is_the_background_energy_of_the_net_charge():
+    """volcorrect (engproc.F90:1022-1038) multiplies each sample's weight by
+    V exp(pi Q^2 / (2 alpha^2 V) / kT) = V exp(-E_bg / kT), E_bg being the
+    energy of the net charge Q with the uniform neutralising background.
+    E_bg is pinned here: with it, the Ewald energy of a lone charge is the
+    alpha-independent Madelung energy; without it, not.  The oracle's weight
+    of a single sample then equals V exp(-E_bg/kT), and at constant volume
+    the factor cancels from every normalised output."""
+    L, Q, kT = 25.0, 3.0 * CH, 0.6
+    V = L ** 3
+    x = np.zeros((1, 3))
+    for alpha in (0.3, 0.35, 0.45):
+        full = ewald_self(np.array([Q]), x, L, alpha)
+        assert abs(full - madelung_cubic_point_charge(Q, L)) <= 1e-9 * abs(full)
+        e_bg = -np.pi * Q * Q / (2 * alpha ** 2 * V)
+        assert abs((full - e_bg) - madelung_cubic_point_charge(Q, L)) > 1e-3 * abs(full)   # needed
+        mesh = np.linspace(-5, 5, 41)
+        d = O.Distribution(3, [41], [41], mesh, temp=kT, volume=V, screen=alpha, total_charge=Q, wgtslf=1,
+                           estype=2)
+        d.update(np.ones(4, np.int32), np.array([0.1, 0.2, -1.0, 3.0]), 0.0)
+        np.testing.assert_allclose(d.d.engnorm, V * np.exp(-e_bg / kT), rtol=1e-13)
+    # constant volume: a net charge rescales engnorm and avslf alike
+    rng = np.random.default_rng(5)
+    rows = [rng.uniform(-4.9, 4.9, 8) for _ in range(30)]
+    u0 = rng.normal(0, 0.4, 30)
+    out = []
+    for q in (0.0, 2.0 * CH):
+        d = O.Distribution(3, [41], [41], np.linspace(-5, 5, 41), temp=kT, volume=V, screen=0.35,
+                           total_charge=q, wgtslf=1, estype=2)
+        for n in range(30):
+            d.update(np.ones(8, np.int32), rows[n], u0[n])
+        out.append((d.edens / d.d.engnorm, d.ecorr / d.d.engnorm, d.d.avslf / d.d.engnorm))
+    for a, b in zip(*out):
+        np.testing.assert_allclose(a, b, rtol=1e-12)
+
+
 def test_energy_below_mesh_halts_eng_min():
     rng = np.random.default_rng(1)
     d, mesh = _dist(rng)
