@@ -99,11 +99,11 @@ class FeVars:
         if fv.numprm <= 0:
             fv.numprm = 11 if fv.infchk == "yes" else 5
         if fv.pickgr < fv.msemin:
-            raise SlvfeError("Incorrect setting: pickgr < msemin not allowed")
+            raise SlvfeError("pickgr must not be below msemin")
         if fv.pickgr > fv.msemax:
-            raise SlvfeError("Incorrect setting: pickgr > msemax not allowed")
+            raise SlvfeError("pickgr must not exceed msemax")
         if fv.pickgr > fv.numprm:
-            raise SlvfeError("Incorrect setting: pickgr > numprm not allowed")
+            raise SlvfeError("pickgr must not exceed numprm")
         if fv.et != 0.0:
             fv.functional = "thnc"
         if fv.ljlrc == "yes":
@@ -198,7 +198,7 @@ class Distributions:
             if previous is not None:
                 bad |= _nint(factor) != _nint(previous[pti - 1]) or _nint(ampl) != _nint(previous[pti - 1])
             if bad:
-                raise SlvfeError(f"Incorrect normalization at {pti}")
+                raise SlvfeError(f"species {pti}: the normalised distribution does not sum to its molecule count")
             nm[pti - 1] = float(_nint(factor))
         return nm if previous is None else previous
 
@@ -248,7 +248,7 @@ def regroup(d: Distributions, rduvmax, rduvcore, group: int, inft: int, slncor: 
             idrduv[iduv] = kk + j - 1
         gemax += m
     if int(np.sum(rduvmax)) != ermax:
-        raise SlvfeError("The total no. of meshes does not match with input")
+        raise SlvfeError("the bin count read from the distribution files differs from the one given")
     uvcrd = np.zeros(gemax)
     gpnum = np.zeros(gemax, np.int64)
     uvspec = np.zeros(gemax, np.int64)
@@ -313,8 +313,8 @@ def distnorm(g: Grouped, nummol: np.ndarray, fv: FeVars, slncor: bool) -> None:
                 errtmp = float(np.max(np.abs(correc[pos] - 1.0))) if pos.any() else 0.0
                 itrcnt += 1
                 if itrcnt >= fv.itrmax:
-                    raise SlvfeError(f"The optimization of the correlation matrix did not converge "
-                                     f"with an error of {errtmp}")
+                    raise SlvfeError(f"distnorm stopped after {itrcnt} sweeps with the marginals still "
+                                     f"off by {errtmp}")
         if cnt == 1:
             g.edist = edhst
             if slncor:
@@ -350,12 +350,12 @@ class _State:
             if self.fv.wgtf2smpl == "yes":
                 jdg = 2
         if jdg not in (1, 2):
-            raise SlvfeError("Bug in the program (wgtdst)")
+            raise SlvfeError("internal error: weight function asked for an unknown system or weight type")
         if jdg == 2:
             return fref
         if wgttype == "smpl":
             if cnt == 2:
-                raise SlvfeError("Bug in the program (wgtdst)")
+                raise SlvfeError("internal error: weight function asked for an unknown system or weight type")
             return fsln
         if wgttype == "geom":
             f = fsln * fref
@@ -377,7 +377,7 @@ class _State:
             weight[sp] = np.exp(-ampl / self.kT) * weight[sp]
         ampl = float(np.sum(weight[sp]))
         if not ampl > 0.0:
-            raise SlvfeError(f"Zero weight at {pti}")
+            raise SlvfeError(f"species {pti}: every bin has zero weight")
         return weight / ampl
 
     # cvfcen, slvfe/slvfe.F90:1243-1279
@@ -386,14 +386,14 @@ class _State:
         sp = self.g.uvspec == pti
         if systype == "slncv":
             if cnt == 2:
-                raise SlvfeError("Bug in the program (cvfcen)")
+                raise SlvfeError("internal error: central-value average asked for an unknown kind")
             cv = self.slncv
         elif systype == "inscv":
             cv = inscv_now if inscv_now is not None else (self.sdrcv if cnt == 1 else self.inscv)
         elif systype == "uvcrd":
             cv = self.g.uvcrd
         else:
-            raise SlvfeError("Bug in the program (cvfcen)")
+            raise SlvfeError("internal error: central-value average asked for an unknown kind")
         return float(np.sum(cv[sp] * weight[sp]))
 
     # zeroec, slvfe/slvfe.F90:1318-1355: bin nearest to zero energy
@@ -414,7 +414,7 @@ class _State:
                 elif abs(ampl - factor) <= tiny:
                     k = iduv if dist[iduv] >= dist[iduv + 1] else iduv + 1
                 else:
-                    raise SlvfeError("Bug in the program (zeroec)")
+                    raise SlvfeError("internal error: zero-energy bin search asked for an unknown system")
         if k is None:
             raise SlvfeError(f"no energy-coordinate bin brackets zero for species {pti}")
         return k
@@ -472,11 +472,11 @@ def getslncv(st: _State, d: Distributions) -> None:
             elif ext_target[k]:
                 j = k
             else:
-                raise SlvfeError(f"Extrapolation is not possible at {g.uvcrd[iduv]}")
+                raise SlvfeError(f"no sampled bin to extrapolate from at energy {g.uvcrd[iduv]}")
             work = np.zeros(n)
             for iduvp in np.nonzero(same)[0]:
                 if iduvp == iduv:
-                    raise SlvfeError("A bug in program or data")
+                    raise SlvfeError("extrapolation found no usable partner bin (inconsistent data)")
                 factor = g.uvcrd[iduvp] - g.uvcrd[j]
                 if iduvp < iduv:
                     factor = -factor - 2.0 * (g.uvcrd[j] - g.uvcrd[iduv])
@@ -509,7 +509,7 @@ def getslncv(st: _State, d: Distributions) -> None:
         elif zs == "cntr":
             cvzero = st.cvfcen(pti, 1, "slncv", fv.wgtfnform, "not")
         else:
-            raise SlvfeError("zerosft not properly set")
+            raise SlvfeError("unknown zerosft choice")
         st.slncv[g.uvspec == pti] -= cvzero
         st.zrsln[pti - 1] = cvzero
 
@@ -571,8 +571,8 @@ def getinscv(st: _State) -> dict:
                     A[np.ix_(sel, sel)] += regfac
             x, info = posv(A, work)
             if info != 0:
-                print(" Cholesky-based linear solver failed to converge. "
-                      "Falling back to slower EVD-based solver.", file=sys.stderr)
+                print(" the regularised correlation matrix is not positive definite; "
+                      "solving by eigendecomposition instead", file=sys.stderr)
                 mode = "evd"
                 x = None
         if mode == "evd":
@@ -610,7 +610,7 @@ def getinscv(st: _State) -> dict:
             elif zs == "cntr":
                 cvzero = st.cvfcen(pti, cnt, "inscv", fv.wgtfnform, "not")
             else:
-                raise SlvfeError("zerosft not properly set")
+                raise SlvfeError("unknown zerosft choice")
             sp = g.uvspec == pti
             if cnt == 1:
                 st.sdrcv[sp] -= cvzero
@@ -645,7 +645,7 @@ def pyhnc(indpmf: float, cnt: int, kT: float) -> float:
         if factor > 0.0:
             return 1.0 - math.log(1.0 + factor) * (1.0 / factor + 1.0)
         return -factor / 2.0
-    raise SlvfeError("Incorrct cnt argument in pyhnc")
+    raise SlvfeError("py-hnc switch called with a system index other than 1 or 2")
 
 
 def thnc(indpmf: float, et: float, cnt: int, kT: float) -> float:
@@ -655,7 +655,7 @@ def thnc(indpmf: float, et: float, cnt: int, kT: float) -> float:
         return factor / 2.0
     if cnt == 2:
         return (0.0 if indpmf < et else factor) / 2.0
-    raise SlvfeError("Incorrct cnt argument in thnc")
+    raise SlvfeError("hnc switch called with a system index other than 1 or 2")
 
 
 @dataclass
@@ -702,7 +702,7 @@ def chmpot(d: Distributions, fv: FeVars, group: int, inft: int = 0, *, kT: float
     refs_zero = fv.minthres_refs * float(np.min(d.rddns[d.rddns > 0.0]))
     func = fv.functional.lower()
     if func not in ("py-hnc", "hnc", "none", "thnc"):
-        raise SlvfeError("Incorrct functional")
+        raise SlvfeError("unknown functional")
     chmpt = np.zeros(numslv + 1)
     cumsfe = np.zeros(g.gemax)
     for pti in range(1, numslv + 1):
@@ -807,7 +807,7 @@ def _count_files(direc: str, trunk: str):
     if os.path.exists(os.path.join(direc, f"{trunk}.tt")):
         for i in range(1, 100):
             if os.path.exists(os.path.join(direc, f"{trunk}.{i:02d}")):
-                raise SlvfeError(f"{trunk}.tt is not supposed to coexist with {trunk}.01, ...")
+                raise SlvfeError(f"{trunk}.tt and numbered {trunk}.NN files are both present; keep one kind")
         return 1, True
     n = 0
     for i in range(1, 100):
@@ -815,7 +815,7 @@ def _count_files(direc: str, trunk: str):
             n = i
         else:
             if i == 1:
-                raise SlvfeError(f"Neither {trunk}.01 nor {trunk}.tt exists in {direc}")
+                raise SlvfeError(f"{direc} holds no {trunk}.tt and no {trunk}.01")
             break
     return n, False
 
@@ -884,7 +884,7 @@ def run_merge(fv: FeVars, base: str = ".", log=None) -> MergeResult:
         numdiv = i
     if fv.refmerge == "not":
         if numdiv > numref:
-            raise SlvfeError("With refmerge = 'not', numdiv needs to be not larger than numref")
+            raise SlvfeError("refmerge='not' pairs solution blocks with reference blocks one to one, so numdiv may not exceed numref")
         numref = numdiv * (numref // numdiv)
     maxsln, maxref, numrun, prmmax = numsln, numref, numdiv, fv.numprm
     groups = group_table(prmmax, fv.infchk)
@@ -893,7 +893,7 @@ def run_merge(fv: FeVars, base: str = ".", log=None) -> MergeResult:
     _, rdcrd, rdspec, _ = read_engfile(path1)
     ermax = len(rdcrd)
     if ermax > fv.ermax_limit:
-        raise SlvfeError("The number of energy bins is too large")
+        raise SlvfeError("ermax is above the supported bin count")
     rduvmax, rduvcore = mesh_layout(rdcrd, rdspec, fv.tiny)
     numslv = len(rduvmax)
     kT = kt_of(fv.inptemp)
@@ -926,10 +926,10 @@ def run_merge(fv: FeVars, base: str = ".", log=None) -> MergeResult:
     slfeng = 0.0
     if slfslt == "yes":
         if fv.readwgtfl == "not":
-            raise SlvfeError("readwgtfl needs to be yes when slfslt is yes")
+            raise SlvfeError("slfslt='yes' needs readwgtfl='yes' (the self energy comes from the weight file)")
         p = os.path.join(refs, fv.wgtreffl)
         if not os.path.exists(p):
-            raise SlvfeError("weight_refs is absent although slfslt is set to yes")
+            raise SlvfeError("slfslt='yes' but the weight_refs file is missing")
         with open(p) as fh:
             for i in range(maxref):
                 parts = fh.readline().split()
@@ -982,7 +982,7 @@ def run_merge(fv: FeVars, base: str = ".", log=None) -> MergeResult:
             for i in range(refini, reffin + 1):
                 lb, crd, spec, ratio = read_engfile(os.path.join(refs, f"{fv.refdnspf}.{_suffix(i, ref_tt)}"))
                 if not np.array_equal(lb, leftsln):
-                    raise SlvfeError("Solution and reference system energy coordinates are inconsitent")
+                    raise SlvfeError("the solution and reference meshes differ")
                 rddns += wgtref[i - 1] * ratio
                 rdspec = _species_run_index(spec)
                 rdcrd = crd
