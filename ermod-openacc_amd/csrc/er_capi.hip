@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -63,6 +64,22 @@ template <typename T> struct DBuf {
     }
 };
 
+}  // namespace
+
+// Scratch of er_run_noalloc: the split-K partial sums of edens/ecorr and the
+// weight kernel's block partials, in device memory that the library never
+// allocates at run time.  Each GPU holds its own copy for the life of the
+// process.  In host code `scratch` names a host-side stand-in, not a device
+// address: kernels receive the per-device address that hipGetSymbolAddress
+// returns at er_create.  Every er_run_noalloc on one GPU shares the array,
+// so such calls must not overlap on that GPU (one stream at a time).
+#define ER_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+static __device__ __align__(256) double scratch[ER_NOALLOC_SCRATCH_BYTES / sizeof(double)];
+
+namespace {
+constexpr int ER_MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[ER_MAX_DEVICES] = {};
 }  // namespace
 
 // One solute analysed in the context: its sites, its insertions of the
@@ -195,6 +212,10 @@ struct er_ctx {
 
     // accumulators
     DBuf<double> d_part, d_epart;  // split-K partial sums of ecorr / edens
+    // er_run_noalloc: this device's address of the file-scope scratch array,
+    // and whether the run in progress may allocate
+    double *scratch_dev = nullptr;
+    bool noalloc = false;
 
     // diagnostics: event pairs recorded around launches while profiling
     bool prof = false;
@@ -527,6 +548,17 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
         int dev = 0;
         HIPCHK(c, hipGetDevice(&dev));
         HIPCHK(c, hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        // this device's address of the no-allocation scratch, once per device
+        if (dev < 0 || dev >= ER_MAX_DEVICES) return bad(fail(c, ER_E_UNSUPPORTED, "er_create: device %d", dev));
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (!g_scratch_ptr[dev]) {
+            void *p = nullptr;
+            const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p)
+                return bad(fail(c, ER_E_HIP, "er_create: hipGetSymbolAddress(scratch): %s", hipGetErrorString(e)));
+            g_scratch_ptr[dev] = static_cast<double *>(p);
+        }
+        c->scratch_dev = g_scratch_ptr[dev];
     }
     hipStream_t s = c->stream;
     HIPCHK(c, c->d_charge.ensure(c->numatm));
@@ -1278,44 +1310,68 @@ static hipError_t prep_frame(er_ctx *c)
     return hipGetLastError();
 }
 
+// a run buffer of at least `count` elements: grown as needed, except in an
+// er_run_noalloc call, where a buffer too small is an error (nothing is
+// allocated)
+template <class T> static int want(er_ctx *c, DBuf<T> &b, size_t count, const char *name)
+{
+    if (!c->noalloc) {
+        HIPCHK(c, b.ensure(count));
+        return ER_OK;
+    }
+    if (b.p && count <= b.n) return ER_OK;
+    return fail(c, ER_E_STATE,
+                "er_run_noalloc: buffer %s holds %zu elements, this run needs %zu; an er_run of at least as many "
+                "insertions (same launch shapes) must size it first",
+                name, b.n, count);
+}
+
 static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
 {
+    int rc;
+#define ER_WANT(buf, count)                                     \
+    do {                                                        \
+        if ((rc = want(c, buf, (size_t)(count), #buf)) != ER_OK) \
+            return rc;                                          \
+    } while (0)
     const size_t nslt = (size_t)c->S().nslt;
-    HIPCHK(c, c->d_insposn.ensure(nins * nslt));
-    HIPCHK(c, c->d_insw.ensure(nins * nslt * 3 * c->ord));
-    HIPCHK(c, c->d_insg.ensure(nins * nslt));
-    HIPCHK(c, c->d_insref.ensure(nins));
-    HIPCHK(c, c->d_selfreal.ensure(nins));
-    HIPCHK(c, c->d_resself.ensure(nins));
-    HIPCHK(c, c->d_selfpart.ensure((size_t)nins * c->H));
-    HIPCHK(c, c->d_Ftab.ensure((size_t)nins * nslt * (c->H + c->K2 + c->K3)));
-    HIPCHK(c, c->d_T.ensure((size_t)chunk * c->Gc));
+    ER_WANT(c->d_insposn, nins * nslt);
+    ER_WANT(c->d_insw, nins * nslt * 3 * c->ord);
+    ER_WANT(c->d_insg, nins * nslt);
+    ER_WANT(c->d_insref, nins);
+    ER_WANT(c->d_selfreal, nins);
+    ER_WANT(c->d_resself, nins);
+    ER_WANT(c->d_selfpart, (size_t)nins * c->H);
+    ER_WANT(c->d_Ftab, (size_t)nins * nslt * (c->H + c->K2 + c->K3));
+    ER_WANT(c->d_T, (size_t)chunk * c->Gc);
     const size_t ntile = (size_t)(chunk + NI - 1) / NI;
-    HIPCHK(c, c->d_phi.ensure(ntile * NI * c->G));
+    ER_WANT(c->d_phi, ntile * NI * c->G);
     const size_t ntile_all = (size_t)(nins + NI - 1) / NI;
-    HIPCHK(c, c->S().d_U.ensure(ntile_all * NI * c->nummol));
-    HIPCHK(c, c->S().d_u0.ensure(nins));
-    HIPCHK(c, c->d_C.ensure((size_t)nins * c->ermax));
-    HIPCHK(c, c->d_w.ensure(nins));
-    HIPCHK(c, c->d_wpart.ensure((size_t)4 * std::max(1, (nins + 255) / 256)));
-    HIPCHK(c, c->d_perm.ensure(c->nummol));
-    HIPCHK(c, c->d_keyrank.ensure(2 * (size_t)c->nummol));
-    HIPCHK(c, c->d_ccnt.ensure(c->ncell));
-    HIPCHK(c, c->d_cstart.ensure(c->ncell + 1));
-    if (c->ord == 4 || c->ord == 6) HIPCHK(c, c->d_brec.ensure((size_t)c->numatm * brick_rec_floats(c->ord)));
-    HIPCHK(c, c->d_akeyrank.ensure(2 * (size_t)c->numatm));
-    HIPCHK(c, c->d_bcnt.ensure(c->nbrick));
-    HIPCHK(c, c->d_bstart.ensure(c->nbrick + 1));
-    if (NI > 1 && c->ord <= 6) HIPCHK(c, c->d_Eat.ensure(ntile * NI * (size_t)c->numatm));
+    ER_WANT(c->S().d_U, ntile_all * NI * c->nummol);
+    ER_WANT(c->S().d_u0, nins);
+    ER_WANT(c->d_C, (size_t)nins * c->ermax);
+    ER_WANT(c->d_w, nins);
+    // the weights' block partials: the scratch array in an er_run_noalloc call
+    if (!c->noalloc) ER_WANT(c->d_wpart, (size_t)4 * std::max(1, (nins + 255) / 256));
+    ER_WANT(c->d_perm, c->nummol);
+    ER_WANT(c->d_keyrank, 2 * (size_t)c->nummol);
+    ER_WANT(c->d_ccnt, c->ncell);
+    ER_WANT(c->d_cstart, c->ncell + 1);
+    if (c->ord == 4 || c->ord == 6) ER_WANT(c->d_brec, (size_t)c->numatm * brick_rec_floats(c->ord));
+    ER_WANT(c->d_akeyrank, 2 * (size_t)c->numatm);
+    ER_WANT(c->d_bcnt, c->nbrick);
+    ER_WANT(c->d_bstart, c->nbrick + 1);
+    if (NI > 1 && c->ord <= 6) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
     // enough histogram workgroups to fill the chip
     const int ntile_chunk = (chunk + NI - 1) / NI;
     c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
     // soln runs keep the built-in slicing: their flceng partial sums follow it
     if (c->hist_slices > 0 && c->prm.slttype != 1) c->nslice = c->hist_slices;
     if (c->prm.slttype == 1) {
-        HIPCHK(c, c->d_flceng.ensure((size_t)nins * c->numslv));
-        HIPCHK(c, c->d_flpart.ensure((size_t)nins * c->nslice * c->numslv));
+        ER_WANT(c->d_flceng, (size_t)nins * c->numslv);
+        ER_WANT(c->d_flpart, (size_t)nins * c->nslice * c->numslv);
     }
+#undef ER_WANT
     return ER_OK;
 }
 
@@ -1420,9 +1476,50 @@ template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, i
 
 // weights, edens and ecorr of insertions [n_off, n_off+nch) (after their
 // histograms); update_histogram's accumulation part
+// insertion slices of the split-K edens/ecorr sums over nch insertions:
+// slices of kper (a multiple of 4, at least ecorr_kmin) insertions
+static void ecorr_slices(const er_ctx *c, int nch, int &kper, int &nks)
+{
+    kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
+    nks = (nch + kper - 1) / kper;
+}
+
+// the split-K partials (ecorr, edens) and the weights' block partials of an
+// accumulation over nch insertions: the context's buffers, or in an
+// er_run_noalloc call consecutive ranges of the scratch array
+struct AccPartials {
+    double *part, *epart, *wpart;
+};
+
+static size_t noalloc_doubles(const er_ctx *c, int nch)
+{
+    int kper, nks;
+    ecorr_slices(c, nch, kper, nks);
+    const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
+    return (mfma ? (size_t)nks * c->ermax * c->ermax : 0) + (size_t)nks * c->ermax +
+           (size_t)4 * std::max(1, (nch + 255) / 256);
+}
+
+static AccPartials acc_partials(er_ctx *c, int nch)
+{
+    if (!c->noalloc) return {c->d_part.p, c->d_epart.p, c->d_wpart.p};
+    int kper, nks;
+    ecorr_slices(c, nch, kper, nks);
+    const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
+    double *p = c->scratch_dev;
+    AccPartials a{};
+    a.part = p;
+    p += mfma ? (size_t)nks * c->ermax * c->ermax : 0;
+    a.epart = p;
+    p += (size_t)nks * c->ermax;
+    a.wpart = p;
+    return a;
+}
+
 static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
 {
     const bool soln = c->prm.slttype == 1;
+    const AccPartials ap = acc_partials(c, nch);
     {
         ProfScope ps(c, ER_K_WEIGHTS);
         WeightArgs wa{};
@@ -1451,29 +1548,28 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
                            c->d_resself.p, c->S().wslt_given ? c->S().d_wslt.p : (const float *)nullptr,
                            c->S().d_state.p, c->S().d_fstate.p, c->S().d_u0.p, c->d_w.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
-                           c->d_wpart.p);
-        hipLaunchKernelGGL(k_weights_fin, dim3(1), dim3(256), 0, c->stream, wa, nblk, c->d_wpart.p, c->S().d_state.p,
+                           ap.wpart);
+        hipLaunchKernelGGL(k_weights_fin, dim3(1), dim3(256), 0, c->stream, wa, nblk, ap.wpart, c->S().d_state.p,
                            c->S().d_u0.p, c->d_w.p, c->d_escrd.p, c->S().d_eself.p, c->S().d_minmax.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
                            c->S().d_slnuv.p, c->S().d_errkey.p, c->S().d_warnkey.p);
     }
-    // insertion slices of at most kper (a multiple of 4) for the split-K sums
-    const int kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
-    const int nks = (nch + kper - 1) / kper;
+    int kper, nks;
+    ecorr_slices(c, nch, kper, nks);
     const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
     {
         ProfScope ps(c, ER_K_ECORR);
         const int nb = (c->ermax + 63) / 64;
         if (mfma) {
             hipLaunchKernelGGL(k_ecorr, dim3(nb, nb, nks), dim3(256), 0, c->stream, nch, n_off, c->ermax, kper,
-                               c->d_C.p, c->d_w.p, c->d_dmap.p, c->d_part.p, c->d_epart.p);
+                               c->d_C.p, c->d_w.p, c->d_dmap.p, ap.part, ap.epart);
         } else {
             hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256, nks), dim3(256), 0, c->stream, nch, n_off,
-                               c->ermax, kper, c->d_C.p, c->d_w.p, c->d_epart.p);
+                               c->ermax, kper, c->d_C.p, c->d_w.p, ap.epart);
         }
         const bool full = mfma;
         hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, full ? c->ermax : 1), dim3(256), 0,
-                           c->stream, c->ermax, nks, c->d_part.p, c->d_epart.p,
+                           c->stream, c->ermax, nks, ap.part, ap.epart,
                            full ? c->S().d_ecorr.p : (double *)nullptr, c->S().d_edens.p);
         if (c->prm.corrcal == 1 && !c->use_mfma) {
             const int nb16 = (c->ermax + 15) / 16;
@@ -1578,6 +1674,36 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
     const int rc = run_plan(c, "er_run", todo);
     if (rc) return rc;
     return run_slots(c, todo, mode);
+}
+
+// er_run without allocation: every run buffer must already hold this run's
+// shapes, and the partial sums go to the scratch array; no events are
+// recorded (profiling is suspended for the call)
+extern "C" int er_run_noalloc(er_ctx *c, int32_t mode)
+{
+    if (!c) return ER_E_ARG;
+    if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run_noalloc: mode");
+    if (!c->scratch_dev) return fail(c, ER_E_STATE, "er_run_noalloc: no scratch address for this device");
+    std::vector<int> todo;
+    int rc = run_plan(c, "er_run_noalloc", todo);
+    if (rc) return rc;
+    const size_t cap = ER_NOALLOC_SCRATCH_BYTES / sizeof(double);
+    for (int k : todo) {
+        const int nins = (c->prm.slttype == 1) ? c->nsltmol : c->slots[k].nins;
+        const size_t need = noalloc_doubles(c, mode == ER_BATCHED ? nins : 1);
+        if (need > cap)
+            return fail(c, ER_E_UNSUPPORTED,
+                        "er_run_noalloc: %d insertions over %d bins need %zu bytes of partial sums, above the "
+                        "%zu-byte scratch (use er_run)",
+                        nins, c->ermax, need * sizeof(double), (size_t)ER_NOALLOC_SCRATCH_BYTES);
+    }
+    const bool prof = c->prof;
+    c->prof = false;
+    c->noalloc = true;
+    rc = run_slots(c, todo, mode);
+    c->noalloc = false;
+    c->prof = prof;
+    return rc;
 }
 
 // the halt (or, warn = true, the 'mbin' warning) of slot c->S():

@@ -19,7 +19,8 @@ ER_PLAIN, ER_BATCHED = 0, 1
 
 EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_select_solute",
             "er_set_solute_molecules", "er_set_frame",
-            "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_sync",
+            "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_run_noalloc",
+            "er_sync",
             "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
             "er_destroy", "er_profile", "er_profile_read", "er_autotune", "er_set_tuning", "er_get_tuning"]
 EXPORTED_TRAJ = ["er_traj_open", "er_traj_read", "er_traj_info", "er_traj_close"]
@@ -72,6 +73,7 @@ def load() -> C.CDLL:
     L.er_set_insertions.argtypes = [vp, i32, vp, vp]
     L.er_set_insertions_device.argtypes = [vp, i32, vp, vp]
     L.er_run.argtypes = [vp, i32]
+    L.er_run_noalloc.argtypes = [vp, i32]
     L.er_sync.argtypes = [vp]
     L.er_get_uvengy.argtypes = [vp, vp, vp]
     L.er_fetch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
@@ -268,6 +270,11 @@ class Context:
 
     def run(self, mode=ER_BATCHED):
         self._check(self.L.er_run(self.h, int(mode)))
+
+    def run_noalloc(self, mode=ER_BATCHED):
+        """er_run_noalloc: the same run with no device allocation; an earlier
+        er_run of at least as many insertions must have sized the buffers."""
+        self._check(self.L.er_run_noalloc(self.h, int(mode)))
 
     def sync(self):
         self._check(self.L.er_sync(self.h))

@@ -139,6 +139,21 @@ int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sltpos_dev,
  * ER_BATCHED runs every insertion of the frame in one launch sequence. */
 int er_run(er_ctx *c, int32_t mode);
 
+/* No counterpart in the reference: er_run for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It calls no hipMalloc, hipFree or
+ * allocating library routine: every per-run buffer of the context must
+ * already hold this run's shapes (an earlier er_run of at least as many
+ * insertions per solute, same launch shapes, sizes them; otherwise
+ * ER_E_STATE), and the split-K partial sums of edens/ecorr and the weight
+ * partials live in a 64 MiB array of the library's own that each GPU holds
+ * until the process ends.  A run whose partial sums exceed it (about
+ * nslices x ermax^2 x 8 bytes with ecorr on) returns ER_E_UNSUPPORTED.
+ * Results equal er_run's.  All contexts on one GPU share that array, so
+ * er_run_noalloc calls that overlap on one GPU race: issue them from one
+ * stream at a time.  Profiling (er_profile) is suspended for the call. */
+int er_run_noalloc(er_ctx *c, int32_t mode);
+
 /* Waits for the context's stream; returns ER_E_HALT if a run hit a halt
  * condition (getiduv's 'eng_min' / 'eng_sft', engproc.F90:1142-1156) or,
  * unless force_calculation is set, the 'mbin' warning: an energy below
