@@ -197,6 +197,48 @@ def cpu_baseline_sample(job, budget_s=10.0, nthreads=None):
                 sample=f"frame 0, first {n} insertions of solute 1 ({t:.1f} s)")
 
 
+def result_errors(job, capi, frame=0, nins=64):
+    """Per-result errors of the batched FP32 path against the FP64 oracle on
+    one frame's first nins insertions (the first solute): per-(insertion,
+    molecule) energies by regime, uvengy0, histogram flips (bins that differ
+    between the GPU's energies and the oracle's), and edens / ecorr of that
+    block against the oracle's distribution of its own FP64 energies."""
+    import erutil as U
+    w, p = job.w, job.params
+    if job.soln:
+        return None
+    ctx = capi.Context(w.topo, p)
+    ctx.set_mesh(*w.uvmax_uvsoft_uvcrd())
+    ctx.set_solute(w.topo.slt_ljtype, w.topo.slt_charge32)
+    ctx.set_frame(w.frame(frame), w.cell)
+    ctx.set_insertions(w.insertions(frame)[:nins])
+    ctx.run(capi.ER_BATCHED)
+    ctx.sync()
+    u, u0 = ctx.uvengy(nins)
+    g = ctx.fetch()
+    ctx.close()
+    threads = max(1, min(os.cpu_count() or 1, 16))
+    uo, u0o = U.oracle_refs(w, frame, p, nins=nins, nthreads=threads)
+    uvmax, _, uvcrd = w.uvmax_uvsoft_uvcrd()
+    d = U.oracle_distribution(w, p)
+    for n in range(nins):
+        d.update(w.topo.uvspec, uo[n], float(u0o[n]))
+    ermax = int(np.sum(uvmax))
+    ed_g, ed_o = g["edens"] / g["engnorm"], d.edens / d.d.engnorm
+    ec_g = g["ecorr"] / g["engnorm"]
+    ec_o = d.ecorr.reshape(ermax, ermax) / d.d.engnorm
+    nbad, worst, _ = U.energy_mismatch(u, uo)
+    return dict(sample=f"frame {frame}, first {nins} insertions x {w.topo.nmol} molecules",
+                energy=U.energy_report(u, uo), outside_bar=nbad, worst_excess_over_bar=float(f"{worst:.3e}"),
+                bar=f"|dE| <= r|E| + {U.ATOL} kcal/mol, r = {U.RTOL} (|E| < {U.E_MID}), {U.RTOL_MID} "
+                    f"(< {U.E_HIGH}), {U.RTOL_HIGH} above",
+                uvengy0_max_abs=float(np.abs(u0 - u0o).max()),
+                hist=U.flip_report(u, uo, w.topo.uvspec, uvmax, uvcrd),
+                edens_max_abs=float(np.abs(ed_g - ed_o).max()), edens_max=float(np.abs(ed_o).max()),
+                ecorr_max_abs=float(np.abs(ec_g - ec_o).max()), ecorr_max=float(np.abs(ec_o).max()),
+                unit="kcal/mol (energies); molecules per bin (edens, ecorr)")
+
+
 def dmu_error(soln, refs, sg, rg, so, ro, workdir):
     """Solvation free energy (slvfe's merge over engstore's files) from the
     GPU's and the oracle's distributions of the same frames and insertions."""
@@ -343,6 +385,7 @@ def main():
                      "launch_ms": round(avg_s * 1e3, 5), "units_per_launch": round(nch, 2)},
         "cpu_baseline": None,
         "dmu": None,
+        "errors": None,
     }
     ctx.close()
     if not args.no_cpu:
@@ -355,6 +398,7 @@ def main():
         else:
             rec = cpu_baseline_sample(job)
         out["cpu_baseline"] = rec
+        out["errors"] = result_errors(job, capi, frame=W)
         out["config"]["gpu_over_cpu"] = round(value / rec["value"], 1)
     print(json.dumps(out))
 

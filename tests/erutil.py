@@ -109,20 +109,22 @@ REL_FLOOR = 1.0  # kcal/mol: relative errors are reported for |E| at or above it
 
 
 def energy_report(e_gpu, e_ref) -> dict:
-    """Error summary of FP32 energies against the FP64 oracle: the largest
-    absolute error below E_MID (the regime ATOL bounds), the largest relative
-    error of energies of at least REL_FLOOR in magnitude below E_MID, and the
-    largest relative error on the LJ wall (above E_MID)."""
+    """Error summary of FP32 energies against the FP64 oracle, by regime:
+    the largest absolute error where |E| < REL_FLOOR (sums that cancel, where
+    only an absolute bar means anything), the largest relative error for
+    REL_FLOOR <= |E| < E_MID, and the largest relative error on the LJ wall
+    (|E| >= E_MID)."""
     e_gpu = np.asarray(e_gpu, np.float64)
     e_ref = np.asarray(e_ref, np.float64)
     d = np.abs(e_gpu - e_ref)
     a = np.abs(e_ref)
-    low = a < E_MID
-    rel = low & (a >= REL_FLOOR)
-    wall = ~low
-    return dict(max_abs=float(d[low].max()) if low.any() else 0.0,
-                max_rel=float((d[rel] / a[rel]).max()) if rel.any() else 0.0,
+    small = a < REL_FLOOR
+    mid = (a >= REL_FLOOR) & (a < E_MID)
+    wall = a >= E_MID
+    return dict(max_abs_small=float(d[small].max()) if small.any() else 0.0,
+                max_rel_mid=float((d[mid] / a[mid]).max()) if mid.any() else 0.0,
                 max_rel_wall=float((d[wall] / a[wall]).max()) if wall.any() else 0.0,
+                max_abs=float(d[~wall].max()) if (~wall).any() else 0.0,
                 n=int(e_ref.size))
 
 
@@ -130,12 +132,15 @@ def flip_report(e_gpu, e_ref, species, uvmax, uvcrd) -> dict:
     """Molecules whose bin differs between the GPU's FP32 energies and the
     oracle's FP64 energies.  e_*: [nins, nmol]; species: [nmol] (0 = the
     solute itself, skipped).  Every flip must have a bin edge between the two
-    energies within the energy tolerance; each is reported with the FP32
-    ulps between the FP64 energy and that edge."""
+    energies within the energy tolerance.  Reported: the count, the largest
+    distance of the FP64 energy from that edge in kcal/mol and as a fraction
+    of the energy tolerance, and the largest such distance in float32 ulps of
+    the larger of |E| and |edge| (edges at exactly 0 included)."""
     e_gpu = np.asarray(e_gpu, np.float64)
     e_ref = np.asarray(e_ref, np.float64)
     idmin = np.concatenate([[0], np.cumsum(uvmax)[:-1]])
-    nflip, max_ulp, unexplained = 0, 0.0, 0
+    nflip, unexplained = 0, 0
+    max_dist, max_frac, max_ulp = 0.0, 0.0, 0.0
     for sp in range(1, len(uvmax) + 1):
         cols = np.nonzero(np.asarray(species) == sp)[0]
         if cols.size == 0:
@@ -150,14 +155,19 @@ def flip_report(e_gpu, e_ref, species, uvmax, uvcrd) -> dict:
         nflip += int(diff.sum())
         lo = np.minimum(g, r)[diff]
         hi = np.maximum(g, r)[diff]
-        tol = energy_tolerance(r[diff])
+        rd = r[diff]
+        tol = energy_tolerance(rd)
         ilo = np.searchsorted(mesh, lo, side="left")
         edge = mesh[np.minimum(ilo, mesh.size - 1)]
         ok = (np.searchsorted(mesh, hi, side="left") > ilo) & (hi - lo <= tol)
         unexplained += int((~ok).sum())
-        ulp = np.spacing(np.abs(edge).astype(np.float32)).astype(np.float64)
-        max_ulp = max(max_ulp, float((np.abs(r[diff] - edge) / ulp).max()))
-    return dict(flips=nflip, max_ulp_to_edge=max_ulp, unexplained=unexplained, n=int(e_ref.size))
+        dist = np.abs(rd - edge)
+        ulp = np.spacing(np.maximum(np.abs(edge), np.abs(rd)).astype(np.float32)).astype(np.float64)
+        max_dist = max(max_dist, float(dist.max()))
+        max_frac = max(max_frac, float((dist / tol).max()))
+        max_ulp = max(max_ulp, float((dist / ulp).max()))
+    return dict(flips=nflip, max_dist_to_edge=max_dist, max_dist_over_tol=max_frac,
+                max_ulp_to_edge=max_ulp, unexplained=unexplained, n=int(e_ref.size))
 
 
 def oracle_block(d) -> dict:
