@@ -4,10 +4,14 @@
 Default workload (configuration 2, the headline): 30 000 rigid 3-site waters
 (90 000 atoms) in a 67.0 A cube, 1000 insertions per frame of the 5-site
 tetrahedral test particle, PME 72^3 order 4, cutoff 12 A, alpha 0.35,
-600-bin energy mesh, ecorr on.  A step is one frame: energies of 1000
-insertions x 30 000 molecules, their histograms, edens and the 600 x 600
-ecorr update.  Frames and insertions are generated on the host and uploaded
-before timing, so the timed region starts with every input resident in HBM.
+600-bin energy mesh, ecorr on.  A step is one batch of 16 frames, recorded
+once as a HIP graph (er_graph_begin / er_graph_end) and replayed with one
+launch (er_graph_launch): per frame, energies of 1000 insertions x 30 000
+molecules, their histograms, edens and the 600 x 600 ecorr update.  The
+workload's first 192 frames (12 batches, one graph each) are generated on
+the host and uploaded before timing, so the timed region starts with every
+input resident in HBM; the timed steps replay the batches in turn.
+--frames-per-step 0 times eager er_run calls, one frame per step.
 
 --workload 0|1|3|4 runs the other configurations the same way (3: all 16
 solutes of a frame in one context and one er_run; 1: the solution run, one
@@ -270,6 +274,9 @@ def main():
     ap.add_argument("--workload", type=int, default=2, choices=[0, 1, 2, 3, 4])
     ap.add_argument("--plain-insertions", type=int, default=200)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--frames-per-step", type=int, default=16,
+                    help="frames per step, recorded as one HIP graph per batch and replayed (at most the "
+                         "workload's frames); 0: eager er_run per frame, one frame per step")
     ap.add_argument("--builtin-shapes", action="store_true",
                     help="skip er_autotune, which otherwise times launch shapes on the last warm-up frame "
                          "before the timed steps")
@@ -288,29 +295,73 @@ def main():
     device = torch.device("cuda:0")
     job = Job(args.workload)
     W, K = args.warmup, args.steps
-    pos, ins = job.inputs(range(W + K), torch, device)
+    B = min(args.frames_per_step, job.w.nframes)
+    if B > 0:  # the workload's frames in whole batches, at most 12 of them
+        nbatch = max(1, min(12, job.w.nframes // B))
+        frames = range(nbatch * B)
+    else:
+        nbatch, frames = 0, range(W + K)
+    pos, ins = job.inputs(frames, torch, device)
     torch.cuda.synchronize()
     ctx = job.context(capi)
 
     # warm-up (compiles nothing; first-touch of buffers, caches, clocks)
-    job.run_frames(ctx, pos[:W], ins[:W], capi.ER_BATCHED)
+    nwarm = max(1, min(W, len(pos)))
+    job.run_frames(ctx, pos[:nwarm], ins[:nwarm], capi.ER_BATCHED)
     ctx.sync()
     tuning, tune_s = None, None
     if args.autotune:
         ta = time.perf_counter()
         tuning = ctx.autotune(reps=3)
         tune_s = round(time.perf_counter() - ta, 3)
-    ctx.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    job.run_frames(ctx, pos[W:], ins[W:], capi.ER_BATCHED)
-    ctx.sync()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = ctx.profile_read()
-    ctx.profile(False)
-    elapsed = t1 - t0
-    nins_total = K * job.ins_per_frame
+    eager_rate, record_s = None, None
+    if B > 0:
+        # one graph per batch of B frames, recorded after the tuning (its launch shapes)
+        tr = time.perf_counter()
+        graphs = []
+        for b in range(nbatch):
+            ctx.graph_begin()
+            job.run_frames(ctx, pos[b * B:(b + 1) * B], ins[b * B:(b + 1) * B], capi.ER_BATCHED)
+            graphs.append(ctx.graph_end())
+        record_s = round(time.perf_counter() - tr, 3)
+        for s in range(W):
+            ctx.graph_launch(graphs[s % nbatch])
+        ctx.sync()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(K):
+            ctx.graph_launch(graphs[s % nbatch])
+        ctx.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        elapsed = t1 - t0
+        nins_total = K * B * job.ins_per_frame
+        # per-kernel HIP-event times: events cannot be recorded inside a
+        # replayed graph, so the same kernels run once more eagerly over the
+        # first batch's frames with the events on (also the eager rate)
+        ctx.profile(True)
+        torch.cuda.synchronize()
+        te = time.perf_counter()
+        job.run_frames(ctx, pos[:B], ins[:B], capi.ER_BATCHED)
+        ctx.sync()
+        torch.cuda.synchronize()
+        eager_rate = B * job.ins_per_frame / (time.perf_counter() - te)
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        prof_units = B * job.ins_per_frame
+    else:
+        ctx.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        job.run_frames(ctx, pos[W:], ins[W:], capi.ER_BATCHED)
+        ctx.sync()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prof = ctx.profile_read()
+        ctx.profile(False)
+        elapsed = t1 - t0
+        nins_total = K * job.ins_per_frame
+        prof_units = nins_total
     value = nins_total / elapsed
 
     # plain path: one insertion per launch sequence, same inputs (the first
@@ -321,12 +372,13 @@ def main():
         pjob.__dict__.update(job.__dict__)
         pjob.multi, pjob.solutes = None, job.solutes[:1]
     pctx = pjob.context(capi) if pjob is not job else ctx
+    fp = min(len(pos) - 1, W)
     if job.soln:
-        pl_pos, pl_ins, nplain = pos[W:W + 1], ins[W:W + 1], job.ins_per_frame
+        pl_pos, pl_ins, nplain = pos[fp:fp + 1], ins[fp:fp + 1], job.ins_per_frame
         pjob.run_frames(pctx, pos[:1], ins[:1], capi.ER_PLAIN)
     else:
         nplain = min(args.plain_insertions, job.solutes[0].nins)
-        pl_pos, pl_ins = pos[W:W + 1], [ins[W][:1]]
+        pl_pos, pl_ins = pos[fp:fp + 1], [ins[fp][:1]]
         pjob.run_frames(pctx, pos[:1], [ins[0][:1]], capi.ER_PLAIN, nins=8)
     pctx.sync()
     torch.cuda.synchronize()
@@ -344,14 +396,16 @@ def main():
     busy = {k: v for k, v in prof.items() if v[1] > 0}
     dom = max((k for k in busy if algorithmic(k, job, 1)[0]), key=lambda k: busy[k][0])
     ms, launches = busy[dom]
-    nch = nins_total / launches
+    nch = prof_units / launches
     bound, amount = algorithmic(dom, job, nch)
     avg_s = ms / launches / 1e3
     if bound == "hbm":
         achieved, peak, unit = amount / avg_s / 1e9, HBM_PEAK_GBS, "GB/s"
     else:
         achieved, peak, unit = amount / avg_s / 1e12, FP64_MFMA_PEAK_TFS, "TFLOP/s"
-    kernels = {k: dict(ms_per_step=v[0] / K, launches=v[1]) for k, v in busy.items()}
+    frames_profiled = prof_units / job.ins_per_frame
+    kernels = {k: dict(ms_per_frame=v[0] / frames_profiled, launches_per_frame=v[1] / frames_profiled)
+               for k, v in busy.items()}
     traffic = pmc_traffic(dom, args.workload, nch)
 
     out = {
@@ -368,9 +422,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": DESCRIPTION[args.workload] + "; one step = one frame",
-            "insertions_per_step": job.ins_per_frame,
-            "frames_per_s": round(K / elapsed, 3),
+            "workload": DESCRIPTION[args.workload] + (
+                f"; one step = {B} frames replayed as one recorded HIP graph ({nbatch} batches of the "
+                f"workload's frames, replayed in turn)" if B > 0 else "; one step = one frame, eager er_run"),
+            "frames_per_step": max(B, 1),
+            "insertions_per_step": job.ins_per_frame * max(B, 1),
+            "frames_per_s": round(K * max(B, 1) / elapsed, 3),
+            "eager_insertions_per_s": None if eager_rate is None else round(eager_rate, 1),
+            "graph_record_s": record_s,
             "plain_insertions_per_s": round(plain, 1),
             "batched_over_plain": round(value / plain, 2),
             "precision": "pair/PME f32; self energy, weights, edens, ecorr f64",
@@ -398,7 +457,7 @@ def main():
         else:
             rec = cpu_baseline_sample(job)
         out["cpu_baseline"] = rec
-        out["errors"] = result_errors(job, capi, frame=W)
+        out["errors"] = result_errors(job, capi, frame=fp)
         out["config"]["gpu_over_cpu"] = round(value / rec["value"], 1)
     print(json.dumps(out))
 

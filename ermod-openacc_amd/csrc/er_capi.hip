@@ -95,7 +95,9 @@ struct Slot {
     bool have_ins = false, wslt_given = false;
     DBuf<float> d_sltpos_own, d_wslt;
     const float *d_sltpos = nullptr;
-    DBuf<float> d_U, d_u0;         // uvengy / uvengy0 of the last run
+    // uvengy / uvengy0 of the last run: this solute's rows of the run's
+    // energies (valid until the next run)
+    const float *U_view = nullptr, *u0_view = nullptr;
     int last_NI = 16, last_nins = 0;
     // accumulators
     DBuf<double> d_edens, d_ecorr, d_eself, d_state, d_slnuv;
@@ -104,12 +106,12 @@ struct Slot {
     DBuf<unsigned int> d_minmax;
     // halt key, 'mbin' warning key; captured (key, energy) of each
     DBuf<unsigned long long> d_errkey, d_warnkey, d_halt;
-    unsigned int seq = 0;                         // runs of this slot
+    DBuf<unsigned int> d_seq;      // runs of this slot (device counter: graph replays count too)
     int pair_split = 0;            // pair-search workgroups per insertion (0: built-in choice)
     void release()
     {
-        d_sltq.release(); d_sltlj.release(); d_sltpos_own.release(); d_wslt.release(); d_U.release();
-        d_u0.release(); d_edens.release(); d_ecorr.release(); d_eself.release(); d_state.release();
+        d_sltq.release(); d_sltlj.release(); d_sltpos_own.release(); d_wslt.release(); d_seq.release();
+        d_edens.release(); d_ecorr.release(); d_eself.release(); d_state.release();
         d_slnuv.release(); d_flags.release(); d_warnkey.release(); d_fstate.release(); d_minmax.release();
         d_errkey.release(); d_halt.release();
     }
@@ -212,6 +214,32 @@ struct er_ctx {
 
     // accumulators
     DBuf<double> d_part, d_epart;  // split-K partial sums of ecorr / edens
+
+    // the run in progress: its solutes end to end (RunPlan), their device
+    // descriptors, the insertions' coordinates and the run's energies
+    struct RunPlan {
+        int NI = NI_B;
+        std::vector<int> order;    // slot ids in run order (few-site solutes first)
+        std::vector<SlotDesc> desc;
+        int ntot = 0, nsite = 0, ndense = 0, pair_blocks = 0, nz = 0, wrow = 1;
+    } plan;
+    DBuf<SlotDesc> d_sd;
+    std::vector<SlotDesc> sd_up;   // what d_sd holds
+    DBuf<float> d_sltcat;          // [nsite][3] the run's insertion coordinates
+    DBuf<float> d_Ucat, d_u0cat;   // [ntot/NI][nummol][NI], [ntot]
+    bool capturing = false;        // a HIP graph is being recorded
+    bool capture_bad = false;      // a call failed while recording
+    bool capture_engfac = false;   // the recording computes the influence function before its first run
+    int capture_runs = 0;          // runs recorded so far
+    bool prof_before_capture = false;
+    bool cap_have_prev = false;    // influence-function cell when recording began
+    float cap_prev_cell[9] = {0};
+    unsigned long long cap_gen = 0;
+    // device buffers of the run path are reallocated or freed only where
+    // this counter moves; a graph replays only while it is unchanged
+    unsigned long long alloc_gen = 0;
+    struct GraphRec;
+    std::vector<GraphRec *> graphs;
     // er_run_noalloc: this device's address of the file-scope scratch array,
     // and whether the run in progress may allocate
     double *scratch_dev = nullptr;
@@ -241,6 +269,7 @@ struct er_ctx {
 static int fail(er_ctx *c, int code, const char *fmt, ...)
 {
     if (c) {
+        if (c->capturing) c->capture_bad = true;
         char buf[512];
         va_list ap;
         va_start(ap, fmt);
@@ -304,6 +333,14 @@ static void prof_collect(er_ctx *c)
         if (e_ != hipSuccess)                                                                \
             return fail((c), ER_E_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), \
                         __FILE__, __LINE__);                                                 \
+    } while (0)
+
+// the entry points that cannot be recorded into a graph
+#define ER_NO_CAPTURE(c, who)                                                                          \
+    do {                                                                                               \
+        if ((c)->capturing)                                                                            \
+            return fail((c), ER_E_STATE, "%s: not allowed while a graph is recorded (er_graph_begin)", \
+                        who);                                                                          \
     } while (0)
 
 static bool fft_len_ok(int n)
@@ -375,6 +412,7 @@ static int slot_state_init(er_ctx *c, Slot &sl)
     HIPCHK(c, sl.d_warnkey.ensure(1));
     HIPCHK(c, sl.d_errkey.ensure(1));
     HIPCHK(c, sl.d_halt.ensure(4));
+    HIPCHK(c, sl.d_seq.ensure(1));
     HIPCHK(c, sl.d_minmax.ensure(2 * (size_t)(c->numslv + 1)));
     HIPCHK(c, sl.d_slnuv.ensure(c->numslv));
     HIPCHK(c, hipMemsetAsync(sl.d_state.p, 0, 4 * sizeof(double), s));
@@ -383,7 +421,7 @@ static int slot_state_init(er_ctx *c, Slot &sl)
     HIPCHK(c, hipMemsetAsync(sl.d_warnkey.p, 0xff, sizeof(unsigned long long), s));
     HIPCHK(c, hipMemsetAsync(sl.d_errkey.p, 0xff, sizeof(unsigned long long), s));
     HIPCHK(c, hipMemsetAsync(sl.d_halt.p, 0xff, 4 * sizeof(unsigned long long), s));
-    sl.seq = 0;
+    HIPCHK(c, hipMemsetAsync(sl.d_seq.p, 0, sizeof(unsigned int), s));
     HIPCHK(c, hipMemsetAsync(sl.d_slnuv.p, 0, c->numslv * sizeof(double), s));
     std::vector<unsigned int> mm(2 * (size_t)(c->numslv + 1));
     auto ord = [](float f) {
@@ -654,6 +692,8 @@ extern "C" int er_set_mesh(er_ctx *c, const int32_t *uvmax, const int32_t *uvsof
                            int32_t esmax, const double *escrd)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_mesh");
+    ++c->alloc_gen;
     if (!uvmax || !uvsoft || !uvcrd) return fail(c, ER_E_ARG, "er_set_mesh: null argument");
     c->uvmax.assign(uvmax, uvmax + c->numslv);
     c->uvsoft.assign(uvsoft, uvsoft + c->numslv);
@@ -751,6 +791,8 @@ static int set_slot_solute(er_ctx *c, Slot &sl, int32_t nsite, const int32_t *lj
 extern "C" int er_set_solute(er_ctx *c, int32_t nsite, const int32_t *ljtype, const float *charge)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_solute");
+    ++c->alloc_gen;
     if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "er_set_solute: soln runs use er_set_solute_molecules");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return set_slot_solute(c, c->S(), nsite, ljtype, charge);
@@ -760,6 +802,8 @@ extern "C" int er_set_solutes(er_ctx *c, int32_t nsolute, const int32_t *nsite, 
                               const float *charge)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_solutes");
+    ++c->alloc_gen;
     if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "er_set_solutes: only for refs runs");
     if (nsolute <= 0 || !nsite || !ljtype || !charge) return fail(c, ER_E_ARG, "er_set_solutes: bad arguments");
     HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -793,6 +837,8 @@ extern "C" int er_select_solute(er_ctx *c, int32_t k)
 extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *tagslt)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_solute_molecules");
+    ++c->alloc_gen;
     if (c->prm.slttype != 1) return fail(c, ER_E_STATE, "er_set_solute_molecules: only for soln runs");
     if (nslt <= 0 || !tagslt) return fail(c, ER_E_ARG, "er_set_solute_molecules: bad arguments");
     c->tagslt.resize(nslt);
@@ -926,6 +972,7 @@ static int frame_cell(er_ctx *c, const float *cell)
 extern "C" int er_set_frame(er_ctx *c, const float *sitepos, const float *cell, float stat_weight_system)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_frame (use er_set_frame_device)");
     if (!sitepos || !cell) return fail(c, ER_E_ARG, "er_set_frame: null argument");
     int rc = frame_cell(c, cell);
     if (rc) return rc;
@@ -957,10 +1004,18 @@ static int ins_common(er_ctx *c, int32_t nins, const float *wslt, bool copies)
     if (c->prm.slttype == 1) return fail(c, ER_E_STATE, "insertions are for refs runs");
     if (c->S().nslt <= 0) return fail(c, ER_E_STATE, "er_set_solute must precede insertions");
     if (nins <= 0) return fail(c, ER_E_ARG, "nins must be positive");
+    if (c->capturing && (copies || wslt))
+        return fail(c, ER_E_STATE, "a recorded graph takes er_set_insertions_device without stat_weight_solute");
     // a run still in flight may read the buffers about to be overwritten
     if (copies || wslt) HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (c->capturing && c->S().d_wslt.n < (size_t)nins)
+        return fail(c, ER_E_STATE, "er_graph_begin: run these insertions once before recording");
     c->S().nins = nins;
-    HIPCHK(c, c->S().d_wslt.ensure(nins));
+    {
+        const float *before = c->S().d_wslt.p;
+        HIPCHK(c, c->S().d_wslt.ensure(nins));
+        if (c->S().d_wslt.p != before) ++c->alloc_gen;
+    }
     c->S().wslt_given = wslt != nullptr;
     if (wslt) HIPCHK(c, hipMemcpy(c->S().d_wslt.p, wslt, nins * sizeof(float), hipMemcpyHostToDevice));
     c->S().have_ins = true;
@@ -974,7 +1029,7 @@ extern "C" int er_set_insertions(er_ctx *c, int32_t nins, const float *sltpos, c
     int rc = ins_common(c, nins, stat_weight_solute, true);
     if (rc) return rc;
     const size_t n = (size_t)3 * c->S().nslt * nins;
-    HIPCHK(c, c->S().d_sltpos_own.ensure(n));
+    HIPCHK(c, c->S().d_sltpos_own.ensure(n));  // read only by the runs' copy of the coordinates
     HIPCHK(c, hipMemcpy(c->S().d_sltpos_own.p, sltpos, n * sizeof(float), hipMemcpyHostToDevice));
     c->S().d_sltpos = c->S().d_sltpos_own.p;
     return ER_OK;
@@ -1006,7 +1061,10 @@ template <> struct Split<96> { static constexpr int A = 8, B = 12; };
 template <> struct Split<108> { static constexpr int A = 12, B = 9; };
 template <> struct Split<128> { static constexpr int A = 8, B = 16; };
 
-template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch)
+// the plane kernel over insertions [n_off, n_off + nch) of the run, whose T
+// rows start at insertion t_off; sparse: the scatter + forward DFT form (the
+// solutes of more than ER_SPARSE_SITES sites)
+template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off, int nch, int t_off, bool sparse)
 {
     constexpr int R1 = Split<K3>::A, R2 = Split<K3>::B, C1 = Split<K2>::A, C2 = Split<K2>::B;
     constexpr int NT = (K2 * K3 >= 5184) ? 512 : 256;  // large planes: more threads, fewer values per thread
@@ -1020,35 +1078,38 @@ template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off
                                   hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         attr = true;
     }
+    const int nslot = (int)c->plan.desc.size();
+    float2 *T = c->d_T.p + (size_t)(n_off - t_off) * c->H * K2 * K3;  // the launch's first T row
     // many sites: scatter + forward plane DFT beats nslt * K2 * K3 products
-    if (c->S().nslt > ER_SPARSE_SITES)
+    if (sparse)
         hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT, true>), dim3(c->H, nch), dim3(NT), lds,
-                           c->stream, c->S().nslt, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_insw.p, c->d_insg.p,
-                           c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
+                           c->stream, nslot, c->d_sd.p, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_insw.p,
+                           c->d_insg.p, c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, T, c->d_selfpart.p);
     else
         hipLaunchKernelGGL((k_pme_spec2d<K2, K3, R1, R2, C1, C2, NT, false>), dim3(c->H, nch), dim3(NT), lds,
-                           c->stream, c->S().nslt, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_insw.p, c->d_insg.p,
-                           c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, c->d_T.p, c->d_selfpart.p);
+                           c->stream, nslot, c->d_sd.p, c->ord, c->K1, n_off, c->d_Ftab.p, c->d_insw.p,
+                           c->d_insg.p, c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, c->d_tws.p, T, c->d_selfpart.p);
     return hipGetLastError();
 }
 
-static hipError_t launch_spec2d(er_ctx *c, int n_off, int nch)
+static hipError_t launch_spec2d(er_ctx *c, int n_off, int nch, int t_off, bool sparse)
 {
+    if (nch <= 0) return hipSuccess;
     if (c->K2 == c->K3) {
         switch (c->K2) {
-        case 32: return launch_spec2d_k<32, 32>(c, n_off, nch);
-        case 36: return launch_spec2d_k<36, 36>(c, n_off, nch);
-        case 40: return launch_spec2d_k<40, 40>(c, n_off, nch);
-        case 48: return launch_spec2d_k<48, 48>(c, n_off, nch);
-        case 64: return launch_spec2d_k<64, 64>(c, n_off, nch);
-        case 72: return launch_spec2d_k<72, 72>(c, n_off, nch);
-        case 80: return launch_spec2d_k<80, 80>(c, n_off, nch);
-        case 96: return launch_spec2d_k<96, 96>(c, n_off, nch);
-        case 108: return launch_spec2d_k<108, 108>(c, n_off, nch);
-        case 128: return launch_spec2d_k<128, 128>(c, n_off, nch);
+        case 32: return launch_spec2d_k<32, 32>(c, n_off, nch, t_off, sparse);
+        case 36: return launch_spec2d_k<36, 36>(c, n_off, nch, t_off, sparse);
+        case 40: return launch_spec2d_k<40, 40>(c, n_off, nch, t_off, sparse);
+        case 48: return launch_spec2d_k<48, 48>(c, n_off, nch, t_off, sparse);
+        case 64: return launch_spec2d_k<64, 64>(c, n_off, nch, t_off, sparse);
+        case 72: return launch_spec2d_k<72, 72>(c, n_off, nch, t_off, sparse);
+        case 80: return launch_spec2d_k<80, 80>(c, n_off, nch, t_off, sparse);
+        case 96: return launch_spec2d_k<96, 96>(c, n_off, nch, t_off, sparse);
+        case 108: return launch_spec2d_k<108, 108>(c, n_off, nch, t_off, sparse);
+        case 128: return launch_spec2d_k<128, 128>(c, n_off, nch, t_off, sparse);
         }
     } else if (c->K2 == 128 && c->K3 == 108) {
-        return launch_spec2d_k<128, 108>(c, n_off, nch);
+        return launch_spec2d_k<128, 108>(c, n_off, nch, t_off, sparse);
     }
     return hipErrorInvalidValue;
 }
@@ -1118,8 +1179,8 @@ template <int ORD, int NI, int LPM> static hipError_t launch_gather_k(er_ctx *c,
     const bool soln = c->prm.slttype == 1;
     hipLaunchKernelGGL((k_gather<ORD, NI, LPM>), dim3(ngx * ngy), dim3(256), 0, c->stream, c->nummol, nch, n_off,
                        c->K1, c->K2, c->K3, c->d_perm.p, ngx, NXCD, c->d_molbeg.p, c->d_posn.p, c->d_splw.p,
-                       c->d_splg.p, c->d_molq.p, c->d_phi.p, soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt,
-                       c->prm.screen, c->fc.volume, c->S().d_U.p);
+                       c->d_splg.p, c->d_molq.p, c->d_phi.p, soln ? c->d_tag.p : (const int *)nullptr,
+                       (int)c->plan.desc.size(), c->d_sd.p, c->prm.screen, c->fc.volume, c->d_Ucat.p);
     return hipGetLastError();
 }
 
@@ -1156,7 +1217,8 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
     hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + mpb - 1) / mpb), ntile), dim3(256),
                        0, c->stream, c->nummol,
                        nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
-                       soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt, c->prm.screen, c->fc.volume, c->S().d_U.p);
+                       soln ? c->d_tag.p : (const int *)nullptr, (int)c->plan.desc.size(), c->d_sd.p, c->prm.screen,
+                       c->fc.volume, c->d_Ucat.p);
     return hipGetLastError();
 }
 
@@ -1175,8 +1237,8 @@ template <int NI> static hipError_t launch_gather_stream_k(er_ctx *c, int n_off,
     const int mpb = ER_MR_MPT * (256 / NI);  // molecules per block
     hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + mpb - 1) / mpb), ntile), dim3(256), 0,
                        c->stream, c->nummol, nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
-                       soln ? c->d_tag.p : (const int *)nullptr, c->S().qslt, c->prm.screen, c->fc.volume,
-                       c->S().d_U.p);
+                       soln ? c->d_tag.p : (const int *)nullptr, (int)c->plan.desc.size(), c->d_sd.p, c->prm.screen,
+                       c->fc.volume, c->d_Ucat.p);
     return hipGetLastError();
 }
 
@@ -1190,13 +1252,13 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
     if (lds > 65536)
         (void)hipFuncSetAttribute((const void *)k_hist<NI>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     const bool soln = c->prm.slttype == 1;
-    HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice, c->S().seq};
+    HistArgs ha{c->nummol, nch, n_off, c->ermax, c->numslv, c->prm.slttype, c->nslice};
     const int ntile = (nch + NI - 1) / NI;
-    hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha, c->S().d_U.p,
+    hipLaunchKernelGGL((k_hist<NI>), dim3(ntile, c->nslice), dim3(NT), lds, c->stream, ha,
+                       (int)c->plan.desc.size(), c->d_sd.p, c->d_Ucat.p,
                        soln ? c->d_tag.p : (const int *)nullptr, c->d_meshf.p, c->d_blut.p, c->d_spb.p, c->d_spe.p,
-                       c->d_idmin.p,
-                       c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p, c->S().d_minmax.p, soln ? c->d_flpart.p : (double *)nullptr,
-                       c->S().d_errkey.p, c->S().d_warnkey.p);
+                       c->d_idmin.p, c->d_uvmax.p, c->d_uvsoft.p, c->d_C.p,
+                       soln ? c->d_flpart.p : (double *)nullptr);
     return hipGetLastError();
 }
 
@@ -1246,6 +1308,7 @@ static hipError_t join_atoms(er_ctx *c)
 static hipError_t prep_frame(er_ctx *c)
 {
     const int B = 256;
+    if (c->capturing && c->capture_runs == 0 && c->cell_changed) c->capture_engfac = true;
     if (c->cell_changed && c->prm.cltype == 0) {  // bare Coulomb: no influence function
         std::memcpy(c->prev_cell, c->cell, sizeof c->cell);
         c->have_prev_cell = true;
@@ -1270,10 +1333,13 @@ static hipError_t prep_frame(er_ctx *c)
         c->cell_changed = false;
     }
     if (c->frame_dirty) {
-        // after the frame's upload on the main stream
-        hipStream_t s2 = c->stream2;
-        HIPCHK_E(hipEventRecord(c->ev_in, c->stream));
-        HIPCHK_E(hipStreamWaitEvent(s2, c->ev_in, 0));
+        // after the frame's upload on the main stream; while a graph is
+        // recorded the work stays on the main stream
+        hipStream_t s2 = c->capturing ? c->stream : c->stream2;
+        if (!c->capturing) {
+            HIPCHK_E(hipEventRecord(c->ev_in, c->stream));
+            HIPCHK_E(hipStreamWaitEvent(s2, c->ev_in, 0));
+        }
         hipLaunchKernelGGL(k_prep_atoms, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm, c->d_sitepos,
                            c->d_charge.p, c->fc, c->K1, c->K2, c->K3, c->ord, c->d_posn.p, c->d_splw.p, c->d_splg.p);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s2, (size_t)1,
@@ -1303,56 +1369,208 @@ static hipError_t prep_frame(er_ctx *c)
             hipLaunchKernelGGL(k_brick_records<6>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
                                c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
                                c->K3, c->nbx, c->nby, c->d_brec.p);
-        HIPCHK_E(hipEventRecord(c->ev_atoms, s2));
-        c->atoms_pending = true;
+        if (!c->capturing) {
+            HIPCHK_E(hipEventRecord(c->ev_atoms, s2));
+            c->atoms_pending = true;
+        }
         c->frame_dirty = false;
     }
     return hipGetLastError();
 }
 
 // a run buffer of at least `count` elements: grown as needed, except in an
-// er_run_noalloc call, where a buffer too small is an error (nothing is
-// allocated)
+// er_run_noalloc call or while a graph is recorded, where a buffer too small
+// is an error (nothing is allocated)
 template <class T> static int want(er_ctx *c, DBuf<T> &b, size_t count, const char *name)
 {
-    if (!c->noalloc) {
+    if (!c->noalloc && !c->capturing) {
+        const T *before = b.p;
         HIPCHK(c, b.ensure(count));
+        if (b.p != before) ++c->alloc_gen;
         return ER_OK;
     }
     if (b.p && count <= b.n) return ER_OK;
     return fail(c, ER_E_STATE,
-                "er_run_noalloc: buffer %s holds %zu elements, this run needs %zu; an er_run of at least as many "
-                "insertions (same launch shapes) must size it first",
-                name, b.n, count);
+                "%s: buffer %s holds %zu elements, this run needs %zu; an er_run of the same solutes and at least "
+                "as many insertions (same launch shapes) must size it first",
+                c->capturing ? "er_graph_begin" : "er_run_noalloc", name, b.n, count);
 }
 
-static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
+// insertion slices of the split-K edens/ecorr sums over nch insertions:
+// slices of kper (a multiple of 4, at least ecorr_kmin) insertions
+static void ecorr_slices(const er_ctx *c, int nch, int &kper, int &nks)
 {
+    kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
+    nks = (nch + kper - 1) / kper;
+}
+
+// workgroups per insertion in the pair search: the slot's tuned value in a
+// batched run (the shape it was tuned for), else enough to fill the chip (at
+// least 4 per CU); a many-site solute reaches far, so its candidates are
+// spread over 16.  Every (insertion, molecule) sum has one owner whatever
+// the split, so the split never changes a result.
+static int pair_split(const Slot &sl, int nins, bool batched)
+{
+    if (batched && sl.pair_split > 0) return sl.pair_split;
+    if (sl.nslt > 8) return 16;
+    return std::max(1, std::min(16, (4 * 256 + nins - 1) / nins));
+}
+
+// The run's solutes end to end: the solutes of few sites first (dense plane
+// spectra), then the many-site ones (sparse spectra), each solute's
+// insertion range padded to the interleave NI.
+static int build_plan(er_ctx *c, const std::vector<int> &todo, int mode)
+{
+    auto &P = c->plan;
+    const bool soln = c->prm.slttype == 1;
+    const bool batched = mode == ER_BATCHED;
+    P.NI = batched ? NI_B : 1;
+    P.order.clear();
+    for (int pass = 0; pass < 2; ++pass)
+        for (int k : todo)
+            if ((c->slots[k].nslt > ER_SPARSE_SITES) == (pass == 1)) P.order.push_back(k);
+    P.desc.assign(P.order.size(), SlotDesc{});
+    long long ins = 0, site = 0, blk = 0;
+    int z = 0, wrow = 1;
+    P.ndense = 0;
+    for (size_t i = 0; i < P.order.size(); ++i) {
+        Slot &sl = c->slots[P.order[i]];
+        const int nins = soln ? c->nsltmol : sl.nins;
+        SlotDesc &d = P.desc[i];
+        d.nslt = sl.nslt;
+        d.ins_off = (int)ins;
+        d.nins = nins;
+        d.site_off = (int)site;
+        d.pair_split = pair_split(sl, batched ? nins : 1, batched);
+        d.pair_blk = (int)blk;
+        ecorr_slices(c, nins, d.kper, d.nks);
+        d.slice_off = z;
+        d.qslt = sl.qslt;
+        d.sltq = sl.d_sltq.p;
+        d.sltlj = sl.d_sltlj.p;
+        d.wslt = sl.wslt_given ? sl.d_wslt.p : nullptr;
+        d.seq = sl.d_seq.p;
+        d.state = sl.d_state.p;
+        d.flags = sl.d_flags.p;
+        d.fstate = sl.d_fstate.p;
+        d.minmax = sl.d_minmax.p;
+        d.edens = sl.d_edens.p;
+        d.ecorr = sl.d_ecorr.p;
+        d.eself = sl.d_eself.p;
+        d.slnuv = sl.d_slnuv.p;
+        d.errkey = sl.d_errkey.p;
+        d.warnkey = sl.d_warnkey.p;
+        d.halt = sl.d_halt.p;
+        ins += (nins + P.NI - 1) / P.NI * P.NI;
+        site += (long long)nins * sl.nslt;
+        blk += (long long)nins * d.pair_split;
+        z += d.nks;
+        wrow = std::max(wrow, (nins + 255) / 256);
+        if (sl.nslt <= ER_SPARSE_SITES) P.ndense = (int)ins;
+    }
+    if (ins >= (1ll << 30) || site >= (1ll << 30) || blk >= (1ll << 31) - 1)
+        return fail(c, ER_E_UNSUPPORTED, "er_run: %lld insertions, %lld sites in one run exceed this build's range",
+                    ins, site);
+    P.ntot = (int)ins;
+    P.nsite = (int)site;
+    P.pair_blocks = (int)blk;
+    P.nz = z;
+    P.wrow = wrow;
+    return ER_OK;
+}
+
+// the descriptors on the device; an upload waits for the runs in flight,
+// which read the previous ones (the descriptors change only with the shapes)
+static int upload_plan(er_ctx *c)
+{
+    const auto &d = c->plan.desc;
+    if (d.size() == c->sd_up.size() && std::memcmp(d.data(), c->sd_up.data(), d.size() * sizeof(SlotDesc)) == 0)
+        return ER_OK;
+    if (c->capturing)
+        return fail(c, ER_E_STATE,
+                    "er_graph_begin: the solutes, insertion counts or launch shapes differ from the last er_run; "
+                    "run them once before recording");
+    int rc = want(c, c->d_sd, d.size(), "descriptors");
+    if (rc) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    HIPCHK(c, hipMemcpy(c->d_sd.p, d.data(), d.size() * sizeof(SlotDesc), hipMemcpyHostToDevice));
+    c->sd_up = d;
+    return ER_OK;
+}
+
+// the split-K partials (ecorr, edens) and the weights' block partials of
+// the run: the context's buffers, or in an er_run_noalloc call consecutive
+// ranges of the scratch array
+struct AccPartials {
+    double *part, *epart, *wpart;
+};
+
+static void partial_sizes(const er_ctx *c, bool batched, size_t &npart, size_t &nepart, size_t &nwpart)
+{
+    const auto &P = c->plan;
+    int kper1, nks1;
+    ecorr_slices(c, 1, kper1, nks1);
+    const size_t nz = batched ? (size_t)P.nz : (size_t)nks1;
+    const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
+    npart = mfma ? nz * c->ermax * c->ermax : 0;
+    nepart = nz * c->ermax;
+    nwpart = batched ? (size_t)4 * P.desc.size() * P.wrow : 4;
+}
+
+static AccPartials acc_partials(er_ctx *c, bool batched)
+{
+    if (!c->noalloc) return {c->d_part.p, c->d_epart.p, c->d_wpart.p};
+    size_t np, ne, nw;
+    partial_sizes(c, batched, np, ne, nw);
+    double *p = c->scratch_dev;
+    return {p, p + np, p + np + ne};
+}
+
+static int ensure_run_buffers(er_ctx *c, int chunk)
+{
+    const auto &P = c->plan;
+    const int NI = P.NI;
+    const bool soln = c->prm.slttype == 1;
+    const size_t nsite = (size_t)P.nsite, ntot = (size_t)P.ntot;
     int rc;
-#define ER_WANT(buf, count)                                     \
-    do {                                                        \
+#define ER_WANT(buf, count)                                      \
+    do {                                                         \
         if ((rc = want(c, buf, (size_t)(count), #buf)) != ER_OK) \
-            return rc;                                          \
+            return rc;                                           \
     } while (0)
-    const size_t nslt = (size_t)c->S().nslt;
-    ER_WANT(c->d_insposn, nins * nslt);
-    ER_WANT(c->d_insw, nins * nslt * 3 * c->ord);
-    ER_WANT(c->d_insg, nins * nslt);
-    ER_WANT(c->d_insref, nins);
-    ER_WANT(c->d_selfreal, nins);
-    ER_WANT(c->d_resself, nins);
-    ER_WANT(c->d_selfpart, (size_t)nins * c->H);
-    ER_WANT(c->d_Ftab, (size_t)nins * nslt * (c->H + c->K2 + c->K3));
+    // the partial sums first: a run beyond the no-allocation scratch is refused outright
+    {
+        size_t np, ne, nw;
+        partial_sizes(c, NI > 1, np, ne, nw);
+        if (c->noalloc) {
+            if (np + ne + nw > ER_NOALLOC_SCRATCH_BYTES / sizeof(double))
+                return fail(c, ER_E_UNSUPPORTED,
+                            "er_run_noalloc: the run's partial sums (%d insertions, %d bins) need %zu bytes, above "
+                            "the %zu-byte scratch (use er_run)",
+                            P.ntot, c->ermax, (np + ne + nw) * sizeof(double), (size_t)ER_NOALLOC_SCRATCH_BYTES);
+        } else {
+            // at least the slices set_mesh sized, so small runs never shrink them
+            ER_WANT(c->d_wpart, nw);
+            ER_WANT(c->d_epart, std::max(ne, (size_t)ER_KSLICES * c->ermax));
+            if (np) ER_WANT(c->d_part, std::max(np, (size_t)ER_KSLICES * c->ermax * c->ermax));
+        }
+    }
+    ER_WANT(c->d_insposn, nsite);
+    ER_WANT(c->d_insw, nsite * 3 * c->ord);
+    ER_WANT(c->d_insg, nsite);
+    if (!soln) ER_WANT(c->d_sltcat, nsite * 3);
+    ER_WANT(c->d_Ftab, nsite * (c->H + c->K2 + c->K3));
+    ER_WANT(c->d_insref, ntot);
+    ER_WANT(c->d_selfreal, ntot);
+    ER_WANT(c->d_resself, ntot);
+    ER_WANT(c->d_selfpart, ntot * c->H);
     ER_WANT(c->d_T, (size_t)chunk * c->Gc);
     const size_t ntile = (size_t)(chunk + NI - 1) / NI;
     ER_WANT(c->d_phi, ntile * NI * c->G);
-    const size_t ntile_all = (size_t)(nins + NI - 1) / NI;
-    ER_WANT(c->S().d_U, ntile_all * NI * c->nummol);
-    ER_WANT(c->S().d_u0, nins);
-    ER_WANT(c->d_C, (size_t)nins * c->ermax);
-    ER_WANT(c->d_w, nins);
-    // the weights' block partials: the scratch array in an er_run_noalloc call
-    if (!c->noalloc) ER_WANT(c->d_wpart, (size_t)4 * std::max(1, (nins + 255) / 256));
+    ER_WANT(c->d_Ucat, ntot * c->nummol);
+    ER_WANT(c->d_u0cat, ntot);
+    ER_WANT(c->d_C, ntot * c->ermax);
+    ER_WANT(c->d_w, ntot);
     ER_WANT(c->d_perm, c->nummol);
     ER_WANT(c->d_keyrank, 2 * (size_t)c->nummol);
     ER_WANT(c->d_ccnt, c->ncell);
@@ -1367,164 +1585,99 @@ static int ensure_run_buffers(er_ctx *c, int nins, int chunk, int NI)
     c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
     // soln runs keep the built-in slicing: their flceng partial sums follow it
     if (c->hist_slices > 0 && c->prm.slttype != 1) c->nslice = c->hist_slices;
-    if (c->prm.slttype == 1) {
-        ER_WANT(c->d_flceng, (size_t)nins * c->numslv);
-        ER_WANT(c->d_flpart, (size_t)nins * c->nslice * c->numslv);
+    if (soln) {
+        ER_WANT(c->d_flceng, ntot * c->numslv);
+        ER_WANT(c->d_flpart, ntot * c->nslice * c->numslv);
     }
 #undef ER_WANT
     return ER_OK;
 }
 
-template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, int nch);
-
-// energies and histograms of insertions [n_off, n_off+nch); NI = 16 for the
+// reciprocal part of insertions [n_off, n_off+nch) of the run (a chunk; T
+// rows start at n_off): plane spectra (few-site solutes, then many-site
+// ones), axis-1 transform, gather and molecule sums into U; NI = 16 for the
 // batched path, 1 for the plain path
 template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
 {
     hipError_t e;
-    if (c->prm.cltype == 0) {
-        // bare Coulomb (engproc.F90:761-769, 819-826): no reciprocal part; the
-        // chunk's rows of U start at zero for the pair sums
-        const size_t t0 = (size_t)n_off / NI, t1 = ((size_t)n_off + nch + NI - 1) / NI;
-        const size_t n = (t1 - t0) * NI * c->nummol;
-        hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
-                           reinterpret_cast<uint32_t *>(c->S().d_U.p + t0 * NI * c->nummol));
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-        return run_chunk_pairs_hist<NI>(c, n_off, nch);
-    }
     {
         ProfScope ps(c, ER_K_SPEC2D);
-        if ((e = launch_spec2d(c, n_off, nch)) != hipSuccess) return e;
+        const int nd = std::max(0, std::min(n_off + nch, c->plan.ndense) - n_off);  // few-site insertions
+        if ((e = launch_spec2d(c, n_off, nd, n_off, false)) != hipSuccess) return e;
+        if ((e = launch_spec2d(c, n_off + nd, nch - nd, n_off, true)) != hipSuccess) return e;
     }
     {
         ProfScope ps(c, ER_K_AXIS1);
         if ((e = launch_axis1<NI>(c, nch)) != hipSuccess) return e;
     }
     if ((e = join_atoms(c)) != hipSuccess) return e;
-    {
-        ProfScope ps(c, ER_K_GATHER);
-        if constexpr (NI == 1) e = launch_gather<1, 1>(c, n_off, nch);
-        else {
-            if (c->ord == 4) e = launch_gather_stream_k<NI>(c, n_off, nch);
-            else if (c->ord == 6) e = launch_gather_brick_k<6, NI>(c, n_off, nch);
-            else e = launch_gather<NI, NI>(c, n_off, nch);
-        }
+    ProfScope ps(c, ER_K_GATHER);
+    if constexpr (NI == 1) return launch_gather<1, 1>(c, n_off, nch);
+    else {
+        if (c->ord == 4) return launch_gather_stream_k<NI>(c, n_off, nch);
+        if (c->ord == 6) return launch_gather_brick_k<6, NI>(c, n_off, nch);
+        return launch_gather<NI, NI>(c, n_off, nch);
     }
-    if (e != hipSuccess) return e;
-    return run_chunk_pairs_hist<NI>(c, n_off, nch);
 }
 
-// workgroups per insertion in the pair search: the slot's tuned value in a
-// batched run (the shape it was tuned for), else enough to fill the chip (at
-// least 4 per CU); a many-site solute reaches far, so its candidates are
-// spread over 16.  Every (insertion, molecule) sum has one owner whatever
-// the split, so the split never changes a result.
-static int pair_split(er_ctx *c, int nch, bool batched)
-{
-    if (batched && c->S().pair_split > 0) return c->S().pair_split;
-    if (c->S().nslt > 8) return 16;
-    return std::max(1, std::min(16, (4 * 256 + nch - 1) / nch));
-}
-
-// the real-space pairs and the histograms of insertions [n_off, n_off+nch)
-template <int NI> static hipError_t run_chunk_pairs_hist(er_ctx *c, int n_off, int nch)
+// the real-space pairs (workgroups [blk0, blk0 + nblk) of the run; open
+// systems: insertions [n_off, n_off + nch)) and the histograms of
+// insertions [n_off, n_off + nch)
+template <int NI> static hipError_t run_pairs_hist(er_ctx *c, int n_off, int nch, int blk0, int nblk)
 {
     hipError_t e;
     if ((e = join_atoms(c)) != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
-    if (c->prm.boxshp == 0) {  // open system: every molecule, no cell list
-        ProfScope ps(c, ER_K_PAIRS);
-        PairArgs pa{};
-        pa.nmol = c->nummol;
-        pa.nchunk = nch;
-        pa.n_off = n_off;
-        pa.nslt = c->S().nslt;
-        pa.NI = NI;
-        pa.ntype = c->ntype;
-        pa.pp = c->pp;
-        hipLaunchKernelGGL(k_pairs_open, dim3((unsigned)((c->nummol + 255) / 256), (unsigned)nch), dim3(256), 0,
-                           c->stream, pa, c->d_molbeg.p, c->d_posn.p, c->d_ljtype.p, c->d_insposn.p,
-                           c->S().d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
-                           c->d_ljlensq.p, c->S().d_U.p);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    } else {
-        ProfScope ps(c, ER_K_PAIRS);
-        PairArgs pa{};
-        pa.nsplit = pair_split(c, nch, NI > 1);
-        pa.nmol = c->nummol;
-        pa.nchunk = nch;
-        pa.n_off = n_off;
-        pa.nslt = c->S().nslt;
-        pa.NI = NI;
-        pa.ntype = c->ntype;
-        pa.nc = c->nc;
-        pa.pp = c->pp;
-        pa.cell_w = c->cell_w;
-        hipLaunchKernelGGL(k_pairs, dim3(nch * pa.nsplit), dim3(256), 0, c->stream, pa, c->fc, c->d_molbeg.p, c->d_posn.p,
-                           c->d_ljtype.p, c->d_molref.p, c->d_segref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p,
-                           c->d_insposn.p,
-                           c->d_insref.p, c->S().d_sltlj.p, soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p,
-                           c->d_ljlensq.p, c->S().d_U.p);
-        if ((e = hipGetLastError()) != hipSuccess) return e;
-    }
+    const int nslot = (int)c->plan.desc.size();
+    PairArgs pa{};
+    pa.nmol = c->nummol;
+    pa.nchunk = nch;
+    pa.n_off = n_off;
+    pa.NI = NI;
+    pa.ntype = c->ntype;
+    pa.nc = c->nc;
+    pa.pp = c->pp;
+    pa.cell_w = c->cell_w;
     {
-        ProfScope ps(c, ER_K_HIST);
-        if ((e = launch_hist<NI>(c, n_off, nch)) != hipSuccess) return e;
+        ProfScope ps(c, ER_K_PAIRS);
+        if (c->prm.boxshp == 0) {  // open system: every molecule, no cell list; y = insertions of a chunk
+            const int step = std::max(NI, std::min(32768, c->chunk) / NI * NI);
+            for (int s0 = 0; s0 < nch; s0 += step) {
+                pa.n_off = n_off + s0;
+                pa.nchunk = std::min(step, nch - s0);
+                hipLaunchKernelGGL(k_pairs_open, dim3((unsigned)((c->nummol + 255) / 256), (unsigned)pa.nchunk),
+                                   dim3(256), 0, c->stream, pa, nslot, c->d_sd.p, c->d_molbeg.p, c->d_posn.p,
+                                   c->d_ljtype.p, c->d_insposn.p, soln ? c->d_tag.p : (const int *)nullptr,
+                                   c->d_ljene.p, c->d_ljlensq.p, c->d_Ucat.p);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
+        } else if (nblk > 0) {
+            hipLaunchKernelGGL(k_pairs, dim3((unsigned)nblk), dim3(256), 0, c->stream, pa, blk0, nslot, c->d_sd.p,
+                               c->fc, c->d_molbeg.p, c->d_posn.p, c->d_ljtype.p, c->d_molref.p, c->d_segref.p,
+                               c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p, c->d_insref.p,
+                               soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p, c->d_ljlensq.p, c->d_Ucat.p);
+            if ((e = hipGetLastError()) != hipSuccess) return e;
+        }
     }
-    return hipSuccess;
+    ProfScope ps(c, ER_K_HIST);
+    return launch_hist<NI>(c, n_off, nch);
 }
 
-// weights, edens and ecorr of insertions [n_off, n_off+nch) (after their
-// histograms); update_histogram's accumulation part
-// insertion slices of the split-K edens/ecorr sums over nch insertions:
-// slices of kper (a multiple of 4, at least ecorr_kmin) insertions
-static void ecorr_slices(const er_ctx *c, int nch, int &kper, int &nks)
-{
-    kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
-    nks = (nch + kper - 1) / kper;
-}
-
-// the split-K partials (ecorr, edens) and the weights' block partials of an
-// accumulation over nch insertions: the context's buffers, or in an
-// er_run_noalloc call consecutive ranges of the scratch array
-struct AccPartials {
-    double *part, *epart, *wpart;
-};
-
-static size_t noalloc_doubles(const er_ctx *c, int nch)
-{
-    int kper, nks;
-    ecorr_slices(c, nch, kper, nks);
-    const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
-    return (mfma ? (size_t)nks * c->ermax * c->ermax : 0) + (size_t)nks * c->ermax +
-           (size_t)4 * std::max(1, (nch + 255) / 256);
-}
-
-static AccPartials acc_partials(er_ctx *c, int nch)
-{
-    if (!c->noalloc) return {c->d_part.p, c->d_epart.p, c->d_wpart.p};
-    int kper, nks;
-    ecorr_slices(c, nch, kper, nks);
-    const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
-    double *p = c->scratch_dev;
-    AccPartials a{};
-    a.part = p;
-    p += mfma ? (size_t)nks * c->ermax * c->ermax : 0;
-    a.epart = p;
-    p += (size_t)nks * c->ermax;
-    a.wpart = p;
-    return a;
-}
-
-static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
+// weights, edens and ecorr (after the histograms); update_histogram's
+// accumulation part over the insertions `ar` names (every solute of the run,
+// or one insertion range of one solute)
+static hipError_t run_accumulate(er_ctx *c, AccRange ar)
 {
     const bool soln = c->prm.slttype == 1;
-    const AccPartials ap = acc_partials(c, nch);
+    const bool all = ar.nch < 0;
+    const auto &P = c->plan;
+    const int nslot = (int)P.desc.size();
+    const int ns = all ? nslot : 1;  // solutes of this accumulation (grid dimension)
+    const AccPartials ap = acc_partials(c, all);
+    const SlotDesc *sd = c->d_sd.p;
     {
         ProfScope ps(c, ER_K_WEIGHTS);
         WeightArgs wa{};
-        wa.nchunk = nch;
-        wa.n_off = n_off;
         wa.H = (c->prm.cltype == 0) ? 0 : c->H;   // no reciprocal self part for bare Coulomb
         wa.bare = c->prm.cltype == 0;
         wa.slttype = c->prm.slttype;
@@ -1535,103 +1688,139 @@ static hipError_t run_accumulate(er_ctx *c, int n_off, int nch)
         wa.rigid = c->prm.slttype == 2;
         wa.numslv = c->numslv;
         wa.nslice = c->nslice;
-        wa.seq = c->S().seq;
         wa.w_sys = c->w_sys;
         wa.volume = c->fc.volume;
         wa.screen = c->prm.screen;
         wa.qtot = c->qtot;
         wa.temp = c->prm.temp;
-        const int nblk = std::max(1, (nch + 255) / 256);
-        hipLaunchKernelGGL(k_weights_init, dim3(1), dim3(64), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
-                           c->d_resself.p, c->S().d_state.p, c->S().d_flags.p, c->S().d_fstate.p);
-        hipLaunchKernelGGL(k_weights, dim3(nblk), dim3(256), 0, c->stream, wa, c->d_selfpart.p, c->d_selfreal.p,
-                           c->d_resself.p, c->S().wslt_given ? c->S().d_wslt.p : (const float *)nullptr,
-                           c->S().d_state.p, c->S().d_fstate.p, c->S().d_u0.p, c->d_w.p,
+        const int wrow = all ? P.wrow : std::max(1, (ar.nch + 255) / 256);
+        hipLaunchKernelGGL(k_weights_init, dim3(ns), dim3(64), 0, c->stream, wa, ar, sd, c->d_selfpart.p,
+                           c->d_selfreal.p, c->d_resself.p);
+        hipLaunchKernelGGL(k_weights, dim3(wrow, ns), dim3(256), 0, c->stream, wa, ar, sd, c->d_selfpart.p,
+                           c->d_selfreal.p, c->d_resself.p, c->d_u0cat.p, c->d_w.p,
                            soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
                            ap.wpart);
-        hipLaunchKernelGGL(k_weights_fin, dim3(1), dim3(256), 0, c->stream, wa, nblk, ap.wpart, c->S().d_state.p,
-                           c->S().d_u0.p, c->d_w.p, c->d_escrd.p, c->S().d_eself.p, c->S().d_minmax.p,
-                           soln ? c->d_flpart.p : (const double *)nullptr, soln ? c->d_flceng.p : (double *)nullptr,
-                           c->S().d_slnuv.p, c->S().d_errkey.p, c->S().d_warnkey.p);
+        hipLaunchKernelGGL(k_weights_fin, dim3(ns), dim3(256), 0, c->stream, wa, ar, sd, wrow, ap.wpart,
+                           c->d_u0cat.p, c->d_w.p, c->d_escrd.p, soln ? c->d_flpart.p : (const double *)nullptr,
+                           soln ? c->d_flceng.p : (const double *)nullptr);
     }
-    int kper, nks;
-    ecorr_slices(c, nch, kper, nks);
+    const int nz = all ? P.nz : (ar.nch + ar.kper - 1) / ar.kper;
     const bool mfma = c->prm.corrcal == 1 && c->use_mfma;
     {
         ProfScope ps(c, ER_K_ECORR);
         const int nb = (c->ermax + 63) / 64;
-        if (mfma) {
-            hipLaunchKernelGGL(k_ecorr, dim3(nb, nb, nks), dim3(256), 0, c->stream, nch, n_off, c->ermax, kper,
-                               c->d_C.p, c->d_w.p, c->d_dmap.p, ap.part, ap.epart);
-        } else {
-            hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256, nks), dim3(256), 0, c->stream, nch, n_off,
-                               c->ermax, kper, c->d_C.p, c->d_w.p, ap.epart);
-        }
-        const bool full = mfma;
-        hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, full ? c->ermax : 1), dim3(256), 0,
-                           c->stream, c->ermax, nks, ap.part, ap.epart,
-                           full ? c->S().d_ecorr.p : (double *)nullptr, c->S().d_edens.p);
+        if (mfma)
+            hipLaunchKernelGGL(k_ecorr, dim3(nb, nb, nz), dim3(256), 0, c->stream, ar, nslot, sd, c->ermax, c->d_C.p,
+                               c->d_w.p, c->d_dmap.p, ap.part, ap.epart);
+        else
+            hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256, nz), dim3(256), 0, c->stream, ar, nslot, sd,
+                               c->ermax, c->d_C.p, c->d_w.p, ap.epart);
+        hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, mfma ? c->ermax : 1, ns), dim3(256), 0,
+                           c->stream, ar, sd, c->ermax, (int)mfma, ap.part, ap.epart);
         if (c->prm.corrcal == 1 && !c->use_mfma) {
             const int nb16 = (c->ermax + 15) / 16;
-            hipLaunchKernelGGL(k_ecorr_valu, dim3(nb16, nb16), dim3(16, 16), 0, c->stream, nch, n_off, c->ermax,
-                               c->d_C.p, c->d_w.p, c->S().d_ecorr.p);
+            hipLaunchKernelGGL(k_ecorr_valu, dim3(nb16, nb16, ns), dim3(16, 16), 0, c->stream, ar, sd, c->ermax,
+                               c->d_C.p, c->d_w.p);
         }
     }
     return hipGetLastError();
 }
 
-// one slot's insertions of the frame: per-insertion prep, chunks, accumulation
-static int run_slot(er_ctx *c, int mode, bool prep_frame_now)
+// one run over the solutes `todo` of the frame: per-insertion preparation,
+// the reciprocal part in chunks, pairs, histograms and accumulation, each a
+// single launch over every solute (the plain path: one insertion at a time)
+static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
 {
     const bool soln = c->prm.slttype == 1;
-    const int nins = soln ? c->nsltmol : c->S().nins;
-    if (nins >= (1 << 21) || c->nummol >= (1 << 21) - 1)
-        return fail(c, ER_E_UNSUPPORTED, "er_run: at most 2^21-1 insertions and 2^21-2 molecules per run");
-    c->S().seq += 1;
-    const int NI = (mode == ER_BATCHED) ? NI_B : 1;
-    const int chunk = (mode == ER_BATCHED) ? c->chunk : 1;
-    int rc = ensure_run_buffers(c, nins, chunk, NI);
+    const bool batched = mode == ER_BATCHED;
+    for (int k : todo) {
+        const int nins = soln ? c->nsltmol : c->slots[k].nins;
+        if (nins >= (1 << 21) || c->nummol >= (1 << 21) - 1)
+            return fail(c, ER_E_UNSUPPORTED, "er_run: at most 2^21-1 insertions per solute and 2^21-2 molecules");
+    }
+    int rc = build_plan(c, todo, mode);
     if (rc) return rc;
+    const auto &P = c->plan;
+    const int NI = P.NI;
+    const int nslot = (int)P.desc.size();
+    const int chunk = batched ? c->chunk : 1;
+    if ((rc = ensure_run_buffers(c, chunk))) return rc;
+    if ((rc = upload_plan(c))) return rc;
+    const SlotDesc *sd = c->d_sd.p;
     {
         ProfScope ps(c, ER_K_PREP);
-        if (prep_frame_now) HIPCHK(c, prep_frame(c));
-        const size_t nsite = (size_t)nins * c->S().nslt;
-        hipLaunchKernelGGL(k_prep_sites, dim3((unsigned)((nsite + 255) / 256)), dim3(256), 0, c->stream, nins,
-                           c->S().nslt, soln ? (const float *)nullptr : c->S().d_sltpos, c->S().d_sltq.p,
-                           c->d_sitepos, c->d_charge.p, c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr,
-                           c->fc, c->K1, c->K2, c->K3, c->ord, c->d_insposn.p, c->d_insw.p, c->d_insg.p);
+        hipLaunchKernelGGL(k_seq_bump, dim3((nslot + 63) / 64), dim3(64), 0, c->stream, nslot, sd);
         HIPCHK(c, hipGetLastError());
-        hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)((nins + 3) / 4)), dim3(256), 0, c->stream, nins, c->S().nslt,
-                           c->fc, c->prm.screen, (int)(c->prm.cltype == 0), c->d_insposn.p, c->d_insref.p,
+        HIPCHK(c, prep_frame(c));
+        if (!soln)  // the insertions' coordinates, solutes end to end
+            for (int i = 0; i < nslot; ++i) {
+                const Slot &sl = c->slots[P.order[i]];
+                const size_t n = (size_t)P.desc[i].nins * sl.nslt * 3;
+                HIPCHK(c, hipMemcpyAsync(c->d_sltcat.p + (size_t)P.desc[i].site_off * 3, sl.d_sltpos,
+                                         n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+            }
+        hipLaunchKernelGGL(k_prep_sites, dim3((unsigned)((P.nsite + 255) / 256)), dim3(256), 0, c->stream, P.nsite,
+                           nslot, sd, soln ? (const float *)nullptr : c->d_sltcat.p, c->d_sitepos, c->d_charge.p,
+                           c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3,
+                           c->ord, c->d_insposn.p, c->d_insw.p, c->d_insg.p);
+        HIPCHK(c, hipGetLastError());
+        hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)((P.ntot + 3) / 4)), dim3(256), 0, c->stream, P.ntot, nslot,
+                           sd, c->fc, c->prm.screen, (int)(c->prm.cltype == 0), c->d_insposn.p, c->d_insref.p,
                            c->d_selfreal.p, c->d_resself.p);
         HIPCHK(c, hipGetLastError());
-        const size_t nf = (size_t)nins * c->S().nslt * (c->H + c->K2 + c->K3);
-        hipLaunchKernelGGL(k_ins_factors, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, c->stream,
-                           nins * c->S().nslt, c->ord, c->K1, c->K2, c->K3, c->H, c->d_insposn.p, c->d_insw.p,
-                           c->d_insg.p, c->d_tw1.p, c->d_tw2.p, c->d_tw3.p, c->d_Ftab.p);
+        const size_t nf = (size_t)P.nsite * (c->H + c->K2 + c->K3);
+        hipLaunchKernelGGL(k_ins_factors, dim3((unsigned)((nf + 255) / 256)), dim3(256), 0, c->stream, P.nsite,
+                           c->ord, c->K1, c->K2, c->K3, c->H, c->d_insposn.p, c->d_insw.p, c->d_insg.p, c->d_tw1.p,
+                           c->d_tw2.p, c->d_tw3.p, c->d_Ftab.p);
         HIPCHK(c, hipGetLastError());
-        const size_t nC = (size_t)nins * c->ermax;
+        const size_t nC = (size_t)P.ntot * c->ermax;
         hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((nC + 255) / 256)), dim3(256), 0, c->stream, nC, c->d_C.p);
         HIPCHK(c, hipGetLastError());
+        if (c->prm.cltype == 0) {
+            // bare Coulomb (engproc.F90:761-769, 819-826): no reciprocal
+            // part; U starts at zero for the pair sums
+            const size_t n = (size_t)P.ntot * c->nummol;
+            hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, c->stream, n,
+                               reinterpret_cast<uint32_t *>(c->d_Ucat.p));
+            HIPCHK(c, hipGetLastError());
+        }
     }
-    if (mode == ER_BATCHED) {
-        for (int n0 = 0; n0 < nins; n0 += chunk) HIPCHK(c, run_chunk<NI_B>(c, n0, std::min(chunk, nins - n0)));
-        HIPCHK(c, run_accumulate(c, 0, nins));
+    if (batched) {
+        if (c->prm.cltype != 0)
+            for (int n0 = 0; n0 < P.ntot; n0 += chunk) HIPCHK(c, run_chunk<NI_B>(c, n0, std::min(chunk, P.ntot - n0)));
+        HIPCHK(c, run_pairs_hist<NI_B>(c, 0, P.ntot, 0, P.pair_blocks));
+        HIPCHK(c, run_accumulate(c, AccRange{0, 0, -1, 0}));
     } else {
         // one insertion per launch sequence, synchronised like the reference's
         // per-insertion host loop
-        for (int n0 = 0; n0 < nins; ++n0) {
-            HIPCHK(c, run_chunk<1>(c, n0, 1));
-            HIPCHK(c, run_accumulate(c, n0, 1));
-            HIPCHK(c, hipStreamSynchronize(c->stream));
+        int kper1, nks1;
+        ecorr_slices(c, 1, kper1, nks1);
+        for (int i = 0; i < nslot; ++i) {
+            const SlotDesc &d = P.desc[i];
+            for (int j = 0; j < d.nins; ++j) {
+                const int n = d.ins_off + j;
+                if (c->prm.cltype != 0) HIPCHK(c, run_chunk<1>(c, n, 1));
+                HIPCHK(c, run_pairs_hist<1>(c, n, 1, d.pair_blk + j * d.pair_split, d.pair_split));
+                HIPCHK(c, run_accumulate(c, AccRange{i, j, 1, kper1}));
+                HIPCHK(c, hipStreamSynchronize(c->stream));
+            }
         }
     }
     HIPCHK(c, join_atoms(c));  // a run without chunks still orders the next upload after the atom work
-    hipLaunchKernelGGL(k_halt_capture, dim3(1), dim3(64), 0, c->stream, c->S().seq, NI, c->nummol,
-                       c->S().d_errkey.p, c->S().d_warnkey.p, c->S().d_U.p, c->S().d_u0.p, c->S().d_halt.p);
+    if (c->capturing) ++c->capture_runs;
+    hipLaunchKernelGGL(k_halt_capture, dim3(nslot), dim3(64), 0, c->stream, sd, NI, c->nummol, c->d_Ucat.p,
+                       c->d_u0cat.p);
     HIPCHK(c, hipGetLastError());
-    c->S().last_NI = NI;
-    c->S().last_nins = nins;
+    for (int i = 0; i < nslot; ++i) {
+        Slot &sl = c->slots[P.order[i]];
+        sl.U_view = c->d_Ucat.p + (size_t)P.desc[i].ins_off * c->nummol;
+        sl.u0_view = c->d_u0cat.p + P.desc[i].ins_off;
+        sl.last_NI = NI;
+        sl.last_nins = P.desc[i].nins;
+    }
+    // solutes outside this run keep no energies (their rows were reused)
+    for (size_t k = 0; k < c->slots.size(); ++k)
+        if (std::find(todo.begin(), todo.end(), (int)k) == todo.end()) c->slots[k].last_nins = 0;
     return ER_OK;
 }
 
@@ -1653,23 +1842,12 @@ static int run_plan(er_ctx *c, const char *who, std::vector<int> &todo)
     return ER_OK;
 }
 
-// every solute of the frame in turn; the frame's preparation is shared
-static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
-{
-    const int keep = c->cur;
-    int rc = ER_OK;
-    for (size_t t = 0; t < todo.size() && rc == ER_OK; ++t) {
-        c->cur = todo[t];
-        rc = run_slot(c, mode, t == 0);
-    }
-    c->cur = keep;
-    return rc;
-}
-
 extern "C" int er_run(er_ctx *c, int32_t mode)
 {
     if (!c) return ER_E_ARG;
     if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run: mode");
+    if (c->capturing && mode != ER_BATCHED)
+        return fail(c, ER_E_STATE, "er_run: a graph records ER_BATCHED runs only (ER_PLAIN waits per insertion)");
     std::vector<int> todo;
     const int rc = run_plan(c, "er_run", todo);
     if (rc) return rc;
@@ -1683,20 +1861,11 @@ extern "C" int er_run_noalloc(er_ctx *c, int32_t mode)
 {
     if (!c) return ER_E_ARG;
     if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run_noalloc: mode");
+    if (c->capturing) return fail(c, ER_E_STATE, "er_run_noalloc: not while a graph is recorded");
     if (!c->scratch_dev) return fail(c, ER_E_STATE, "er_run_noalloc: no scratch address for this device");
     std::vector<int> todo;
     int rc = run_plan(c, "er_run_noalloc", todo);
     if (rc) return rc;
-    const size_t cap = ER_NOALLOC_SCRATCH_BYTES / sizeof(double);
-    for (int k : todo) {
-        const int nins = (c->prm.slttype == 1) ? c->nsltmol : c->slots[k].nins;
-        const size_t need = noalloc_doubles(c, mode == ER_BATCHED ? nins : 1);
-        if (need > cap)
-            return fail(c, ER_E_UNSUPPORTED,
-                        "er_run_noalloc: %d insertions over %d bins need %zu bytes of partial sums, above the "
-                        "%zu-byte scratch (use er_run)",
-                        nins, c->ermax, need * sizeof(double), (size_t)ER_NOALLOC_SCRATCH_BYTES);
-    }
     const bool prof = c->prof;
     c->prof = false;
     c->noalloc = true;
@@ -1748,6 +1917,7 @@ static void decode_halt(er_ctx *c, unsigned long long key, bool warn)
 extern "C" int er_sync(er_ctx *c)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_sync");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const int keep = c->cur;
     for (int k = 0; k < (int)c->slots.size(); ++k) {
@@ -1778,17 +1948,18 @@ extern "C" int er_sync(er_ctx *c)
 extern "C" int er_get_uvengy(er_ctx *c, float *uvengy, float *uvengy0)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_get_uvengy");
     if (c->S().last_nins <= 0) return fail(c, ER_E_STATE, "er_get_uvengy: no run yet");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     const int nins = c->S().last_nins, NI = c->S().last_NI, nmol = c->nummol;
     if (uvengy) {
         const size_t ntile = (size_t)(nins + NI - 1) / NI;
         std::vector<float> u(ntile * NI * nmol);
-        HIPCHK(c, hipMemcpy(u.data(), c->S().d_U.p, u.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, hipMemcpy(u.data(), c->S().U_view, u.size() * sizeof(float), hipMemcpyDeviceToHost));
         for (int n = 0; n < nins; ++n)
             for (int j = 0; j < nmol; ++j) uvengy[(size_t)n * nmol + j] = u[((size_t)(n / NI) * nmol + j) * NI + n % NI];
     }
-    if (uvengy0) HIPCHK(c, hipMemcpy(uvengy0, c->S().d_u0.p, nins * sizeof(float), hipMemcpyDeviceToHost));
+    if (uvengy0) HIPCHK(c, hipMemcpy(uvengy0, c->S().u0_view, nins * sizeof(float), hipMemcpyDeviceToHost));
     return ER_OK;
 }
 
@@ -1796,6 +1967,7 @@ extern "C" int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, 
                         float *maxuv, double *slnuv)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_fetch");
     if (!c->have_mesh) return fail(c, ER_E_STATE, "er_fetch: no mesh");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     if (edens) HIPCHK(c, hipMemcpy(edens, c->S().d_edens.p, c->ermax * sizeof(double), hipMemcpyDeviceToHost));
@@ -1824,6 +1996,7 @@ extern "C" int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, 
 extern "C" int er_get_flceng(er_ctx *c, double *flceng)
 {
     if (!c || !flceng) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_get_flceng");
     if (c->prm.slttype != 1 || c->S().last_nins <= 0) return fail(c, ER_E_STATE, "er_get_flceng: no soln run");
     HIPCHK(c, hipStreamSynchronize(c->stream));
     HIPCHK(c, hipMemcpy(flceng, c->d_flceng.p, (size_t)c->S().last_nins * c->numslv * sizeof(double), hipMemcpyDeviceToHost));
@@ -1833,6 +2006,7 @@ extern "C" int er_get_flceng(er_ctx *c, double *flceng)
 extern "C" int er_clear(er_ctx *c)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_clear");
     if (!c->have_mesh) return fail(c, ER_E_STATE, "er_clear: no mesh");
     for (Slot &sl : c->slots) {
         const int rc = slot_clear(c, sl);
@@ -1860,6 +2034,7 @@ extern "C" const char *er_error_string(er_ctx *c)
 extern "C" int er_profile(er_ctx *c, int32_t enable)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_profile");
     prof_collect(c);
     c->prof = enable != 0;
     return ER_OK;
@@ -1868,6 +2043,7 @@ extern "C" int er_profile(er_ctx *c, int32_t enable)
 extern "C" int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int64_t *launches)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_profile_read");
     if (kernel < 0 || kernel >= ER_K_COUNT) return fail(c, ER_E_ARG, "er_profile_read: kernel id");
     prof_collect(c);
     if (total_ms) *total_ms = c->prof_ms[kernel];
@@ -1935,20 +2111,24 @@ constexpr double TUNE_MARGIN = 0.02;
 // current shapes are always among the candidates.
 static int autotune_runs(er_ctx *c, const std::vector<int> &todo, int reps)
 {
-    int maxins = 1;
-    for (int k : todo) maxins = std::max(maxins, c->prm.slttype == 1 ? c->nsltmol : c->slots[k].nins);
-    // plane spectra and potentials of a chunk; candidates beyond 6 GiB of
+    // the run's insertions, every solute end to end (each padded to NI_B)
+    int ntot = 0;
+    for (int k : todo) ntot += ((c->prm.slttype == 1 ? c->nsltmol : c->slots[k].nins) + NI_B - 1) / NI_B * NI_B;
+    // plane spectra and potentials of a chunk; candidates beyond 40 GiB of
     // them are skipped unless the current chunk already takes more.  The
     // histogram's slicing follows the chunk, so a chunk beyond the
     // insertions still differs from a smaller one; past the larger of the
-    // current chunk and the insertions nothing changes.
+    // current chunk and the run's insertions nothing changes.  The whole run
+    // in one chunk is a candidate (one launch per kernel class).
     const int cur_chunk = c->chunk;
-    const size_t per_ins = c->Gc * sizeof(float2) + c->G * sizeof(float);
-    const size_t cap = std::max<size_t>((size_t)6 << 30, (size_t)cur_chunk * per_ins);
-    const int reach = std::max(cur_chunk, (maxins + NI_B - 1) / NI_B * NI_B);
+    const size_t per_ins = c->Gc * sizeof(float2) + c->G * sizeof(float) + (size_t)c->numatm * sizeof(float);
+    const size_t cap = std::max<size_t>((size_t)40 << 30, (size_t)cur_chunk * per_ins);
+    const int reach = std::max(cur_chunk, ntot);
     std::vector<int> chunks = {cur_chunk};
-    for (int ch : {256, 512, 1024, 2048, 4096})
-        if (ch != cur_chunk && ch <= reach && (size_t)ch * per_ins <= cap) chunks.push_back(ch);
+    for (int ch : {256, 512, 1024, 2048, 4096, 8192, 16384, ntot})
+        if (ch != cur_chunk && ch <= reach && (size_t)ch * per_ins <= cap &&
+            std::find(chunks.begin(), chunks.end(), ch) == chunks.end())
+            chunks.push_back(ch);
     EventPair ev;
     HIPCHK(c, hipEventCreate(&ev.a));
     HIPCHK(c, hipEventCreate(&ev.b));
@@ -2007,11 +2187,12 @@ static int autotune_runs(er_ctx *c, const std::vector<int> &todo, int reps)
             if (sp != c->S().pair_split) splits.push_back(sp);
         for (size_t i = 0; i < splits.size(); ++i) {
             c->S().pair_split = splits[i];
-            if ((rc = run_slot(c, ER_BATCHED, true))) return rc;
+            const std::vector<int> one = {k};
+            if ((rc = run_slots(c, one, ER_BATCHED))) return rc;
             prof_collect(c);
             c->prof_ms[ER_K_PAIRS] = 0.0;
             for (int r = 0; r < reps; ++r)
-                if ((rc = run_slot(c, ER_BATCHED, true))) return rc;
+                if ((rc = run_slots(c, one, ER_BATCHED))) return rc;
             prof_collect(c);
             if (i == 0 || c->prof_ms[ER_K_PAIRS] < best_ms * (1.0 - TUNE_MARGIN)) {
                 best_ms = c->prof_ms[ER_K_PAIRS];
@@ -2026,6 +2207,7 @@ static int autotune_runs(er_ctx *c, const std::vector<int> &todo, int reps)
 extern "C" int er_autotune(er_ctx *c, int32_t reps)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_autotune");
     if (reps < 0 || reps > 100) return fail(c, ER_E_ARG, "er_autotune: reps must lie in 0..100");
     if (reps == 0) reps = 3;
     std::vector<int> todo;
@@ -2054,26 +2236,33 @@ extern "C" int er_autotune(er_ctx *c, int32_t reps)
         for (size_t k = 0; k < c->slots.size(); ++k) c->slots[k].pair_split = split_keep[k];
     }
     (void)hipStreamSynchronize(c->stream);
+    // spectra and potentials sized for a larger candidate chunk than the one
+    // kept are given back
+    {
+        const size_t ntile = (size_t)(c->chunk + NI_B - 1) / NI_B;
+        ++c->alloc_gen;
+        if (c->d_T.n > (size_t)c->chunk * c->Gc) c->d_T.release();
+        if (c->d_phi.n > ntile * NI_B * c->G) c->d_phi.release();
+        if (c->d_Eat.n > ntile * NI_B * (size_t)c->numatm) c->d_Eat.release();
+    }
+    // one run with the kept shapes sizes the buffers and the device
+    // descriptors for them, so a graph can be recorded right after tuning
+    // (its accumulation is undone below with the tuning runs')
+    if (rc == ER_OK) rc = run_slots(c, todo, ER_BATCHED);
+    (void)hipStreamSynchronize(c->stream);
     prof_collect(c);
     std::memcpy(c->prof_ms, ms_keep, sizeof ms_keep);
     std::memcpy(c->prof_n, n_keep, sizeof n_keep);
     c->prof = prof_keep;
     for (size_t k = 0; k < c->slots.size(); ++k) HIPCHK(c, slot_snap(c->slots[k], snap[k], c->stream, false));
     HIPCHK(c, hipStreamSynchronize(c->stream));
-    // spectra and potentials sized for a larger candidate chunk than the one
-    // kept are given back (the next run sizes them for the kept chunk)
-    {
-        const size_t ntile = (size_t)(c->chunk + NI_B - 1) / NI_B;
-        if (c->d_T.n > (size_t)c->chunk * c->Gc) c->d_T.release();
-        if (c->d_phi.n > ntile * NI_B * c->G) c->d_phi.release();
-        if (c->d_Eat.n > ntile * NI_B * (size_t)c->numatm) c->d_Eat.release();
-    }
     return rc;
 }
 
 extern "C" int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split, int32_t hist_slices)
 {
     if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_tuning");
     if (chunk != 0 && (chunk < NI_B || chunk > 65536 || chunk % NI_B != 0))
         return fail(c, ER_E_ARG, "er_set_tuning: chunk must be 0 or a multiple of %d in %d..65536", NI_B, NI_B);
     if (pair_split < 0 || pair_split > 64) return fail(c, ER_E_ARG, "er_set_tuning: pair_split must lie in 0..64");
@@ -2093,9 +2282,214 @@ extern "C" int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split, int
     return ER_OK;
 }
 
+// ---- HIP graphs of whole runs ----
+//
+// er_graph_begin .. er_graph_end records the context's work (frames set on
+// the device, their insertions, batched runs) into one HIP graph that
+// er_graph_launch replays with a single launch.  The recorded kernels keep
+// the device pointers, cells, weights and launch shapes of the recording;
+// the caller refills the same device buffers between replays.  The host-side
+// state a replay leaves behind (frame, cell, per-solute energies) is the one
+// the recording left, restored at each launch.
+
+struct HostRunState {
+    bool have_frame, frame_dirty, cell_changed, have_prev_cell;
+    float cell[9], prev_cell[9];
+    FrameConst fc;
+    double invcl_d[9], volume_d;
+    float w_sys;
+    const float *d_sitepos;
+    int nc, ncell;
+    float cell_w;
+    struct SlotState {
+        bool have_ins, wslt_given;
+        int nins, last_NI, last_nins;
+        const float *d_sltpos, *U_view, *u0_view;
+    };
+    std::vector<SlotState> slots;
+};
+
+struct er_ctx::GraphRec {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    unsigned long long gen = 0;
+    std::vector<SlotDesc> sd;
+    bool engfac_at_start = false;  // the graph computes the influence function before its first run
+    bool have_prev_begin = false;
+    float prev_begin[9] = {0};
+    HostRunState end;
+    size_t nslots = 0;
+};
+
+static void save_state(const er_ctx *c, HostRunState &h)
+{
+    h.have_frame = c->have_frame;
+    h.frame_dirty = c->frame_dirty;
+    h.cell_changed = c->cell_changed;
+    h.have_prev_cell = c->have_prev_cell;
+    std::memcpy(h.cell, c->cell, sizeof h.cell);
+    std::memcpy(h.prev_cell, c->prev_cell, sizeof h.prev_cell);
+    h.fc = c->fc;
+    std::memcpy(h.invcl_d, c->invcl_d, sizeof h.invcl_d);
+    h.volume_d = c->volume_d;
+    h.w_sys = c->w_sys;
+    h.d_sitepos = c->d_sitepos;
+    h.nc = c->nc;
+    h.ncell = c->ncell;
+    h.cell_w = c->cell_w;
+    h.slots.resize(c->slots.size());
+    for (size_t k = 0; k < c->slots.size(); ++k) {
+        const Slot &sl = c->slots[k];
+        h.slots[k] = {sl.have_ins, sl.wslt_given, sl.nins, sl.last_NI, sl.last_nins, sl.d_sltpos, sl.U_view,
+                      sl.u0_view};
+    }
+}
+
+static void restore_state(er_ctx *c, const HostRunState &h)
+{
+    c->have_frame = h.have_frame;
+    c->frame_dirty = h.frame_dirty;
+    c->cell_changed = h.cell_changed;
+    c->have_prev_cell = h.have_prev_cell;
+    std::memcpy(c->cell, h.cell, sizeof h.cell);
+    std::memcpy(c->prev_cell, h.prev_cell, sizeof h.prev_cell);
+    c->fc = h.fc;
+    std::memcpy(c->invcl_d, h.invcl_d, sizeof h.invcl_d);
+    c->volume_d = h.volume_d;
+    c->w_sys = h.w_sys;
+    c->d_sitepos = h.d_sitepos;
+    c->nc = h.nc;
+    c->ncell = h.ncell;
+    c->cell_w = h.cell_w;
+    for (size_t k = 0; k < c->slots.size() && k < h.slots.size(); ++k) {
+        Slot &sl = c->slots[k];
+        const auto &s = h.slots[k];
+        sl.have_ins = s.have_ins;
+        sl.wslt_given = s.wslt_given;
+        sl.nins = s.nins;
+        sl.last_NI = s.last_NI;
+        sl.last_nins = s.last_nins;
+        sl.d_sltpos = s.d_sltpos;
+        sl.U_view = s.U_view;
+        sl.u0_view = s.u0_view;
+    }
+}
+
+static void graph_free(er_ctx::GraphRec *g)
+{
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+extern "C" int er_graph_begin(er_ctx *c)
+{
+    if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_graph_begin");
+    if (!c->have_mesh) return fail(c, ER_E_STATE, "er_graph_begin: er_set_mesh first");
+    // the frame work of an earlier eager run has joined the main stream
+    HIPCHK(c, join_atoms(c));
+    c->prof_before_capture = c->prof;
+    c->prof = false;
+    c->cap_have_prev = c->have_prev_cell;
+    std::memcpy(c->cap_prev_cell, c->prev_cell, sizeof c->cap_prev_cell);
+    c->cap_gen = c->alloc_gen;
+    c->capture_bad = false;
+    c->capture_engfac = false;
+    c->capture_runs = 0;
+    const hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) {
+        c->prof = c->prof_before_capture;
+        return fail(c, ER_E_HIP, "er_graph_begin: hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    c->capturing = true;
+    return ER_OK;
+}
+
+extern "C" int er_graph_end(er_ctx *c, int32_t *graph)
+{
+    if (!c || !graph) return ER_E_ARG;
+    *graph = 0;
+    if (!c->capturing) return fail(c, ER_E_STATE, "er_graph_end: no graph is being recorded");
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->capturing = false;
+    c->prof = c->prof_before_capture;
+    auto drop = [&](int code, const char *why) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(c, code, "er_graph_end: %s", why);
+    };
+    if (e != hipSuccess) return drop(ER_E_HIP, hipGetErrorString(e));
+    if (c->capture_bad) return drop(ER_E_STATE, "a call failed while recording; the graph is discarded");
+    if (c->capture_runs == 0) return drop(ER_E_STATE, "no er_run was recorded");
+    if (c->alloc_gen != c->cap_gen) return drop(ER_E_STATE, "device buffers changed while recording");
+    auto *r = new er_ctx::GraphRec();
+    r->graph = g;
+    const hipError_t ei = hipGraphInstantiate(&r->exec, g, nullptr, nullptr, 0);
+    if (ei != hipSuccess) {
+        graph_free(r);
+        return fail(c, ER_E_HIP, "er_graph_end: hipGraphInstantiate: %s", hipGetErrorString(ei));
+    }
+    r->gen = c->alloc_gen;
+    r->sd = c->sd_up;
+    r->engfac_at_start = c->capture_engfac;
+    r->have_prev_begin = c->cap_have_prev;
+    std::memcpy(r->prev_begin, c->cap_prev_cell, sizeof r->prev_begin);
+    save_state(c, r->end);
+    r->nslots = c->slots.size();
+    // the recording launched nothing: the state it leaves is what a replay leaves
+    c->graphs.push_back(r);
+    *graph = (int32_t)c->graphs.size();
+    return ER_OK;
+}
+
+extern "C" int er_graph_launch(er_ctx *c, int32_t graph)
+{
+    if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_graph_launch");
+    if (graph < 1 || graph > (int32_t)c->graphs.size() || !c->graphs[graph - 1])
+        return fail(c, ER_E_ARG, "er_graph_launch: no graph %d", graph);
+    er_ctx::GraphRec *r = c->graphs[graph - 1];
+    if (r->gen != c->alloc_gen || r->nslots != c->slots.size())
+        return fail(c, ER_E_STATE, "er_graph_launch: device buffers were reallocated since graph %d was recorded; "
+                                   "record it again", graph);
+    if (r->sd.size() != c->sd_up.size() ||
+        std::memcmp(r->sd.data(), c->sd_up.data(), r->sd.size() * sizeof(SlotDesc)) != 0)
+        return fail(c, ER_E_STATE, "er_graph_launch: the solutes or launch shapes changed since graph %d was "
+                                   "recorded; record it again", graph);
+    if (!r->engfac_at_start &&
+        (c->have_prev_cell != r->have_prev_begin ||
+         (r->have_prev_begin && std::memcmp(c->prev_cell, r->prev_begin, sizeof r->prev_begin) != 0)))
+        return fail(c, ER_E_STATE, "er_graph_launch: graph %d was recorded for another cell's influence function",
+                    graph);
+    HIPCHK(c, join_atoms(c));
+    HIPCHK(c, hipGraphLaunch(r->exec, c->stream));
+    restore_state(c, r->end);
+    return ER_OK;
+}
+
+extern "C" int er_graph_destroy(er_ctx *c, int32_t graph)
+{
+    if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_graph_destroy");
+    if (graph < 1 || graph > (int32_t)c->graphs.size() || !c->graphs[graph - 1])
+        return fail(c, ER_E_ARG, "er_graph_destroy: no graph %d", graph);
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    graph_free(c->graphs[graph - 1]);
+    c->graphs[graph - 1] = nullptr;
+    return ER_OK;
+}
+
 extern "C" void er_destroy(er_ctx *c)
 {
     if (!c) return;
+    if (c->capturing) {  // a recording left open is discarded
+        hipGraph_t g = nullptr;
+        (void)hipStreamEndCapture(c->stream, &g);
+        if (g) (void)hipGraphDestroy(g);
+        c->capturing = false;
+    }
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     prof_collect(c);
     for (hipEvent_t e : c->prof_pool) (void)hipEventDestroy(e);
@@ -2116,5 +2510,6 @@ extern "C" void er_destroy(er_ctx *c)
     if (c->stream2) (void)hipStreamDestroy(c->stream2);
     if (c->ev_in) (void)hipEventDestroy(c->ev_in);
     if (c->ev_atoms) (void)hipEventDestroy(c->ev_atoms);
+    for (auto *g : c->graphs) graph_free(g);
     delete c;
 }

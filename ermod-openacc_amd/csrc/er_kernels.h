@@ -41,6 +41,63 @@ struct PairParams {
     float repA, repB, repC, attA, attB, attC;
 };
 
+// One solute of a run, device-resident.  A run lays the insertions of all its
+// solutes end to end (the run's insertion space): solute k owns insertions
+// [ins_off, ins_off + nins), ins_off a multiple of the insertion interleave
+// so that no tile of interleaved insertions spans two solutes; the
+// insertions between nins and the next solute's ins_off are padding (zero
+// spectra, never binned).  Their sites lie end to end likewise (site_off).
+// The k_pairs workgroups of solute k start at pair_blk (pair_split per
+// insertion) and its k_ecorr insertion slices at slice_off.  The pointers
+// name the solute's own inputs and block accumulators.
+struct SlotDesc {
+    int nslt, ins_off, nins, site_off;
+    int pair_split, pair_blk;
+    int kper, nks, slice_off;
+    float qslt;
+    const float *sltq;             // [nslt] charges (refs)
+    const int *sltlj;              // [nslt] LJ types, 0-based (refs)
+    const float *wslt;             // [nins] stat_weight_solute, or null
+    unsigned int *seq;             // runs of this solute (halt keys order runs first)
+    double *state;                 // engnorm, engsmpl, avslf, voffset
+    int *flags;
+    float *fstate;
+    unsigned int *minmax;          // minuv/maxuv as order keys
+    double *edens, *ecorr, *eself, *slnuv;
+    unsigned long long *errkey, *warnkey, *halt;
+};
+
+// the solute whose range holds x: the largest k with key(k) <= x (keys
+// ascending in k; slot 0 when x lies before every range)
+template <class F> __device__ __forceinline__ int slot_find(int nslot, int x, F key)
+{
+    int lo = 0, hi = nslot - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (key(mid) <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+#define ER_SLOT_OF_INS(sd, nslot, n) slot_find((nslot), (n), [&](int m_) { return (sd)[m_].ins_off; })
+
+// A pointer read from a descriptor is generic to the compiler, so its
+// accesses would be flat instructions (which also count against the LDS
+// wait counter).  Every descriptor pointer addresses device memory: these
+// helpers access it as such.
+template <class T> __device__ __forceinline__ __attribute__((address_space(1))) T *gptr(T *p)
+{
+    return (__attribute__((address_space(1))) T *)p;
+}
+template <class T> __device__ __forceinline__ void gatomic_min(T *p, T v)
+{
+    __hip_atomic_fetch_min(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T> __device__ __forceinline__ void gatomic_max(T *p, T v)
+{
+    __hip_atomic_fetch_max(gptr(p), v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ int pmod(int a, int K)
 {
     int r = a % K;
@@ -183,30 +240,34 @@ __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const floa
 // spline; k_prep_ins then adds the reach radius and the real-space self term
 // realcal_self[_refs] (realcal.F90:669-771).  For soln the sites come from
 // the frame.
-__global__ void k_prep_sites(int nins, int nslt, const float *__restrict__ sltpos, const float *__restrict__ sltq,
+__global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ sd, const float *__restrict__ sltpos,
                              const float *__restrict__ frame_pos, const float *__restrict__ charge,
                              const int *__restrict__ molbeg, const int *__restrict__ tag, FrameConst fc, int K1,
                              int K2, int K3, int ord, float4 *__restrict__ insposn, float *__restrict__ insw,
                              int4 *__restrict__ insg)
 {
-    // one thread per (insertion, site): spline weights, grid base, wrapped position
-    const size_t is = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (is >= (size_t)nins * nslt) return;
-    const int n = (int)(is / nslt), s = (int)(is % nslt);
+    // one thread per site of the run (every solute's insertions end to end):
+    // spline weights, grid base, wrapped position
+    const int is = blockIdx.x * blockDim.x + threadIdx.x;
+    if (is >= nsite) return;
+    const int k = slot_find(nslot, is, [&](int m) { return sd[m].site_off; });
+    const int nslt = sd[k].nslt;
+    const int local = is - sd[k].site_off;
+    const int n = local / nslt, s = local % nslt;
     float x, y, z, q;
-    if (tag) {
+    if (tag) {  // soln: the solute molecules' sites from the frame
         const size_t a = (size_t)molbeg[tag[n]] + s;
         x = frame_pos[3 * a]; y = frame_pos[3 * a + 1]; z = frame_pos[3 * a + 2];
         q = charge[a];
-    } else {
-        const size_t o = is * 3;
+    } else {  // refs: the run's insertion coordinates, solutes end to end
+        const size_t o = (size_t)is * 3;
         x = sltpos[o]; y = sltpos[o + 1]; z = sltpos[o + 2];
-        q = sltq[s];
+        q = sd[k].sltq[s];
     }
     float w[3 * ER_MAXORD];
     int4 g;
     site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
-    for (int k = 0; k < 3 * ord; ++k) insw[is * 3 * ord + k] = w[k];
+    for (int c = 0; c < 3 * ord; ++c) insw[(size_t)is * 3 * ord + c] = w[c];
     insg[is] = g;
     wrap_site(fc, x, y, z);
     insposn[is] = make_float4(x, y, z, q);
@@ -214,14 +275,26 @@ __global__ void k_prep_sites(int nins, int nslt, const float *__restrict__ sltpo
 
 // One wave per insertion (after k_prep_sites): reference site and reach
 // radius, real-space self energy and residual self term.
-__global__ void __launch_bounds__(256) k_prep_ins(int nins, int nslt, FrameConst fc, float screen, int bare,
+__global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const SlotDesc *__restrict__ sd,
+                                                  FrameConst fc, float screen, int bare,
                                                   const float4 *__restrict__ insposn, float4 *__restrict__ insref,
                                                   float *__restrict__ selfreal, float *__restrict__ resself)
 {
     const int lane = threadIdx.x & 63;
     const int n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-    if (n >= nins) return;
-    const float4 *P = insposn + (size_t)n * nslt;
+    if (n >= ntot) return;
+    const int k = ER_SLOT_OF_INS(sd, nslot, n);
+    const int local = n - sd[k].ins_off;
+    if (local >= sd[k].nins) {  // padding of the solute's insertion range
+        if (lane == 0) {
+            insref[n] = make_float4(0.f, 0.f, 0.f, 0.f);
+            selfreal[n] = 0.0f;
+            resself[n] = 0.0f;
+        }
+        return;
+    }
+    const int nslt = sd[k].nslt;
+    const float4 *P = insposn + sd[k].site_off + (size_t)local * nslt;
     const float4 c0 = P[0];
     float R = 0.0f;
     for (int s = 1 + lane; s < nslt; s += 64) {
@@ -568,7 +641,8 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
 //   2. FP64 half-plane self-energy partial and the engfac multiply.
 //   3. inverse DFT along axes 3 and 2; T[nl][m1][x2][x3] for stage B.
 template <int K2, int K3, int R1, int R2, int C1, int C2, int NT, bool SPARSE>
-__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, int n_off,
+__global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__restrict__ sd, int ord, int K1,
+                                                    int n_off,
                                                     const float2 *__restrict__ Ftab,
                                                     const float *__restrict__ insw, const int4 *__restrict__ insg,
                                                     const float *__restrict__ engfac,
@@ -593,6 +667,17 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
     const int nl = blockIdx.y;
     const int n = n_off + nl;
     const int tid = threadIdx.x;
+    const int ks = ER_SLOT_OF_INS(sd, nslot, n);
+    const int local = n - sd[ks].ins_off;
+    if (local >= sd[ks].nins) {
+        // padding insertion: a zero plane and no self energy
+        float2 *Tp = T + ((size_t)nl * H + m1) * (size_t)NE;
+        for (int e = tid; e < NE; e += NT) Tp[e] = make_float2(0.f, 0.f);
+        if (tid == 0) selfpart[(size_t)n * H + m1] = 0.0;
+        return;
+    }
+    const int nslt = sd[ks].nslt;
+    const size_t rowb = (size_t)sd[ks].site_off + (size_t)local * nslt;  // the insertion's first site row
     for (int k = tid; k < K2; k += NT) t2[k] = tw2[k];
     for (int k = tid; k < K3; k += NT) t3[k] = tw3[k];
     const int per = H + K2 + K3;
@@ -614,7 +699,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
             const int ns = min(ER_SCH, nslt - s0);
             for (int f = tid; f < ns * (K2 + K3); f += NT) {
                 const int s = f / (K2 + K3), r = f % (K2 + K3);
-                const float2 *Fs = Ftab + ((size_t)n * nslt + s0 + s) * per;
+                const float2 *Fs = Ftab + (rowb + s0 + s) * per;
                 if (r < K2) F2[s * K2 + r] = erfft::cmul(Fs[m1], Fs[H + r]);
                 else F3[s * K3 + (r - K2)] = Fs[H + r];
             }
@@ -660,7 +745,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslt, int ord, int K1, in
         if (tid < ord * ord) {
             const int c2 = tid / ord, c3 = tid % ord;
             for (int s = 0; s < nslt; ++s) {
-                const size_t site = (size_t)n * nslt + s;
+                const size_t site = rowb + s;
                 const int4 g = insg[site];
                 const float *w = insw + site * 3 * ord;
                 const float2 A = Ftab[site * per + m1];
@@ -936,8 +1021,9 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
                                                 const int *__restrict__ molbeg, const float4 *__restrict__ posn,
                                                 const float *__restrict__ splw, const int4 *__restrict__ splg,
                                                 const float *__restrict__ molq, const float *__restrict__ phi,
-                                                const int *__restrict__ tag, float qslt_refs, float screen,
-                                                float volume, float *__restrict__ U)
+                                                const int *__restrict__ tag, int nslot,
+                                                const SlotDesc *__restrict__ sd, float screen, float volume,
+                                                float *__restrict__ U)
 {
     constexpr int MPW = 64 / LPM;
     // XCD-aware remap: workgroup b runs on XCD b % nxcd; give each XCD a
@@ -951,6 +1037,8 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
     const int pos = (gx * (blockDim.x >> 6) + wave) * MPW + lane / LPM;
     const int nl = gy * LPM + lane % LPM;
     if (pos >= nmol || nl >= nchunk) return;
+    const int ks = ER_SLOT_OF_INS(sd, nslot, n_off + nl);
+    if (n_off + nl - sd[ks].ins_off >= sd[ks].nins) return;  // padding of the solute's insertion range
     const int mol = perm[pos];
     const int tile = nl / NI, i = nl % NI;
     const size_t G = (size_t)K1 * K2 * K3;
@@ -988,7 +1076,7 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
         acc += __fmul_rn(q, ea);
     }
     const int n = n_off + nl;
-    float qs = qslt_refs;
+    float qs = tag ? 0.0f : sd[ks].qslt;
     if (tag) {
         if (mol == tag[n]) acc = 0.0f;
         qs = molq[tag[n]];
@@ -1473,8 +1561,8 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
 template <int NI>
 __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const int *__restrict__ molbeg,
                              const float *__restrict__ molq, const float *__restrict__ Eat,
-                             const int *__restrict__ tag, float qslt_refs, float screen, float volume,
-                             float *__restrict__ U)
+                             const int *__restrict__ tag, int nslot, const SlotDesc *__restrict__ sd,
+                             float screen, float volume, float *__restrict__ U)
 {
     // block = ER_MR_MPT * 256 / NI molecules x NI insertions of one tile
     // (blockIdx.y); thread (m, i) takes molecules m, m + 256 / NI, ...
@@ -1484,8 +1572,10 @@ __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const i
     const int nl = tile * NI + i;
     if (nl >= nchunk) return;
     const int n = n_off + nl;
+    const int ks = ER_SLOT_OF_INS(sd, nslot, n);
+    if (n - sd[ks].ins_off >= sd[ks].nins) return;  // padding of the solute's insertion range
     const int skip = tag ? tag[n] : -1;
-    const float qs = tag ? molq[skip] : qslt_refs;
+    const float qs = tag ? molq[skip] : sd[ks].qslt;
     const float *E = Eat + (size_t)tile * natom * NI + i;
     float *Urow = U + (size_t)(n / NI) * nmol * NI + (n % NI);
     // every bound, charge and the rows of molecules of up to four atoms are
@@ -1608,10 +1698,11 @@ struct PairArgs {
     PairParams pp;
 };
 
+template <class LJP>
 __device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const FrameConst &fc, int n, int jm,
                                                    const int *__restrict__ molbeg, const float4 *__restrict__ posn,
                                                    const int *__restrict__ ljtype,
-                                                   const float4 *__restrict__ insposn, const int *__restrict__ sltlj,
+                                                   const float4 *__restrict__ insposn, LJP sltlj,
                                                    int slt_a0, bool soln, const float *__restrict__ ljene,
                                                    const float *__restrict__ ljlensq)
 {
@@ -1643,10 +1734,10 @@ __device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const Fra
 // pair, LJ within upljcut.  One thread per (insertion, molecule) of the
 // chunk, solute sites then molecule atoms in order; the sum is stored in U
 // (every molecule is written; a soln solute's own entry is 0).
-__global__ void __launch_bounds__(256) k_pairs_open(PairArgs pa, const int *__restrict__ molbeg,
+__global__ void __launch_bounds__(256) k_pairs_open(PairArgs pa, int nslot, const SlotDesc *__restrict__ sd,
+                                                    const int *__restrict__ molbeg,
                                                     const float4 *__restrict__ posn, const int *__restrict__ ljtype,
-                                                    const float4 *__restrict__ insposn,
-                                                    const int *__restrict__ sltlj, const int *__restrict__ tag,
+                                                    const float4 *__restrict__ insposn, const int *__restrict__ tag,
                                                     const float *__restrict__ ljene,
                                                     const float *__restrict__ ljlensq, float *__restrict__ U)
 {
@@ -1654,15 +1745,19 @@ __global__ void __launch_bounds__(256) k_pairs_open(PairArgs pa, const int *__re
     const int nl = blockIdx.y;
     if (jm >= pa.nmol || nl >= pa.nchunk) return;
     const int n = pa.n_off + nl;
+    const int k = ER_SLOT_OF_INS(sd, nslot, n);
+    const int local = n - sd[k].ins_off;
     PairParams p = pa.pp;
     p.elecut = __builtin_huge_valf();
     float e = 0.0f;
     const bool soln = tag != nullptr;
-    if (!(soln && jm == tag[n])) {
+    if (local < sd[k].nins && !(soln && jm == tag[n])) {
         const int slt_a0 = soln ? molbeg[tag[n]] : 0;
-        for (int s = 0; s < pa.nslt; ++s) {
-            const float4 xa = insposn[(size_t)n * pa.nslt + s];
-            const int ti = soln ? ljtype[slt_a0 + s] : sltlj[s];
+        const int nslt = sd[k].nslt;
+        const float4 *ins = insposn + sd[k].site_off + (size_t)local * nslt;
+        for (int s = 0; s < nslt; ++s) {
+            const float4 xa = ins[s];
+            const int ti = soln ? ljtype[slt_a0 + s] : sd[k].sltlj[s];
             for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
                 const float4 xb = posn[b];
                 const float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
@@ -1709,14 +1804,15 @@ __device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const
     return e;
 }
 
-__global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const int *__restrict__ molbeg,
+__global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot, const SlotDesc *__restrict__ sd,
+                                               FrameConst fc, const int *__restrict__ molbeg,
                                                const float4 *__restrict__ posn, const int *__restrict__ ljtype,
                                                const float4 *__restrict__ molref,
                                                const float4 *__restrict__ segref,
                                                const float *__restrict__ rmolmax,
                                                const unsigned int *__restrict__ cstart,
-                                               const int *__restrict__ perm, const float4 *__restrict__ insposn,
-                                               const float4 *__restrict__ insref, const int *__restrict__ sltlj,
+                                               const int *__restrict__ perm, const float4 *__restrict__ insposn_all,
+                                               const float4 *__restrict__ insref,
                                                const int *__restrict__ tag, const float *__restrict__ ljene,
                                                const float *__restrict__ ljlensq, float *__restrict__ U)
 {
@@ -1725,10 +1821,21 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     __shared__ float4 sx[ER_PAIR_SCAP];
     __shared__ int stj[ER_PAIR_SCAP];
     __shared__ float te[ER_PAIR_TCAP], ts[ER_PAIR_TCAP];
-    // nsplit workgroups per insertion; part p owns the cells c with c % nsplit == p,
-    // so every (insertion, molecule) sum still has a single owner
-    const int nl = blockIdx.x / pa.nsplit, part = blockIdx.x % pa.nsplit;
-    const int n = pa.n_off + nl;
+    // workgroup b of the run (blk0 + blockIdx.x) belongs to the solute whose
+    // pair_blk range holds it; pair_split workgroups per insertion, part p
+    // owning the cells c with c % pair_split == p, so every (insertion,
+    // molecule) sum still has a single owner
+    const int b = blk0 + blockIdx.x;
+    const int ks = slot_find(nslot, b, [&](int m) { return sd[m].pair_blk; });
+    PairArgs pa = pa0;
+    pa.nsplit = sd[ks].pair_split;
+    pa.nslt = sd[ks].nslt;
+    const int bl = b - sd[ks].pair_blk;
+    const int nl = bl / pa.nsplit, part = bl % pa.nsplit;
+    const int n = sd[ks].ins_off + nl;
+    const __attribute__((address_space(1))) int *sltlj = gptr(sd[ks].sltlj);
+    // the insertion's sites (molecule_pair_sum reads them as insertion 0)
+    const float4 *__restrict__ insposn = insposn_all + sd[ks].site_off + (size_t)nl * pa.nslt;
     const int tid = threadIdx.x, NT = blockDim.x;
     const float4 ref = insref[n];
     const float cut = fmaxf(pa.pp.upljcut, pa.pp.elecut) + ref.w + 1e-3f;
@@ -1742,7 +1849,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     if (tid == 0) ncand = 0;
     if (in_lds) {
         for (int k = tid; k < pa.nslt; k += NT) {
-            sx[k] = insposn[(size_t)n * pa.nslt + k];
+            sx[k] = insposn[k];
             stj[k] = soln ? ljtype[slt_a0 + k] : sltlj[k];
         }
         for (int k = tid; k < nt2; k += NT) {
@@ -1753,7 +1860,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     __syncthreads();
     auto pair_sum = [&](int j) {
         return in_lds ? molecule_pair_sum_lds(pa, fc, j, molbeg, posn, ljtype, sx, stj, te, ts)
-                      : molecule_pair_sum(pa, fc, n, j, molbeg, posn, ljtype, insposn, sltlj, slt_a0, soln, ljene,
+                      : molecule_pair_sum(pa, fc, 0, j, molbeg, posn, ljtype, insposn, sltlj, slt_a0, soln, ljene,
                                           ljlensq);
     };
     int cx, cy, cz;
@@ -1812,7 +1919,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa, FrameConst fc, const
     const float skip2 = pair_skip2(pa.pp);
     const float skipc = fmaxf(pa.pp.upljcut, pa.pp.elecut) * 1.0001f + 1e-3f;
     auto site_sum = [&](int s, int jm) {
-        const float4 xa = in_lds ? sx[s] : insposn[(size_t)n * pa.nslt + s];
+        const float4 xa = in_lds ? sx[s] : insposn[s];
         const int ti = in_lds ? stj[s] : (soln ? ljtype[slt_a0 + s] : sltlj[s]);
         float e = 0.0f;
         // the molecule's reference site and reach radius: beyond cutoff +
@@ -1891,7 +1998,6 @@ __device__ __forceinline__ unsigned int f2ord(float f)
 
 struct HistArgs {
     int nmol, nchunk, n_off, ermax, numslv, slttype, nslice;
-    unsigned int seq;  // run number of the slot (halt keys order runs first)
 };
 
 // Halt key: the earliest (run, insertion, molecule) that hit a halt condition
@@ -1903,16 +2009,14 @@ __device__ __forceinline__ unsigned long long halt_key(unsigned int seq, int n, 
 }
 
 template <int NI>
-__global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restrict__ U, const int *__restrict__ tag,
+__global__ void __launch_bounds__(256) k_hist(HistArgs ha, int nslot, const SlotDesc *__restrict__ sd,
+                                              const float *__restrict__ U, const int *__restrict__ tag,
                                               const float *__restrict__ meshf,
                                               const unsigned short *__restrict__ lut,
                                               const int *__restrict__ sp_begin,
                                               const int *__restrict__ sp_end, const int *__restrict__ idmin,
                                               const int *__restrict__ uvmax, const int *__restrict__ uvsoft,
-                                              uint32_t *__restrict__ Cnt, unsigned int *__restrict__ minmax,
-                                              double *__restrict__ flpart,
-                                              unsigned long long *__restrict__ errkey,
-                                              unsigned long long *__restrict__ warnkey)
+                                              uint32_t *__restrict__ Cnt, double *__restrict__ flpart)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     double *rsum = reinterpret_cast<double *>(smem);                 // [NT]
@@ -1931,8 +2035,16 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
 
     const int i = tid % NI, jj = tid / NI, jstep = NT / NI;
     const int nl = tile * NI + i;
-    const bool valid = nl < ha.nchunk;
+    // the tile's solute (a tile never spans two); its padding rows are not binned
+    const int ks = ER_SLOT_OF_INS(sd, nslot, ha.n_off + tile * NI);
+    const int live = sd[ks].ins_off + sd[ks].nins;  // end of the solute's insertions
+    const unsigned int seq = *gptr(sd[ks].seq);
+    unsigned int *__restrict__ minmax = sd[ks].minmax;
+    unsigned long long *__restrict__ errkey = sd[ks].errkey;
+    unsigned long long *__restrict__ warnkey = sd[ks].warnkey;
+    const bool valid = nl < ha.nchunk && ha.n_off + nl < live;
     const int n = ha.n_off + nl;
+    const int nloc = n - sd[ks].ins_off;  // the insertion within its solute (halt keys)
     const int skip = (tag && valid) ? tag[n] : -1;
     const size_t gt = (size_t)(ha.n_off / NI + tile);
     for (int sp = 1; sp <= ha.numslv; ++sp) {
@@ -1970,10 +2082,10 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
                     else hi = mid;
                 }
                 const int idpti = lo;
-                if (idpti <= 0 || (ha.slttype == 1 && idpti > soft)) atomicMin(errkey, halt_key(ha.seq, n, j + 1));
+                if (idpti <= 0 || (ha.slttype == 1 && idpti > soft)) gatomic_min(errkey, halt_key(seq, nloc, j + 1));
                 // the 'mbin' warning (engproc.F90:1158-1167): an energy below
                 // 1e3 in the open top bin; the earliest one is kept
-                if (idpti == num && v < 1.0e3f) atomicMin(warnkey, halt_key(ha.seq, n, j + 1));
+                if (idpti == num && v < 1.0e3f) gatomic_min(warnkey, halt_key(seq, nloc, j + 1));
                 atomicAdd(&hist[i * ha.ermax + base + max(idpti, 1) - 1], 1u);
                 lmin = fminf(lmin, v);
                 lmax = fmaxf(lmax, v);
@@ -1993,10 +2105,10 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
                 mx = fmaxf(mx, rmax[tid + k * NI]);
                 sm += rsum[tid + k * NI];
             }
-            if (tile * NI + tid < ha.nchunk) {
+            if (tile * NI + tid < ha.nchunk && ha.n_off + tile * NI + tid < live) {
                 if (mn <= mx) {
-                    atomicMin(&minmax[2 * sp], f2ord(mn));
-                    atomicMax(&minmax[2 * sp + 1], f2ord(mx));
+                    gatomic_min(&minmax[2 * sp], f2ord(mn));
+                    gatomic_max(&minmax[2 * sp + 1], f2ord(mx));
                 }
                 if (flpart)
                     flpart[((size_t)(ha.n_off + tile * NI + tid) * ha.nslice + slice) * ha.numslv + sp - 1] = sm;
@@ -2007,8 +2119,16 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, const float *__restri
     for (int f = tid; f < NI * ha.ermax; f += NT) {
         const int ii = f / ha.ermax, b = f % ha.ermax;
         const uint32_t h = hist[f];
-        if (h && tile * NI + ii < ha.nchunk) atomicAdd(&Cnt[(size_t)(ha.n_off + tile * NI + ii) * ha.ermax + b], h);
+        if (h && tile * NI + ii < ha.nchunk && ha.n_off + tile * NI + ii < live)
+            atomicAdd(&Cnt[(size_t)(ha.n_off + tile * NI + ii) * ha.ermax + b], h);
     }
+}
+
+// the start of a run: every solute of the run counts one more run
+__global__ void k_seq_bump(int nslot, const SlotDesc *__restrict__ sd)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nslot) *sd[k].seq += 1u;
 }
 
 __global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
@@ -2021,15 +2141,18 @@ __global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
 // 1) belongs to this run and has not been captured yet, record the
 // offending energy while U / uvengy0 still hold it (cap[2t] = captured key,
 // cap[2t + 1] = the energy's bits).
-__global__ void k_halt_capture(unsigned int seq, int NI, int nmol, const unsigned long long *__restrict__ errkey,
-                               const unsigned long long *__restrict__ warnkey, const float *__restrict__ U,
-                               const float *__restrict__ u0, unsigned long long *__restrict__ cap)
+__global__ void k_halt_capture(const SlotDesc *__restrict__ sd, int NI, int nmol, const float *__restrict__ U,
+                               const float *__restrict__ u0)
 {
     const int t = threadIdx.x;
     if (t > 1) return;
-    const unsigned long long key = t == 0 ? *errkey : *warnkey;
+    const SlotDesc &d = sd[blockIdx.x];  // one block per solute of the run
+    const unsigned int seq = *d.seq;
+    unsigned long long *cap = d.halt;
+    const unsigned long long key = t == 0 ? *d.errkey : *d.warnkey;
     if (key == ~0ull || (unsigned int)(key >> 42) != (seq & 0x3fffffu) || cap[2 * t] == key) return;
-    const int n = (int)((key >> 21) & 0x1fffffu), jp1 = (int)(key & 0x1fffffu);
+    // the key holds the insertion within its solute
+    const int n = d.ins_off + (int)((key >> 21) & 0x1fffffu), jp1 = (int)(key & 0x1fffffu);
     const float v = jp1 == 0 ? u0[n] : U[((size_t)(n / NI) * nmol + (jp1 - 1)) * NI + (n % NI)];
     cap[2 * t] = key;
     cap[2 * t + 1] = (unsigned long long)__float_as_uint(v);
@@ -2043,12 +2166,35 @@ __global__ void k_halt_capture(unsigned int seq, int NI, int nmol, const unsigne
 // flags: [0] voffset_initialized [1] usreal_set; usreal in fstate[0]
 // ---------------------------------------------------------------------------
 struct WeightArgs {
-    int nchunk, n_off, H, slttype, wgtslf, estype, selfcal, esmax, rigid, numslv, nslice;
+    int H, slttype, wgtslf, estype, selfcal, esmax, rigid, numslv, nslice;
     int bare;  // cltype 0: no reciprocal self part, no Ewald factor in volcorrect
-    unsigned int seq;
     float w_sys, volume, screen, qtot;
     double temp;
 };
+
+// The insertions an accumulation covers.  nch < 0: every solute of the run,
+// solute = blockIdx.y (k_weights) / blockIdx.x (init, fin) / the slice's
+// solute (k_ecorr), each over all its insertions.  nch >= 0: solute `slot`
+// only, its insertions [rel, rel + nch) (the plain path), kper insertions
+// per ecorr slice.
+struct AccRange {
+    int slot, rel, nch, kper;
+};
+
+// (solute, first insertion in the run's space, insertion count) of a block
+__device__ __forceinline__ void acc_span(const AccRange &r, const SlotDesc *__restrict__ sd, int bslot, int &k,
+                                         int &n_off, int &nchunk)
+{
+    if (r.nch < 0) {
+        k = bslot;
+        n_off = sd[k].ins_off;
+        nchunk = sd[k].nins;
+    } else {
+        k = r.slot;
+        n_off = sd[k].ins_off + r.rel;
+        nchunk = r.nch;
+    }
+}
 
 // uvengy0 of insertion n: uvrecp (FP64 half-plane partials) + pairep +
 // residual, in the reference's real arithmetic
@@ -2060,19 +2206,24 @@ __device__ __forceinline__ float weights_uv0(const WeightArgs &wa, const double 
     return (float)rec + real + resself[n];
 }
 
-// first run of a block: the rigid solute's real self term and voffset
-__global__ void k_weights_init(WeightArgs wa, const double *__restrict__ selfpart,
-                               const float *__restrict__ selfreal, const float *__restrict__ resself,
-                               double *__restrict__ state, int *__restrict__ flags, float *__restrict__ fstate)
+// first run of a block: the rigid solute's real self term and voffset (one
+// block per solute)
+__global__ void k_weights_init(WeightArgs wa, AccRange ar, const SlotDesc *__restrict__ sd,
+                               const double *__restrict__ selfpart, const float *__restrict__ selfreal,
+                               const float *__restrict__ resself)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (threadIdx.x != 0) return;
+    int k, n_off, nchunk;
+    acc_span(ar, sd, blockIdx.x, k, n_off, nchunk);
+    int *flags = sd[k].flags;
+    float *fstate = sd[k].fstate;
     if (wa.rigid && !flags[1]) {
-        fstate[0] = selfreal[wa.n_off];
+        fstate[0] = selfreal[n_off];
         flags[1] = 1;
     }
     if (wa.wgtslf && !flags[0]) {
-        const float real0 = wa.rigid ? fstate[0] : selfreal[wa.n_off];
-        state[3] = (double)weights_uv0(wa, selfpart, resself, wa.n_off, real0);
+        const float real0 = wa.rigid ? fstate[0] : selfreal[n_off];
+        sd[k].state[3] = (double)weights_uv0(wa, selfpart, resself, n_off, real0);
         flags[0] = 1;
     }
 }
@@ -2080,10 +2231,10 @@ __global__ void k_weights_init(WeightArgs wa, const double *__restrict__ selfpar
 // per insertion (one thread each): uvengy0, the weight engnmfc
 // (engproc.F90:858-912), flceng; per block: fixed-order tree partials of
 // sum w, sum w_tot and the min/max of uvengy0 into wpart[block][4]
-__global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__restrict__ selfpart,
+__global__ void __launch_bounds__(256) k_weights(WeightArgs wa, AccRange ar, const SlotDesc *__restrict__ sd,
+                                                 const double *__restrict__ selfpart,
                                                  const float *__restrict__ selfreal,
-                                                 const float *__restrict__ resself, const float *__restrict__ wslt,
-                                                 const double *__restrict__ state, const float *__restrict__ fstate,
+                                                 const float *__restrict__ resself,
                                                  float *__restrict__ u0, double *__restrict__ wout,
                                                  const double *__restrict__ flpart, double *__restrict__ flceng,
                                                  double *__restrict__ wpart)
@@ -2091,13 +2242,16 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
     __shared__ double r_w[256], r_t[256];
     __shared__ float r_mn[256], r_mx[256];
     const int tid = threadIdx.x, NT = blockDim.x;
-    const double voffset = state[3];
-    const float usreal = fstate[0];
+    int k, n_off, nchunk;
+    acc_span(ar, sd, blockIdx.y, k, n_off, nchunk);  // grid.y: the run's solutes
+    const double voffset = gptr(sd[k].state)[3];
+    const float usreal = gptr(sd[k].fstate)[0];
+    const __attribute__((address_space(1))) float *wslt = sd[k].wslt ? gptr(sd[k].wslt) : nullptr;
     double sw = 0.0, st = 0.0;
     float mn = 3.402823466e38f, mx = -3.402823466e38f;
     const int nl = blockIdx.x * NT + tid;
-    if (nl < wa.nchunk) {
-        const int n = wa.n_off + nl;
+    if (nl < nchunk) {
+        const int n = n_off + nl;
         const float real = wa.rigid ? usreal : selfreal[n];
         const float u = weights_uv0(wa, selfpart, resself, n, real);
         u0[n] = u;
@@ -2108,7 +2262,7 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
             const float factor = (float)((double)u - voffset);
             engnmfc = (wa.slttype == 1) ? exp((double)factor / wa.temp) : exp(-(double)factor / wa.temp);
         }
-        float tw = wa.w_sys * (wslt ? wslt[n] : 1.0f);
+        float tw = wa.w_sys * (wslt ? wslt[n - sd[k].ins_off] : 1.0f);
         if (wa.estype == 2) {  // volcorrect (engproc.F90:1022-1038)
             tw = tw * wa.volume;
             if (!wa.bare) {
@@ -2142,27 +2296,37 @@ __global__ void __launch_bounds__(256) k_weights(WeightArgs wa, const double *__
         }
         __syncthreads();
     }
-    if (tid == 0) {
-        wpart[4 * blockIdx.x + 0] = r_w[0];
-        wpart[4 * blockIdx.x + 1] = r_t[0];
-        wpart[4 * blockIdx.x + 2] = (double)r_mn[0];
-        wpart[4 * blockIdx.x + 3] = (double)r_mx[0];
+    if (tid == 0) {  // each solute's partials in its own row of gridDim.x blocks
+        double *wp = wpart + 4 * ((size_t)blockIdx.y * gridDim.x + blockIdx.x);
+        wp[0] = r_w[0];
+        wp[1] = r_t[0];
+        wp[2] = (double)r_mn[0];
+        wp[3] = (double)r_mx[0];
     }
 }
 
 // the block partials in block order, then the sequential parts: the
 // self-energy distribution and slnuv, in n as the reference adds them
-__global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, const double *__restrict__ wpart,
-                                                     double *__restrict__ state, const float *__restrict__ u0,
+__global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, AccRange ar, const SlotDesc *__restrict__ sd,
+                                                     int wrow, const double *__restrict__ wpart_all,
+                                                     const float *__restrict__ u0,
                                                      const double *__restrict__ wout,
-                                                     const double *__restrict__ escrd, double *__restrict__ eself,
-                                                     unsigned int *__restrict__ minmax,
+                                                     const double *__restrict__ escrd,
                                                      const double *__restrict__ flpart,
-                                                     const double *__restrict__ flceng, double *__restrict__ slnuv,
-                                                     unsigned long long *__restrict__ errkey,
-                                                     unsigned long long *__restrict__ warnkey)
+                                                     const double *__restrict__ flceng)
 {
+    // one block per solute; wrow = k_weights' blocks per solute row of wpart
     const int tid = threadIdx.x;
+    int k, n_off, nchunk;
+    acc_span(ar, sd, blockIdx.x, k, n_off, nchunk);
+    const int nblk = max(1, (nchunk + 255) / 256);
+    const double *__restrict__ wpart = wpart_all + 4 * (size_t)blockIdx.x * wrow;
+    double *__restrict__ state = sd[k].state;
+    double *__restrict__ eself = sd[k].eself;
+    double *__restrict__ slnuv = sd[k].slnuv;
+    unsigned int *__restrict__ minmax = sd[k].minmax;
+    const unsigned int seq = *sd[k].seq;
+    const int base = sd[k].ins_off;  // halt keys hold the insertion within its solute
     if (tid == 0) {
         double sw = 0.0, st = 0.0;
         float mn = 3.402823466e38f, mx = -3.402823466e38f;
@@ -2173,17 +2337,17 @@ __global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, co
             mx = fmaxf(mx, (float)wpart[4 * b + 3]);
         }
         state[0] += sw;
-        state[1] += (double)wa.nchunk;
+        state[1] += (double)nchunk;
         state[2] += st;
         // solute self-energy extremes (engproc.F90:914-915)
-        if (wa.nchunk > 0) {
-            atomicMin(&minmax[0], f2ord(mn));
-            atomicMax(&minmax[1], f2ord(mx));
+        if (nchunk > 0) {
+            gatomic_min(&minmax[0], f2ord(mn));
+            gatomic_max(&minmax[1], f2ord(mx));
         }
         // self-energy distribution, sequential in n as the reference adds it
         if (wa.selfcal) {
-            for (int nl = 0; nl < wa.nchunk; ++nl) {
-                const int n = wa.n_off + nl;
+            for (int nl = 0; nl < nchunk; ++nl) {
+                const int n = n_off + nl;
                 const float u = u0[n];
                 int lo = 0, hi = wa.esmax;
                 while (lo < hi) {
@@ -2191,17 +2355,17 @@ __global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, co
                     if ((double)u > escrd[mid]) lo = mid + 1;
                     else hi = mid;
                 }
-                if (lo <= 0) atomicMin(errkey, halt_key(wa.seq, n, 0));
-                if (lo == wa.esmax && u < 1.0e3f) atomicMin(warnkey, halt_key(wa.seq, n, 0));
-                eself[max(lo, 1) - 1] += wout[n];
+                if (lo <= 0) gatomic_min(sd[k].errkey, halt_key(seq, n - base, 0));
+                if (lo == wa.esmax && u < 1.0e3f) gatomic_min(sd[k].warnkey, halt_key(seq, n - base, 0));
+                gptr(eself)[max(lo, 1) - 1] += wout[n];
             }
         }
     }
     // slnuv += flceng * engnmfc (engproc.F90:939-941), sequential in n
     if (flpart && tid < wa.numslv) {
         double s = slnuv[tid];
-        for (int nl = 0; nl < wa.nchunk; ++nl) {
-            const int n = wa.n_off + nl;
+        for (int nl = 0; nl < nchunk; ++nl) {
+            const int n = n_off + nl;
             s += flceng[(size_t)n * wa.numslv + tid] * wout[n];
         }
         slnuv[tid] = s;
@@ -2211,11 +2375,33 @@ __global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, int nblk, co
 // edens(b) += engnmfc * count (engproc.F90:943-947), sequential in n per bin
 // edens partial of insertion slice blockIdx.y (used when no ecorr is formed
 // or on the VALU ecorr path); reduced by k_ecorr_reduce with ecorr = null
-__global__ void k_edens(int nchunk, int n_off, int ermax, int kper, const uint32_t *__restrict__ Cnt,
-                        const double *__restrict__ w, double *__restrict__ epart)
+// the solute and insertion range of ecorr/edens slice z (grid z or y): over
+// every solute of the run (ar.nch < 0: z runs over each solute's slices in
+// turn) or over one solute's range (z = the slice)
+__device__ __forceinline__ void slice_span(const AccRange &ar, const SlotDesc *__restrict__ sd, int nslot, int z,
+                                           int &slice, int &n_off, int &nchunk, int &kper)
 {
-    const int b = blockIdx.x * blockDim.x + threadIdx.x, slice = blockIdx.y;
+    if (ar.nch < 0) {
+        const int k = slot_find(nslot, z, [&](int m) { return sd[m].slice_off; });
+        slice = z - sd[k].slice_off;
+        n_off = sd[k].ins_off;
+        nchunk = sd[k].nins;
+        kper = sd[k].kper;
+    } else {
+        slice = z;
+        n_off = sd[ar.slot].ins_off + ar.rel;
+        nchunk = ar.nch;
+        kper = ar.kper;
+    }
+}
+
+__global__ void k_edens(AccRange ar, int nslot, const SlotDesc *__restrict__ sd, int ermax,
+                        const uint32_t *__restrict__ Cnt, const double *__restrict__ w, double *__restrict__ epart)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
     if (b >= ermax) return;
+    int slice, n_off, nchunk, kper;
+    slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper);
     const int k_begin = slice * kper, k_end = min(nchunk, k_begin + kper);
     double e = 0.0;
     for (int nl = k_begin; nl < k_end; ++nl) {
@@ -2223,7 +2409,7 @@ __global__ void k_edens(int nchunk, int n_off, int ermax, int kper, const uint32
         const uint32_t k = Cnt[(size_t)n * ermax + b];
         if (k) e += w[n] * (double)k;
     }
-    epart[(size_t)slice * ermax + b] = e;
+    epart[(size_t)z * ermax + b] = e;
 }
 
 // ---------------------------------------------------------------------------
@@ -2258,14 +2444,16 @@ __global__ void k_mfma_layout(double *__restrict__ out)
 // results do not depend on scheduling.  Insertions are taken four per MFMA
 // step (kq = lane / 16), the next step's operands loaded before the current
 // step's MFMAs.
-__global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax, int kper,
+__global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const SlotDesc *__restrict__ sd, int ermax,
                                                const uint32_t *__restrict__ Cnt, const double *__restrict__ w,
                                                const int *__restrict__ dmap, double *__restrict__ part,
                                                double *__restrict__ epart)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int by = blockIdx.y, bx = blockIdx.x, slice = blockIdx.z;
+    const int by = blockIdx.y, bx = blockIdx.x, z = blockIdx.z;
     if (by > bx) return;  // lower block triangle: mirrored by the reduce
+    int slice, n_off, nchunk, kper;
+    slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper);
     const int b1w = by * 64 + (wave >> 1) * 32;
     const int b2w = bx * 64 + (wave & 1) * 32;
     const int k_begin = slice * kper, k_end = min(nchunk, k_begin + kper);
@@ -2306,7 +2494,7 @@ __global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax,
         av[0] = an[0]; av[1] = an[1];
         bv[0] = bn[0]; bv[1] = bn[1];
     }
-    double *P = part + (size_t)slice * plane;
+    double *P = part + (size_t)z * plane;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ij = dmap[lane * 4 + r];
@@ -2329,20 +2517,28 @@ __global__ void __launch_bounds__(256) k_ecorr(int nchunk, int n_off, int ermax,
                          e3 = __shfl(ed[a], mr + 48, 64);
             const double e = ((ed[a] + e1) + e2) + e3;
             const int b1 = b1w + a * 16 + mr;
-            if (kq == 0 && b1 < ermax) epart[(size_t)slice * ermax + b1] = e;
+            if (kq == 0 && b1 < ermax) epart[(size_t)z * ermax + b1] = e;
         }
     }
 }
 
 // ecorr(b1,b2) += sum over slices of part (upper triangle, mirrored);
 // edens(b) += sum over slices of epart
-__global__ void k_ecorr_reduce(int ermax, int nslice, const double *__restrict__ part,
-                               const double *__restrict__ epart, double *__restrict__ ecorr,
-                               double *__restrict__ edens)
+__global__ void k_ecorr_reduce(AccRange ar, const SlotDesc *__restrict__ sd, int ermax, int full,
+                               const double *__restrict__ part_all, const double *__restrict__ epart_all)
 {
+    // grid z: the run's solutes (ar.nch < 0) or solute ar.slot alone; each
+    // adds its own slices in slice order
     const int b2 = blockIdx.x * blockDim.x + threadIdx.x, b1 = blockIdx.y;
     if (b2 >= ermax) return;
+    const int ks = ar.nch < 0 ? (int)blockIdx.z : ar.slot;
+    const int z0 = ar.nch < 0 ? sd[ks].slice_off : 0;
+    const int nslice = ar.nch < 0 ? sd[ks].nks : (ar.nch + ar.kper - 1) / ar.kper;
     const size_t plane = (size_t)ermax * ermax;
+    const double *__restrict__ part = part_all + (size_t)z0 * plane;
+    const double *__restrict__ epart = epart_all + (size_t)z0 * ermax;
+    __attribute__((address_space(1))) double *ecorr = full ? gptr(sd[ks].ecorr) : nullptr;
+    __attribute__((address_space(1))) double *edens = gptr(sd[ks].edens);
     if (ecorr && b1 <= b2) {
         double s = 0.0;
         for (int k = 0; k < nslice; ++k) s += part[k * plane + (size_t)b1 * ermax + b2];
@@ -2357,11 +2553,14 @@ __global__ void k_ecorr_reduce(int ermax, int nslice, const double *__restrict__
 }
 
 // VALU fallback of the same update (one thread per upper-triangle element)
-__global__ void k_ecorr_valu(int nchunk, int n_off, int ermax, const uint32_t *__restrict__ Cnt,
-                             const double *__restrict__ w, double *__restrict__ ecorr)
+__global__ void k_ecorr_valu(AccRange ar, const SlotDesc *__restrict__ sd, int ermax,
+                             const uint32_t *__restrict__ Cnt, const double *__restrict__ w)
 {
     const int b1 = blockIdx.y * 16 + threadIdx.y, b2 = blockIdx.x * 16 + threadIdx.x;
     if (b1 >= ermax || b2 >= ermax || b1 > b2) return;
+    int k, n_off, nchunk;
+    acc_span(ar, sd, blockIdx.z, k, n_off, nchunk);
+    __attribute__((address_space(1))) double *ecorr = gptr(sd[k].ecorr);
     double s = 0.0;
     for (int nl = 0; nl < nchunk; ++nl) {
         const int n = n_off + nl;

@@ -22,7 +22,8 @@ EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_s
             "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_run_noalloc",
             "er_sync",
             "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
-            "er_destroy", "er_profile", "er_profile_read", "er_autotune", "er_set_tuning", "er_get_tuning"]
+            "er_destroy", "er_profile", "er_profile_read", "er_autotune", "er_set_tuning", "er_get_tuning",
+            "er_graph_begin", "er_graph_end", "er_graph_launch", "er_graph_destroy"]
 EXPORTED_TRAJ = ["er_traj_open", "er_traj_read", "er_traj_info", "er_traj_close"]
 ER_TRAJ_EOF, ER_TRAJ_IO, ER_TRAJ_FORMAT = 100, 101, 102
 TRAJ_FLAGS = {"charmm": 1, "cell": 2, "4d": 4, "rec64": 8, "swapped": 16}
@@ -89,6 +90,10 @@ def load() -> C.CDLL:
     L.er_autotune.argtypes = [vp, i32]
     L.er_set_tuning.argtypes = [vp, i32, i32, i32]
     L.er_get_tuning.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+    L.er_graph_begin.argtypes = [vp]
+    L.er_graph_end.argtypes = [vp, C.POINTER(i32)]
+    L.er_graph_launch.argtypes = [vp, i32]
+    L.er_graph_destroy.argtypes = [vp, i32]
     for name in EXPORTED:
         if name not in ("er_error_string", "er_destroy"):
             getattr(L, name).restype = C.c_int
@@ -323,6 +328,23 @@ class Context:
             self._check(self.L.er_profile_read(self.h, k, C.byref(ms), C.byref(n)))
             out[name] = (ms.value, n.value)
         return out
+
+    def graph_begin(self):
+        """er_graph_begin: record the following device-side setters and
+        batched runs into a HIP graph."""
+        self._check(self.L.er_graph_begin(self.h))
+
+    def graph_end(self):
+        """er_graph_end: the recorded graph's handle."""
+        g = C.c_int32()
+        self._check(self.L.er_graph_end(self.h, C.byref(g)))
+        return int(g.value)
+
+    def graph_launch(self, graph):
+        self._check(self.L.er_graph_launch(self.h, int(graph)))
+
+    def graph_destroy(self, graph):
+        self._check(self.L.er_graph_destroy(self.h, int(graph)))
 
     def autotune(self, reps=0):
         """Times launch shapes on the current frame and insertions and keeps

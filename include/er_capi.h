@@ -136,7 +136,10 @@ int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sltpos_dev,
  * realcal_prepare + recpcal_spline_greenfunc + recpcal_prepare_solvent +
  * get_uv_energy_refs|soln + update_histogram x maxdst.  Results accumulate
  * into the block accumulators.  ER_PLAIN launches one insertion at a time;
- * ER_BATCHED runs every insertion of the frame in one launch sequence. */
+ * ER_BATCHED runs every insertion of the frame, of every solute that has
+ * insertions (er_set_solutes), in one launch per kernel class (the
+ * reciprocal part in chunks of insertions, one chunk by default when the
+ * tuner picks it). */
 int er_run(er_ctx *c, int32_t mode);
 
 /* No counterpart in the reference: er_run for callers that cannot allocate
@@ -153,6 +156,32 @@ int er_run(er_ctx *c, int32_t mode);
  * er_run_noalloc calls that overlap on one GPU race: issue them from one
  * stream at a time.  Profiling (er_profile) is suspended for the call. */
 int er_run_noalloc(er_ctx *c, int32_t mode);
+
+/* HIP graphs of whole runs; no counterpart in the reference, whose frame
+ * loop (erdst/enganal.F90:86-95 -> engconst per frame) launches every
+ * kernel from the host.  Between er_graph_begin and er_graph_end the calls
+ * er_set_frame_device, er_select_solute, er_set_insertions_device (with
+ * stat_weight_solute NULL) and er_run(ER_BATCHED) are recorded, not
+ * executed; any other call on the context fails with ER_E_STATE and spoils
+ * the recording.  er_graph_end instantiates the recording and returns a
+ * handle >= 1; er_graph_launch replays it with one launch on the context's
+ * stream, asynchronously (er_sync waits), adding to the block accumulators
+ * exactly what the recorded calls would add.
+ *  - The graph keeps the device pointers, cells, weights and launch shapes
+ *    of the recording: the caller refills the same device buffers between
+ *    replays.  A replay needs the same solutes, insertion counts and launch
+ *    shapes as the recording; a run of them before er_graph_begin sizes the
+ *    buffers (nothing is allocated while recording).
+ *  - er_graph_launch refuses (ER_E_STATE) a graph whose device buffers were
+ *    reallocated since, or whose solutes, shapes or influence-function cell
+ *    differ from the context's; record it again then.
+ *  - Halts and warnings of replayed runs are reported by er_sync as for
+ *    er_run.  A typical use records one batch of frames (16 frames of 1000
+ *    insertions) and replays it per batch. */
+int er_graph_begin(er_ctx *c);
+int er_graph_end(er_ctx *c, int32_t *graph);
+int er_graph_launch(er_ctx *c, int32_t graph);
+int er_graph_destroy(er_ctx *c, int32_t graph);
 
 /* Waits for the context's stream; returns ER_E_HALT if a run hit a halt
  * condition (getiduv's 'eng_min' / 'eng_sft', engproc.F90:1142-1156) or,
