@@ -910,6 +910,35 @@ static int frame_cell(er_ctx *c, const float *cell)
     for (int i = 0; i < 3; ++i)
         if (cn[0] < 0.f)
             for (int r = 0; r < 3; ++r) cn[r + 3 * i] = -cn[r + 3 * i];
+    // its lattice reduction (realcal.F90:832-847): a column whose (1,2),
+    // (1,3) or (2,3) element exceeds half the diagonal by more than 1e-4 is
+    // shifted by the nearest whole multiple of column 1 (or 2), in float as
+    // the reference's real.  The reference scales by invcell_normal as the
+    // previous frame left it (normalize_periodic sets it afterwards); this
+    // uses 1/cell_normal(i,i) of the frame itself, which is the same value
+    // for every frame after the first of a constant cell.
+    bool reduced = false;
+    {
+        auto anint = [](float x) { return x >= 0.f ? std::floor(x + 0.5f) : -std::floor(-x + 0.5f); };
+        const float thres = 1.0e-4f;
+        const float inv0 = 1.0f / std::fabs(cn[0]);
+        if (std::fabs(cn[3]) > 0.5f * cn[0] + thres) {
+            const float f = anint(cn[3] * inv0);
+            for (int r = 0; r < 3; ++r) cn[r + 3] -= cn[r] * f;
+            reduced = true;
+        }
+        if (std::fabs(cn[6]) > 0.5f * cn[0] + thres) {
+            const float f = anint(cn[6] * inv0);
+            for (int r = 0; r < 3; ++r) cn[r + 6] -= cn[r] * f;
+            reduced = true;
+        }
+        const float inv1 = 1.0f / std::fabs(cn[4]);
+        if (std::fabs(cn[7]) > 0.5f * cn[4] + thres) {
+            const float f = anint(cn[7] * inv1);
+            for (int r = 0; r < 3; ++r) cn[r + 6] -= cn[r + 3] * f;
+            reduced = true;
+        }
+    }
     const float cutoff = std::min(c->prm.elecut, c->prm.upljcut);
     for (int i = 0; i < 3; ++i)
         if (cutoff > cn[i + 3 * i] * 0.5f)
@@ -944,21 +973,24 @@ static int frame_cell(er_ctx *c, const float *cell)
         fc.invcl[k] = (float)iv[k];
         fc.invP[k] = fc.invcl[k];
     }
-    if (fc.rotate) {  // inverse of the upper-triangular cell_normal
+    if (fc.rotate || reduced) {  // inverse of the upper-triangular cell_normal the pairs use
         const double a = cn[0], b = cn[3], d = cn[6], e = cn[4], f = cn[7], g = cn[8];
         double ip[9] = {1.0 / a, 0, 0, -b / (a * e), 1.0 / e, 0, (b * f - d * e) / (a * e * g), -f / (e * g), 1.0 / g};
         for (int k = 0; k < 9; ++k) fc.invP[k] = (float)ip[k];
     }
     c->volume_d = v;
     fc.volume = (float)v;
-    // cell-list resolution: nc cells per axis of >= 3.5 A perpendicular width
+    // cell-list resolution: nc cells per axis of >= 3.5 A perpendicular
+    // width, in the cell the pair positions are wrapped into (cell_normal)
     {
-        double wmin = 1e300;
+        double wmin = 1e300, pm[9];
+        for (int k = 0; k < 9; ++k) pm[k] = cn[k];
+        const double pv = std::fabs(pm[0] * pm[4] * pm[8]);  // upper triangular
         for (int k = 0; k < 3; ++k) {
             const int i1 = (k + 1) % 3, i2 = (k + 2) % 3;
-            const double *a = m + 3 * i1, *b = m + 3 * i2;
+            const double *a = pm + 3 * i1, *b = pm + 3 * i2;
             const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
-            wmin = std::min(wmin, std::fabs(v) / std::sqrt(cx * cx + cy * cy + cz * cz));
+            wmin = std::min(wmin, pv / std::sqrt(cx * cx + cy * cy + cz * cz));
         }
         c->nc = std::max(1, std::min(32, (int)(wmin / 3.5)));
         c->ncell = 1 << (3 * 5);  // Morton keys of 5 bits per axis

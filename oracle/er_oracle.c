@@ -46,6 +46,12 @@ void orc_inverted_cell(const double *cell, double *invcl, double *volume)
     *volume = v;
 }
 
+/* Fortran anint: nearest integer, halves away from zero */
+static double orc_anint(double t)
+{
+    return (t >= 0.0) ? floor(t + 0.5) : -floor(-t + 0.5);
+}
+
 /* erdst/realcal.F90:776-896.  Only the upper-triangular branch of
  * make_cell_uppertriangular (realcal.F90:784-790) is restated: every cell the
  * tests build is upper triangular, and the general branch divides every
@@ -85,6 +91,28 @@ int orc_normalize_periodic(const double *cell, double elecut, double upljcut,
     for (int i = 0; i < 3; i++)
         if (C(box->cell_normal, 0, 0) < 0.0)
             for (int r = 0; r < 3; r++) C(box->cell_normal, r, i) = -C(box->cell_normal, r, i);
+    /* its lattice reduction, realcal.F90:832-847: columns 2 and 3 shifted by
+     * whole multiples of columns 1 and 2 when an off-diagonal element
+     * exceeds half the diagonal by more than cutoff_thres = 1e-4.  The
+     * multiple is anint(element / |diagonal|) with this frame's diagonal
+     * (the reference reads invcell_normal as its previous frame left it,
+     * the same value for a constant cell from the second frame on). */
+    {
+        double *cn = box->cell_normal;
+        const double thres = 1.0e-4;
+        if (fabs(C(cn,0,1)) > 0.5 * C(cn,0,0) + thres) {
+            const double f = orc_anint(C(cn,0,1) / fabs(C(cn,0,0)));
+            for (int r = 0; r < 3; r++) C(cn,r,1) -= C(cn,r,0) * f;
+        }
+        if (fabs(C(cn,0,2)) > 0.5 * C(cn,0,0) + thres) {
+            const double f = orc_anint(C(cn,0,2) / fabs(C(cn,0,0)));
+            for (int r = 0; r < 3; r++) C(cn,r,2) -= C(cn,r,0) * f;
+        }
+        if (fabs(C(cn,1,2)) > 0.5 * C(cn,1,1) + thres) {
+            const double f = orc_anint(C(cn,1,2) / fabs(C(cn,1,1)));
+            for (int r = 0; r < 3; r++) C(cn,r,2) -= C(cn,r,1) * f;
+        }
+    }
     /* realcal.F90:849-856 */
     double cutoff = elecut < upljcut ? elecut : upljcut;
     for (int i = 0; i < 3; i++)

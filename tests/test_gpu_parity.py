@@ -343,6 +343,37 @@ def test_rotated_cell_cfg0(cfg0):
         assert U.energy_mismatch(u0, u0o_plain)[0] == 0
 
 
+def test_triclinic_cell_cfg0(cfg0):
+    """The configuration-0 system sheared into a skewed cell whose (1,2)
+    and (2,3) elements exceed half the diagonal: normalize_cell_vector's
+    reduction (realcal.F90:832-847), the non-cuboid minimum image
+    (realcal.F90:272-282), the cell list in the reduced cell and PME of the
+    skewed lattice, against the oracle on the same inputs (batched and
+    plain)."""
+    L = float(cfg0.cell[0, 0])
+    tri = np.array([[L, 0.7 * L, 0.2 * L], [0.0, L, -0.6 * L], [0.0, 0.0, L]])
+    S = tri / L   # maps the cube onto the skewed cell: a periodic configuration stays periodic
+    p = U.workload_params(cfg0)
+    pos = (cfg0.frame(0).astype(np.float64) @ S.T).astype(np.float32)
+    ins = (cfg0.insertions(0)[:32].astype(np.float64) @ S.T).astype(np.float32)
+    cell = tri.astype(np.float32)
+    topo = cfg0.topo
+    uo, u0o = U.O.energies_refs(U.oracle_system(topo), U.oracle_params(p), cell.astype(np.float64),
+                                pos.astype(np.float64), topo.slt_charge64, topo.slt_ljtype, ins.astype(np.float64),
+                                rigid=(p["slttype"] == 2))
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx = _ctx(cfg0, p)
+        ctx.set_frame(pos, cell)
+        ctx.set_insertions(ins)
+        ctx.run(mode)
+        u, u0 = ctx.uvengy(32)
+        nbad, worst, at = U.energy_mismatch(u, uo)
+        print(f"triclinic mode {mode}: {U.energy_report(u, uo)}, worst excess {worst:.3e}")
+        assert nbad == 0, (mode, worst, at, u[at], uo[at])
+        assert U.energy_mismatch(u0, u0o)[0] == 0
+        ctx.close()
+
+
 def test_pppm_rejects_odd_order(cfg0):
     with pytest.raises(capi.ErError):
         capi.Context(cfg0.topo, U.workload_params(cfg0, cltype=3, splodr=4) | {"splodr": 5})

@@ -626,3 +626,78 @@ def test_open_system_bare_coulomb_is_the_all_pairs_sum():
     with pytest.raises(ValueError, match="bare Coulomb"):
         O.energies_refs(sysm, O.make_params(cltype=2, boxshp=0), np.zeros((3, 3)), pos, qs,
                         np.zeros(3, np.int32), slt, nthreads=2)
+
+
+def _triclinic(L):
+    """Columns a, b, c of a cell whose (1,2) and (2,3) elements exceed half
+    the diagonal, so normalize_cell_vector reduces columns 2 and 3."""
+    return np.array([[L, 0.7 * L, 0.2 * L],
+                     [0.0, L, -0.6 * L],
+                     [0.0, 0.0, L]])
+
+
+def test_triclinic_bare_coulomb_is_the_image_sum():
+    """A skewed periodic cell (realcal.F90:272-282 minimum image after
+    normalize_cell_vector's reduction, realcal.F90:832-847): each pair adds
+    q_i q_j / r over every periodic image within elecut, computed here by
+    brute force over lattice translations; the same lattice given by a
+    reduced or an unreduced basis gives the same energies."""
+    rng = np.random.default_rng(17)
+    L, n = 30.0, 40
+    cell = _triclinic(L)
+    frac = rng.uniform(0, 1, (n, 3))
+    pos = np.empty((2 * n, 3))
+    pos[::2] = frac @ cell.T
+    pos[1::2] = pos[::2] + rng.normal(0, 0.9, (n, 3))
+    q = np.tile([0.4, -0.4], n) * CH
+    slt = (rng.uniform(0, 1, 3) @ cell.T) + np.vstack([np.zeros(3), rng.normal(0, 0.9, (2, 3))])
+    qs = np.array([0.3, -0.1, -0.2]) * CH
+    p = O.make_params(screen=0.35, splodr=4, ms=(32, 32, 32), cltype=0)
+    sysm = O.OracleSystem(np.arange(0, 2 * n + 1, 2), q, np.zeros(2 * n, np.int32), np.array([[0.0]]),
+                          np.array([[1.0]]))
+    uv, _ = O.energies_refs(sysm, p, cell, pos, qs, np.zeros(3, np.int32), slt[None], nthreads=2)
+    shifts = np.array([[i, j, k] for i in (-2, -1, 0, 1, 2) for j in (-2, -1, 0, 1, 2) for k in (-2, -1, 0, 1, 2)])
+    lat = shifts @ cell.T
+    want = np.zeros(n)
+    for m in range(n):
+        for a in (2 * m, 2 * m + 1):
+            for s in range(3):
+                r = np.linalg.norm(slt[s] - pos[a] + lat, axis=1)
+                want[m] += np.sum(qs[s] * q[a] / r[r < 12.0])
+    np.testing.assert_allclose(uv[0], want, rtol=1e-12, atol=1e-12)
+    # the reduced basis of the same lattice: b - a, then c + (b - a)
+    red = cell.copy()
+    red[:, 1] -= red[:, 0]
+    red[:, 2] += red[:, 1]
+    assert abs(red[0, 1]) <= 0.5 * L and abs(red[1, 2]) <= 0.5 * L
+    uv2, _ = O.energies_refs(sysm, p, red, pos, qs, np.zeros(3, np.int32), slt[None], nthreads=2)
+    np.testing.assert_allclose(uv2[0], want, rtol=1e-12, atol=1e-12)
+
+
+def test_triclinic_pme_pair_energy_converges_to_ewald():
+    """PME in the skewed cell (engfac from the full inverse cell, splines of
+    the fractional coordinates, recpcal.F90:121-163, 504-538) against a
+    brute-force Ewald sum over reciprocal vectors of that lattice."""
+    rng = np.random.default_rng(19)
+    L = 30.0
+    cell = _triclinic(L)
+    sites = (rng.uniform(0, 1, (2, 3)) @ cell.T)
+    q = np.array([0.5, -0.5]) * CH
+    alpha = 0.35
+    p = O.make_params(screen=alpha, splodr=8, ms=(64, 64, 64), elecut=12.0)
+    sysm = O.OracleSystem(np.array([0, 1]), q[1:], np.zeros(1, np.int32), np.array([[0.0]]), np.array([[1.0]]))
+    uv, _ = O.energies_refs(sysm, p, cell, sites[1:], q[:1], np.zeros(1, np.int32), sites[None, :1], nthreads=2)
+    # real-space part over images (the oracle's pair term), plus the reciprocal sum
+    vol = abs(np.linalg.det(cell))
+    recip = 2 * np.pi * np.linalg.inv(cell).T
+    m = np.array([[i, j, k] for i in range(-12, 13) for j in range(-12, 13) for k in range(-12, 13)
+                  if (i, j, k) != (0, 0, 0)])
+    kv = m @ recip.T
+    k2 = np.einsum("ij,ij->i", kv, kv)
+    d = sites[0] - sites[1]
+    rec = q[0] * q[1] * 4 * np.pi / vol * np.sum(np.exp(-k2 / (4 * alpha ** 2)) / k2 * np.cos(kv @ d))
+    shifts = np.array([[i, j, k] for i in (-2, -1, 0, 1, 2) for j in (-2, -1, 0, 1, 2) for k in (-2, -1, 0, 1, 2)])
+    r = np.linalg.norm(d + shifts @ cell.T, axis=1)
+    real = np.sum(q[0] * q[1] * erfc(alpha * r[r < 12.0]) / r[r < 12.0])
+    resid = -np.pi * q[0] * q[1] / alpha ** 2 / vol
+    np.testing.assert_allclose(uv[0, 0], real + rec + resid, rtol=0, atol=2e-6)
