@@ -480,13 +480,20 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
     }
     if (prm->slttype < 1 || prm->slttype > 3) return bad(fail(c, ER_E_ARG, "er_create: slttype"));
     if (prm->ljswitch < 0 || prm->ljswitch > 3) return bad(fail(c, ER_E_ARG, "er_create: ljswitch"));
-    if (!(prm->splodr == 4 || prm->splodr == 6 || prm->splodr == 8))
-        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: splodr must be 4, 6 or 8"));
-    if (!fft_len_ok(prm->ms[0]) || !plane_ok(prm->ms[1], prm->ms[2]))
+    if (prm->splodr < 3 || prm->splodr > 10)
+        return bad(fail(c, ER_E_UNSUPPORTED, "er_create: splodr must lie in 3..10"));
+    // any grid: the compiled shapes run the four-step kernels, the others the
+    // direct-DFT ones, whose plane must fit a workgroup's LDS
+    for (int d = 0; d < 3; ++d)
+        if (prm->ms[d] < std::max(16, 2 * prm->splodr) || prm->ms[d] > 512)
+            return bad(fail(c, ER_E_UNSUPPORTED, "er_create: ms%dmax = %d outside %d..512", d + 1, prm->ms[d],
+                            std::max(16, 2 * prm->splodr)));
+    if (!plane_ok(prm->ms[1], prm->ms[2]) &&
+        (prm->ms[1] > 256 || prm->ms[2] > 256 || spec2d_gen_lds(prm->ms[1], prm->ms[2]) > 160 * 1024))
         return bad(fail(c, ER_E_UNSUPPORTED,
-                        "er_create: PME grid %dx%dx%d not supported (ms1max one of 32 36 40 48 64 72 80 96 "
-                        "108 128; ms2max = ms3max from the same list, or 128 x 108)",
-                        prm->ms[0], prm->ms[1], prm->ms[2]));
+                        "er_create: PME plane %d x %d exceeds one workgroup's LDS (ms2max x ms3max up to about "
+                        "128 x 128, or a compiled shape)",
+                        prm->ms[1], prm->ms[2]));
     c->numatm = topo->numatm;
     c->nummol = topo->nummol;
     c->ntype = topo->ljtype_max;
@@ -1124,9 +1131,28 @@ template <int K2, int K3> static hipError_t launch_spec2d_k(er_ctx *c, int n_off
     return hipGetLastError();
 }
 
+// the direct-DFT plane kernel for shapes without a compiled one
+static hipError_t launch_spec2d_gen(er_ctx *c, int n_off, int nch, int t_off, bool sparse)
+{
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void *)k_pme_spec2d_gen, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+        attr = true;
+    }
+    const size_t lds = spec2d_gen_lds(c->K2, c->K3);
+    float2 *T = c->d_T.p + (size_t)(n_off - t_off) * c->Gc;
+    hipLaunchKernelGGL(k_pme_spec2d_gen, dim3(c->H, nch), dim3(ER_GEN_NT), lds, c->stream,
+                       (int)c->plan.desc.size(), c->d_sd.p, c->ord, c->K1, c->K2, c->K3, (int)sparse, n_off,
+                       c->d_Ftab.p, c->d_insw.p, c->d_insg.p, c->d_engfac.p, c->d_tw2.p, c->d_tw3.p, T,
+                       c->d_selfpart.p);
+    return hipGetLastError();
+}
+
 static hipError_t launch_spec2d(er_ctx *c, int n_off, int nch, int t_off, bool sparse)
 {
     if (nch <= 0) return hipSuccess;
+    if (!plane_ok(c->K2, c->K3)) return launch_spec2d_gen(c, n_off, nch, t_off, sparse);
     if (c->K2 == c->K3) {
         switch (c->K2) {
         case 32: return launch_spec2d_k<32, 32>(c, n_off, nch, t_off, sparse);
@@ -1185,6 +1211,13 @@ template <int K1, int NI> static hipError_t launch_axis1_k(er_ctx *c, int nch)
 
 template <int NI> static hipError_t launch_axis1(er_ctx *c, int nch)
 {
+    if (!fft_len_ok(c->K1)) {  // the direct C2R kernel
+        const int ntile = (nch + NI - 1) / NI;
+        const size_t lds = ((size_t)NI * c->H + c->K1) * sizeof(float2);
+        hipLaunchKernelGGL(k_pme_axis1_gen, dim3(c->K3, c->K2, ntile), dim3(ER_GEN_NT), lds, c->stream, nch, NI,
+                           c->K1, c->K2, c->K3, c->d_T.p, c->d_phi.p, c->d_tw1.p);
+        return hipGetLastError();
+    }
     switch (c->K1) {
     case 32: return launch_axis1_k<32, NI>(c, nch);
     case 36: return launch_axis1_k<36, NI>(c, nch);
@@ -1219,9 +1252,14 @@ template <int ORD, int NI, int LPM> static hipError_t launch_gather_k(er_ctx *c,
 template <int NI, int LPM> static hipError_t launch_gather(er_ctx *c, int n_off, int nch)
 {
     switch (c->ord) {
+    case 3: return launch_gather_k<3, NI, LPM>(c, n_off, nch);
     case 4: return launch_gather_k<4, NI, LPM>(c, n_off, nch);
+    case 5: return launch_gather_k<5, NI, LPM>(c, n_off, nch);
     case 6: return launch_gather_k<6, NI, LPM>(c, n_off, nch);
+    case 7: return launch_gather_k<7, NI, LPM>(c, n_off, nch);
     case 8: return launch_gather_k<8, NI, LPM>(c, n_off, nch);
+    case 9: return launch_gather_k<9, NI, LPM>(c, n_off, nch);
+    case 10: return launch_gather_k<10, NI, LPM>(c, n_off, nch);
     }
     return hipErrorInvalidValue;
 }
@@ -1302,7 +1340,7 @@ static PPPMCoef pppm_coef(const er_ctx *c)
     PPPMCoef pc{};
     const int P = c->ord;
     pc.order = P;
-    double b[8] = {1.0, 0, 0, 0, 0, 0, 0, 0};
+    double b[ER_MAXORD] = {1.0};
     for (int m = 1; m <= P - 1; ++m) {
         for (int l = m; l >= 1; --l)
             b[l] = 4.0 * (b[l] * (l - m) * (l - m - 0.5) - b[l - 1] * (l - m - 1) * (l - m - 1));
@@ -1611,7 +1649,7 @@ static int ensure_run_buffers(er_ctx *c, int chunk)
     ER_WANT(c->d_akeyrank, 2 * (size_t)c->numatm);
     ER_WANT(c->d_bcnt, c->nbrick);
     ER_WANT(c->d_bstart, c->nbrick + 1);
-    if (NI > 1 && c->ord <= 6) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
+    if (NI > 1 && (c->ord == 4 || c->ord == 6)) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
     // enough histogram workgroups to fill the chip
     const int ntile_chunk = (chunk + NI - 1) / NI;
     c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));

@@ -381,7 +381,7 @@ __global__ void k_engfac(int K1, int K2, int K3, int ord, const double *__restri
 // (the third alias component scales with K2, as the reference writes it).
 // Computed in FP64, stored FP32; runs only when the cell changes.
 struct PPPMCoef {
-    double b[8];      // gf_b(1:order)
+    double b[ER_MAXORD];  // gf_b(1:order)
     int order, a1, a2, a3;
 };
 
@@ -902,6 +902,214 @@ __global__ void __launch_bounds__(RL * 16) k_pme_axis1_t(int nchunk, int K2, int
     for (int f = tid; f < RL * PERLINE; f += RL * 16) {
         const int line = f / PERLINE, rem = f % PERLINE;
         dst[f] = *reinterpret_cast<const float4 *>(sml + line * LS + rem * 4);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// PME for grid shapes without a compiled plane or line kernel (any K, any
+// factors): the same two stages with direct DFTs, each output a sum over its
+// line with twiddles e^{+2 pi i k / K} from the tables, O(K) work per
+// element instead of O(log K).  The reference transforms any K1 x K2 x K3
+// (erdst/fft_iface.F90:38-50); these kernels keep that generality.
+// ---------------------------------------------------------------------------
+#define ER_GEN_NT 256
+
+// nline lines of length K in LDS (line L at base + L * lstride, element m at
+// + m * estride), transformed in place: out(x) = sum_m v(m) e^{SIGN 2 pi i m x / K}.
+// One wave per line; lane l computes the outputs x = l, l + 64, ... (K <= 256)
+// in registers before they are stored, so a line is read whole first.
+template <int SIGN>
+__device__ __forceinline__ void lds_dft_lines(float2 *base, int nline, int lstride, int estride, int K,
+                                              const float2 *tw, int tid)
+{
+    const int lane = tid & 63, wave = tid >> 6;
+    constexpr int NW = ER_GEN_NT / 64;
+    for (int L = wave; L < nline; L += NW) {
+        float2 *line = base + (size_t)L * lstride;
+        float2 out[4];
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int x = lane + 64 * o;
+            float2 acc = make_float2(0.f, 0.f);
+            if (x < K) {
+                int idx = 0;  // (m * x) mod K
+                for (int m = 0; m < K; ++m) {
+                    const float2 v = line[m * estride];
+                    const float2 t = tw[idx];
+                    const float ty = SIGN > 0 ? t.y : -t.y;
+                    acc.x += v.x * t.x - v.y * ty;
+                    acc.y += v.x * ty + v.y * t.x;
+                    idx += x;
+                    if (idx >= K) idx -= K;
+                }
+            }
+            out[o] = acc;
+        }
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int o = 0; o < 4; ++o) {
+            const int x = lane + 64 * o;
+            if (x < K) line[x * estride] = out[o];
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// LDS bytes of k_pme_spec2d_gen for a K2 x K3 plane
+__host__ __device__ constexpr size_t spec2d_gen_lds(int K2, int K3)
+{
+    return ((size_t)K2 * (K3 + 1) + K2 + K3 + (size_t)ER_SCH * (K2 + K3)) * 8 + (ER_GEN_NT / 64) * 8;
+}
+
+// Stage A for any K2 x K3 (one (insertion, m1) plane per workgroup): the
+// structure factor (separable product, or for many-site solutes the scatter
+// + forward DFT along axes 3 and 2), the FP64 half-plane self energy and the
+// engfac multiply, the inverse DFT along axes 3 and 2, T[nl][m1][x2][x3].
+__global__ void __launch_bounds__(ER_GEN_NT) k_pme_spec2d_gen(int nslot, const SlotDesc *__restrict__ sd, int ord,
+                                                              int K1, int K2, int K3, int sparse, int n_off,
+                                                              const float2 *__restrict__ Ftab,
+                                                              const float *__restrict__ insw,
+                                                              const int4 *__restrict__ insg,
+                                                              const float *__restrict__ engfac,
+                                                              const float2 *__restrict__ tw2,
+                                                              const float2 *__restrict__ tw3, float2 *__restrict__ T,
+                                                              double *__restrict__ selfpart)
+{
+    constexpr int NT = ER_GEN_NT;
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int PS = K3 + 1, NE = K2 * K3, H = K1 / 2 + 1;
+    float2 *plane = reinterpret_cast<float2 *>(smem);  // [K2][PS]
+    float2 *t2 = plane + (size_t)K2 * PS;               // [K2]
+    float2 *t3 = t2 + K2;                               // [K3]
+    float2 *F2 = t3 + K3;                               // [SCH][K2]: F1(m1) F2(m2)
+    float2 *F3 = F2 + ER_SCH * K2;                      // [SCH][K3]
+    double *red = reinterpret_cast<double *>(F3 + ER_SCH * K3);
+    const int m1 = blockIdx.x, nl = blockIdx.y, n = n_off + nl, tid = threadIdx.x;
+    float2 *Tp = T + ((size_t)nl * H + m1) * (size_t)NE;
+    const int ks = ER_SLOT_OF_INS(sd, nslot, n);
+    const int local = n - sd[ks].ins_off;
+    if (local >= sd[ks].nins) {  // padding insertion: a zero plane and no self energy
+        for (int e = tid; e < NE; e += NT) Tp[e] = make_float2(0.f, 0.f);
+        if (tid == 0) selfpart[(size_t)n * H + m1] = 0.0;
+        return;
+    }
+    const int nslt = sd[ks].nslt;
+    const size_t rowb = (size_t)sd[ks].site_off + (size_t)local * nslt;
+    const int per = H + K2 + K3;
+    for (int k = tid; k < K2; k += NT) t2[k] = tw2[k];
+    for (int k = tid; k < K3; k += NT) t3[k] = tw3[k];
+    for (int e = tid; e < K2 * PS; e += NT) plane[e] = make_float2(0.f, 0.f);
+    __syncthreads();
+    if (!sparse) {
+        // element e = m2 * K3 + m3 owned by one thread, sites added in order
+        for (int s0 = 0; s0 < nslt; s0 += ER_SCH) {
+            const int ns = min(ER_SCH, nslt - s0);
+            for (int f = tid; f < ns * (K2 + K3); f += NT) {
+                const int s = f / (K2 + K3), r = f % (K2 + K3);
+                const float2 *Fs = Ftab + (rowb + s0 + s) * per;
+                if (r < K2) F2[s * K2 + r] = erfft::cmul(Fs[m1], Fs[H + r]);
+                else F3[s * K3 + (r - K2)] = Fs[H + r];
+            }
+            __syncthreads();
+            for (int e = tid; e < NE; e += NT) {
+                const int m2 = e / K3, m3 = e % K3;
+                float2 acc = plane[m2 * PS + m3];
+                for (int s = 0; s < ns; ++s) {
+                    const float2 B = F2[s * K2 + m2], f3 = F3[s * K3 + m3];
+                    acc.x += B.x * f3.x - B.y * f3.y;
+                    acc.y += B.x * f3.y + B.y * f3.x;
+                }
+                plane[m2 * PS + m3] = acc;
+            }
+            __syncthreads();
+        }
+    } else {
+        // scatter of sum_s F1_s(m1) w2_s(x2) w3_s(x3), one lane per stencil
+        // point, sites in order; then the forward DFT along axes 3 and 2
+        if (tid < ord * ord) {
+            const int c2 = tid / ord, c3 = tid % ord;
+            for (int s = 0; s < nslt; ++s) {
+                const size_t site = rowb + s;
+                const int4 g = insg[site];
+                const float *w = insw + site * 3 * ord;
+                const float2 A = Ftab[site * per + m1];
+                const int x2 = pmod(g.y - c2, K2), x3 = pmod(g.z - c3, K3);
+                const float2 v = erfft::cscale(A, w[ord + c2] * w[2 * ord + c3]);
+                float2 *q = plane + x2 * PS + x3;
+                *q = make_float2(q->x + v.x, q->y + v.y);
+            }
+        }
+        __syncthreads();
+        lds_dft_lines<-1>(plane, K2, PS, 1, K3, t3, tid);
+        __syncthreads();
+        lds_dft_lines<-1>(plane, K3, 1, PS, K2, t2, tid);
+        __syncthreads();
+    }
+    // engfac multiply + FP64 half-plane self energy (engfac is [m1][m3][m2])
+    const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
+    const float *ef = engfac + (size_t)m1 * NE;
+    double sp = 0.0;
+    for (int e = tid; e < NE; e += NT) {
+        const int m2 = e % K2, m3 = e / K2;
+        const float f = ef[e];
+        float2 *q = plane + m2 * PS + m3;
+        const float2 a = *q;
+        sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+        *q = erfft::cscale(a, f);
+    }
+    __syncthreads();
+    lds_dft_lines<+1>(plane, K2, PS, 1, K3, t3, tid);
+    __syncthreads();
+    lds_dft_lines<+1>(plane, K3, 1, PS, K2, t2, tid);
+    __syncthreads();
+    for (int e = tid; e < NE; e += NT) Tp[e] = plane[(e / K3) * PS + (e % K3)];
+    for (int off = 32; off > 0; off >>= 1) sp += __shfl_down(sp, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = sp;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int wv = 0; wv < NT / 64; ++wv) t += red[wv];
+        selfpart[(size_t)n * H + m1] = t;
+    }
+}
+
+// Stage B for any K1: the complex-to-real DFT along axis 1 of one (x2, x3)
+// line of NI insertions per workgroup, phi(x1) = Re X(0) + 2 sum_{m=1}^{M'}
+// Re(X(m) e^{+2 pi i m x1 / K1}) [+ (-1)^x1 Re X(K1/2) for even K1], the
+// unnormalised C2R transform of the Hermitian half spectrum; phi layout as
+// k_pme_axis1 ([tile][x2][x3][x1][NI]).
+__global__ void __launch_bounds__(ER_GEN_NT) k_pme_axis1_gen(int nchunk, int NI, int K1, int K2, int K3,
+                                                             const float2 *__restrict__ T, float *__restrict__ phi,
+                                                             const float2 *__restrict__ tw1)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int H = K1 / 2 + 1;
+    float2 *X = reinterpret_cast<float2 *>(smem);  // [NI][H]
+    float2 *tw = X + (size_t)NI * H;               // [K1]
+    const int x3 = blockIdx.x, x2 = blockIdx.y, tile = blockIdx.z, tid = threadIdx.x;
+    const size_t plane = (size_t)K2 * K3;
+    for (int k = tid; k < NI * H; k += ER_GEN_NT) {
+        const int i = k / H, m = k % H, nl = tile * NI + i;
+        X[k] = (nl < nchunk) ? T[((size_t)nl * H + m) * plane + (size_t)x2 * K3 + x3] : make_float2(0.f, 0.f);
+    }
+    for (int k = tid; k < K1; k += ER_GEN_NT) tw[k] = tw1[k];
+    __syncthreads();
+    const int mh = (K1 % 2 == 0) ? H - 1 : H;  // terms m = 1 .. mh-1 appear with their conjugates
+    float *dst = phi + (size_t)tile * plane * K1 * NI + ((size_t)x2 * K3 + x3) * K1 * NI;
+    for (int k = tid; k < K1 * NI; k += ER_GEN_NT) {
+        const int x1 = k / NI, i = k % NI;
+        const float2 *Xi = X + (size_t)i * H;
+        float s = 0.0f;
+        int idx = 0;  // (m * x1) mod K1
+        for (int m = 1; m < mh; ++m) {
+            idx += x1;
+            if (idx >= K1) idx -= K1;
+            const float2 t = tw[idx];
+            s += Xi[m].x * t.x - Xi[m].y * t.y;
+        }
+        s = Xi[0].x + 2.0f * s;
+        if (K1 % 2 == 0) s += (x1 & 1) ? -Xi[H - 1].x : Xi[H - 1].x;
+        dst[k] = s;
     }
 }
 

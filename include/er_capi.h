@@ -70,8 +70,10 @@ typedef struct {
     float elecut, lwljcut, upljcut, screen;
     int32_t ljswitch;           /* 0..3, erdst/engmain.F90:358              */
     int32_t cltype;             /* 0 bare Coulomb, 2 PME, 3 PPPM (even splodr) */
-    int32_t splodr;             /* B-spline order, 2..12                    */
-    int32_t ms[3];              /* ms1max, ms2max, ms3max                    */
+    int32_t splodr;             /* B-spline order, 3..10                    */
+    int32_t ms[3];              /* ms1max, ms2max, ms3max: any size from
+                                   max(16, 2 splodr) to 512, the plane
+                                   ms2max x ms3max up to about 128 x 128    */
     int32_t boxshp;             /* 1 periodic; 0 open (cltype 0, estype 1)  */
     int32_t slttype;            /* 1 soln, 2 refs rigid, 3 refs flexible    */
     int32_t wgtslf, estype, corrcal, selfcal;

@@ -164,6 +164,38 @@ def test_noncubic_grid_and_order6():
     assert U.energy_mismatch(u0, u0o)[0] == 0
 
 
+@pytest.mark.parametrize("ms,order", [((60, 60, 60), 5), ((50, 45, 36), 6), ((100, 100, 100), 4),
+                                      ((32, 32, 32), 3), ((32, 32, 32), 7), ((45, 40, 40), 10)])
+def test_any_grid_and_order(ms, order):
+    """PME grids without a compiled four-step shape (the direct-DFT plane and
+    axis-1 kernels: 60^3, 50 x 45 x 36 with K2 != K3, 100^3, 45 x 40 x 40)
+    and spline orders 3 to 10 (erdst/fft_iface.F90:38-50 plans any size,
+    erdst/spline.F90:25-58 any order; odd orders zero engfac at m = K/2,
+    erdst/recpcal.F90:142): batched and plain energies against the oracle."""
+    w = synth.refs_water(0, 1000, 31.04, 24, 1, 32)
+    w.params["ms"] = ms
+    p = U.workload_params(w, splodr=order)
+    uo, u0o = U.oracle_refs(w, 0, p)
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx = _ctx(w, p)
+        u, u0 = _run_refs(ctx, w, 0, mode)
+        nbad, worst, at = U.energy_mismatch(u, uo)
+        print(f"grid {ms} order {order} mode {mode}: {U.energy_report(u, uo)}, uvengy0 "
+              f"{np.abs(u0 - u0o).max():.2e}, worst excess {worst:.3e}", flush=True)
+        assert nbad == 0, (mode, worst, at, u[at], uo[at])
+        assert U.energy_mismatch(u0, u0o)[0] == 0
+        ctx.close()
+
+
+def test_grid_and_order_limits():
+    w = synth.refs_water(0, 1000, 31.04, 4, 1, 32)
+    for ms, order in (((32, 32, 32), 2), ((32, 32, 32), 11), ((12, 32, 32), 4), ((32, 200, 200), 4)):
+        w.params["ms"] = ms
+        with pytest.raises(capi.ErError) as e:
+            capi.Context(w.topo, U.workload_params(w, splodr=order))
+        assert e.value.status == capi.ER_E_UNSUPPORTED
+
+
 def test_dense_bricks_read_overflow_records():
     """8000 waters in a 62.1 A cube on a 32^3 grid: about 250 atoms per 8x8x6
     brick, so the batched gather runs atoms beyond its 128 LDS-staged records
