@@ -739,20 +739,41 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
     } else {
         // ---- sparse spread of the plane, then forward DFT along axes 3, 2 ----
         for (int e = tid; e < K2 * PS; e += NT) plane[e] = make_float2(0.f, 0.f);
-        __syncthreads();
-        // one wave, one lane per stencil point (c2, c3): the ORD^2 points of a
-        // site are distinct, and the wave's LDS updates stay in site order
-        if (tid < ord * ord) {
-            const int c2 = tid / ord, c3 = tid % ord;
-            for (int s = 0; s < nslt; ++s) {
-                const size_t site = rowb + s;
+        // the sites' factors F1(m1), grid bases and axis-2/3 weights are
+        // staged in LDS (in the factor-table area, unused on this path) by the
+        // whole workgroup, chunk by chunk, so that the scatter below reads no
+        // global memory
+        float2 *sA = F2;
+        const int CH = (int)((size_t)ER_SCH * (K2 + K3) * sizeof(float2) / (sizeof(float2) + sizeof(int2) +
+                                                                             2 * ord * sizeof(float)));
+        int2 *sg = reinterpret_cast<int2 *>(sA + CH);
+        float *sw = reinterpret_cast<float *>(sg + CH);  // [CH][2 ord]: w2 then w3
+        for (int s0 = 0; s0 < nslt; s0 += CH) {
+            const int ns = min(CH, nslt - s0);
+            __syncthreads();
+            for (int f = tid; f < ns; f += NT) {
+                const size_t site = rowb + s0 + f;
                 const int4 g = insg[site];
-                const float *w = insw + site * 3 * ord;
-                const float2 A = Ftab[site * per + m1];
-                const int x2 = pmod(g.y - c2, K2), x3 = pmod(g.z - c3, K3);
-                const float2 v = erfft::cscale(A, w[ord + c2] * w[2 * ord + c3]);
-                float2 *q = plane + x2 * PS + x3;
-                *q = make_float2(q->x + v.x, q->y + v.y);
+                sA[f] = Ftab[site * per + m1];
+                sg[f] = make_int2(g.y, g.z);
+            }
+            for (int f = tid; f < ns * 2 * ord; f += NT) {
+                const int sl = f / (2 * ord), c = f % (2 * ord);
+                sw[f] = insw[(rowb + s0 + sl) * 3 * ord + ord + c];
+            }
+            __syncthreads();
+            // one wave, one lane per stencil point (c2, c3): the ORD^2 points of
+            // a site are distinct, and the wave's LDS updates stay in site order
+            if (tid < ord * ord) {
+                const int c2 = tid / ord, c3 = tid % ord;
+                for (int s = 0; s < ns; ++s) {
+                    const int2 g = sg[s];
+                    const float2 A = sA[s];
+                    const int x2 = pmod(g.x - c2, K2), x3 = pmod(g.y - c3, K3);
+                    const float2 v = erfft::cscale(A, sw[s * 2 * ord + c2] * sw[s * 2 * ord + ord + c3]);
+                    float2 *q = plane + x2 * PS + x3;
+                    *q = make_float2(q->x + v.x, q->y + v.y);
+                }
             }
         }
         __syncthreads();
