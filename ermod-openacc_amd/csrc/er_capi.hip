@@ -186,6 +186,8 @@ struct er_ctx {
     // spatial order / cell list (per frame)
     int nc = 1, ncell = 1;
     float cell_w = 1.f;            // smallest cell width (perpendicular), Angstrom
+    float cell_min = 0.f;          // tuned least cell width, Angstrom (0: the built-in 3.5)
+    double cell_wmin = 0.0;        // the frame cell's smallest perpendicular width
     DBuf<int> d_perm;
     DBuf<unsigned int> d_keyrank, d_ccnt, d_cstart;
     DBuf<float> d_rmolmax;
@@ -999,7 +1001,8 @@ static int frame_cell(er_ctx *c, const float *cell)
             const double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
             wmin = std::min(wmin, pv / std::sqrt(cx * cx + cy * cy + cz * cz));
         }
-        c->nc = std::max(1, std::min(32, (int)(wmin / 3.5)));
+        c->cell_wmin = wmin;
+        c->nc = std::max(1, std::min(32, (int)(wmin / (c->cell_min > 0.f ? c->cell_min : 3.5f))));
         c->ncell = 1 << (3 * 5);  // Morton keys of 5 bits per axis
         c->cell_w = (float)(wmin / c->nc);
     }
@@ -2172,6 +2175,8 @@ struct EventPair {
     }
 };
 
+extern "C" int er_set_cell_width(er_ctx *c, float angstrom);
+
 // a candidate replaces the current shape only when it is faster by more
 // than this fraction, so that timing noise keeps the current one
 constexpr double TUNE_MARGIN = 0.02;
@@ -2247,7 +2252,34 @@ static int autotune_runs(er_ctx *c, const std::vector<int> &todo, int reps)
         }
         c->hist_slices = best;
     }
-    if (c->prm.boxshp == 0) return ER_OK;  // the open-system pair kernel has no split
+    if (c->prm.boxshp == 0) return ER_OK;  // the open-system pair kernel has no split or cell list
+    // the cell list's least cell width: the pair kernel's own time over whole
+    // runs (the cell list is rebuilt for each candidate; cells partition the
+    // molecules whatever their size, so no result changes)
+    {
+        const float keep_w = c->cell_min;
+        float best = keep_w;
+        double best_ms = 0.0;
+        std::vector<float> ws = {keep_w};
+        for (float w : {3.5f, 5.0f, 7.0f, 10.0f})
+            if (w != keep_w && !(keep_w == 0.f && w == 3.5f)) ws.push_back(w);
+        for (size_t i = 0; i < ws.size(); ++i) {
+            if ((rc = er_set_cell_width(c, ws[i]))) return rc;
+            if ((rc = run_slots(c, todo, ER_BATCHED))) return rc;
+            prof_collect(c);
+            c->prof_ms[ER_K_PAIRS] = 0.0;
+            for (int r = 0; r < reps; ++r) {
+                c->frame_dirty = true;  // each run rebuilds the cell list, as a new frame would
+                if ((rc = run_slots(c, todo, ER_BATCHED))) return rc;
+            }
+            prof_collect(c);
+            if (i == 0 || c->prof_ms[ER_K_PAIRS] < best_ms * (1.0 - TUNE_MARGIN)) {
+                best_ms = c->prof_ms[ER_K_PAIRS];
+                best = ws[i];
+            }
+        }
+        if ((rc = er_set_cell_width(c, best))) return rc;
+    }
     for (int k : todo) {
         c->cur = k;
         int best = c->S().pair_split;
@@ -2295,6 +2327,7 @@ extern "C" int er_autotune(er_ctx *c, int32_t reps)
     const int keep = c->cur;
     // the shapes in force, restored if tuning fails part way
     const int chunk_keep = c->chunk, slices_keep = c->hist_slices;
+    const float cellw_keep = c->cell_min;
     std::vector<int> split_keep(c->slots.size());
     for (size_t k = 0; k < c->slots.size(); ++k) split_keep[k] = c->slots[k].pair_split;
     rc = autotune_runs(c, todo, reps);
@@ -2303,6 +2336,7 @@ extern "C" int er_autotune(er_ctx *c, int32_t reps)
     if (rc != ER_OK) {
         c->chunk = chunk_keep;
         c->hist_slices = slices_keep;
+        (void)er_set_cell_width(c, cellw_keep);
         for (size_t k = 0; k < c->slots.size(); ++k) c->slots[k].pair_split = split_keep[k];
     }
     (void)hipStreamSynchronize(c->stream);
@@ -2548,6 +2582,30 @@ extern "C" int er_graph_destroy(er_ctx *c, int32_t graph)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     graph_free(c->graphs[graph - 1]);
     c->graphs[graph - 1] = nullptr;
+    return ER_OK;
+}
+
+// the least cell width of the pair search's cell list: the cells of the next
+// run are rebuilt for it (the frame's cell is kept)
+extern "C" int er_set_cell_width(er_ctx *c, float angstrom)
+{
+    if (!c) return ER_E_ARG;
+    ER_NO_CAPTURE(c, "er_set_cell_width");
+    if (!(angstrom == 0.f || (angstrom >= 1.0f && angstrom <= 1000.0f)))
+        return fail(c, ER_E_ARG, "er_set_cell_width: 0 (built-in) or 1..1000 Angstrom");
+    c->cell_min = angstrom;
+    if (c->have_frame && c->prm.boxshp != 0 && c->cell_wmin > 0.0) {
+        c->nc = std::max(1, std::min(32, (int)(c->cell_wmin / (c->cell_min > 0.f ? c->cell_min : 3.5f))));
+        c->cell_w = (float)(c->cell_wmin / c->nc);
+        c->frame_dirty = true;
+    }
+    return ER_OK;
+}
+
+extern "C" int er_get_cell_width(er_ctx *c, float *angstrom)
+{
+    if (!c || !angstrom) return ER_E_ARG;
+    *angstrom = c->cell_min;
     return ER_OK;
 }
 

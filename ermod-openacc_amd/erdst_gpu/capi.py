@@ -23,7 +23,8 @@ EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_s
             "er_sync",
             "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
             "er_destroy", "er_profile", "er_profile_read", "er_autotune", "er_set_tuning", "er_get_tuning",
-            "er_graph_begin", "er_graph_end", "er_graph_launch", "er_graph_destroy"]
+            "er_graph_begin", "er_graph_end", "er_graph_launch", "er_graph_destroy", "er_set_cell_width",
+            "er_get_cell_width"]
 EXPORTED_TRAJ = ["er_traj_open", "er_traj_read", "er_traj_info", "er_traj_close"]
 ER_TRAJ_EOF, ER_TRAJ_IO, ER_TRAJ_FORMAT = 100, 101, 102
 TRAJ_FLAGS = {"charmm": 1, "cell": 2, "4d": 4, "rec64": 8, "swapped": 16}
@@ -91,6 +92,8 @@ def load() -> C.CDLL:
     L.er_set_tuning.argtypes = [vp, i32, i32, i32]
     L.er_get_tuning.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
     L.er_graph_begin.argtypes = [vp]
+    L.er_set_cell_width.argtypes = [vp, f32]
+    L.er_get_cell_width.argtypes = [vp, C.POINTER(f32)]
     L.er_graph_end.argtypes = [vp, C.POINTER(i32)]
     L.er_graph_launch.argtypes = [vp, i32]
     L.er_graph_destroy.argtypes = [vp, i32]
@@ -354,8 +357,9 @@ class Context:
 
     def tuning(self):
         """{"chunk": insertions per launch sequence, "hist_slices": histogram
-        workgroups per 16 insertions, "pair_split": [per solute]} (0: the
-        built-in choice); leaves solute 0 selected."""
+        workgroups per 16 insertions, "pair_split": [per solute],
+        "cell_width": least cell width of the pair search in Angstrom} (0:
+        the built-in choice); leaves solute 0 selected."""
         ch, sp, hs = C.c_int32(), C.c_int32(), C.c_int32()
         splits = []
         for k in range(self.nsolute):
@@ -365,7 +369,9 @@ class Context:
             splits.append(sp.value)
         if self.nsolute > 1:
             self.select_solute(0)
-        return {"chunk": ch.value, "hist_slices": hs.value, "pair_split": splits}
+        cw = C.c_float()
+        self._check(self.L.er_get_cell_width(self.h, C.byref(cw)))
+        return {"chunk": ch.value, "hist_slices": hs.value, "pair_split": splits, "cell_width": cw.value}
 
     def set_tuning(self, tuning):
         """Applies a tuning() dictionary (for instance one kept in a cache
@@ -378,6 +384,7 @@ class Context:
                                              int(tuning.get("hist_slices", 0))))
         if len(splits) > 1:
             self.select_solute(0)
+        self._check(self.L.er_set_cell_width(self.h, float(tuning.get("cell_width", 0.0))))
 
     def close(self):
         if self.h is not None and self.h.value:

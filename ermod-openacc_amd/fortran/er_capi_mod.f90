@@ -29,7 +29,7 @@ module er_capi
   public :: er_create, er_set_mesh, er_set_solute, er_set_solutes, er_select_solute, er_set_solute_molecules, &
        er_set_frame, er_set_insertions, er_run, er_sync, er_get_uvengy, er_fetch, &
        er_get_flceng, er_clear, er_last_error, er_destroy, er_traj_open, er_traj_read, er_traj_close, &
-       er_autotune, er_set_tuning, er_get_tuning
+       er_autotune, er_set_tuning, er_get_tuning, er_set_cell_width, er_get_cell_width
 
   interface
      integer(c_int) function er_create(topo, prm, ctx) bind(C, name="er_create")
@@ -144,6 +144,18 @@ module er_capi
        type(c_ptr), value :: ctx
        integer(c_int32_t), intent(out) :: chunk, pair_split, hist_slices
      end function er_get_tuning
+
+     integer(c_int) function er_set_cell_width(ctx, angstrom) bind(C, name="er_set_cell_width")
+       import :: c_int, c_float, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_float), value :: angstrom
+     end function er_set_cell_width
+
+     integer(c_int) function er_get_cell_width(ctx, angstrom) bind(C, name="er_get_cell_width")
+       import :: c_int, c_float, c_ptr
+       type(c_ptr), value :: ctx
+       real(c_float), intent(out) :: angstrom
+     end function er_get_cell_width
 
      integer(c_int) function er_clear(ctx) bind(C, name="er_clear")
        import :: c_int, c_ptr

@@ -242,7 +242,8 @@ int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int64_t *launch
  *   hist_slices histogram workgroups per 16 insertions of a refs run: the
  *               current one, 1 .. 64 (the histogram kernel's own time);
  *   pair_split  pair-search workgroups per insertion, per solute: the
- *               current one, 1 .. 64 (the pair kernel's own time).
+ *               current one, 1 .. 64 (the pair kernel's own time);
+ *   cell width  the pair search's least cell width (er_set_cell_width).
  * A candidate replaces the current shape only when it is more than 2%
  * faster, so the result is never worse than the current shapes by more than
  * timing noise; 0 stands for the built-in choice.  It restores every
@@ -258,6 +259,13 @@ int er_profile_read(er_ctx *c, int32_t kernel, double *total_ms, int64_t *launch
 int er_autotune(er_ctx *c, int32_t reps);
 int er_set_tuning(er_ctx *c, int32_t chunk, int32_t pair_split, int32_t hist_slices);
 int er_get_tuning(er_ctx *c, int32_t *chunk, int32_t *pair_split, int32_t *hist_slices);
+/* The least cell width (Angstrom) of the pair search's cell list, also tuned
+ * by er_autotune (candidates 3.5, 5, 7 and 10; 0 = the built-in 3.5).  The
+ * cells partition the molecules whatever their width, so no result changes;
+ * the next run rebuilds the frame's cell list.  No counterpart in the
+ * reference, which tests every (insertion, molecule) pair. */
+int er_set_cell_width(er_ctx *c, float angstrom);
+int er_get_cell_width(er_ctx *c, float *angstrom);
 
 #ifdef __cplusplus
 }

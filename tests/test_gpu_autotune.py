@@ -56,6 +56,7 @@ def test_autotune_keeps_accumulators_and_results(cfg0):
     assert t["chunk"] in (256, 512, 1024, 2048, 4096)
     assert t["hist_slices"] in (0, 1, 2, 4, 8, 16, 32, 64)
     assert len(t["pair_split"]) == 1 and t["pair_split"][0] in (0, 1, 2, 4, 8, 16, 32, 64)
+    assert t["cell_width"] in (0.0, 5.0, 7.0, 10.0)
     _same_block(tuned.fetch(), before)
     # the energies tuning leaves behind are the current frame's
     plain.run(capi.ER_BATCHED)
@@ -68,13 +69,14 @@ def test_autotune_keeps_accumulators_and_results(cfg0):
     _same_block(tuned.fetch(), plain.fetch())
 
 
-@pytest.mark.parametrize("chunk,split,slices", [(16, 1, 1), (48, 3, 7), (4096, 64, 64)])
-def test_explicit_tunings_equal_builtin(cfg0, chunk, split, slices):
+@pytest.mark.parametrize("chunk,split,slices,cellw", [(16, 1, 1, 5.0), (48, 3, 7, 10.0), (4096, 64, 64, 1.0)])
+def test_explicit_tunings_equal_builtin(cfg0, chunk, split, slices, cellw):
     w, nins = cfg0, 100
     p = U.workload_params(w)
     ref, alt = _ctx(w, p), _ctx(w, p)
-    alt.set_tuning({"chunk": chunk, "hist_slices": slices, "pair_split": [split]})
-    assert alt.tuning() == {"chunk": chunk, "hist_slices": slices, "pair_split": [split]}
+    want = {"chunk": chunk, "hist_slices": slices, "pair_split": [split], "cell_width": cellw}
+    alt.set_tuning(want)
+    assert alt.tuning() == want
     for f in (0, 1):
         for c in (ref, alt):
             _frame(c, w, f, nins)
@@ -84,7 +86,7 @@ def test_explicit_tunings_equal_builtin(cfg0, chunk, split, slices):
             np.testing.assert_array_equal(a, b)
     _same_block(alt.fetch(), ref.fetch())
     alt.set_tuning({"chunk": 0, "hist_slices": 0, "pair_split": [0]})
-    assert alt.tuning() == {"chunk": chunk, "hist_slices": 0, "pair_split": [0]}
+    assert alt.tuning() == {"chunk": chunk, "hist_slices": 0, "pair_split": [0], "cell_width": 0.0}
 
 
 @pytest.mark.parametrize("over", [dict(cltype=0, boxshp=0, estype=1), dict(cltype=0), dict(cltype=3)],
