@@ -389,6 +389,13 @@ def main():
     pctx.sync()
     torch.cuda.synchronize()
     plain = nrep * nplain / (time.perf_counter() - tp0)
+    # the same launches without the host waiting after each insertion
+    tp0 = time.perf_counter()
+    for _ in range(nrep):
+        pjob.run_frames(pctx, pl_pos, pl_ins, capi.ER_PLAIN_ASYNC, nins=None if job.soln else nplain)
+    pctx.sync()
+    torch.cuda.synchronize()
+    plain_async = nrep * nplain / (time.perf_counter() - tp0)
     if pctx is not ctx:
         pctx.close()
 
@@ -432,6 +439,10 @@ def main():
             "graph_record_s": record_s,
             "plain_insertions_per_s": round(plain, 1),
             "batched_over_plain": round(value / plain, 2),
+            "plain_async_insertions_per_s": round(plain_async, 1),
+            "batched_over_plain_async": round(value / plain_async, 2),
+            "plain_note": "plain = one insertion per launch sequence with a host wait after each (the reference's "
+                          "per-insertion loop); plain_async = the same launches without the waits",
             "precision": "pair/PME f32; self energy, weights, edens, ecorr f64",
             "tuning": tuning if tuning is not None else "built-in launch shapes",
             "tuning_s": tune_s,

@@ -1864,8 +1864,8 @@ static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
         HIPCHK(c, run_pairs_hist<NI_B>(c, 0, P.ntot, 0, P.pair_blocks));
         HIPCHK(c, run_accumulate(c, AccRange{0, 0, -1, 0}));
     } else {
-        // one insertion per launch sequence, synchronised like the reference's
-        // per-insertion host loop
+        // one insertion per launch sequence; ER_PLAIN also waits for each,
+        // like the reference's per-insertion host loop
         int kper1, nks1;
         ecorr_slices(c, 1, kper1, nks1);
         for (int i = 0; i < nslot; ++i) {
@@ -1875,7 +1875,7 @@ static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
                 if (c->prm.cltype != 0) HIPCHK(c, run_chunk<1>(c, n, 1));
                 HIPCHK(c, run_pairs_hist<1>(c, n, 1, d.pair_blk + j * d.pair_split, d.pair_split));
                 HIPCHK(c, run_accumulate(c, AccRange{i, j, 1, kper1}));
-                HIPCHK(c, hipStreamSynchronize(c->stream));
+                if (mode == ER_PLAIN) HIPCHK(c, hipStreamSynchronize(c->stream));
             }
         }
     }
@@ -1918,7 +1918,7 @@ static int run_plan(er_ctx *c, const char *who, std::vector<int> &todo)
 extern "C" int er_run(er_ctx *c, int32_t mode)
 {
     if (!c) return ER_E_ARG;
-    if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run: mode");
+    if (mode != ER_PLAIN && mode != ER_BATCHED && mode != ER_PLAIN_ASYNC) return fail(c, ER_E_ARG, "er_run: mode");
     if (c->capturing && mode != ER_BATCHED)
         return fail(c, ER_E_STATE, "er_run: a graph records ER_BATCHED runs only (ER_PLAIN waits per insertion)");
     std::vector<int> todo;
@@ -1933,7 +1933,8 @@ extern "C" int er_run(er_ctx *c, int32_t mode)
 extern "C" int er_run_noalloc(er_ctx *c, int32_t mode)
 {
     if (!c) return ER_E_ARG;
-    if (mode != ER_PLAIN && mode != ER_BATCHED) return fail(c, ER_E_ARG, "er_run_noalloc: mode");
+    if (mode != ER_PLAIN && mode != ER_BATCHED && mode != ER_PLAIN_ASYNC)
+        return fail(c, ER_E_ARG, "er_run_noalloc: mode");
     if (c->capturing) return fail(c, ER_E_STATE, "er_run_noalloc: not while a graph is recorded");
     if (!c->scratch_dev) return fail(c, ER_E_STATE, "er_run_noalloc: no scratch address for this device");
     std::vector<int> todo;

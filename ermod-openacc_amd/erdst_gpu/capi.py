@@ -15,7 +15,7 @@ PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "liberdst_hip.so")
 
 ER_OK, ER_E_ARG, ER_E_UNSUPPORTED, ER_E_HIP, ER_E_HALT, ER_E_STATE = range(6)
-ER_PLAIN, ER_BATCHED = 0, 1
+ER_PLAIN, ER_BATCHED, ER_PLAIN_ASYNC = 0, 1, 2
 
 EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_select_solute",
             "er_set_solute_molecules", "er_set_frame",

@@ -9,7 +9,7 @@ module er_capi
 
   integer(c_int), parameter, public :: ER_OK = 0, ER_E_ARG = 1, ER_E_UNSUPPORTED = 2, &
        ER_E_HIP = 3, ER_E_HALT = 4, ER_E_STATE = 5
-  integer(c_int), parameter, public :: ER_PLAIN = 0, ER_BATCHED = 1
+  integer(c_int), parameter, public :: ER_PLAIN = 0, ER_BATCHED = 1, ER_PLAIN_ASYNC = 2
   integer(c_int), parameter, public :: ER_TRAJ_EOF = 100, ER_TRAJ_IO = 101, ER_TRAJ_FORMAT = 102
 
   type, bind(C), public :: er_topology

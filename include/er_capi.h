@@ -46,7 +46,11 @@ enum {
 };
 
 /* run modes for er_run */
-enum { ER_PLAIN = 0, ER_BATCHED = 1 };
+/* ER_PLAIN: one insertion per launch sequence, the host waiting for each
+ * (the reference's per-insertion host loop); ER_PLAIN_ASYNC: the same
+ * launches without the waits; ER_BATCHED: every insertion of the frame in one
+ * launch per kernel class.  All three give the same results to rounding. */
+enum { ER_PLAIN = 0, ER_BATCHED = 1, ER_PLAIN_ASYNC = 2 };
 
 /* Topology of the physical system, as setparam leaves it in module engmain
  * (erdst/setconf.F90:516-844).  "Physical" = molecules whose coordinates are
@@ -137,8 +141,8 @@ int er_set_insertions_device(er_ctx *c, int32_t nins, const float *sltpos_dev,
 /* Replaces: the body of engconst for one frame (engproc.F90:419-458):
  * realcal_prepare + recpcal_spline_greenfunc + recpcal_prepare_solvent +
  * get_uv_energy_refs|soln + update_histogram x maxdst.  Results accumulate
- * into the block accumulators.  ER_PLAIN launches one insertion at a time;
- * ER_BATCHED runs every insertion of the frame, of every solute that has
+ * into the block accumulators.  ER_PLAIN / ER_PLAIN_ASYNC launch one
+ * insertion at a time; ER_BATCHED runs every insertion of the frame, of every solute that has
  * insertions (er_set_solutes), in one launch per kernel class (the
  * reciprocal part in chunks of insertions, one chunk by default when the
  * tuner picks it). */

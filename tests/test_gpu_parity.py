@@ -71,6 +71,25 @@ def test_plain_matches_batched(cfg0):
     np.testing.assert_allclose(u0b, u0p, rtol=1e-6, atol=1e-6)
 
 
+def test_plain_async_equals_plain(cfg0):
+    """ER_PLAIN_ASYNC issues ER_PLAIN's launches without the host waiting
+    after each insertion: energies and distributions bit for bit equal."""
+    p = U.workload_params(cfg0)
+    out = []
+    for mode in (capi.ER_PLAIN, capi.ER_PLAIN_ASYNC):
+        ctx = _ctx(cfg0, p)
+        u, u0 = _run_refs(ctx, cfg0, 0, mode, nins=20)
+        out.append((u, u0, ctx.fetch()))
+        ctx.close()
+    (ua, u0a, ga), (ub, u0b, gb) = out
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(u0a, u0b)
+    for k in ("edens", "ecorr", "minuv", "maxuv"):
+        np.testing.assert_array_equal(ga[k], gb[k])
+    for k in ("engnorm", "engsmpl", "avslf", "voffset"):
+        assert ga[k] == gb[k]
+
+
 def test_multi_chunk_equals_single_chunk(cfg0):
     p = U.workload_params(cfg0)
     u1, _ = _run_refs(_ctx(cfg0, p, chunk=32), cfg0, 2, capi.ER_BATCHED)
