@@ -231,13 +231,13 @@ def result_errors(job, capi, frame=0, nins=64):
     ed_g, ed_o = g["edens"] / g["engnorm"], d.edens / d.d.engnorm
     ec_g = g["ecorr"] / g["engnorm"]
     ec_o = d.ecorr.reshape(ermax, ermax) / d.d.engnorm
-    nbad, worst, _ = U.energy_mismatch(u, uo)
+    ns = U.nsites(w)
+    nbad, worst, _ = U.energy_mismatch(u, uo, ns)
     return dict(sample=f"frame {frame}, first {nins} insertions x {w.topo.nmol} molecules",
                 energy=U.energy_report(u, uo), outside_bar=nbad, worst_excess_over_bar=float(f"{worst:.3e}"),
-                bar=f"|dE| <= r|E| + {U.ATOL} kcal/mol, r = {U.RTOL} (|E| < {U.E_MID}), {U.RTOL_MID} "
-                    f"(< {U.E_HIGH}), {U.RTOL_HIGH} above",
+                bar=U.bar_text(ns),
                 uvengy0_max_abs=float(np.abs(u0 - u0o).max()),
-                hist=U.flip_report(u, uo, w.topo.uvspec, uvmax, uvcrd),
+                hist=U.flip_report(u, uo, w.topo.uvspec, uvmax, uvcrd, ns),
                 edens_max_abs=float(np.abs(ed_g - ed_o).max()), edens_max=float(np.abs(ed_o).max()),
                 ecorr_max_abs=float(np.abs(ec_g - ec_o).max()), ecorr_max=float(np.abs(ec_o).max()),
                 unit="kcal/mol (energies); molecules per bin (edens, ecorr)")

@@ -161,6 +161,7 @@ struct er_ctx {
     DBuf<float> d_sitepos_own;
     const float *d_sitepos = nullptr;
     DBuf<float4> d_posn, d_molref;
+    DBuf<uint4> d_posu;            // lattice coordinates (pair energies)
     DBuf<float4> d_segref;         // segment reference + radius at each segment's first atom
     DBuf<float> d_splw;
     DBuf<int4> d_splg;
@@ -168,6 +169,7 @@ struct er_ctx {
 
     // insertions
     DBuf<float4> d_insposn, d_insref;
+    DBuf<uint4> d_insposu;
     DBuf<float> d_insw, d_selfreal;
     DBuf<int4> d_insg;
 
@@ -652,6 +654,7 @@ extern "C" int er_create(const er_topology *topo, const er_params *prm, er_ctx *
 
     // per-atom frame buffers
     HIPCHK(c, c->d_posn.ensure(c->numatm));
+    HIPCHK(c, c->d_posu.ensure(c->numatm));
     HIPCHK(c, c->d_splw.ensure((size_t)c->numatm * 3 * c->ord));
     HIPCHK(c, c->d_splg.ensure(c->numatm));
     HIPCHK(c, c->d_molref.ensure(c->nummol));
@@ -887,9 +890,14 @@ static int frame_cell(er_ctx *c, const float *cell)
         std::memcpy(c->cell, cell, sizeof c->cell);
         return ER_OK;
     }
-    // column-major: cell(i,j) = cell[i + 3j]; sub-diagonal (2,1),(3,1),(3,2)
+    // column-major: cell(i,j) = cell[i + 3j]; sub-diagonal (2,1),(3,1),(3,2).
+    // cn is cell_normal in FP32 as the reference's real; cd follows it in
+    // FP64 (the same rotation, flip and lattice shifts) for the lattice
+    // coordinates of the pair energies.
     float cn[9];
+    double cd[9], qd[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
     std::memcpy(cn, cell, sizeof cn);
+    for (int k = 0; k < 9; ++k) cd[k] = cell[k];
     FrameConst &fc = c->fc;
     fc.rotate = 0;
     if (cell[1] != 0.f || cell[2] != 0.f || cell[5] != 0.f) {
@@ -911,14 +919,21 @@ static int frame_cell(er_ctx *c, const float *cell)
             }
         }
         for (int k = 0; k < 9; ++k) cn[k] = (float)r[k];
+        for (int k = 0; k < 9; ++k) cd[k] = r[k];
         for (int i = 0; i < 3; ++i)
-            for (int k = 0; k < 3; ++k) fc.rotT[3 * i + k] = (float)q[k + 3 * i];
+            for (int k = 0; k < 3; ++k) {
+                fc.rotT[3 * i + k] = (float)q[k + 3 * i];
+                qd[3 * i + k] = q[k + 3 * i];
+            }
         fc.rotate = 1;
     }
     // normalize_cell_vector (realcal.F90:826-830): only column 1 can be flipped
     for (int i = 0; i < 3; ++i)
         if (cn[0] < 0.f)
-            for (int r = 0; r < 3; ++r) cn[r + 3 * i] = -cn[r + 3 * i];
+            for (int r = 0; r < 3; ++r) {
+                cn[r + 3 * i] = -cn[r + 3 * i];
+                cd[r + 3 * i] = -cd[r + 3 * i];
+            }
     // its lattice reduction (realcal.F90:832-847): a column whose (1,2),
     // (1,3) or (2,3) element exceeds half the diagonal by more than 1e-4 is
     // shifted by the nearest whole multiple of column 1 (or 2), in float as
@@ -934,17 +949,20 @@ static int frame_cell(er_ctx *c, const float *cell)
         if (std::fabs(cn[3]) > 0.5f * cn[0] + thres) {
             const float f = anint(cn[3] * inv0);
             for (int r = 0; r < 3; ++r) cn[r + 3] -= cn[r] * f;
+            for (int r = 0; r < 3; ++r) cd[r + 3] -= cd[r] * f;
             reduced = true;
         }
         if (std::fabs(cn[6]) > 0.5f * cn[0] + thres) {
             const float f = anint(cn[6] * inv0);
             for (int r = 0; r < 3; ++r) cn[r + 6] -= cn[r] * f;
+            for (int r = 0; r < 3; ++r) cd[r + 6] -= cd[r] * f;
             reduced = true;
         }
         const float inv1 = 1.0f / std::fabs(cn[4]);
         if (std::fabs(cn[7]) > 0.5f * cn[4] + thres) {
             const float f = anint(cn[7] * inv1);
             for (int r = 0; r < 3; ++r) cn[r + 6] -= cn[r + 3] * f;
+            for (int r = 0; r < 3; ++r) cd[r + 6] -= cd[r + 3] * f;
             reduced = true;
         }
     }
@@ -958,6 +976,20 @@ static int frame_cell(er_ctx *c, const float *cell)
     for (int i = 0; i < 3; ++i) {
         fc.lenN[i] = std::fabs(cn[i + 3 * i]);
         fc.invN[i] = 1.0f / fc.lenN[i];
+    }
+    // lattice coordinates: s = cd^-1 Q^T x (cd upper triangular), and back
+    // to Cartesian differences through cell_normal / 2^32
+    {
+        const double a = cd[0], b = cd[3], d = cd[6], e = cd[4], f = cd[7], g = cd[8];
+        const double ic[9] = {1.0 / a, 0, 0, -b / (a * e), 1.0 / e, 0, (b * f - d * e) / (a * e * g), -f / (e * g),
+                              1.0 / g};
+        for (int k = 0; k < 3; ++k)
+            for (int j = 0; j < 3; ++j) {
+                double m = 0.0;
+                for (int r = 0; r < 3; ++r) m += ic[k + 3 * r] * qd[3 * r + j];
+                fc.fracM[k + 3 * j] = m;
+            }
+        for (int k = 0; k < 9; ++k) fc.dsc[k] = (float)std::ldexp(cd[k], -32);
     }
     fc.is_cuboid = !(std::fabs(cell[3]) > 1e-8f || std::fabs(cell[6]) > 1e-8f || std::fabs(cell[7]) > 1e-8f);
     // inverse cell in double (engproc.F90:1049-1070)
@@ -1414,7 +1446,8 @@ static hipError_t prep_frame(er_ctx *c)
             HIPCHK_E(hipStreamWaitEvent(s2, c->ev_in, 0));
         }
         hipLaunchKernelGGL(k_prep_atoms, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm, c->d_sitepos,
-                           c->d_charge.p, c->fc, c->K1, c->K2, c->K3, c->ord, c->d_posn.p, c->d_splw.p, c->d_splg.p);
+                           c->d_charge.p, c->fc, c->K1, c->K2, c->K3, c->ord, c->d_posn.p, c->d_posu.p, c->d_splw.p,
+                           c->d_splg.p);
         hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s2, (size_t)1,
                            reinterpret_cast<uint32_t *>(c->d_rmolmax.p));
         hipLaunchKernelGGL(k_prep_mols, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
@@ -1629,6 +1662,7 @@ static int ensure_run_buffers(er_ctx *c, int chunk)
         }
     }
     ER_WANT(c->d_insposn, nsite);
+    ER_WANT(c->d_insposu, nsite);
     ER_WANT(c->d_insw, nsite * 3 * c->ord);
     ER_WANT(c->d_insg, nsite);
     if (!soln) ER_WANT(c->d_sltcat, nsite * 3);
@@ -1726,8 +1760,9 @@ template <int NI> static hipError_t run_pairs_hist(er_ctx *c, int n_off, int nch
             }
         } else if (nblk > 0) {
             hipLaunchKernelGGL(k_pairs, dim3((unsigned)nblk), dim3(256), 0, c->stream, pa, blk0, nslot, c->d_sd.p,
-                               c->fc, c->d_molbeg.p, c->d_posn.p, c->d_ljtype.p, c->d_molref.p, c->d_segref.p,
-                               c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p, c->d_insref.p,
+                               c->fc, c->d_molbeg.p, c->d_posu.p, c->d_ljtype.p, c->d_molref.p, c->d_segref.p,
+                               c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p, c->d_insposu.p,
+                               c->d_insref.p,
                                soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p, c->d_ljlensq.p, c->d_Ucat.p);
             if ((e = hipGetLastError()) != hipSuccess) return e;
         }
@@ -1835,7 +1870,7 @@ static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
         hipLaunchKernelGGL(k_prep_sites, dim3((unsigned)((P.nsite + 255) / 256)), dim3(256), 0, c->stream, P.nsite,
                            nslot, sd, soln ? (const float *)nullptr : c->d_sltcat.p, c->d_sitepos, c->d_charge.p,
                            c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3,
-                           c->ord, c->d_insposn.p, c->d_insw.p, c->d_insg.p);
+                           c->ord, c->d_insposn.p, c->d_insposu.p, c->d_insw.p, c->d_insg.p);
         HIPCHK(c, hipGetLastError());
         hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)((P.ntot + 3) / 4)), dim3(256), 0, c->stream, P.ntot, nslot,
                            sd, c->fc, c->prm.screen, (int)(c->prm.cltype == 0), c->d_insposn.p, c->d_insref.p,
@@ -2628,8 +2663,8 @@ extern "C" void er_destroy(er_ctx *c)
     for (Slot &sl : c->slots) sl.release();
     c->d_tag.release();
     c->d_uvcrd.release(); c->d_escrd.release(); c->d_uvmax.release(); c->d_uvsoft.release(); c->d_idmin.release();
-    c->d_sitepos_own.release(); c->d_posn.release(); c->d_molref.release(); c->d_segref.release(); c->d_splw.release(); c->d_splg.release();
-    c->d_insposn.release(); c->d_insref.release(); c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
+    c->d_sitepos_own.release(); c->d_posn.release(); c->d_posu.release(); c->d_molref.release(); c->d_segref.release(); c->d_splw.release(); c->d_splg.release();
+    c->d_insposn.release(); c->d_insposu.release(); c->d_insref.release(); c->d_insw.release(); c->d_selfreal.release(); c->d_insg.release();
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_Ftab.release(); c->d_T.release(); c->d_phi.release();

@@ -1,7 +1,11 @@
 // er_kernels.h -- gfx950 kernels of the erdst per-frame hot path.
 //
 // Data layout in HBM (one context, one frame in flight):
-//   posn   [numatm] float4 (x,y,z wrapped as realcal_prepare; w = charge)
+//   posn   [numatm] float4 (x,y,z as read, rotated into the upper-triangular
+//          frame when the cell needs it; w = charge)
+//   posu   [numatm] uint4  (lattice coordinates: the fractional coordinates
+//          along the cell_normal axes times 2^32, wrapped modulo 2^32;
+//          w = the charge's bits) for the pair energies
 //   splw   [numatm][3][ORD] float    B-spline weights (recpcal.F90:504-538)
 //   splg   [numatm] int4             base grid index per axis
 //   T      [chunk][H][K2][K3] float2 spectrum after the axis-2/3 inverse DFT
@@ -20,6 +24,9 @@
 #define ER_MAXORD 12
 
 struct FrameConst {
+    double fracM[9];  // x (as read) -> fractional coordinates along the
+                      // cell_normal axes, s_k = sum_j fracM[k + 3j] x_j
+    float dsc[9];     // cell_normal / 2^32: lattice-coordinate differences -> Cartesian
     float cellN[9];  // cell_normal, column-major (realcal.F90:24)
     float invN[3];   // invcell_normal
     float lenN[3];   // cell_len_normal
@@ -104,14 +111,19 @@ __device__ __forceinline__ int pmod(int a, int K)
     return r < 0 ? r + K : r;
 }
 
-// minimum image of a difference vector (realcal.F90:271-284)
+// minimum image of a difference vector (realcal.F90:271-284): the nearest
+// image along each axis of a cuboid cell, else the reference's Z, Y, X
+// sequence of nearest-integer shifts.  The positions are not wrapped into the
+// cell first (realcal.F90:885-895): the minimum image of a difference is the
+// same for every lattice image of its ends, and unwrapped FP32 positions keep
+// their own rounding only.  Pair energies take their distances from lattice
+// coordinates instead (lattice_d2).
 __device__ __forceinline__ void min_image(const FrameConst &fc, float &dx, float &dy, float &dz)
 {
     if (fc.is_cuboid) {
-        float h0 = 0.5f * fc.lenN[0], h1 = 0.5f * fc.lenN[1], h2 = 0.5f * fc.lenN[2];
-        dx = h0 - fabsf(h0 - fabsf(dx));
-        dy = h1 - fabsf(h1 - fabsf(dy));
-        dz = h2 - fabsf(h2 - fabsf(dz));
+        dx = fmaf(-fc.lenN[0], rintf(dx * fc.invN[0]), dx);
+        dy = fmaf(-fc.lenN[1], rintf(dy * fc.invN[1]), dy);
+        dz = fmaf(-fc.lenN[2], rintf(dz * fc.invN[2]), dz);
     } else {
         float f = roundf(dz * fc.invN[2]);
         dx -= fc.cellN[6] * f; dy -= fc.cellN[7] * f; dz -= fc.cellN[8] * f;
@@ -122,10 +134,26 @@ __device__ __forceinline__ void min_image(const FrameConst &fc, float &dx, float
     }
 }
 
+// min_image in FP64 (the solute's self energy)
+__device__ __forceinline__ void min_image_d(const FrameConst &fc, double &dx, double &dy, double &dz)
+{
+    if (fc.is_cuboid) {
+        dx = fma(-(double)fc.lenN[0], rint(dx / (double)fc.lenN[0]), dx);
+        dy = fma(-(double)fc.lenN[1], rint(dy / (double)fc.lenN[1]), dy);
+        dz = fma(-(double)fc.lenN[2], rint(dz / (double)fc.lenN[2]), dz);
+    } else {
+        double f = round(dz / (double)fc.lenN[2]);
+        dx -= fc.cellN[6] * f; dy -= fc.cellN[7] * f; dz -= fc.cellN[8] * f;
+        f = round(dy / (double)fc.lenN[1]);
+        dx -= fc.cellN[3] * f; dy -= fc.cellN[4] * f; dz -= fc.cellN[5] * f;
+        f = round(dx / (double)fc.lenN[0]);
+        dx -= fc.cellN[0] * f; dy -= fc.cellN[1] * f; dz -= fc.cellN[2] * f;
+    }
+}
+
 // rotate into the upper-triangular frame when the cell needs it
-// (realcal.F90:814-816), then wrap into the cell with Z, then Y, then X
-// shifts (realcal.F90:885-895)
-__device__ __forceinline__ void wrap_site(const FrameConst &fc, float &x, float &y, float &z)
+// (realcal.F90:814-816)
+__device__ __forceinline__ void rotate_site(const FrameConst &fc, float &x, float &y, float &z)
 {
     if (fc.rotate) {
         const float rx = fc.rotT[0] * x + fc.rotT[1] * y + fc.rotT[2] * z;
@@ -133,12 +161,51 @@ __device__ __forceinline__ void wrap_site(const FrameConst &fc, float &x, float 
         const float rz = fc.rotT[6] * x + fc.rotT[7] * y + fc.rotT[8] * z;
         x = rx; y = ry; z = rz;
     }
-    float f = floorf(fc.invN[2] * z);
-    x -= fc.cellN[6] * f; y -= fc.cellN[7] * f; z -= fc.cellN[8] * f;
-    f = floorf(fc.invN[1] * y);
-    x -= fc.cellN[3] * f; y -= fc.cellN[4] * f; z -= fc.cellN[5] * f;
-    f = floorf(fc.invN[0] * x);
-    x -= fc.cellN[0] * f; y -= fc.cellN[1] * f; z -= fc.cellN[2] * f;
+}
+
+// Lattice coordinates of a site: its fractional coordinates along the
+// cell_normal axes (from the position as read, in FP64), wrapped into
+// [0, 1) and held as 32-bit fixed point.  The difference of two of them in
+// 32-bit integer arithmetic is the fractional difference vector wrapped into
+// [-1/2, 1/2) with no rounding at all, so the periodic wrap of a pair costs
+// nothing and its distance carries only the FP32 rounding of the short
+// difference itself (1e-7 relative), not that of coordinates as large as the
+// box (the reference's single build, wrapping and folding FP32 coordinates,
+// loses up to half an ulp of the box length per axis: 1e-5 relative at
+// 2 A in an 80 A box).
+__device__ __forceinline__ uint4 lattice_site(const FrameConst &fc, float x, float y, float z, float q)
+{
+    const double X = x, Y = y, Z = z;
+    unsigned int u[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        double f = fc.fracM[k] * X + fc.fracM[k + 3] * Y + fc.fracM[k + 6] * Z;
+        f -= floor(f);
+        u[k] = (unsigned int)(unsigned long long)(long long)rint(f * 4294967296.0);  // 2^32 wraps to 0
+    }
+    return make_uint4(u[0], u[1], u[2], __float_as_uint(q));
+}
+
+// squared minimum-image distance of two sites from their lattice coordinates
+// (realcal.F90:271-285): the wrapped fractional difference to Cartesian
+// (cell_normal is upper triangular), then for a skewed cell the reference's
+// Z, Y, X shift sequence, which depends only on the difference modulo the
+// lattice
+__device__ __forceinline__ float lattice_d2(const FrameConst &fc, const uint4 &a, const uint4 &b)
+{
+    const float sx = (float)(int)(a.x - b.x), sy = (float)(int)(a.y - b.y), sz = (float)(int)(a.z - b.z);
+    float dx, dy, dz;
+    if (fc.is_cuboid) {
+        dx = sx * fc.dsc[0];
+        dy = sy * fc.dsc[4];
+        dz = sz * fc.dsc[8];
+    } else {
+        dx = fmaf(fc.dsc[0], sx, fmaf(fc.dsc[3], sy, fc.dsc[6] * sz));
+        dy = fmaf(fc.dsc[4], sy, fc.dsc[7] * sz);
+        dz = fc.dsc[8] * sz;
+        min_image(fc, dx, dy, dz);
+    }
+    return dx * dx + dy * dy + dz * dz;
 }
 
 // B-spline weights w[c] = M_ord(t + c), c = 0..ord-1, by the Cox-de Boor
@@ -182,14 +249,15 @@ __device__ __forceinline__ void site_spline(const FrameConst &fc, float x, float
 // (recpcal.F90:340-357) for every physical atom
 __global__ void k_prep_atoms(int natom, const float *__restrict__ pos, const float *__restrict__ charge,
                              FrameConst fc, int K1, int K2, int K3, int ord, float4 *__restrict__ posn,
-                             float *__restrict__ splw, int4 *__restrict__ splg)
+                             uint4 *__restrict__ posu, float *__restrict__ splw, int4 *__restrict__ splg)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     if (a >= natom) return;
     const float x = pos[3 * (size_t)a], y = pos[3 * (size_t)a + 1], z = pos[3 * (size_t)a + 2];
     float wx = x, wy = y, wz = z;
-    wrap_site(fc, wx, wy, wz);
+    rotate_site(fc, wx, wy, wz);
     posn[a] = make_float4(wx, wy, wz, charge[a]);
+    posu[a] = lattice_site(fc, x, y, z, charge[a]);
     float w[3 * ER_MAXORD];
     int4 g;
     site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
@@ -243,11 +311,11 @@ __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const floa
 __global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ sd, const float *__restrict__ sltpos,
                              const float *__restrict__ frame_pos, const float *__restrict__ charge,
                              const int *__restrict__ molbeg, const int *__restrict__ tag, FrameConst fc, int K1,
-                             int K2, int K3, int ord, float4 *__restrict__ insposn, float *__restrict__ insw,
-                             int4 *__restrict__ insg)
+                             int K2, int K3, int ord, float4 *__restrict__ insposn, uint4 *__restrict__ insposu,
+                             float *__restrict__ insw, int4 *__restrict__ insg)
 {
     // one thread per site of the run (every solute's insertions end to end):
-    // spline weights, grid base, wrapped position
+    // spline weights, grid base, position and lattice coordinates
     const int is = blockIdx.x * blockDim.x + threadIdx.x;
     if (is >= nsite) return;
     const int k = slot_find(nslot, is, [&](int m) { return sd[m].site_off; });
@@ -269,7 +337,8 @@ __global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ 
     site_spline(fc, x, y, z, K1, K2, K3, ord, w, g);
     for (int c = 0; c < 3 * ord; ++c) insw[(size_t)is * 3 * ord + c] = w[c];
     insg[is] = g;
-    wrap_site(fc, x, y, z);
+    insposu[is] = lattice_site(fc, x, y, z, q);
+    rotate_site(fc, x, y, z);
     insposn[is] = make_float4(x, y, z, q);
 }
 
@@ -311,21 +380,24 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
         }
         return;
     }
-    // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r;
-    // FP32 terms as in the reference's single build, summed in FP64 (a
-    // 150-site solute has 11 175 terms that largely cancel): lane l takes
-    // the pairs (s, t) with t - s - 1 = l mod 64, then a fixed-order tree
+    // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r,
+    // in FP64 from the FP32 positions and charges: a 150-site solute has
+    // 11 175 terms of up to tens of kcal/mol that largely cancel, so FP32
+    // terms would leave ~1e-4 kcal/mol in this weight-bearing self energy;
+    // once per insertion, FP64 costs nothing here.  Lane l takes the pairs
+    // (s, t) with t - s - 1 = l mod 64, then a fixed-order tree.
     double e = 0.0;
-    const float rsqpi = 0.564189583547756286948079451560772586f;  // 1/sqrt(pi)
-    for (int s = lane; s < nslt; s += 64) e += (double)(-P[s].w * P[s].w * screen * rsqpi);
+    const double rsqpi = 0.564189583547756286948079451560772586;  // 1/sqrt(pi)
+    const double scr = screen;
+    for (int s = lane; s < nslt; s += 64) e += -(double)P[s].w * (double)P[s].w * scr * rsqpi;
     for (int s = 0; s < nslt; ++s) {
         const float4 a = P[s];
         for (int t = s + 1 + lane; t < nslt; t += 64) {
             const float4 b = P[t];
-            float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
-            min_image(fc, dx, dy, dz);
-            const float r = sqrtf(dx * dx + dy * dy + dz * dz);
-            e += (double)(-a.w * b.w * erff(screen * r) / r);
+            double dx = (double)a.x - (double)b.x, dy = (double)a.y - (double)b.y, dz = (double)a.z - (double)b.z;
+            min_image_d(fc, dx, dy, dz);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            e += -(double)a.w * (double)b.w * erf(scr * r) / r;
         }
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
@@ -1929,24 +2001,23 @@ struct PairArgs {
 
 template <class LJP>
 __device__ __forceinline__ float molecule_pair_sum(const PairArgs &pa, const FrameConst &fc, int n, int jm,
-                                                   const int *__restrict__ molbeg, const float4 *__restrict__ posn,
+                                                   const int *__restrict__ molbeg, const uint4 *__restrict__ posu,
                                                    const int *__restrict__ ljtype,
-                                                   const float4 *__restrict__ insposn, LJP sltlj,
+                                                   const uint4 *__restrict__ insposu, LJP sltlj,
                                                    int slt_a0, bool soln, const float *__restrict__ ljene,
                                                    const float *__restrict__ ljlensq)
 {
     float e = 0.0f;
     for (int s = 0; s < pa.nslt; ++s) {
-        const float4 xa = insposn[(size_t)n * pa.nslt + s];
+        const uint4 xa = insposu[(size_t)n * pa.nslt + s];
         const int ti = soln ? ljtype[slt_a0 + s] : sltlj[s];
         for (int b = molbeg[jm]; b < molbeg[jm + 1]; ++b) {
-            const float4 xb = posn[b];
-            float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
-            min_image(fc, dx, dy, dz);
-            const float d2 = dx * dx + dy * dy + dz * dz;
+            const uint4 xb = posu[b];
+            const float d2 = lattice_d2(fc, xa, xb);
             if (d2 > pair_skip2(pa.pp)) continue;
             const int tj = ljtype[b];
-            e += pair_energy(pa.pp, d2, xa.w, xb.w, ljene[ti + pa.ntype * tj], ljlensq[ti + pa.ntype * tj]);
+            e += pair_energy(pa.pp, d2, __uint_as_float(xa.w), __uint_as_float(xb.w), ljene[ti + pa.ntype * tj],
+                             ljlensq[ti + pa.ntype * tj]);
         }
     }
     return e;
@@ -2001,46 +2072,46 @@ __global__ void __launch_bounds__(256) k_pairs_open(PairArgs pa, int nslot, cons
 
 __device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const FrameConst &fc, int jm,
                                                        const int *__restrict__ molbeg,
-                                                       const float4 *__restrict__ posn,
-                                                       const int *__restrict__ ljtype, const float4 *sx,
+                                                       const uint4 *__restrict__ posu,
+                                                       const int *__restrict__ ljtype, const uint4 *sxu,
                                                        const int *stj, const float *te, const float *ts)
 {
     const int b0 = molbeg[jm], b1 = molbeg[jm + 1];
-    float4 xb4[4];
+    uint4 xb4[4];
     int tb4[4];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        xb4[k] = (b0 + k < b1) ? posn[b0 + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+        xb4[k] = (b0 + k < b1) ? posu[b0 + k] : make_uint4(0u, 0u, 0u, 0u);
         tb4[k] = (b0 + k < b1) ? ljtype[b0 + k] : 0;
     }
     float e = 0.0f;
     const float skip2 = pair_skip2(pa.pp);
-    auto term = [&](const float4 &xa, int ti, const float4 &xb, int tj) {
-        float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
-        min_image(fc, dx, dy, dz);
-        const float d2 = dx * dx + dy * dy + dz * dz;
+    auto term = [&](const uint4 &xa, int ti, const uint4 &xb, int tj) {
+        const float d2 = lattice_d2(fc, xa, xb);
         if (d2 > skip2) return;
-        e += pair_energy(pa.pp, d2, xa.w, xb.w, te[ti + pa.ntype * tj], ts[ti + pa.ntype * tj]);
+        e += pair_energy(pa.pp, d2, __uint_as_float(xa.w), __uint_as_float(xb.w), te[ti + pa.ntype * tj],
+                         ts[ti + pa.ntype * tj]);
     };
     for (int s = 0; s < pa.nslt; ++s) {
-        const float4 xa = sx[s];
+        const uint4 xa = sxu[s];
         const int ti = stj[s];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
             if (b0 + k < b1) term(xa, ti, xb4[k], tb4[k]);
-        for (int b = b0 + 4; b < b1; ++b) term(xa, ti, posn[b], ljtype[b]);
+        for (int b = b0 + 4; b < b1; ++b) term(xa, ti, posu[b], ljtype[b]);
     }
     return e;
 }
 
 __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot, const SlotDesc *__restrict__ sd,
                                                FrameConst fc, const int *__restrict__ molbeg,
-                                               const float4 *__restrict__ posn, const int *__restrict__ ljtype,
+                                               const uint4 *__restrict__ posu, const int *__restrict__ ljtype,
                                                const float4 *__restrict__ molref,
                                                const float4 *__restrict__ segref,
                                                const float *__restrict__ rmolmax,
                                                const unsigned int *__restrict__ cstart,
                                                const int *__restrict__ perm, const float4 *__restrict__ insposn_all,
+                                               const uint4 *__restrict__ insposu_all,
                                                const float4 *__restrict__ insref,
                                                const int *__restrict__ tag, const float *__restrict__ ljene,
                                                const float *__restrict__ ljlensq, float *__restrict__ U)
@@ -2048,6 +2119,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     __shared__ int cand[ER_PAIR_CAP];
     __shared__ int ncand;
     __shared__ float4 sx[ER_PAIR_SCAP];
+    __shared__ uint4 sxu[ER_PAIR_SCAP];
     __shared__ int stj[ER_PAIR_SCAP];
     __shared__ float te[ER_PAIR_TCAP], ts[ER_PAIR_TCAP];
     // workgroup b of the run (blk0 + blockIdx.x) belongs to the solute whose
@@ -2065,6 +2137,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     const __attribute__((address_space(1))) int *sltlj = gptr(sd[ks].sltlj);
     // the insertion's sites (molecule_pair_sum reads them as insertion 0)
     const float4 *__restrict__ insposn = insposn_all + sd[ks].site_off + (size_t)nl * pa.nslt;
+    const uint4 *__restrict__ insposu = insposu_all + sd[ks].site_off + (size_t)nl * pa.nslt;
     const int tid = threadIdx.x, NT = blockDim.x;
     const float4 ref = insref[n];
     const float cut = fmaxf(pa.pp.upljcut, pa.pp.elecut) + ref.w + 1e-3f;
@@ -2079,6 +2152,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     if (in_lds) {
         for (int k = tid; k < pa.nslt; k += NT) {
             sx[k] = insposn[k];
+            sxu[k] = insposu[k];
             stj[k] = soln ? ljtype[slt_a0 + k] : sltlj[k];
         }
         for (int k = tid; k < nt2; k += NT) {
@@ -2088,8 +2162,8 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     }
     __syncthreads();
     auto pair_sum = [&](int j) {
-        return in_lds ? molecule_pair_sum_lds(pa, fc, j, molbeg, posn, ljtype, sx, stj, te, ts)
-                      : molecule_pair_sum(pa, fc, 0, j, molbeg, posn, ljtype, insposn, sltlj, slt_a0, soln, ljene,
+        return in_lds ? molecule_pair_sum_lds(pa, fc, j, molbeg, posu, ljtype, sxu, stj, te, ts)
+                      : molecule_pair_sum(pa, fc, 0, j, molbeg, posu, ljtype, insposu, sltlj, slt_a0, soln, ljene,
                                           ljlensq);
     };
     int cx, cy, cz;
@@ -2149,6 +2223,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     const float skipc = fmaxf(pa.pp.upljcut, pa.pp.elecut) * 1.0001f + 1e-3f;
     auto site_sum = [&](int s, int jm) {
         const float4 xa = in_lds ? sx[s] : insposn[s];
+        const uint4 xu = in_lds ? sxu[s] : insposu[s];
         const int ti = in_lds ? stj[s] : (soln ? ljtype[slt_a0 + s] : sltlj[s]);
         float e = 0.0f;
         // the molecule's reference site and reach radius: beyond cutoff +
@@ -2173,15 +2248,13 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
                 if (qx * qx + qy * qy + qz * qz > ql * ql) continue;
             }
             for (int b = s0; b < min(s0 + ER_SEG, b1); ++b) {
-                const float4 xb = posn[b];
+                const uint4 xb = posu[b];
                 const int tj = ljtype[b];
-                float dx = xa.x - xb.x, dy = xa.y - xb.y, dz = xa.z - xb.z;
-                min_image(fc, dx, dy, dz);
-                const float d2 = dx * dx + dy * dy + dz * dz;
+                const float d2 = lattice_d2(fc, xu, xb);
                 if (d2 > skip2) continue;
                 const float le = in_lds ? te[ti + pa.ntype * tj] : ljene[ti + pa.ntype * tj];
                 const float ls = in_lds ? ts[ti + pa.ntype * tj] : ljlensq[ti + pa.ntype * tj];
-                e += pair_energy(pa.pp, d2, xa.w, xb.w, le, ls);
+                e += pair_energy(pa.pp, d2, xa.w, __uint_as_float(xb.w), le, ls);
             }
         }
         return e;

@@ -41,11 +41,12 @@ def test_config3_solute_energies(cfg3, k):
     ctx.sync()
     u, u0 = ctx.uvengy(8)
     uo, u0o = U.oracle_refs(w, 0, p, nins=8)
-    nbad, worst, at = U.energy_mismatch(u, uo)
+    ns = U.nsites(w)
+    nbad, worst, at = U.energy_mismatch(u, uo, ns)
     print(f"ENERGY config3 solute {k} ({w.topo.slt_charge32.size} sites): {U.energy_report(u, uo)}, "
           f"uvengy0 {U.energy_report(u0, u0o)}, worst excess {worst:.3e}")
     assert nbad == 0, (worst, at, u[at], uo[at])
-    assert U.energy_mismatch(u0, u0o)[0] == 0
+    assert U.self_mismatch(u0, u0o, ns)[0] == 0
 
 
 @pytest.fixture(scope="module")
@@ -73,12 +74,13 @@ def test_config2_energies_and_distributions(cfg2):
         ctx.sync()
         u, u0 = ctx.uvengy(nins)
         uo, u0o = U.oracle_refs(w, f, p, nins=nins)
-        nbad, worst, at = U.energy_mismatch(u, uo)
-        fl = U.flip_report(u, uo, w.topo.uvspec, uvmax, uvcrd)
+        ns = U.nsites(w)
+        nbad, worst, at = U.energy_mismatch(u, uo, ns)
+        fl = U.flip_report(u, uo, w.topo.uvspec, uvmax, uvcrd, ns)
         print(f"ENERGY config2 frame {f}: {U.energy_report(u, uo)}, uvengy0 {U.energy_report(u0, u0o)}, "
               f"worst excess {worst:.3e}; flips {fl}")
         assert nbad == 0, (worst, at, u[at], uo[at])
-        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.self_mismatch(u0, u0o, ns)[0] == 0
         assert fl["unexplained"] == 0
         for n in range(nins):
             d_gpu.update(w.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
@@ -142,18 +144,18 @@ def test_config4_slab_energies(cfg4):
     ctx.sync()
     u, u0 = ctx.uvengy(6)
     uo, u0o = U.oracle_refs(w, 0, p, nins=6)
-    nbad, worst, at = U.energy_mismatch(u, uo)
-    print(f"config 4: max |dE| {np.abs(u - uo).max():.3e}, worst excess {worst:.3e}, "
+    nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(w))
+    print(f"ENERGY config 4: {U.energy_report(u, uo)}, worst excess {worst:.3e}, "
           f"uvengy0 {u0.tolist()} vs {u0o.tolist()}")
     assert nbad == 0, (worst, at, u[at], uo[at])
-    assert U.energy_mismatch(u0, u0o)[0] == 0
+    assert U.self_mismatch(u0, u0o, U.nsites(w))[0] == 0
 
 
 def test_config4_distribution_plain_equals_batched(cfg4):
     w = cfg4
     p = U.workload_params(w)
     res = U.plain_and_batched(w, p, 1, 20, capi)
-    U.assert_plain_matches_batched(res)
+    U.assert_plain_matches_batched(res, U.nsites(w))
     nm = np.bincount(w.topo.uvspec, minlength=3)[1:]
     g = res[0][2]
     en = g["edens"] / g["engnorm"]

@@ -140,7 +140,7 @@ def test_fortran_host_soln_flcuv(tmp_path):
         want += engstore.flcuv_lines(f + 1, fl)
         uo, _ = U.oracle_soln(w, f, p)
         ref = np.array([uo[0][sp == s + 1].sum() for s in range(w.topo.numslv)])
-        tol = U.energy_tolerance(uo[0][sp > 0]).sum()
+        tol = U.energy_tolerance(uo[0][sp > 0], U.nsites(w)).sum()
         assert np.all(np.abs(fl[0] - ref) <= tol), (fl[0], ref, tol)
     got = open(tmp_path / "flcuv.tt").read().splitlines()
     assert got == want and len(got) == 2

@@ -86,7 +86,7 @@ def test_graph_replays_equal_eager_runs(cfg0):
             gr.clear()
     # the replayed energies are the FP64 oracle's to the energy bar
     uo, _ = U.oracle_refs(w, 3, p, nins=nins)
-    assert U.energy_mismatch(ug, uo)[0] == 0
+    assert U.energy_mismatch(ug, uo, U.nsites(w))[0] == 0
     gr.graph_destroy(h)
     eager.close()
     gr.close()

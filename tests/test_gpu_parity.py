@@ -1,7 +1,7 @@
 """Parity of the HIP path (through the C ABI) against the FP64 oracle.
 
-Energies: per (insertion, molecule) within |dE| <= RTOL*|E| + ATOL (see
-erutil).  Histograms: the GPU's counts must equal the oracle's binning of
+Energies: per (insertion, molecule) within |dE| <= ATOL_SITE sqrt(n) +
+RTOL |E| for a solute of n sites (erutil).  Histograms: the GPU's counts must equal the oracle's binning of
 the GPU's own FP32 energies exactly, so edens/ecorr/engnorm agree to FP64
 summation-order rounding; bins that differ when the oracle bins its FP64
 energies are counted and must all lie within the energy tolerance of an
@@ -55,11 +55,11 @@ def test_refs_energies_cfg0(cfg0, mode):
     ctx = _ctx(cfg0, p)
     u, u0 = _run_refs(ctx, cfg0, 0, mode)
     uo, u0o = U.oracle_refs(cfg0, 0, p)
-    nbad, worst, at = U.energy_mismatch(u, uo)
+    nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(cfg0))
     print(f"mode {mode}: max |dE| {np.abs(u - uo).max():.3e}, worst excess {worst:.3e} at {at}, "
           f"E_gpu {u[at]:.6f} E_64 {uo[at]:.6f}")
     assert nbad == 0
-    nbad0, _, at0 = U.energy_mismatch(u0, u0o)
+    nbad0, _, at0 = U.self_mismatch(u0, u0o, U.nsites(cfg0))
     assert nbad0 == 0, (u0[at0], u0o[at0])
 
 
@@ -118,7 +118,7 @@ def _distribution_check(ctx, w, p, frames, mode):
             # every flip must be explained by the energy tolerance straddling an edge
             lo = np.minimum(u.astype(np.float64), uo)[diff]
             hi = np.maximum(u.astype(np.float64), uo)[diff]
-            tol = U.energy_tolerance(uo[diff])
+            tol = U.energy_tolerance(uo[diff], U.nsites(w))
             edge_between = np.searchsorted(mesh, hi, side="left") > np.searchsorted(mesh, lo, side="left")
             assert edge_between.all()
             assert (hi - lo <= tol).all()
@@ -162,7 +162,7 @@ def test_histograms_cfg0(cfg0):
 
 def test_plain_histograms_equal_batched(cfg0):
     p = U.workload_params(cfg0)
-    U.assert_plain_matches_batched(U.plain_and_batched(cfg0, p, 3, 50, capi))
+    U.assert_plain_matches_batched(U.plain_and_batched(cfg0, p, 3, 50, capi), U.nsites(cfg0))
 
 
 @pytest.mark.parametrize("ljswitch", [1, 2, 3])
@@ -170,7 +170,7 @@ def test_lj_switch_forms(cfg0, ljswitch):
     p = U.workload_params(cfg0, ljswitch=ljswitch)
     u, _ = _run_refs(_ctx(cfg0, p), cfg0, 0, capi.ER_BATCHED, nins=20)
     uo, _ = U.oracle_refs(cfg0, 0, p, nins=20)
-    assert U.energy_mismatch(u, uo)[0] == 0
+    assert U.energy_mismatch(u, uo, U.nsites(cfg0))[0] == 0
 
 
 def test_noncubic_grid_and_order6():
@@ -179,8 +179,8 @@ def test_noncubic_grid_and_order6():
     p = U.workload_params(w, splodr=6)
     u, u0 = _run_refs(_ctx(w, p), w, 0, capi.ER_BATCHED)
     uo, u0o = U.oracle_refs(w, 0, p)
-    assert U.energy_mismatch(u, uo)[0] == 0
-    assert U.energy_mismatch(u0, u0o)[0] == 0
+    assert U.energy_mismatch(u, uo, U.nsites(w))[0] == 0
+    assert U.self_mismatch(u0, u0o, U.nsites(w))[0] == 0
 
 
 @pytest.mark.parametrize("ms,order", [((60, 60, 60), 5), ((50, 45, 36), 6), ((100, 100, 100), 4),
@@ -198,11 +198,11 @@ def test_any_grid_and_order(ms, order):
     for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
         ctx = _ctx(w, p)
         u, u0 = _run_refs(ctx, w, 0, mode)
-        nbad, worst, at = U.energy_mismatch(u, uo)
+        nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(w))
         print(f"grid {ms} order {order} mode {mode}: {U.energy_report(u, uo)}, uvengy0 "
               f"{np.abs(u0 - u0o).max():.2e}, worst excess {worst:.3e}", flush=True)
         assert nbad == 0, (mode, worst, at, u[at], uo[at])
-        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.self_mismatch(u0, u0o, U.nsites(w))[0] == 0
         ctx.close()
 
 
@@ -223,8 +223,8 @@ def test_dense_bricks_read_overflow_records():
     p = U.workload_params(w)
     u, u0 = _run_refs(_ctx(w, p), w, 0, capi.ER_BATCHED)
     uo, u0o = U.oracle_refs(w, 0, p)
-    assert U.energy_mismatch(u, uo)[0] == 0
-    assert U.energy_mismatch(u0, u0o)[0] == 0
+    assert U.energy_mismatch(u, uo, U.nsites(w))[0] == 0
+    assert U.self_mismatch(u0, u0o, U.nsites(w))[0] == 0
 
 
 def test_soln_config1_two_frames():
@@ -239,9 +239,9 @@ def test_soln_config1_two_frames():
         u, u0 = ctx.uvengy(1)
         uo, u0o = U.oracle_soln(w, f, p)
         assert u[0, w.topo.tagslt[0]] == 0.0
-        nbad, worst, at = U.energy_mismatch(u, uo)
+        nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(w))
         assert nbad == 0, (worst, at, u[at], uo[at])
-        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.self_mismatch(u0, u0o, U.nsites(w))[0] == 0
         sp = U.species_of(w.topo, 1, w.topo.tagslt)
         d_gpu_e.update(sp[0], u[0].astype(np.float64), float(u0[0]))
         fl = ctx.flceng(1)
@@ -264,7 +264,7 @@ def test_selfcal_self_energy_distribution(cfg0):
         u, u0 = _run_refs(ctx, cfg0, f, capi.ER_BATCHED)
         _run_refs(ctx0, cfg0, f, capi.ER_BATCHED)
         uo, u0o = U.oracle_refs(cfg0, f, p)
-        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.self_mismatch(u0, u0o, U.nsites(cfg0))[0] == 0
         for n in range(u.shape[0]):
             d.update(cfg0.topo.uvspec, u[n].astype(np.float64), float(u0[n]))
     g, g0 = ctx.fetch(), ctx0.fetch()
@@ -286,9 +286,9 @@ def test_pppm_energies_cfg0(cfg0, order):
     for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
         u, u0 = _run_refs(ctx, cfg0, 0, mode, nins=24)
         uo, u0o = U.oracle_refs(cfg0, 0, p, nins=24)
-        nbad, worst, at = U.energy_mismatch(u, uo)
+        nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(cfg0))
         assert nbad == 0, (mode, worst, at, u[at], uo[at])
-        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.self_mismatch(u0, u0o, U.nsites(cfg0))[0] == 0
     # the GPU's PPPM - PME difference follows the oracle's (both near
     # convergence at order 6, where they differ by a few 1e-5 kcal/mol)
     pme = U.workload_params(cfg0, splodr=order)
@@ -303,7 +303,7 @@ def test_pppm_energies_cfg0(cfg0, order):
     # above applies.
     if order == 4:
         ug = u.astype(np.float64)
-        sel = (np.abs(uo) < U.E_MID) & (np.abs(uo - uop) > 1e-5)
+        sel = (np.abs(uo) < U.E_WALL) & (np.abs(uo - uop) > 1e-5)
         assert sel.sum() >= 20
         rms = lambda a: float(np.sqrt(np.mean(a[sel] ** 2)))
         assert rms(ug - uo) < 0.3 * rms(ug - uop)
@@ -320,7 +320,7 @@ def test_bare_coulomb_cfg0(cfg0):
     for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
         u, u0 = _run_refs(ctx, cfg0, 0, mode, nins=32)
         uo, u0o = U.oracle_refs(cfg0, 0, p, nins=32)
-        nbad, worst, at = U.energy_mismatch(u, uo)
+        nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(cfg0))
         assert nbad == 0, (mode, worst, at, u[at], uo[at])
         assert np.all(u0 == 0.0) and np.all(u0o == 0.0)
         for n in range(u.shape[0]):
@@ -342,7 +342,7 @@ def test_open_system_bare_coulomb_cfg0(cfg0, mode):
     d = U.oracle_distribution(cfg0, p)
     u, u0 = _run_refs(ctx, cfg0, 0, mode, nins=24)
     uo, u0o = U.oracle_refs(cfg0, 0, p, nins=24)
-    nbad, worst, at = U.energy_mismatch(u, uo)
+    nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(cfg0))
     assert nbad == 0, (worst, at, u[at], uo[at])
     assert np.all(u0 == 0.0)
     # far molecules still interact (no Coulomb cutoff in an open system)
@@ -387,11 +387,13 @@ def test_rotated_cell_cfg0(cfg0):
         ctx.run(mode)
         ctx.sync()
         u, u0 = ctx.uvengy(32)
-        nbad, worst, at = U.energy_mismatch(u, uo)
+        nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(cfg0))
         assert nbad == 0, (mode, worst, at, u[at], uo[at])
-        assert U.energy_mismatch(u0, u0o)[0] == 0
-        assert U.energy_mismatch(u, uo_plain)[0] == 0
-        assert U.energy_mismatch(u0, u0o_plain)[0] == 0
+        assert U.self_mismatch(u0, u0o, U.nsites(cfg0))[0] == 0
+        # against the unrotated system: the rotated FP32 inputs differ from it by their own
+        # rounding (half an ulp of 31 A, 1e-6 A), which the LJ wall multiplies to ~1e-5
+        assert U.energy_mismatch(u, uo_plain, U.nsites(cfg0), rtol=1e-4)[0] == 0
+        assert U.self_mismatch(u0, u0o_plain, U.nsites(cfg0))[0] == 0
 
 
 def test_triclinic_cell_cfg0(cfg0):
@@ -418,10 +420,10 @@ def test_triclinic_cell_cfg0(cfg0):
         ctx.set_insertions(ins)
         ctx.run(mode)
         u, u0 = ctx.uvengy(32)
-        nbad, worst, at = U.energy_mismatch(u, uo)
+        nbad, worst, at = U.energy_mismatch(u, uo, U.nsites(cfg0))
         print(f"triclinic mode {mode}: {U.energy_report(u, uo)}, worst excess {worst:.3e}")
         assert nbad == 0, (mode, worst, at, u[at], uo[at])
-        assert U.energy_mismatch(u0, u0o)[0] == 0
+        assert U.self_mismatch(u0, u0o, U.nsites(cfg0))[0] == 0
         ctx.close()
 
 
