@@ -265,14 +265,43 @@ __global__ void k_prep_atoms(int natom, const float *__restrict__ pos, const flo
     splg[a] = g;
 }
 
-// reference site and reach radius of each molecule (for cutoff pruning)
+// centre and reach radius of each molecule (for cutoff pruning)
 #ifndef ER_SEG
 #define ER_SEG 8  // atoms per molecule segment of the pair search's pruning
 #endif
 
-// Per molecule: its reference site (first atom) and reach radius, and for
-// every segment of ER_SEG consecutive atoms (molecule-relative) the same pair
-// for the segment, stored at the segment's first atom in segref
+// centroid of the atoms [a0, a1) (minimum-image offsets from atom a0,
+// averaged) and their largest minimum-image distance from it: a sphere that
+// holds every atom, for the pair search's pruning (any centre is valid by
+// the triangle inequality on minimum images; the centroid reaches least far
+// for extended molecules)
+__device__ __forceinline__ float4 atom_sphere(const FrameConst &fc, const float4 *__restrict__ posn, int a0, int a1)
+{
+    const float4 r = posn[a0];
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
+    for (int a = a0 + 1; a < a1; ++a) {
+        const float4 p = posn[a];
+        float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
+        min_image(fc, dx, dy, dz);
+        ox += dx;
+        oy += dy;
+        oz += dz;
+    }
+    const float inv = 1.0f / (float)(a1 - a0);
+    const float cx = r.x + ox * inv, cy = r.y + oy * inv, cz = r.z + oz * inv;
+    float R = 0.0f;
+    for (int a = a0; a < a1; ++a) {
+        const float4 p = posn[a];
+        float dx = p.x - cx, dy = p.y - cy, dz = p.z - cz;
+        min_image(fc, dx, dy, dz);
+        R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
+    }
+    return make_float4(cx, cy, cz, R);
+}
+
+// Per molecule: its bounding sphere (atom_sphere), and for every segment of
+// ER_SEG consecutive atoms (molecule-relative) the segment's, stored at the
+// segment's first atom in segref
 __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const float4 *__restrict__ posn,
                             FrameConst fc, float4 *__restrict__ molref, float *__restrict__ rmolmax,
                             float4 *__restrict__ segref)
@@ -280,32 +309,18 @@ __global__ void k_prep_mols(int nmol, const int *__restrict__ molbeg, const floa
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= nmol) return;
     const int a0 = molbeg[j], a1 = molbeg[j + 1];
-    const float4 r = posn[a0];
-    float R = 0.0f;
-    for (int a = a0 + 1; a < a1; ++a) {
-        const float4 p = posn[a];
-        float dx = p.x - r.x, dy = p.y - r.y, dz = p.z - r.z;
-        min_image(fc, dx, dy, dz);
-        R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
-    }
-    molref[j] = make_float4(r.x, r.y, r.z, R);
+    const float4 ms = atom_sphere(fc, posn, a0, a1);
+    const float R = ms.w;
+    molref[j] = ms;
     for (int s0 = a0; s0 < a1; s0 += ER_SEG) {
-        const float4 q = posn[s0];
-        float Rs = 0.0f;
-        for (int a = s0 + 1; a < min(s0 + ER_SEG, a1); ++a) {
-            const float4 p = posn[a];
-            float dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
-            min_image(fc, dx, dy, dz);
-            Rs = fmaxf(Rs, sqrtf(dx * dx + dy * dy + dz * dz));
-        }
-        segref[s0] = make_float4(q.x, q.y, q.z, Rs);
+        segref[s0] = atom_sphere(fc, posn, s0, min(s0 + ER_SEG, a1));
     }
     // frame maximum (non-negative floats order like their bit patterns)
     atomicMax(reinterpret_cast<unsigned int *>(rmolmax), __float_as_uint(R));
 }
 
-// Insertion (refs) or solute molecule (soln) sites: wrapped position and
-// spline; k_prep_ins then adds the reach radius and the real-space self term
+// Insertion (refs) or solute molecule (soln) sites: position, lattice
+// coordinates and spline; k_prep_ins then adds the search sphere and the real-space self term
 // realcal_self[_refs] (realcal.F90:669-771).  For soln the sites come from
 // the frame.
 __global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ sd, const float *__restrict__ sltpos,
@@ -342,7 +357,7 @@ __global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ 
     insposn[is] = make_float4(x, y, z, q);
 }
 
-// One wave per insertion (after k_prep_sites): reference site and reach
+// One wave per insertion (after k_prep_sites): search centre and reach
 // radius, real-space self energy and residual self term.
 __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const SlotDesc *__restrict__ sd,
                                                   FrameConst fc, float screen, int bare,
@@ -364,17 +379,39 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
     }
     const int nslt = sd[k].nslt;
     const float4 *P = insposn + sd[k].site_off + (size_t)local * nslt;
+    // The pair search's centre: the sites' centroid (their minimum-image
+    // offsets from site 0, averaged), with reach R = the largest minimum-image
+    // distance of a site from it.  Any centre bounds the search by the
+    // triangle inequality on minimum images; the centroid of an extended
+    // solute reaches far less than its first site (28 -> ~16 A for a
+    // 150-site chain), and the search volume goes with the cube of cutoff +
+    // reach.
     const float4 c0 = P[0];
-    float R = 0.0f;
+    float ox = 0.0f, oy = 0.0f, oz = 0.0f;
     for (int s = 1 + lane; s < nslt; s += 64) {
         float dx = P[s].x - c0.x, dy = P[s].y - c0.y, dz = P[s].z - c0.z;
+        min_image(fc, dx, dy, dz);
+        ox += dx;
+        oy += dy;
+        oz += dz;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ox += __shfl_xor(ox, off, 64);
+        oy += __shfl_xor(oy, off, 64);
+        oz += __shfl_xor(oz, off, 64);
+    }
+    const float inv = 1.0f / (float)nslt;
+    const float cx = c0.x + ox * inv, cy = c0.y + oy * inv, cz = c0.z + oz * inv;
+    float R = 0.0f;
+    for (int s = lane; s < nslt; s += 64) {
+        float dx = P[s].x - cx, dy = P[s].y - cy, dz = P[s].z - cz;
         min_image(fc, dx, dy, dz);
         R = fmaxf(R, sqrtf(dx * dx + dy * dy + dz * dz));
     }
     for (int off = 32; off > 0; off >>= 1) R = fmaxf(R, __shfl_xor(R, off, 64));
     if (bare) {  // cltype 0: realcal_self and residual_self_ene return at once
         if (lane == 0) {
-            insref[n] = make_float4(c0.x, c0.y, c0.z, R);
+            insref[n] = make_float4(cx, cy, cz, R);
             selfreal[n] = 0.0f;
             resself[n] = 0.0f;
         }
@@ -402,7 +439,7 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
     if (lane == 0) {
-        insref[n] = make_float4(c0.x, c0.y, c0.z, R);
+        insref[n] = make_float4(cx, cy, cz, R);
         selfreal[n] = (float)e;
         // residual_self_ene (engproc.F90:967-977): -(pi Q^2 / alpha^2 / V) / 2 with
         // Q = mol_charge, a plain sum of the site charges (setconf.F90:838-840)
@@ -1208,7 +1245,7 @@ __global__ void __launch_bounds__(ER_GEN_NT) k_pme_axis1_gen(int nchunk, int NI,
 
 // ---------------------------------------------------------------------------
 // Spatial order of the molecules (per frame): counting sort of the molecule
-// reference sites into NC^3 cells keyed by the Morton code of the cell, so
+// centres into NC^3 cells keyed by the Morton code of the cell, so
 // that consecutive molecules are neighbours in space.  The gather walks the
 // molecules in this order (neighbouring stencils hit the same L2 lines) and
 // the pair search uses the cells as a cell list.
@@ -1917,8 +1954,8 @@ __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const i
 
 // ---------------------------------------------------------------------------
 // Real-space LJ + erfc-Coulomb pairs (realcal.F90:207-364 / 51-205) with
-// molecule-level pruning: a molecule whose reference site is further than
-// cutoff + R_solute + R_molecule from the solute's reference site has every
+// molecule-level pruning: a molecule whose bounding-sphere centre is further
+// than cutoff + R_solute + R_molecule from the solute's centre has every
 // site pair beyond both cutoffs (triangle inequality on minimum images), so
 // the per-pair cutoff semantics are kept exactly.  Candidates come from the
 // cells within reach; each (insertion, molecule) sum is owned by one thread
@@ -2168,7 +2205,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     };
     int cx, cy, cz;
     site_cell(fc, ref.x, ref.y, ref.z, pa.nc, cx, cy, cz);
-    // a site within distance d of the reference site lies at most
+    // a molecule centre within distance d of the search centre lies at most
     // ceil(d / width) cells away along every axis
     const int nreach = (int)ceilf((cut + rmolmax[0]) / pa.cell_w);
     const int span = min(2 * nreach + 1, pa.nc);  // distinct cells per axis
@@ -2179,7 +2216,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
         const int qx = pmod(cx + lo + dx, pa.nc), qy = pmod(cy + lo + dy, pa.nc), qz = pmod(cz + lo + dz, pa.nc);
         const unsigned int key = cell_key(qx, qy, qz);
         const unsigned int p0 = cstart[key], p1 = cstart[key + 1];
-        // the cell's molecules four at a time: indices, then reference sites
+        // the cell's molecules four at a time: indices, then centres
         for (unsigned int pb = p0; pb < p1; pb += 4) {
             int jk[4];
             float4 mk[4];
@@ -2226,8 +2263,8 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
         const uint4 xu = in_lds ? sxu[s] : insposu[s];
         const int ti = in_lds ? stj[s] : (soln ? ljtype[slt_a0 + s] : sltlj[s]);
         float e = 0.0f;
-        // the molecule's reference site and reach radius: beyond cutoff +
-        // reach, every site pair is beyond both cutoffs (triangle inequality
+        // the molecule's centre and reach radius: beyond cutoff + reach,
+        // every site pair is beyond both cutoffs (triangle inequality
         // on minimum images), so the site's sum is +0
         const float4 m = molref[jm];
         float rx = xa.x - m.x, ry = xa.y - m.y, rz = xa.z - m.z;
