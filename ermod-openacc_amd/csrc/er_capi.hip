@@ -1299,8 +1299,16 @@ template <int NI, int LPM> static hipError_t launch_gather(er_ctx *c, int n_off,
     return hipErrorInvalidValue;
 }
 
+// spline orders with a brick gather: a region row of RS = 8 + ORD - 1 grid
+// points x 16 insertions must be one wave instruction (RS <= 16) and the
+// region must fit the 160 KB LDS (order 8: 15 x 15 x 11 points, 158 KB)
+static bool brick_order(int ord) { return ord >= 3 && ord <= 8; }
+
 template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, int n_off, int nch)
 {
+    static_assert((size_t)(ER_BRICK + ORD - 1) * (ER_BRICK + ORD - 1) * (brick_z(ORD) + ORD - 1) * (NI / 4) * 16 <=
+                      160 * 1024,
+                  "the staged region fits the LDS");
     constexpr int NXCD = 8;
     constexpr int RS = ER_BRICK + ORD - 1, RZ = brick_z(ORD) + ORD - 1;
     const size_t lds = (size_t)RS * RS * RZ * (NI / 4) * 16;  // the staged region
@@ -1467,14 +1475,17 @@ static hipError_t prep_frame(er_ctx *c)
                            c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, brick_z(c->ord), c->d_bcnt.p,
                            c->d_akeyrank.p);
         hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s2, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
-        if (c->ord == 4)
-            hipLaunchKernelGGL(k_brick_records<4>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
-                               c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
-                               c->K3, c->nbx, c->nby, c->d_brec.p);
-        else if (c->ord == 6)
-            hipLaunchKernelGGL(k_brick_records<6>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
-                               c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2,
-                               c->K3, c->nbx, c->nby, c->d_brec.p);
+        switch (c->ord) {
+#define ER_BREC(O)                                                                                              \
+    case O:                                                                                                     \
+        hipLaunchKernelGGL(k_brick_records<O>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,        \
+                           c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2, \
+                           c->K3, c->nbx, c->nby, c->d_brec.p);                                                 \
+        break;
+            ER_BREC(3) ER_BREC(4) ER_BREC(5) ER_BREC(6) ER_BREC(7) ER_BREC(8)
+#undef ER_BREC
+        default: break;  // orders 9 and 10: the per-molecule gather, no records
+        }
         if (!c->capturing) {
             HIPCHK_E(hipEventRecord(c->ev_atoms, s2));
             c->atoms_pending = true;
@@ -1682,11 +1693,11 @@ static int ensure_run_buffers(er_ctx *c, int chunk)
     ER_WANT(c->d_keyrank, 2 * (size_t)c->nummol);
     ER_WANT(c->d_ccnt, c->ncell);
     ER_WANT(c->d_cstart, c->ncell + 1);
-    if (c->ord == 4 || c->ord == 6) ER_WANT(c->d_brec, (size_t)c->numatm * brick_rec_floats(c->ord));
+    if (brick_order(c->ord)) ER_WANT(c->d_brec, (size_t)c->numatm * brick_rec_floats(c->ord));
     ER_WANT(c->d_akeyrank, 2 * (size_t)c->numatm);
     ER_WANT(c->d_bcnt, c->nbrick);
     ER_WANT(c->d_bstart, c->nbrick + 1);
-    if (NI > 1 && (c->ord == 4 || c->ord == 6)) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
+    if (NI > 1 && brick_order(c->ord)) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
     // enough histogram workgroups to fill the chip
     const int ntile_chunk = (chunk + NI - 1) / NI;
     c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
@@ -1721,9 +1732,15 @@ template <int NI> static hipError_t run_chunk(er_ctx *c, int n_off, int nch)
     ProfScope ps(c, ER_K_GATHER);
     if constexpr (NI == 1) return launch_gather<1, 1>(c, n_off, nch);
     else {
-        if (c->ord == 4) return launch_gather_stream_k<NI>(c, n_off, nch);
-        if (c->ord == 6) return launch_gather_brick_k<6, NI>(c, n_off, nch);
-        return launch_gather<NI, NI>(c, n_off, nch);
+        switch (c->ord) {
+        case 3: return launch_gather_brick_k<3, NI>(c, n_off, nch);
+        case 4: return launch_gather_stream_k<NI>(c, n_off, nch);
+        case 5: return launch_gather_brick_k<5, NI>(c, n_off, nch);
+        case 6: return launch_gather_brick_k<6, NI>(c, n_off, nch);
+        case 7: return launch_gather_brick_k<7, NI>(c, n_off, nch);
+        case 8: return launch_gather_brick_k<8, NI>(c, n_off, nch);
+        default: return launch_gather<NI, NI>(c, n_off, nch);  // 9, 10: regions beyond one wave's row
+        }
     }
 }
 

@@ -1436,7 +1436,9 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
 // molecule by k_mol_reduce in the same atom order as k_gather.
 // ---------------------------------------------------------------------------
 #define ER_BRICK 8
-__host__ __device__ constexpr int brick_z(int ord) { return ord <= 4 ? 6 : 4; }
+// brick depth along z: order 6 takes 2 so that its staged region (13 x 13 x 7
+// points x 64 bytes, 76 KB) lets two workgroups share a CU
+__host__ __device__ constexpr int brick_z(int ord) { return ord <= 4 ? 6 : (ord == 6 ? 2 : 4); }
 
 __global__ void k_atom_brick_count(int natom, const int4 *__restrict__ splg, int K1, int K2, int K3, int nbx,
                                    int nby, int bzs, unsigned int *__restrict__ cnt,
@@ -1489,7 +1491,9 @@ __global__ void k_brick_records(int natom, const unsigned int *__restrict__ key_
     for (int c = 3 * ORD + 3; c < RECF; ++c) r[c] = 0.f;
 }
 
-// value of the lane R places further along the same 16-lane DPP row (row_ror:R)
+// row_ror:R, a rotation by R lanes within each 16-lane DPP row: adding
+// row_ror<8> and then row_ror<4> sums the four lanes l, l+4, l+8, l+12 of a
+// row into each of them whichever way the rotation turns
 template <int R>
 __device__ __forceinline__ float row_ror(float v)
 {
@@ -1524,7 +1528,7 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     constexpr int NROW = RS * RZ;
     constexpr int NT = 512, NW = NT / 64;
     constexpr int RPW = (NROW + NW - 1) / NW;  // rows per wave (at most)
-    constexpr int XL = 4;                   // lanes along the stencil's x offsets
+    constexpr int XL = (ORD == 6) ? 2 : 4;  // lanes along the stencil's x offsets (6 = 2 x 3)
     constexpr int NX = (ORD + XL - 1) / XL; // x offsets per lane
     constexpr int APASS = NT / (XL * NV);   // atoms per workgroup pass
     constexpr int RECF = brick_rec_floats(ORD);
@@ -1627,15 +1631,23 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
                 ea.w += w1 * xs.w;
             }
         }
-        // every lane of the wave takes part in the rotations
-        ea.x += row_ror<8>(ea.x);
-        ea.y += row_ror<8>(ea.y);
-        ea.z += row_ror<8>(ea.z);
-        ea.w += row_ror<8>(ea.w);
-        ea.x += row_ror<4>(ea.x);
-        ea.y += row_ror<4>(ea.y);
-        ea.z += row_ror<4>(ea.z);
-        ea.w += row_ror<4>(ea.w);
+        // every lane of the wave takes part in the exchanges (the XL x-lanes
+        // of an atom are XL consecutive groups of NV lanes in one DPP row)
+        if constexpr (XL == 4) {
+            ea.x += row_ror<8>(ea.x);
+            ea.y += row_ror<8>(ea.y);
+            ea.z += row_ror<8>(ea.z);
+            ea.w += row_ror<8>(ea.w);
+            ea.x += row_ror<4>(ea.x);
+            ea.y += row_ror<4>(ea.y);
+            ea.z += row_ror<4>(ea.z);
+            ea.w += row_ror<4>(ea.w);
+        } else {  // two x-lanes: lanes l and l ^ 4 belong to the same atom
+            ea.x += __shfl_xor(ea.x, 4, 64);
+            ea.y += __shfl_xor(ea.y, 4, 64);
+            ea.z += __shfl_xor(ea.z, 4, 64);
+            ea.w += __shfl_xor(ea.w, 4, 64);
+        }
         if (live && cx == 0)
             Eat4[((size_t)tile * natom + t.a) * NV + j] = make_float4(t.q * ea.x, t.q * ea.y, t.q * ea.z, t.q * ea.w);
     };

@@ -184,11 +184,13 @@ def test_noncubic_grid_and_order6():
 
 
 @pytest.mark.parametrize("ms,order", [((60, 60, 60), 5), ((50, 45, 36), 6), ((100, 100, 100), 4),
-                                      ((32, 32, 32), 3), ((32, 32, 32), 7), ((45, 40, 40), 10)])
+                                      ((32, 32, 32), 3), ((32, 32, 32), 7), ((36, 36, 36), 8),
+                                      ((45, 40, 40), 10)])
 def test_any_grid_and_order(ms, order):
     """PME grids without a compiled four-step shape (the direct-DFT plane and
     axis-1 kernels: 60^3, 50 x 45 x 36 with K2 != K3, 100^3, 45 x 40 x 40)
-    and spline orders 3 to 10 (erdst/fft_iface.F90:38-50 plans any size,
+    and spline orders 3 to 10, the brick gather's 3 to 8 and the
+    per-molecule gather's 10 (erdst/fft_iface.F90:38-50 plans any size,
     erdst/spline.F90:25-58 any order; odd orders zero engfac at m = K/2,
     erdst/recpcal.F90:142): batched and plain energies against the oracle."""
     w = synth.refs_water(0, 1000, 31.04, 24, 1, 32)
