@@ -1453,38 +1453,35 @@ static hipError_t prep_frame(er_ctx *c)
             HIPCHK_E(hipEventRecord(c->ev_in, c->stream));
             HIPCHK_E(hipStreamWaitEvent(s2, c->ev_in, 0));
         }
+        // five launches: atoms (and the counters cleared), molecules, both
+        // counting passes, both scans, then the brick records with the
+        // molecules' scatter into cell order
         hipLaunchKernelGGL(k_prep_atoms, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm, c->d_sitepos,
                            c->d_charge.p, c->fc, c->K1, c->K2, c->K3, c->ord, c->d_posn.p, c->d_posu.p, c->d_splw.p,
-                           c->d_splg.p);
-        hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(64), 0, s2, (size_t)1,
-                           reinterpret_cast<uint32_t *>(c->d_rmolmax.p));
+                           c->d_splg.p, c->d_ccnt.p, c->ncell, c->d_bcnt.p, c->nbrick, c->d_rmolmax.p);
         hipLaunchKernelGGL(k_prep_mols, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
                            c->d_molbeg.p, c->d_posn.p, c->fc, c->d_molref.p, c->d_rmolmax.p, c->d_segref.p);
-        // spatial order of the molecules and the cell list
-        hipLaunchKernelGGL(k_zero_u32, dim3((c->ncell + B - 1) / B), dim3(B), 0, s2, (size_t)c->ncell,
-                           c->d_ccnt.p);
-        hipLaunchKernelGGL(k_cell_count, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
-                           c->d_molref.p, c->fc, c->nc, c->d_ccnt.p, c->d_keyrank.p);
-        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s2, c->ncell, c->d_ccnt.p, c->d_cstart.p);
-        hipLaunchKernelGGL(k_cell_scatter, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
-                           c->d_keyrank.p, c->d_cstart.p, c->d_perm.p);
-        // atoms by PME brick for the brick gather
-        hipLaunchKernelGGL(k_zero_u32, dim3((c->nbrick + B - 1) / B), dim3(B), 0, s2, (size_t)c->nbrick,
-                           c->d_bcnt.p);
-        hipLaunchKernelGGL(k_atom_brick_count, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,
-                           c->d_splg.p, c->K1, c->K2, c->K3, c->nbx, c->nby, brick_z(c->ord), c->d_bcnt.p,
-                           c->d_akeyrank.p);
-        hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s2, c->nbrick, c->d_bcnt.p, c->d_bstart.p);
+        // spatial order of the molecules (the cell list) and of the atoms by PME brick
+        const int nsort = c->nummol + c->numatm;
+        hipLaunchKernelGGL(k_sort_count, dim3((nsort + B - 1) / B), dim3(B), 0, s2, c->nummol, c->d_molref.p,
+                           c->fc, c->nc, c->d_ccnt.p, c->d_keyrank.p, c->numatm, c->d_splg.p, c->K1, c->K2, c->K3,
+                           c->nbx, c->nby, brick_z(c->ord), c->d_bcnt.p, c->d_akeyrank.p);
+        hipLaunchKernelGGL(k_sort_scan, dim3(2), dim3(1024), 0, s2, c->ncell, c->d_ccnt.p, c->d_cstart.p, c->nbrick,
+                           c->d_bcnt.p, c->d_bstart.p);
         switch (c->ord) {
 #define ER_BREC(O)                                                                                              \
     case O:                                                                                                     \
-        hipLaunchKernelGGL(k_brick_records<O>, dim3((c->numatm + B - 1) / B), dim3(B), 0, s2, c->numatm,        \
+        hipLaunchKernelGGL(k_brick_records<O>, dim3((nsort + B - 1) / B), dim3(B), 0, s2, c->numatm,            \
                            c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2, \
-                           c->K3, c->nbx, c->nby, c->d_brec.p);                                                 \
+                           c->K3, c->nbx, c->nby, c->d_brec.p, c->nummol, c->d_keyrank.p, c->d_cstart.p,        \
+                           c->d_perm.p);                                                                        \
         break;
             ER_BREC(3) ER_BREC(4) ER_BREC(5) ER_BREC(6) ER_BREC(7) ER_BREC(8)
 #undef ER_BREC
-        default: break;  // orders 9 and 10: the per-molecule gather, no records
+        default:  // orders 9 and 10: the per-molecule gather, no records
+            hipLaunchKernelGGL(k_cell_scatter, dim3((c->nummol + B - 1) / B), dim3(B), 0, s2, c->nummol,
+                               c->d_keyrank.p, c->d_cstart.p, c->d_perm.p);
+            break;
         }
         if (!c->capturing) {
             HIPCHK_E(hipEventRecord(c->ev_atoms, s2));

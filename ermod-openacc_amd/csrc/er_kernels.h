@@ -22,6 +22,7 @@
 
 #define ER_PI_F 3.14159265358979323846f
 #define ER_MAXORD 12
+#define ER_BRICK 8  // PME brick side along x and y, in grid points (brick gathers)
 
 struct FrameConst {
     double fracM[9];  // x (as read) -> fractional coordinates along the
@@ -247,11 +248,19 @@ __device__ __forceinline__ void site_spline(const FrameConst &fc, float x, float
 
 // realcal_prepare (realcal.F90:30-48) + recpcal_prepare_solvent
 // (recpcal.F90:340-357) for every physical atom
+// The frame's sort counters (cell counts, brick counts and the largest
+// molecule reach) are cleared here too, before the kernels that add to them.
 __global__ void k_prep_atoms(int natom, const float *__restrict__ pos, const float *__restrict__ charge,
                              FrameConst fc, int K1, int K2, int K3, int ord, float4 *__restrict__ posn,
-                             uint4 *__restrict__ posu, float *__restrict__ splw, int4 *__restrict__ splg)
+                             uint4 *__restrict__ posu, float *__restrict__ splw, int4 *__restrict__ splg,
+                             unsigned int *__restrict__ ccnt, int ncell, unsigned int *__restrict__ bcnt, int nbrick,
+                             float *__restrict__ rmolmax)
 {
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int T = gridDim.x * blockDim.x;
+    for (int i = a; i < ncell; i += T) ccnt[i] = 0u;
+    for (int i = a; i < nbrick; i += T) bcnt[i] = 0u;
+    if (a == 0) *rmolmax = 0.0f;
     if (a >= natom) return;
     const float x = pos[3 * (size_t)a], y = pos[3 * (size_t)a + 1], z = pos[3 * (size_t)a + 2];
     float wx = x, wy = y, wz = z;
@@ -1278,29 +1287,42 @@ __device__ __forceinline__ void site_cell(const FrameConst &fc, float x, float y
     cz = min(nc - 1, (int)(u[2] * nc));
 }
 
-__global__ void k_cell_count(int nmol, const float4 *__restrict__ molref, FrameConst fc, int nc,
-                             unsigned int *__restrict__ cnt, unsigned int *__restrict__ key_rank)
+// the two counting passes of the frame's sorts in one launch: threads
+// [0, nmol) count the molecules per cell (the molecule centre's cell), the
+// next natom threads the atoms per PME brick (the brick of the spline base)
+__global__ void k_sort_count(int nmol, const float4 *__restrict__ molref, FrameConst fc, int nc,
+                             unsigned int *__restrict__ ccnt, unsigned int *__restrict__ keyrank, int natom,
+                             const int4 *__restrict__ splg, int K1, int K2, int K3, int nbx, int nby, int bzs,
+                             unsigned int *__restrict__ bcnt, unsigned int *__restrict__ akeyrank)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= nmol) return;
-    const float4 m = molref[j];
-    int cx, cy, cz;
-    site_cell(fc, m.x, m.y, m.z, nc, cx, cy, cz);
-    const unsigned int key = cell_key(cx, cy, cz);
-    const unsigned int rank = atomicAdd(&cnt[key], 1u);
-    key_rank[2 * j] = key;
-    key_rank[2 * j + 1] = rank;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nmol) {
+        const float4 m = molref[t];
+        int cx, cy, cz;
+        site_cell(fc, m.x, m.y, m.z, nc, cx, cy, cz);
+        const unsigned int key = cell_key(cx, cy, cz);
+        const unsigned int rank = atomicAdd(&ccnt[key], 1u);
+        keyrank[2 * t] = key;
+        keyrank[2 * t + 1] = rank;
+        return;
+    }
+    const int a = t - nmol;
+    if (a >= natom) return;
+    const int4 g = splg[a];
+    const unsigned int key = (unsigned int)((pmod(g.z, K3) / bzs * nby + pmod(g.y, K2) / ER_BRICK) * nbx +
+                                            pmod(g.x, K1) / ER_BRICK);
+    akeyrank[2 * a] = key;
+    akeyrank[2 * a + 1] = atomicAdd(&bcnt[key], 1u);
 }
 
-// exclusive scan of ncell counts in one block -> start[ncell + 1]
-__global__ void __launch_bounds__(1024) k_cell_scan(int ncell, const unsigned int *__restrict__ cnt,
-                                                    unsigned int *__restrict__ start)
+// exclusive scan of n counts by one 1024-thread block -> start[n + 1]
+__device__ __forceinline__ void block_scan(int ncell, const unsigned int *__restrict__ cnt,
+                                           unsigned int *__restrict__ start, unsigned int *wsum)
 {
     // exclusive scan in rounds of 1024 coalesced counts (the counts of RR
     // rounds are loaded together): inclusive wave scans by shuffles, wave
     // totals through LDS, a running carry between rounds
     constexpr int RR = 8;
-    __shared__ unsigned int wsum[16];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned int carry = 0;
     for (int base0 = 0; base0 < ncell; base0 += RR * 1024) {
@@ -1334,6 +1356,18 @@ __global__ void __launch_bounds__(1024) k_cell_scan(int ncell, const unsigned in
         }
     }
     if (tid == 0) start[ncell] = carry;
+}
+
+// both scans of the frame's sorts in one launch: block 0 the cells, block 1
+// the bricks
+__global__ void __launch_bounds__(1024) k_sort_scan(int ncell, const unsigned int *__restrict__ ccnt,
+                                                    unsigned int *__restrict__ cstart, int nbrick,
+                                                    const unsigned int *__restrict__ bcnt,
+                                                    unsigned int *__restrict__ bstart)
+{
+    __shared__ unsigned int wsum[16];
+    if (blockIdx.x == 0) block_scan(ncell, ccnt, cstart, wsum);
+    else block_scan(nbrick, bcnt, bstart, wsum);
 }
 
 __global__ void k_cell_scatter(int nmol, const unsigned int *__restrict__ key_rank,
@@ -1435,23 +1469,9 @@ __global__ void __launch_bounds__(256) k_gather(int nmol, int nchunk, int n_off,
 // while the other computes).  Per-atom energies go to Eat and are summed per
 // molecule by k_mol_reduce in the same atom order as k_gather.
 // ---------------------------------------------------------------------------
-#define ER_BRICK 8
 // brick depth along z: order 6 takes 2 so that its staged region (13 x 13 x 7
 // points x 64 bytes, 76 KB) lets two workgroups share a CU
 __host__ __device__ constexpr int brick_z(int ord) { return ord <= 4 ? 6 : (ord == 6 ? 2 : 4); }
-
-__global__ void k_atom_brick_count(int natom, const int4 *__restrict__ splg, int K1, int K2, int K3, int nbx,
-                                   int nby, int bzs, unsigned int *__restrict__ cnt,
-                                   unsigned int *__restrict__ key_rank)
-{
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natom) return;
-    const int4 g = splg[a];
-    const unsigned int key = (unsigned int)((pmod(g.z, K3) / bzs * nby + pmod(g.y, K2) / ER_BRICK) * nbx +
-                                            pmod(g.x, K1) / ER_BRICK);
-    key_rank[2 * a] = key;
-    key_rank[2 * a + 1] = atomicAdd(&cnt[key], 1u);
-}
 
 // wrap a region coordinate that lies in [-K, 2K) into [0, K)
 __device__ __forceinline__ int wrap1(int x, int K)
@@ -1466,15 +1486,23 @@ __device__ __forceinline__ int wrap1(int x, int K)
 // bits), padded to whole float4s.
 __host__ __device__ constexpr int brick_rec_floats(int ord) { return (3 * ord + 3 + 3) / 4 * 4; }
 
+// Atom records in brick order; threads [natom, natom + nmol) scatter the
+// molecules into cell order as k_cell_scatter does (one launch for both).
 template <int ORD>
 __global__ void k_brick_records(int natom, const unsigned int *__restrict__ key_rank,
                                 const unsigned int *__restrict__ bstart, const float4 *__restrict__ posn,
                                 const float *__restrict__ splw, const int4 *__restrict__ splg, int K1, int K2, int K3,
-                                int nbx, int nby, float *__restrict__ rec)
+                                int nbx, int nby, float *__restrict__ rec, int nmol,
+                                const unsigned int *__restrict__ mkeyrank, const unsigned int *__restrict__ cstart,
+                                int *__restrict__ perm)
 {
     constexpr int RECF = brick_rec_floats(ORD), BZ = brick_z(ORD);
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= natom) return;
+    if (a >= natom) {
+        const int j = a - natom;
+        if (j < nmol) perm[cstart[mkeyrank[2 * j]] + mkeyrank[2 * j + 1]] = j;
+        return;
+    }
     const unsigned int key = key_rank[2 * a];
     const unsigned int pos = bstart[key] + key_rank[2 * a + 1];
     const int bx = key % nbx, by = (key / nbx) % nby, bz = key / (nbx * nby);
