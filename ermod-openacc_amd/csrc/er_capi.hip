@@ -871,6 +871,15 @@ extern "C" int er_set_solute_molecules(er_ctx *c, int32_t nslt, const int32_t *t
     return ER_OK;
 }
 
+// Morton keys of nc cells per axis: coordinates below the next power of two
+// p interleave into keys below p^3 (32^3 at most)
+static int morton_keys(int nc)
+{
+    int p = 1;
+    while (p < nc) p <<= 1;
+    return p * p * p;
+}
+
 // realcal_prepare's cell handling (realcal.F90:776-896) + get_inverted_cell
 static int frame_cell(er_ctx *c, const float *cell)
 {
@@ -1035,7 +1044,7 @@ static int frame_cell(er_ctx *c, const float *cell)
         }
         c->cell_wmin = wmin;
         c->nc = std::max(1, std::min(32, (int)(wmin / (c->cell_min > 0.f ? c->cell_min : 3.5f))));
-        c->ncell = 1 << (3 * 5);  // Morton keys of 5 bits per axis
+        c->ncell = morton_keys(c->nc);
         c->cell_w = (float)(wmin / c->nc);
     }
     c->cell_changed = !c->have_prev_cell || std::memcmp(c->prev_cell, cell, sizeof c->prev_cell) != 0;
@@ -2646,6 +2655,7 @@ extern "C" int er_set_cell_width(er_ctx *c, float angstrom)
     c->cell_min = angstrom;
     if (c->have_frame && c->prm.boxshp != 0 && c->cell_wmin > 0.0) {
         c->nc = std::max(1, std::min(32, (int)(c->cell_wmin / (c->cell_min > 0.f ? c->cell_min : 3.5f))));
+        c->ncell = morton_keys(c->nc);
         c->cell_w = (float)(c->cell_wmin / c->nc);
         c->frame_dirty = true;
     }

@@ -1315,45 +1315,46 @@ __global__ void k_sort_count(int nmol, const float4 *__restrict__ molref, FrameC
     akeyrank[2 * a + 1] = atomicAdd(&bcnt[key], 1u);
 }
 
-// exclusive scan of n counts by one 1024-thread block -> start[n + 1]
+// exclusive scan of n counts by one 1024-thread block -> start[n + 1]:
+// each thread scans ER_SCAN_CH consecutive counts in registers, the thread
+// totals go through one wave scan and one LDS exchange, so a 32768-count
+// scan takes a single pass of two barriers
+#define ER_SCAN_CH 32
 __device__ __forceinline__ void block_scan(int ncell, const unsigned int *__restrict__ cnt,
                                            unsigned int *__restrict__ start, unsigned int *wsum)
 {
-    // exclusive scan in rounds of 1024 coalesced counts (the counts of RR
-    // rounds are loaded together): inclusive wave scans by shuffles, wave
-    // totals through LDS, a running carry between rounds
-    constexpr int RR = 8;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     unsigned int carry = 0;
-    for (int base0 = 0; base0 < ncell; base0 += RR * 1024) {
-        unsigned int vr[RR];
+    for (int base = 0; base < ncell; base += 1024 * ER_SCAN_CH) {
+        const int b0 = base + tid * ER_SCAN_CH;
+        unsigned int v[ER_SCAN_CH];
+        unsigned int sum = 0;
 #pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int b = base0 + r * 1024 + tid;
-            vr[r] = (b < ncell) ? cnt[b] : 0u;
+        for (int k = 0; k < ER_SCAN_CH; ++k) {
+            v[k] = (b0 + k < ncell) ? cnt[b0 + k] : 0u;
+            sum += v[k];
         }
+        unsigned int x = sum;  // inclusive scan of the thread totals over the wave
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        if (lane == 63) wsum[wave] = x;
+        __syncthreads();
+        unsigned int before = 0, total = 0;
+        for (int w = 0; w < 16; ++w) {
+            const unsigned int t = wsum[w];
+            before += (w < wave) ? t : 0u;
+            total += t;
+        }
+        unsigned int run = carry + before + x - sum;
 #pragma unroll
-        for (int r = 0; r < RR; ++r) {
-            const int b = base0 + r * 1024 + tid;
-            if (base0 + r * 1024 >= ncell) break;  // uniform
-            const unsigned int v = vr[r];
-            unsigned int x = v;
-            for (int off = 1; off < 64; off <<= 1) {
-                const unsigned int y = __shfl_up(x, off, 64);
-                if (lane >= off) x += y;
-            }
-            if (lane == 63) wsum[wave] = x;
-            __syncthreads();
-            unsigned int before = 0, total = 0;
-            for (int w = 0; w < 16; ++w) {
-                const unsigned int t = wsum[w];
-                before += (w < wave) ? t : 0u;
-                total += t;
-            }
-            if (b < ncell) start[b] = carry + before + x - v;
-            carry += total;
-            __syncthreads();
+        for (int k = 0; k < ER_SCAN_CH; ++k) {
+            if (b0 + k < ncell) start[b0 + k] = run;
+            run += v[k];
         }
+        carry += total;
+        __syncthreads();
     }
     if (tid == 0) start[ncell] = carry;
 }
