@@ -1880,8 +1880,6 @@ static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
     const SlotDesc *sd = c->d_sd.p;
     {
         ProfScope ps(c, ER_K_PREP);
-        hipLaunchKernelGGL(k_seq_bump, dim3((nslot + 63) / 64), dim3(64), 0, c->stream, nslot, sd);
-        HIPCHK(c, hipGetLastError());
         HIPCHK(c, prep_frame(c));
         if (!soln)  // the insertions' coordinates, solutes end to end
             for (int i = 0; i < nslot; ++i) {
@@ -1890,7 +1888,8 @@ static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
                 HIPCHK(c, hipMemcpyAsync(c->d_sltcat.p + (size_t)P.desc[i].site_off * 3, sl.d_sltpos,
                                          n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
             }
-        hipLaunchKernelGGL(k_prep_sites, dim3((unsigned)((P.nsite + 255) / 256)), dim3(256), 0, c->stream, P.nsite,
+        hipLaunchKernelGGL(k_prep_sites, dim3((unsigned)((std::max(P.nsite, nslot) + 255) / 256)), dim3(256), 0,
+                           c->stream, P.nsite,
                            nslot, sd, soln ? (const float *)nullptr : c->d_sltcat.p, c->d_sitepos, c->d_charge.p,
                            c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3,
                            c->ord, c->d_insposn.p, c->d_insposu.p, c->d_insw.p, c->d_insg.p);

@@ -339,8 +339,11 @@ __global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ 
                              float *__restrict__ insw, int4 *__restrict__ insg)
 {
     // one thread per site of the run (every solute's insertions end to end):
-    // spline weights, grid base, position and lattice coordinates
+    // spline weights, grid base, position and lattice coordinates.  The
+    // first nslot threads also count the run for their solute (the run
+    // number of the halt keys, read by the histograms later in the run).
     const int is = blockIdx.x * blockDim.x + threadIdx.x;
+    if (is < nslot) *sd[is].seq += 1u;
     if (is >= nsite) return;
     const int k = slot_find(nslot, is, [&](int m) { return sd[m].site_off; });
     const int nslt = sd[k].nslt;
@@ -2502,13 +2505,6 @@ __global__ void __launch_bounds__(256) k_hist(HistArgs ha, int nslot, const Slot
         if (h && tile * NI + ii < ha.nchunk && ha.n_off + tile * NI + ii < live)
             atomicAdd(&Cnt[(size_t)(ha.n_off + tile * NI + ii) * ha.ermax + b], h);
     }
-}
-
-// the start of a run: every solute of the run counts one more run
-__global__ void k_seq_bump(int nslot, const SlotDesc *__restrict__ sd)
-{
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < nslot) *sd[k].seq += 1u;
 }
 
 __global__ void k_zero_u32(size_t n, uint32_t *__restrict__ p)
