@@ -1021,6 +1021,7 @@ static int frame_cell(er_ctx *c, const float *cell)
     for (int k = 0; k < 9; ++k) {
         iv[k] /= v;
         fc.invcl[k] = (float)iv[k];
+        fc.invcld[k] = iv[k];
         fc.invP[k] = fc.invcl[k];
     }
     if (fc.rotate || reduced) {  // inverse of the upper-triangular cell_normal the pairs use

@@ -27,6 +27,7 @@
 struct FrameConst {
     double fracM[9];  // x (as read) -> fractional coordinates along the
                       // cell_normal axes, s_k = sum_j fracM[k + 3j] x_j
+    double invcld[9]; // inverse of the raw cell in FP64 (the splines' fractions)
     float dsc[9];     // cell_normal / 2^32: lattice-coordinate differences -> Cartesian
     float cellN[9];  // cell_normal, column-major (realcal.F90:24)
     float invN[3];   // invcell_normal
@@ -228,15 +229,21 @@ __device__ __forceinline__ void bspline_weights(float t, int ord, float *w)
 __device__ __forceinline__ void site_spline(const FrameConst &fc, float x, float y, float z, int K1, int K2,
                                             int K3, int ord, float *w /*[3*ord]*/, int4 &g)
 {
+    // the fraction and grid coordinate in FP64 from the FP32 position, so
+    // that neither carries the rounding of a coordinate as large as the box
+    // (or, for an unwrapped trajectory, several boxes); the weights in FP32
     const int Ks[3] = {K1, K2, K3};
+    const double X = x, Y = y, Z = z;
     int gg[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        float f = fc.invcl[k] * x + fc.invcl[k + 3] * y + fc.invcl[k + 6] * z;
-        f = f - floorf(f);
-        const float factor = f * (float)Ks[k];
-        const int rci = (int)factor;
-        bspline_weights(factor - (float)rci, ord, w + k * ord);
+        double f = fc.invcld[k] * X + fc.invcld[k + 3] * Y + fc.invcld[k + 6] * Z;
+        f -= floor(f);
+        const double factor = f * (double)Ks[k];
+        int rci = (int)factor;
+        const float t = (float)(factor - (double)rci);
+        if (rci >= Ks[k]) rci -= Ks[k];  // a fraction that rounded up to 1
+        bspline_weights(t, ord, w + k * ord);
         gg[k] = rci;
     }
     g = make_int4(gg[0], gg[1], gg[2], 0);

@@ -398,6 +398,47 @@ def test_rotated_cell_cfg0(cfg0):
         assert U.self_mismatch(u0, u0o_plain, U.nsites(cfg0))[0] == 0
 
 
+def test_lattice_images_cfg0(cfg0):
+    """Whole molecules and insertions moved by several cell vectors (up to
+    three box lengths out, as an unwrapped trajectory holds them): the
+    positions are not wrapped into the cell, every minimum image comes from
+    the lattice coordinates, so the energies are the oracle's on the same
+    inputs (batched and plain) and those of the unmoved system up to the FP32
+    rounding of the moved inputs."""
+    rng = np.random.default_rng(5)
+    p = U.workload_params(cfg0)
+    topo = cfg0.topo
+    L = np.diag(cfg0.cell).astype(np.float64)
+    pos0 = cfg0.frame(0).astype(np.float64)
+    ins0 = cfg0.insertions(0)[:32].astype(np.float64)
+    mb = topo.mol_begin
+    shift_mol = rng.integers(-3, 4, size=(len(mb) - 1, 3)).astype(np.float64) * L
+    pos = pos0 + np.repeat(shift_mol, np.diff(mb), axis=0)[: pos0.shape[0]]
+    ins = ins0 + (rng.integers(-3, 4, size=(ins0.shape[0], 1, 3)) * L)
+    pos32, ins32 = pos.astype(np.float32), ins.astype(np.float32)
+    orc = lambda ps, x: U.O.energies_refs(U.oracle_system(topo), U.oracle_params(p), cfg0.cell.astype(np.float64),
+                                          ps.astype(np.float64), topo.slt_charge64, topo.slt_ljtype,
+                                          x.astype(np.float64), rigid=(p["slttype"] == 2))
+    uo, u0o = orc(pos32, ins32)
+    uo_in, _ = orc(cfg0.frame(0), cfg0.insertions(0)[:32])
+    ns = U.nsites(cfg0)
+    for mode in (capi.ER_BATCHED, capi.ER_PLAIN):
+        ctx = _ctx(cfg0, p)
+        ctx.set_frame(pos32, cfg0.cell)
+        ctx.set_insertions(ins32)
+        ctx.run(mode)
+        ctx.sync()
+        u, u0 = ctx.uvengy(32)
+        nbad, worst, at = U.energy_mismatch(u, uo, ns)
+        print(f"lattice images mode {mode}: {U.energy_report(u, uo)}, worst excess {worst:.3e}")
+        assert nbad == 0, (mode, worst, at, u[at], uo[at])
+        assert U.self_mismatch(u0, u0o, ns)[0] == 0
+        # against the unmoved system: the moved FP32 inputs carry their own
+        # rounding (half an ulp of up to 4 box lengths, 4e-6 A)
+        assert U.energy_mismatch(u, uo_in, ns, rtol=2e-4)[0] == 0
+        ctx.close()
+
+
 def test_triclinic_cell_cfg0(cfg0):
     """The configuration-0 system sheared into a skewed cell whose (1,2)
     and (2,3) elements exceed half the diagonal: normalize_cell_vector's
