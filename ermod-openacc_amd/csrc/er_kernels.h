@@ -1337,13 +1337,22 @@ __device__ __forceinline__ void block_scan(int ncell, const unsigned int *__rest
     unsigned int carry = 0;
     for (int base = 0; base < ncell; base += 1024 * ER_SCAN_CH) {
         const int b0 = base + tid * ER_SCAN_CH;
+        const bool whole = b0 + ER_SCAN_CH <= ncell;  // the thread's counts lie inside the array
         unsigned int v[ER_SCAN_CH];
         unsigned int sum = 0;
+        if (whole) {  // 16-byte loads (b0 is a multiple of 32, the array 256-byte aligned)
+            const uint4 *c4 = reinterpret_cast<const uint4 *>(cnt + b0);
 #pragma unroll
-        for (int k = 0; k < ER_SCAN_CH; ++k) {
-            v[k] = (b0 + k < ncell) ? cnt[b0 + k] : 0u;
-            sum += v[k];
+            for (int q = 0; q < ER_SCAN_CH / 4; ++q) {
+                const uint4 t = c4[q];
+                v[4 * q] = t.x; v[4 * q + 1] = t.y; v[4 * q + 2] = t.z; v[4 * q + 3] = t.w;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < ER_SCAN_CH; ++k) v[k] = (b0 + k < ncell) ? cnt[b0 + k] : 0u;
         }
+#pragma unroll
+        for (int k = 0; k < ER_SCAN_CH; ++k) sum += v[k];
         unsigned int x = sum;  // inclusive scan of the thread totals over the wave
         for (int off = 1; off < 64; off <<= 1) {
             const unsigned int y = __shfl_up(x, off, 64);
@@ -1358,10 +1367,23 @@ __device__ __forceinline__ void block_scan(int ncell, const unsigned int *__rest
             total += t;
         }
         unsigned int run = carry + before + x - sum;
+        if (whole) {  // start[] holds ncell + 1 entries: the 16-byte stores stay inside
+            uint4 *s4 = reinterpret_cast<uint4 *>(start + b0);
 #pragma unroll
-        for (int k = 0; k < ER_SCAN_CH; ++k) {
-            if (b0 + k < ncell) start[b0 + k] = run;
-            run += v[k];
+            for (int q = 0; q < ER_SCAN_CH / 4; ++q) {
+                uint4 t;
+                t.x = run; run += v[4 * q];
+                t.y = run; run += v[4 * q + 1];
+                t.z = run; run += v[4 * q + 2];
+                t.w = run; run += v[4 * q + 3];
+                s4[q] = t;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < ER_SCAN_CH; ++k) {
+                if (b0 + k < ncell) start[b0 + k] = run;
+                run += v[k];
+            }
         }
         carry += total;
         __syncthreads();
