@@ -438,10 +438,10 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
     }
     // realcal_self_refs: -sum q^2 alpha/sqrt(pi) - sum_{i<j} q_i q_j erf(alpha r)/r,
     // in FP64 from the FP32 positions and charges: a 150-site solute has
-    // 11 175 terms of up to tens of kcal/mol that largely cancel, so FP32
-    // terms would leave ~1e-4 kcal/mol in this weight-bearing self energy;
-    // once per insertion, FP64 costs nothing here.  Lane l takes the pairs
-    // (s, t) with t - s - 1 = l mod 64, then a fixed-order tree.
+    // 11 175 terms of up to tens of kcal/mol that largely cancel, and this
+    // self energy enters the insertion's weight; computed once per
+    // insertion, FP64 costs little here.  Lane l takes the pairs (s, t) with
+    // t - s - 1 = l mod 64, then a fixed-order tree.
     double e = 0.0;
     const double rsqpi = 0.564189583547756286948079451560772586;  // 1/sqrt(pi)
     const double scr = screen;
