@@ -115,7 +115,11 @@ template <int N, int NS, int SIGN, int R, int... Rest> struct Stages<N, NS, SIGN
 #pragma unroll
             for (int t = 1; t < R; ++t) {
                 const int e = (t * k * (N / (NS * R))) % N;
-                if (e != 0) v[t] = twmul<SIGN>(v[t], tw[e]);
+                // half and quarter turns are exact sign and component swaps
+                if (2 * e == N) v[t] = make_float2(-v[t].x, -v[t].y);
+                else if (4 * e == N) v[t] = mul_si<SIGN>(v[t]);
+                else if (4 * e == 3 * N) v[t] = mul_si<-SIGN>(v[t]);
+                else if (e != 0) v[t] = twmul<SIGN>(v[t], tw[e]);
             }
             Dft<R, SIGN>::run(v);
             const int idx = (j / NS) * NS * R + k;

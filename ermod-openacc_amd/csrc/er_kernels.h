@@ -624,6 +624,18 @@ __global__ void k_ins_factors(int nrows, int ord, int K1, int K2, int K3, int H,
     Ftab[f] = a;
 }
 
+// One term engfac(m) |rho^(m)|^2 of the half-plane self energy
+// (recpcal.F90:482-491): formed in FP32 (every term is non-negative, so
+// the plane's sum carries at most a few FP32 roundings of relative error)
+// and accumulated in FP64 over the plane.
+__device__ __forceinline__ double self_term(float f, float2 a) { return (double)(f * fmaf(a.x, a.x, a.y * a.y)); }
+// weight of plane m1 of the half spectrum: 1/2 for the planes that are their
+// own mirror image (m1 = 0 and, for even K1, m1 = K1/2), 1 otherwise
+__device__ __forceinline__ double plane_weight(int K1, int m1)
+{
+    return (m1 == 0 || (K1 % 2 == 0 && m1 == K1 / 2)) ? 0.5 : 1.0;
+}
+
 // In-place 2-D DFT of the LDS plane [K2][PS] (x3/m3 fastest) along axis 3
 // (K3 = R1*R2) and then axis 2 (K2 = C1*C2), each as a four-step transform:
 // pass A = R1-point DFTs over stride-R2 elements + twiddles, pass B = R2-point
@@ -635,6 +647,9 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
                                           float2 *__restrict__ gout = nullptr)
 {
     constexpr int PS = K3 + 1;
+    // the pass-A twiddle index n2 * k1 (n2 < R2, k1 < R1) stays below
+    // (R1 - 1) (R2 - 1) < K3, so it needs no reduction modulo the length
+    static_assert((R1 - 1) * (R2 - 1) < K3 && (C1 - 1) * (C2 - 1) < K2, "twiddle index range");
     // ---- rows (axis 3), pass A in place: each thread's items are loaded
     // together, then transformed, then stored (their LDS latencies overlap) ----
     {
@@ -642,7 +657,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         float2 v[IA][R1];
 #pragma unroll
         for (int j = 0; j < IA; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < K2 * R2) {
                 const int m2 = it / R2, n2 = it % R2;
 #pragma unroll
@@ -651,12 +666,12 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         }
 #pragma unroll
         for (int j = 0; j < IA; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < K2 * R2) {
                 const int m2 = it / R2, n2 = it % R2;
                 erfft::fft<R1, SIGN>(v[j], tws + tws_off(R1));
 #pragma unroll
-                for (int k1 = 1; k1 < R1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t3[(n2 * k1) % K3]);
+                for (int k1 = 1; k1 < R1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t3[n2 * k1]);
 #pragma unroll
                 for (int k1 = 0; k1 < R1; ++k1) plane[m2 * PS + R2 * k1 + n2] = v[j][k1];
             }
@@ -669,7 +684,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         float2 hold[IB][R2];
 #pragma unroll
         for (int j = 0; j < IB; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < K2 * R1) {
                 const int m2 = it % K2, k1 = it / K2;  // rows fastest: lanes one row apart, no bank conflicts
 #pragma unroll
@@ -680,7 +695,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < IB; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < K2 * R1) {
                 const int m2 = it % K2, k1 = it / K2;
 #pragma unroll
@@ -695,7 +710,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         float2 v[IA][C1];
 #pragma unroll
         for (int j = 0; j < IA; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < C2 * K3) {
                 const int n2 = it / K3, x3 = it % K3;
 #pragma unroll
@@ -704,12 +719,12 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         }
 #pragma unroll
         for (int j = 0; j < IA; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < C2 * K3) {
                 const int n2 = it / K3, x3 = it % K3;
                 erfft::fft<C1, SIGN>(v[j], tws + tws_off(C1));
 #pragma unroll
-                for (int k1 = 1; k1 < C1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t2[(n2 * k1) % K2]);
+                for (int k1 = 1; k1 < C1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t2[n2 * k1]);
 #pragma unroll
                 for (int k1 = 0; k1 < C1; ++k1) plane[(C2 * k1 + n2) * PS + x3] = v[j][k1];
             }
@@ -722,7 +737,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         float2 hold[IC][C2];
 #pragma unroll
         for (int j = 0; j < IC; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < C1 * K3) {
                 const int k1 = it / K3, x3 = it % K3;
 #pragma unroll
@@ -735,7 +750,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
             // consecutive lanes write consecutive x3)
 #pragma unroll
             for (int j = 0; j < IC; ++j) {
-                const int it = tid + j * NT;
+                const unsigned int it = (unsigned int)tid + j * NT;
                 if (it < C1 * K3) {
                     const int k1 = it / K3, x3 = it % K3;
 #pragma unroll
@@ -747,7 +762,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         __syncthreads();
 #pragma unroll
         for (int j = 0; j < IC; ++j) {
-            const int it = tid + j * NT;
+            const unsigned int it = (unsigned int)tid + j * NT;
             if (it < C1 * K3) {
                 const int k1 = it / K3, x3 = it % K3;
 #pragma unroll
@@ -840,17 +855,19 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                         const int m3 = g + k * GR;
                         if (m3 < K3) {
                             const float2 f3 = F3[s * K3 + m3];
-                            acc[k].x += B.x * f3.x - B.y * f3.y;
-                            acc[k].y += B.x * f3.y + B.y * f3.x;
+                            // four chained FMAs per complex product
+                            acc[k].x = fmaf(B.x, f3.x, acc[k].x);
+                            acc[k].x = fmaf(-B.y, f3.y, acc[k].x);
+                            acc[k].y = fmaf(B.x, f3.y, acc[k].y);
+                            acc[k].y = fmaf(B.y, f3.x, acc[k].y);
                         }
                     }
                 }
             }
             __syncthreads();
         }
-        // engfac multiply and FP64 half-plane self energy on the registers,
-        // then one store of the plane
-        const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
+        // engfac multiply and half-plane self energy on the registers, then
+        // one store of the plane
         const float *ef = engfac + (size_t)m1 * NE;
         if (owner) {
 #pragma unroll
@@ -859,7 +876,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                 if (m3 < K3) {
                     const float f = ef[m3 * K2 + m2];  // [m1][m3][m2]
                     const float2 a = acc[k];
-                    sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+                    sp += self_term(f, a);
                     plane[m2 * PS + m3] = erfft::cscale(a, f);
                 }
             }
@@ -909,8 +926,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
     }
     if constexpr (SPARSE) {
         __syncthreads();
-        // ---- engfac multiply + FP64 half-plane self energy ----
-        const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
+        // ---- engfac multiply + half-plane self energy ----
         const float *ef = engfac + (size_t)m1 * NE;
 #pragma unroll
         for (int k = 0; k < EPT; ++k) {
@@ -920,7 +936,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                 const float f = ef[e];  // [m1][m3][m2]
                 float2 *q = plane + m2 * PS + m3;
                 const float2 a = *q;
-                sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+                sp += self_term(f, a);
                 *q = erfft::cscale(a, f);
             }
         }
@@ -935,7 +951,7 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
     if (tid == 0) {
         double t = 0.0;
         for (int wv = 0; wv < NT / 64; ++wv) t += red[wv];
-        selfpart[(size_t)n * H + m1] = t;
+        selfpart[(size_t)n * H + m1] = plane_weight(K1, m1) * t;
     }
 }
 
@@ -1194,8 +1210,7 @@ __global__ void __launch_bounds__(ER_GEN_NT) k_pme_spec2d_gen(int nslot, const S
         lds_dft_lines<-1>(plane, K3, 1, PS, K2, t2, tid);
         __syncthreads();
     }
-    // engfac multiply + FP64 half-plane self energy (engfac is [m1][m3][m2])
-    const double wt = ((K1 % 2 == 0) ? (m1 == 0 || m1 == H - 1) : (m1 == 0)) ? 0.5 : 1.0;
+    // engfac multiply + half-plane self energy (engfac is [m1][m3][m2])
     const float *ef = engfac + (size_t)m1 * NE;
     double sp = 0.0;
     for (int e = tid; e < NE; e += NT) {
@@ -1203,7 +1218,7 @@ __global__ void __launch_bounds__(ER_GEN_NT) k_pme_spec2d_gen(int nslot, const S
         const float f = ef[e];
         float2 *q = plane + m2 * PS + m3;
         const float2 a = *q;
-        sp += wt * (double)f * ((double)a.x * a.x + (double)a.y * a.y);
+        sp += self_term(f, a);
         *q = erfft::cscale(a, f);
     }
     __syncthreads();
@@ -1218,7 +1233,7 @@ __global__ void __launch_bounds__(ER_GEN_NT) k_pme_spec2d_gen(int nslot, const S
     if (tid == 0) {
         double t = 0.0;
         for (int wv = 0; wv < NT / 64; ++wv) t += red[wv];
-        selfpart[(size_t)n * H + m1] = t;
+        selfpart[(size_t)n * H + m1] = plane_weight(K1, m1) * t;
     }
 }
 
