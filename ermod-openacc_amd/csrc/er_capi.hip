@@ -2095,6 +2095,13 @@ extern "C" int er_get_uvengy(er_ctx *c, float *uvengy, float *uvengy0)
     return ER_OK;
 }
 
+extern "C" int er_get_nins(er_ctx *c, int32_t *nins)
+{
+    if (!c || !nins) return ER_E_ARG;
+    *nins = std::max(0, c->S().last_nins);
+    return ER_OK;
+}
+
 extern "C" int er_fetch(er_ctx *c, double *edens, double *ecorr, double *eself, double *scal, float *minuv,
                         float *maxuv, double *slnuv)
 {

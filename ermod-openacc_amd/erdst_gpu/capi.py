@@ -21,7 +21,7 @@ EXPORTED = ["er_create", "er_set_mesh", "er_set_solute", "er_set_solutes", "er_s
             "er_set_solute_molecules", "er_set_frame",
             "er_set_frame_device", "er_set_insertions", "er_set_insertions_device", "er_run", "er_run_noalloc",
             "er_sync",
-            "er_get_uvengy", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
+            "er_get_uvengy", "er_get_nins", "er_fetch", "er_get_flceng", "er_clear", "er_last_error", "er_error_string",
             "er_destroy", "er_profile", "er_profile_read", "er_autotune", "er_set_tuning", "er_get_tuning",
             "er_graph_begin", "er_graph_end", "er_graph_launch", "er_graph_destroy", "er_set_cell_width",
             "er_get_cell_width"]
@@ -78,6 +78,7 @@ def load() -> C.CDLL:
     L.er_run_noalloc.argtypes = [vp, i32]
     L.er_sync.argtypes = [vp]
     L.er_get_uvengy.argtypes = [vp, vp, vp]
+    L.er_get_nins.argtypes = [vp, C.POINTER(i32)]
     L.er_fetch.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
     L.er_get_flceng.argtypes = [vp, vp]
     L.er_clear.argtypes = [vp]
@@ -287,7 +288,20 @@ class Context:
     def sync(self):
         self._check(self.L.er_sync(self.h))
 
-    def uvengy(self, nins):
+    def nins(self):
+        """Insertions of the selected solute in the last run (er_get_nins)."""
+        n = C.c_int32()
+        self._check(self.L.er_get_nins(self.h, C.byref(n)))
+        return n.value
+
+    def uvengy(self, nins=None):
+        """(uvengy [nins, nmol], uvengy0 [nins]) of the last run; nins, when
+        given, must be the run's insertion count."""
+        have = self.nins()
+        if nins is None:
+            nins = have
+        elif nins != have:
+            raise ValueError(f"uvengy: the last run has {have} insertions of this solute, not {nins}")
         u = np.zeros((nins, self.topo.nmol), np.float32)
         u0 = np.zeros(nins, np.float32)
         self._check(self.L.er_get_uvengy(self.h, _ptr(u), _ptr(u0)))

@@ -27,7 +27,7 @@ module er_capi
   end type er_params
 
   public :: er_create, er_set_mesh, er_set_solute, er_set_solutes, er_select_solute, er_set_solute_molecules, &
-       er_set_frame, er_set_insertions, er_run, er_sync, er_get_uvengy, er_fetch, &
+       er_set_frame, er_set_insertions, er_run, er_sync, er_get_uvengy, er_get_nins, er_fetch, &
        er_get_flceng, er_clear, er_last_error, er_destroy, er_traj_open, er_traj_read, er_traj_close, &
        er_autotune, er_set_tuning, er_get_tuning, er_set_cell_width, er_get_cell_width
 
@@ -112,6 +112,12 @@ module er_capi
        type(c_ptr), value :: ctx
        real(c_float), intent(out) :: uvengy(*), uvengy0(*)
      end function er_get_uvengy
+
+     integer(c_int) function er_get_nins(ctx, nins) bind(C, name="er_get_nins")
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), intent(out) :: nins
+     end function er_get_nins
 
      integer(c_int) function er_fetch(ctx, edens, ecorr, eself, scal, minuv, maxuv, slnuv) &
           bind(C, name="er_fetch")

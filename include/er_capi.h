@@ -201,8 +201,14 @@ int er_sync(er_ctx *c);
 
 /* Per-insertion energies of the last er_run, for parity checks.
  * uvengy: [nins*nummol] (= uvengy(slvmax, maxdst) column-major); uvengy0:
- * [nins].  Either pointer may be NULL. */
+ * [nins].  Either pointer may be NULL.  nins is the selected solute's
+ * insertion count of that run (er_get_nins); the arrays must hold it. */
 int er_get_uvengy(er_ctx *c, float *uvengy, float *uvengy0);
+
+/* No counterpart in the reference, whose uvengy array has the fixed bound
+ * maxdst: the row count of er_get_uvengy, i.e. the selected solute's
+ * insertions in the last er_run (0 before any run). */
+int er_get_nins(er_ctx *c, int32_t *nins);
 
 /* Replaces: the data engstore reads (engproc.F90:520-725) before its
  * normalisation.  edens [ermax], ecorr [ermax*ermax] (may be NULL),

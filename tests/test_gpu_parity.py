@@ -63,6 +63,25 @@ def test_refs_energies_cfg0(cfg0, mode):
     assert nbad0 == 0, (u0[at0], u0o[at0])
 
 
+def test_uvengy_row_count(cfg0):
+    """er_get_nins gives the rows er_get_uvengy writes (the selected solute's
+    insertions in the last run); the wrapper refuses another count instead of
+    letting the library write past the caller's array."""
+    p = U.workload_params(cfg0)
+    ctx = _ctx(cfg0, p)
+    assert ctx.nins() == 0
+    ctx.set_frame(cfg0.frame(0), cfg0.cell)
+    ctx.set_insertions(cfg0.insertions(0)[:24])
+    ctx.run(capi.ER_BATCHED)
+    ctx.sync()
+    assert ctx.nins() == 24
+    with pytest.raises(ValueError):
+        ctx.uvengy(16)
+    u, u0 = ctx.uvengy()
+    assert u.shape == (24, cfg0.topo.nmol) and u0.shape == (24,)
+    np.testing.assert_array_equal(u, ctx.uvengy(24)[0])
+
+
 def test_plain_matches_batched(cfg0):
     p = U.workload_params(cfg0)
     ub, u0b = _run_refs(_ctx(cfg0, p), cfg0, 1, capi.ER_BATCHED, nins=40)
