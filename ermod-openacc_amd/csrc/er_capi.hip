@@ -1367,7 +1367,11 @@ template <int NI> static hipError_t launch_gather_stream_k(er_ctx *c, int n_off,
 
 template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
 {
-    const int NT = 256;
+    // 512 threads per tile and slice for the batched path (more waves per CU
+    // than 256 at the same slice count and global merge traffic); the plain
+    // path's single-insertion tiles keep 256 (its per-thread partials are
+    // reduced by one thread)
+    const int NT = NI > 1 ? 512 : 256;
     const size_t lds = (size_t)NT * sizeof(double) + (size_t)c->ermax * sizeof(float) +
                        (size_t)NI * c->ermax * sizeof(uint32_t) + (size_t)NT * 2 * sizeof(float) +
                        (size_t)c->numslv * ER_LUT_N * sizeof(unsigned short);

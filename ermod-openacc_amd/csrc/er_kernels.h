@@ -2436,7 +2436,7 @@ __device__ __forceinline__ unsigned long long halt_key(unsigned int seq, int n, 
 }
 
 template <int NI>
-__global__ void __launch_bounds__(256) k_hist(HistArgs ha, int nslot, const SlotDesc *__restrict__ sd,
+__global__ void __launch_bounds__(512) k_hist(HistArgs ha, int nslot, const SlotDesc *__restrict__ sd,
                                               const float *__restrict__ U, const int *__restrict__ tag,
                                               const float *__restrict__ meshf,
                                               const unsigned short *__restrict__ lut,
