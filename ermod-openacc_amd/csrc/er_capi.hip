@@ -198,6 +198,7 @@ struct er_ctx {
     int nbx = 1, nby = 1, nbz = 1, nbrick = 1;
     int ncu = 0;  // compute units of the context's device
     DBuf<int> d_border;           // bricks in Morton order
+    DBuf<uint4> d_bdesc;          // per frame: (brick, first atom, end atom) in Morton order
     DBuf<float> d_brec;           // atom records in brick order
     DBuf<unsigned int> d_akeyrank, d_bcnt, d_bstart;
     DBuf<float> d_Eat;
@@ -1351,8 +1352,8 @@ template <int NI> static hipError_t launch_gather_stream_k(er_ctx *c, int n_off,
 {
     const int nwg = 8 * std::max(1, c->ncu / 8);
     hipLaunchKernelGGL((k_gather_stream<4, NI>), dim3((unsigned)nwg), dim3(ER_GS_NT), 0, c->stream, c->K1, c->K2,
-                       c->K3, c->nbx, c->nby, c->nbrick, c->d_border.p, nch, c->d_bstart.p, c->d_brec.p, c->d_phi.p,
-                       c->numatm, c->d_Eat.p);
+                       c->K3, c->nbx, c->nby, c->nbrick, c->d_bdesc.p, nch, c->d_brec.p, c->d_phi.p, c->numatm,
+                       c->d_Eat.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
@@ -1488,7 +1489,7 @@ static hipError_t prep_frame(er_ctx *c)
         hipLaunchKernelGGL(k_brick_records<O>, dim3((nsort + B - 1) / B), dim3(B), 0, s2, c->numatm,            \
                            c->d_akeyrank.p, c->d_bstart.p, c->d_posn.p, c->d_splw.p, c->d_splg.p, c->K1, c->K2, \
                            c->K3, c->nbx, c->nby, c->d_brec.p, c->nummol, c->d_keyrank.p, c->d_cstart.p,        \
-                           c->d_perm.p);                                                                        \
+                           c->d_perm.p, c->nbrick, c->d_border.p, c->d_bdesc.p);                                \
         break;
             ER_BREC(3) ER_BREC(4) ER_BREC(5) ER_BREC(6) ER_BREC(7) ER_BREC(8)
 #undef ER_BREC
@@ -1708,6 +1709,7 @@ static int ensure_run_buffers(er_ctx *c, int chunk)
     ER_WANT(c->d_akeyrank, 2 * (size_t)c->numatm);
     ER_WANT(c->d_bcnt, c->nbrick);
     ER_WANT(c->d_bstart, c->nbrick + 1);
+    ER_WANT(c->d_bdesc, c->nbrick);
     if (NI > 1 && brick_order(c->ord)) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
     // enough histogram workgroups to fill the chip
     const int ntile_chunk = (chunk + NI - 1) / NI;
@@ -2703,7 +2705,7 @@ extern "C" void er_destroy(er_ctx *c)
     c->d_splfc1.release(); c->d_splfc2.release(); c->d_splfc3.release(); c->d_engfac.release();
     c->d_tw1.release(); c->d_tw2.release(); c->d_tw3.release(); c->d_twm.release(); c->d_tws.release();
     c->d_selfpart.release(); c->d_Ftab.release(); c->d_T.release(); c->d_phi.release();
-    c->d_C.release(); c->d_w.release(); c->d_border.release(); c->d_flceng.release(); c->d_part.release(); c->d_epart.release();
+    c->d_C.release(); c->d_w.release(); c->d_border.release(); c->d_bdesc.release(); c->d_flceng.release(); c->d_part.release(); c->d_epart.release();
     if (c->stream2) (void)hipStreamSynchronize(c->stream2);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     if (c->stream2) (void)hipStreamDestroy(c->stream2);

@@ -1536,16 +1536,24 @@ __host__ __device__ constexpr int brick_rec_floats(int ord) { return (3 * ord + 
 
 // Atom records in brick order; threads [natom, natom + nmol) scatter the
 // molecules into cell order as k_cell_scatter does (one launch for both).
+// The grid also writes bdesc[rank] = (brick, first atom, end atom) for the
+// bricks in Morton order (border), so that the streaming gather reads one
+// 16-byte descriptor per item.
 template <int ORD>
 __global__ void k_brick_records(int natom, const unsigned int *__restrict__ key_rank,
                                 const unsigned int *__restrict__ bstart, const float4 *__restrict__ posn,
                                 const float *__restrict__ splw, const int4 *__restrict__ splg, int K1, int K2, int K3,
                                 int nbx, int nby, float *__restrict__ rec, int nmol,
                                 const unsigned int *__restrict__ mkeyrank, const unsigned int *__restrict__ cstart,
-                                int *__restrict__ perm)
+                                int *__restrict__ perm, int nbrick, const int *__restrict__ border,
+                                uint4 *__restrict__ bdesc)
 {
     constexpr int RECF = brick_rec_floats(ORD), BZ = brick_z(ORD);
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int r = a; r < nbrick; r += gridDim.x * blockDim.x) {
+        const int brick = border[r];
+        bdesc[r] = make_uint4((unsigned int)brick, bstart[brick], bstart[brick + 1], 0u);
+    }
     if (a >= natom) {
         const int j = a - natom;
         if (j < nmol) perm[cstart[mkeyrank[2 * j]] + mkeyrank[2 * j + 1]] = j;
@@ -1788,8 +1796,7 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
 #define ER_GS_NT 1024
 template <int ORD, int NI>
 __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int K3, int nbx, int nby, int nbrick,
-                                                             const int *__restrict__ border, int nchunk,
-                                                             const unsigned int *__restrict__ bstart,
+                                                             const uint4 *__restrict__ bdesc, int nchunk,
                                                              const float *__restrict__ rec,
                                                              const float *__restrict__ phi, int natom,
                                                              float *__restrict__ Eat)
@@ -1828,13 +1835,14 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
         if (jj >= nitem) return it;
         const int rank = xcd * per + jj % per;
         if (rank >= nbrick) return it;
-        const int brick = border[rank];
+        const uint4 d = bdesc[rank];  // (brick, first atom, end atom): one load
+        const int brick = (int)d.x;
         it.tile = jj / per;
         it.bx = brick % nbx;
         it.by = (brick / nbx) % nby;
         it.bz = brick / (nbx * nby);
-        it.a0 = bstart[brick];
-        it.a1 = bstart[brick + 1];
+        it.a0 = d.y;
+        it.a1 = d.z;
         it.live = it.a1 > it.a0;
         return it;
     };
