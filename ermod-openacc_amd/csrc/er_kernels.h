@@ -1536,9 +1536,11 @@ __host__ __device__ constexpr int brick_rec_floats(int ord) { return (3 * ord + 
 
 // Atom records in brick order; threads [natom, natom + nmol) scatter the
 // molecules into cell order as k_cell_scatter does (one launch for both).
-// The grid also writes bdesc[rank] = (brick, first atom, end atom) for the
-// bricks in Morton order (border), so that the streaming gather reads one
-// 16-byte descriptor per item.
+// The grid also writes bdesc[rank] = (bx | by << 10 | bz << 20, first
+// atom, end atom, atom count) for the bricks in Morton order (border), so
+// that the streaming gather reads one 16-byte descriptor per item and
+// divides nothing (grid sides are at most 512 points, so each brick
+// coordinate fits its field).
 template <int ORD>
 __global__ void k_brick_records(int natom, const unsigned int *__restrict__ key_rank,
                                 const unsigned int *__restrict__ bstart, const float4 *__restrict__ posn,
@@ -1552,7 +1554,10 @@ __global__ void k_brick_records(int natom, const unsigned int *__restrict__ key_
     const int a = blockIdx.x * blockDim.x + threadIdx.x;
     for (int r = a; r < nbrick; r += gridDim.x * blockDim.x) {
         const int brick = border[r];
-        bdesc[r] = make_uint4((unsigned int)brick, bstart[brick], bstart[brick + 1], 0u);
+        const unsigned int bxyz = (unsigned int)(brick % nbx) | (unsigned int)((brick / nbx) % nby) << 10 |
+                                  (unsigned int)(brick / (nbx * nby)) << 20;
+        const unsigned int a0 = bstart[brick], a1 = bstart[brick + 1];
+        bdesc[r] = make_uint4(bxyz, a0, a1, a1 - a0);
     }
     if (a >= natom) {
         const int j = a - natom;
@@ -1829,21 +1834,27 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
         unsigned int a0, a1;
         bool live;
     };
-    auto item = [&](int jj) {
+    // the descriptor (brick coordinates, first atom, end atom, atom count)
+    // of item jj: one load, all zero (no atoms) past the XCD's bricks or the
+    // items.  All four words are used, so that no register of a load in
+    // flight is reused (which would wait for it) before the decode.
+    auto desc = [&](int jj) {
+        uint4 d = make_uint4(0u, 0u, 0u, 0u);
+        if (jj < nitem) {
+            const int rank = xcd * per + jj % per;
+            if (rank < nbrick) d = bdesc[rank];
+        }
+        return d;
+    };
+    auto item = [&](int jj, uint4 d) {
         Item it{};
-        it.live = false;
-        if (jj >= nitem) return it;
-        const int rank = xcd * per + jj % per;
-        if (rank >= nbrick) return it;
-        const uint4 d = bdesc[rank];  // (brick, first atom, end atom): one load
-        const int brick = (int)d.x;
         it.tile = jj / per;
-        it.bx = brick % nbx;
-        it.by = (brick / nbx) % nby;
-        it.bz = brick / (nbx * nby);
+        it.bx = (int)(d.x & 1023u);
+        it.by = (int)((d.x >> 10) & 1023u);
+        it.bz = (int)(d.x >> 20);
         it.a0 = d.y;
         it.a1 = d.z;
-        it.live = it.a1 > it.a0;
+        it.live = d.w > 0u;
         return it;
     };
     // issue the copies of an item's region and first records into buf
@@ -1961,15 +1972,19 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
             pass(it, buf, base, it.a1, std::integral_constant<bool, false>{});
     };
 
-    // item descriptors run one step ahead of the copies, the copies one
+    // descriptor loads run three steps ahead (each is decoded a step after
+    // it is requested, so its wait falls on a barrier instead of before the
+    // staging), decoded items one step ahead of the copies, the copies one
     // step ahead of the arithmetic
     int j = k0;
-    Item cur = item(j), nxt = item(j + W);
+    Item cur = item(j, desc(j)), nxt = item(j + W, desc(j + W));
+    uint4 d2 = desc(j + 2 * W);
     if (cur.live) stage(cur, bufA);
     __syncthreads();  // (waits for the copies)
     for (;;) {
         // even step: compute from bufA, fill bufB
-        Item nx2 = item(j + 2 * W);
+        uint4 d3 = desc(j + 3 * W);
+        Item nx2 = item(j + 2 * W, d2);
         if (nxt.live) stage(nxt, bufB);
         compute(cur, bufA);
         __syncthreads();
@@ -1977,8 +1992,10 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
         j += W;
         cur = nxt;
         nxt = nx2;
+        d2 = d3;
         // odd step: compute from bufB, fill bufA
-        nx2 = item(j + 2 * W);
+        d3 = desc(j + 3 * W);
+        nx2 = item(j + 2 * W, d2);
         if (nxt.live) stage(nxt, bufA);
         compute(cur, bufB);
         __syncthreads();
@@ -1986,6 +2003,7 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
         j += W;
         cur = nxt;
         nxt = nx2;
+        d2 = d3;
     }
 }
 
