@@ -1332,8 +1332,8 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
     const int ntile = (nch + NI - 1) / NI;
     const size_t nwg = (size_t)NXCD * ((c->nbrick + NXCD - 1) / NXCD) * ntile;
     hipLaunchKernelGGL((k_gather_brick<ORD, NI>), dim3((unsigned)nwg), dim3(512), lds, c->stream, c->K1, c->K2, c->K3,
-                       c->nbx, c->nby, c->nbrick, c->d_border.p, nch, c->d_bstart.p, c->d_brec.p, c->d_phi.p,
-                       c->numatm, c->d_Eat.p);
+                       c->nbx, c->nby, c->nbrick, c->d_bdesc.p, nch, c->d_brec.p, c->d_phi.p, c->numatm,
+                       c->d_Eat.p);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;

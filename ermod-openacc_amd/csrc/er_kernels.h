@@ -1603,8 +1603,7 @@ __device__ __forceinline__ float row_ror(float v)
 // the association differs (FP32 rounding only).
 template <int ORD, int NI>
 __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, int nbx, int nby, int nbrick,
-                                                      const int *__restrict__ border, int nchunk,
-                                                      const unsigned int *__restrict__ bstart,
+                                                      const uint4 *__restrict__ bdesc, int nchunk,
                                                       const float *__restrict__ rec, const float *__restrict__ phi,
                                                       int natom, float *__restrict__ Eat)
 {
@@ -1633,10 +1632,10 @@ __global__ void __launch_bounds__(512) k_gather_brick(int K1, int K2, int K3, in
     const int xcd = b & 7, k = b >> 3;
     const int rank = xcd * per + k % per, tile = k / per;
     if (rank >= nbrick || tile * NI >= nchunk) return;
-    const int brick = border[rank];
-    const int bx = brick % nbx, by = (brick / nbx) % nby, bz = brick / (nbx * nby);
-    const unsigned int a0 = bstart[brick], a1 = bstart[brick + 1];
-    if (a0 == a1) return;
+    const uint4 d = bdesc[rank];  // (brick coordinates, first atom, end atom, atom count)
+    const int bx = (int)(d.x & 1023u), by = (int)((d.x >> 10) & 1023u), bz = (int)(d.x >> 20);
+    const unsigned int a0 = d.y, a1 = d.z;
+    if (d.w == 0u) return;
     const int ox = bx * ER_BRICK - (ORD - 1), oy = by * ER_BRICK - (ORD - 1), oz = bz * BZ - (ORD - 1);
     const float4 *ph4 = reinterpret_cast<const float4 *>(phi) + (size_t)tile * K1 * K2 * K3 * NV;
     const int tid = threadIdx.x;
