@@ -615,7 +615,8 @@ __global__ void k_ins_factors(int nrows, int ord, int K1, int K2, int K3, int H,
     const float *w = insw + site * 3 * ord + axis * ord;
     float2 a = make_float2(0.f, 0.f);
     for (int c = 0; c < ord; ++c) {
-        const int e = (int)(((long long)m * pmod(gd - c, K)) % K);
+        // m, pmod(.) < K <= 512: the product fits 32 bits
+        const int e = (int)(((unsigned int)m * (unsigned int)pmod(gd - c, K)) % (unsigned int)K);
         const float2 t = tw[e];  // exp(+2 pi i e/K); the forward sum uses the conjugate
         a.x += w[c] * t.x;
         a.y -= w[c] * t.y;
