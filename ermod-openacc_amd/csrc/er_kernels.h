@@ -2825,7 +2825,7 @@ __global__ void __launch_bounds__(256) k_weights_fin(WeightArgs wa, AccRange ar,
 // every solute of the run (ar.nch < 0: z runs over each solute's slices in
 // turn) or over one solute's range (z = the slice)
 __device__ __forceinline__ void slice_span(const AccRange &ar, const SlotDesc *__restrict__ sd, int nslot, int z,
-                                           int &slice, int &n_off, int &nchunk, int &kper)
+                                           int &slice, int &n_off, int &nchunk, int &kper, int &slot, int &nks)
 {
     if (ar.nch < 0) {
         const int k = slot_find(nslot, z, [&](int m) { return sd[m].slice_off; });
@@ -2833,11 +2833,15 @@ __device__ __forceinline__ void slice_span(const AccRange &ar, const SlotDesc *_
         n_off = sd[k].ins_off;
         nchunk = sd[k].nins;
         kper = sd[k].kper;
+        slot = k;
+        nks = sd[k].nks;
     } else {
         slice = z;
         n_off = sd[ar.slot].ins_off + ar.rel;
         nchunk = ar.nch;
         kper = ar.kper;
+        slot = ar.slot;
+        nks = (ar.nch + ar.kper - 1) / ar.kper;
     }
 }
 
@@ -2846,8 +2850,8 @@ __global__ void k_edens(AccRange ar, int nslot, const SlotDesc *__restrict__ sd,
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x, z = blockIdx.y;
     if (b >= ermax) return;
-    int slice, n_off, nchunk, kper;
-    slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper);
+    int slice, n_off, nchunk, kper, slot, nks;
+    slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper, slot, nks);
     const int k_begin = slice * kper, k_end = min(nchunk, k_begin + kper);
     double e = 0.0;
     for (int nl = k_begin; nl < k_end; ++nl) {
@@ -2898,8 +2902,12 @@ __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const Slo
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int by = blockIdx.y, bx = blockIdx.x, z = blockIdx.z;
     if (by > bx) return;  // lower block triangle: mirrored by the reduce
-    int slice, n_off, nchunk, kper;
-    slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper);
+    int slice, n_off, nchunk, kper, slot, nks;
+    slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper, slot, nks);
+    // a solute with one slice adds its tile straight into ecorr and edens
+    // (what k_ecorr_reduce would add: 0 + the one partial), which then
+    // skips it
+    const bool direct = nks == 1;
     const int b1w = by * 64 + (wave >> 1) * 32;
     const int b2w = bx * 64 + (wave & 1) * 32;
     const int k_begin = slice * kper, k_end = min(nchunk, k_begin + kper);
@@ -2941,6 +2949,7 @@ __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const Slo
         bv[0] = bn[0]; bv[1] = bn[1];
     }
     double *P = part + (size_t)z * plane;
+    __attribute__((address_space(1))) double *E = gptr(sd[slot].ecorr);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         const int ij = dmap[lane * 4 + r];
@@ -2951,7 +2960,14 @@ __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const Slo
             for (int b = 0; b < 2; ++b) {
                 const int b1 = b1w + a * 16 + oi;
                 const int b2 = b2w + b * 16 + oj;
-                if (b1 < ermax && b2 < ermax) P[(size_t)b1 * ermax + b2] = acc[a][b][r];
+                if (b1 < ermax && b2 < ermax) {
+                    if (!direct) {
+                        P[(size_t)b1 * ermax + b2] = acc[a][b][r];
+                    } else if (b1 <= b2) {  // the upper triangle and its mirror, as the reduce
+                        E[(size_t)b1 * ermax + b2] += acc[a][b][r];
+                        if (b1 != b2) E[(size_t)b2 * ermax + b1] += acc[a][b][r];
+                    }
+                }
             }
     }
     // edens partial: sum the four kq groups in a fixed order (diagonal blocks,
@@ -2963,7 +2979,10 @@ __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const Slo
                          e3 = __shfl(ed[a], mr + 48, 64);
             const double e = ((ed[a] + e1) + e2) + e3;
             const int b1 = b1w + a * 16 + mr;
-            if (kq == 0 && b1 < ermax) epart[(size_t)z * ermax + b1] = e;
+            if (kq == 0 && b1 < ermax) {
+                if (direct) gptr(sd[slot].edens)[b1] += e;
+                else epart[(size_t)z * ermax + b1] = e;
+            }
         }
     }
 }
@@ -2980,6 +2999,7 @@ __global__ void k_ecorr_reduce(AccRange ar, const SlotDesc *__restrict__ sd, int
     const int ks = ar.nch < 0 ? (int)blockIdx.z : ar.slot;
     const int z0 = ar.nch < 0 ? sd[ks].slice_off : 0;
     const int nslice = ar.nch < 0 ? sd[ks].nks : (ar.nch + ar.kper - 1) / ar.kper;
+    if (full && nslice == 1) return;  // k_ecorr added this solute's one slice itself
     const size_t plane = (size_t)ermax * ermax;
     const double *__restrict__ part = part_all + (size_t)z0 * plane;
     const double *__restrict__ epart = epart_all + (size_t)z0 * ermax;
