@@ -176,7 +176,7 @@ int read_cell_block(FILE *f, const Header &h, float uc[6])
     } else if (std::fseek(f, (long)len, SEEK_CUR) != 0) {
         return ER_TRAJ_IO;
     }
-    return marker(f, h, &len) ? ER_OK : ER_TRAJ_IO;
+    return marker(f, h, &len) ? (int)ER_OK : (int)ER_TRAJ_IO;
 }
 
 int skip_fourth(FILE *f, const Header &h)
@@ -184,7 +184,7 @@ int skip_fourth(FILE *f, const Header &h)
     if (!h.four_d) return ER_OK;
     uint32_t len;
     if (!marker(f, h, &len) || std::fseek(f, (long)len, SEEK_CUR) != 0) return ER_TRAJ_IO;
-    return marker(f, h, &len) ? ER_OK : ER_TRAJ_IO;
+    return marker(f, h, &len) ? (int)ER_OK : (int)ER_TRAJ_IO;
 }
 
 // one coordinate record of n floats
@@ -196,7 +196,7 @@ int read_axis(FILE *f, const Header &h, float *dst, int32_t n)
     if (!get(f, dst, 4 * (size_t)n)) return ER_TRAJ_IO;
     if (h.swap) swap_words(dst, (size_t)n);
     if (!marker(f, h, &len)) return ER_TRAJ_IO;
-    return len == 4u * (uint32_t)n ? ER_OK : ER_TRAJ_FORMAT;
+    return len == 4u * (uint32_t)n ? (int)ER_OK : (int)ER_TRAJ_FORMAT;
 }
 
 }  // namespace
