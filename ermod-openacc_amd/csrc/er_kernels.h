@@ -2268,7 +2268,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
                                                const float *__restrict__ ljlensq, float *__restrict__ U)
 {
     __shared__ int cand[ER_PAIR_CAP];
-    __shared__ int ncand;
+    __shared__ int ncand, cnext;
     __shared__ float4 sx[ER_PAIR_SCAP];
     __shared__ uint4 sxu[ER_PAIR_SCAP];
     __shared__ int stj[ER_PAIR_SCAP];
@@ -2300,6 +2300,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     const int nt2 = pa.ntype * pa.ntype;
     const bool in_lds = pa.nslt <= ER_PAIR_SCAP && nt2 <= ER_PAIR_TCAP;
     if (tid == 0) ncand = 0;
+    if (tid == 0) cnext = 0;  // many-site path: the next candidate a wave takes
     if (in_lds) {
         for (int k = tid; k < pa.nslt; k += NT) {
             sx[k] = insposn[k];
@@ -2368,7 +2369,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     // lane g sums the sites g, g + GS, ... over the molecule's atoms (atom
     // order), then a fixed xor tree adds the lanes (deterministic)
     const int GS = pa.nslt <= 16 ? 16 : (pa.nslt <= 32 ? 32 : 64);
-    const int cpw = 64 / GS, lane = tid & 63, wave = tid >> 6, nwave = NT / 64;
+    const int cpw = 64 / GS, lane = tid & 63;
     const int grp = lane / GS, gl = lane % GS;
     const float skip2 = pair_skip2(pa.pp);
     const float skipc = fmaxf(pa.pp.upljcut, pa.pp.elecut) * 1.0001f + 1e-3f;
@@ -2410,7 +2411,17 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
         }
         return e;
     };
-    for (int c0 = wave * cpw; c0 < nc; c0 += nwave * cpw) {  // wave-uniform
+    // a wave takes its next cpw candidates from a shared counter when it is
+    // done with the last ones (molecules of very different sizes balance
+    // across the waves); each candidate's sum is still one group's, in the
+    // same order, so the results do not depend on which wave takes it
+    int c0;
+    {
+        int take = 0;
+        if (lane == 0) take = atomicAdd(&cnext, cpw);
+        c0 = __shfl(take, 0, 64);
+    }
+    while (c0 < nc) {  // wave-uniform
         const int c = c0 + grp;
         const bool live = c < nc;
         const int jm = live ? cand[c] : 0;
@@ -2419,6 +2430,9 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
             for (int st = gl; st < pa.nslt; st += GS) e += site_sum(st, jm);
         for (int off = GS / 2; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
         if (live && gl == 0) U[urow + (size_t)jm * pa.NI] += e;
+        int take = 0;
+        if (lane == 0) take = atomicAdd(&cnext, cpw);
+        c0 = __shfl(take, 0, 64);
     }
 }
 
