@@ -1789,12 +1789,24 @@ template <int NI> static hipError_t run_pairs_hist(er_ctx *c, int n_off, int nch
                 if ((e = hipGetLastError()) != hipSuccess) return e;
             }
         } else if (nblk > 0) {
-            hipLaunchKernelGGL(k_pairs, dim3((unsigned)nblk), dim3(256), 0, c->stream, pa, blk0, nslot, c->d_sd.p,
-                               c->fc, c->d_molbeg.p, c->d_posu.p, c->d_ljtype.p, c->d_molref.p, c->d_segref.p,
-                               c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p, c->d_insposn.p, c->d_insposu.p,
-                               c->d_insref.p,
-                               soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p, c->d_ljlensq.p, c->d_Ucat.p);
-            if ((e = hipGetLastError()) != hipSuccess) return e;
+            // the few-site and many-site kernels, for the classes of the
+            // solutes whose blocks [pair_blk, pair_blk + nins * pair_split)
+            // meet [blk0, blk0 + nblk); each returns from the other's blocks
+            bool few = false, many = false;
+            for (const SlotDesc &d : c->plan.desc) {
+                const long b0 = d.pair_blk, b1 = (long)d.pair_blk + (long)d.nins * d.pair_split;
+                if (b1 > blk0 && b0 < (long)blk0 + nblk) (d.nslt > 8 ? many : few) = true;
+            }
+            for (int k = 0; k < 2; ++k) {
+                if (!(k == 0 ? few : many)) continue;
+                hipLaunchKernelGGL(k == 0 ? k_pairs : k_pairs_many, dim3((unsigned)nblk), dim3(256), 0, c->stream,
+                                   pa, blk0, nslot, c->d_sd.p, c->fc, c->d_molbeg.p, c->d_posu.p, c->d_ljtype.p,
+                                   c->d_molref.p, c->d_segref.p, c->d_rmolmax.p, c->d_cstart.p, c->d_perm.p,
+                                   c->d_insposn.p, c->d_insposu.p, c->d_insref.p,
+                                   soln ? c->d_tag.p : (const int *)nullptr, c->d_ljene.p, c->d_ljlensq.p,
+                                   c->d_Ucat.p);
+                if ((e = hipGetLastError()) != hipSuccess) return e;
+            }
         }
     }
     ProfScope ps(c, ER_K_HIST);

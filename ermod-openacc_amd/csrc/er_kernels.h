@@ -2254,18 +2254,20 @@ __device__ __forceinline__ float molecule_pair_sum_lds(const PairArgs &pa, const
     return e;
 }
 
-__global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot, const SlotDesc *__restrict__ sd,
-                                               FrameConst fc, const int *__restrict__ molbeg,
-                                               const uint4 *__restrict__ posu, const int *__restrict__ ljtype,
-                                               const float4 *__restrict__ molref,
-                                               const float4 *__restrict__ segref,
-                                               const float *__restrict__ rmolmax,
-                                               const unsigned int *__restrict__ cstart,
-                                               const int *__restrict__ perm, const float4 *__restrict__ insposn_all,
-                                               const uint4 *__restrict__ insposu_all,
-                                               const float4 *__restrict__ insref,
-                                               const int *__restrict__ tag, const float *__restrict__ ljene,
-                                               const float *__restrict__ ljlensq, float *__restrict__ U)
+// The pair search and sums of one (insertion, part) block: solutes of at
+// most 8 sites take one thread per candidate molecule (k_pairs), larger ones
+// a group of lanes per molecule (k_pairs_many).  The two are separate kernels
+// so that each gets its own register allocation: the many-site kernel is
+// compiled for six waves per SIMD (the most its LDS allows), the few-site one
+// needs no such limit.  A launch covers the run's blocks of both classes;
+// each kernel returns at once from the other class's blocks.
+template <bool MANY>
+__device__ __forceinline__ void pairs_block(PairArgs pa0, int blk0, int nslot, const SlotDesc *__restrict__ sd,
+    FrameConst fc, const int *__restrict__ molbeg, const uint4 *__restrict__ posu, const int *__restrict__ ljtype,
+    const float4 *__restrict__ molref, const float4 *__restrict__ segref, const float *__restrict__ rmolmax,
+    const unsigned int *__restrict__ cstart, const int *__restrict__ perm, const float4 *__restrict__ insposn_all,
+    const uint4 *__restrict__ insposu_all, const float4 *__restrict__ insref, const int *__restrict__ tag,
+    const float *__restrict__ ljene, const float *__restrict__ ljlensq, float *__restrict__ U)
 {
     __shared__ int cand[ER_PAIR_CAP];
     __shared__ int ncand, cnext;
@@ -2282,6 +2284,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     PairArgs pa = pa0;
     pa.nsplit = sd[ks].pair_split;
     pa.nslt = sd[ks].nslt;
+    if ((pa.nslt > 8) != MANY) return;  // the other kernel's block
     const int bl = b - sd[ks].pair_blk;
     const int nl = bl / pa.nsplit, part = bl % pa.nsplit;
     const int n = sd[ks].ins_off + nl;
@@ -2358,7 +2361,7 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
     }
     __syncthreads();
     const int nc = min(ncand, ER_PAIR_CAP);
-    if (pa.nslt <= 8) {  // few sites: one thread per candidate molecule
+    if constexpr (!MANY) {  // few sites: one thread per candidate molecule
         for (int c = tid; c < nc; c += NT) {
             const int jm = cand[c];
             U[urow + (size_t)jm * pa.NI] += pair_sum(jm);
@@ -2434,6 +2437,29 @@ __global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot
         if (lane == 0) take = atomicAdd(&cnext, cpw);
         c0 = __shfl(take, 0, 64);
     }
+}
+
+__global__ void __launch_bounds__(256) k_pairs(PairArgs pa0, int blk0, int nslot, const SlotDesc *__restrict__ sd,
+    FrameConst fc, const int *__restrict__ molbeg, const uint4 *__restrict__ posu, const int *__restrict__ ljtype,
+    const float4 *__restrict__ molref, const float4 *__restrict__ segref, const float *__restrict__ rmolmax,
+    const unsigned int *__restrict__ cstart, const int *__restrict__ perm, const float4 *__restrict__ insposn_all,
+    const uint4 *__restrict__ insposu_all, const float4 *__restrict__ insref, const int *__restrict__ tag,
+    const float *__restrict__ ljene, const float *__restrict__ ljlensq, float *__restrict__ U)
+{
+    pairs_block<false>(pa0, blk0, nslot, sd, fc, molbeg, posu, ljtype, molref, segref, rmolmax, cstart, perm,
+                       insposn_all, insposu_all, insref, tag, ljene, ljlensq, U);
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_pairs_many(PairArgs pa0, int blk0, int nslot, const SlotDesc *__restrict__ sd,
+    FrameConst fc, const int *__restrict__ molbeg, const uint4 *__restrict__ posu, const int *__restrict__ ljtype,
+    const float4 *__restrict__ molref, const float4 *__restrict__ segref, const float *__restrict__ rmolmax,
+    const unsigned int *__restrict__ cstart, const int *__restrict__ perm, const float4 *__restrict__ insposn_all,
+    const uint4 *__restrict__ insposu_all, const float4 *__restrict__ insref, const int *__restrict__ tag,
+    const float *__restrict__ ljene, const float *__restrict__ ljlensq, float *__restrict__ U)
+{
+    pairs_block<true>(pa0, blk0, nslot, sd, fc, molbeg, posu, ljtype, molref, segref, rmolmax, cstart, perm,
+                      insposn_all, insposu_all, insref, tag, ljene, ljlensq, U);
 }
 
 // ---------------------------------------------------------------------------
