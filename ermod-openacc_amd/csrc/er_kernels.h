@@ -908,18 +908,72 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                 sw[f] = insw[(rowb + s0 + sl) * 3 * ord + ord + c];
             }
             __syncthreads();
-            // one wave, one lane per stencil point (c2, c3): the ORD^2 points of
-            // a site are distinct, and the wave's LDS updates stay in site order
-            if (tid < ord * ord) {
-                const int c2 = tid / ord, c3 = tid % ord;
-                for (int s = 0; s < ns; ++s) {
-                    const int2 g = sg[s];
-                    const float2 A = sA[s];
-                    const int x2 = pmod(g.x - c2, K2), x3 = pmod(g.y - c3, K3);
-                    const float2 v = erfft::cscale(A, sw[s * 2 * ord + c2] * sw[s * 2 * ord + ord + c3]);
-                    float2 *q = plane + x2 * PS + x3;
-                    *q = make_float2(q->x + v.x, q->y + v.y);
+            // one site's stencil point (c2, c3) into the plane (the grid
+            // bases lie in [0, K), so one conditional add wraps an index)
+            auto add_point = [&](int s, int c2, int c3) {
+                const int2 g = sg[s];
+                const float2 A = sA[s];
+                int x2 = g.x - c2, x3 = g.y - c3;
+                x2 += (x2 < 0) ? K2 : 0;
+                x3 += (x3 < 0) ? K3 : 0;
+                const float2 v = erfft::cscale(A, sw[s * 2 * ord + c2] * sw[s * 2 * ord + ord + c3]);
+                float2 *q = plane + x2 * PS + x3;
+                *q = make_float2(q->x + v.x, q->y + v.y);
+            };
+            const int Q = (ns + 3) >> 2;
+            if (ord * ord <= 16 && (Q + 31) / 32 <= 2 * (NT / 64)) {
+                // one wave, four sites at a time: lane (k, p) adds point p of
+                // site j + k Q (Q = a quarter of the chunk, so that bonded
+                // neighbours fall in different groups).  The ORD^2 points of a
+                // site are distinct; when no two of the group's ORD x ORD
+                // blocks meet (circular distance of their corners below ORD
+                // on both axes) the four sites go in one step, else one after
+                // another in k order.  Every point therefore receives its
+                // additions in the fixed order (group, k), whatever the step.
+                // The groups' meet flags are found first by the whole
+                // workgroup, one thread per (group, pair of sites), as bits in
+                // the self-energy reduction array (unused until the end).
+                unsigned int *meet = reinterpret_cast<unsigned int *>(red);  // Q bits in 2 (NT / 64) words
+                if (tid < (Q + 31) / 32) meet[tid] = 0u;
+                __syncthreads();
+                for (int t = tid; t < Q * 6; t += NT) {
+                    const int jg = t / 6, pr = t % 6;
+                    const int u = pr < 3 ? 0 : (pr < 5 ? 1 : 2);
+                    const int v = pr < 3 ? pr + 1 : (pr < 5 ? pr - 1 : 3);
+                    const int su = jg + u * Q, sv = jg + v * Q;
+                    if (su < ns && sv < ns) {
+                        const int2 a = sg[su], b = sg[sv];
+                        int dx = a.x - b.x, dy = a.y - b.y;
+                        dx += (dx < 0) ? K2 : 0;
+                        dy += (dy < 0) ? K3 : 0;
+                        dx = min(dx, K2 - dx);
+                        dy = min(dy, K3 - dy);
+                        if (dx < ord && dy < ord) atomicOr(&meet[jg >> 5], 1u << (jg & 31));
+                    }
                 }
+                __syncthreads();
+                if (tid < 64) {
+                    const int k = tid >> 4, pnt = tid & 15;
+                    const bool on = pnt < ord * ord;
+                    const int c2 = on ? pnt / ord : 0, c3 = on ? pnt % ord : 0;
+                    for (int jg = 0; jg < Q; ++jg) {
+                        const bool clash = (meet[jg >> 5] >> (jg & 31)) & 1u;
+                        const int s = jg + k * Q;
+                        const bool mine = on && s < ns;
+                        if (!clash) {
+                            if (mine) add_point(s, c2, c3);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u)
+                                if (k == u && mine) add_point(s, c2, c3);
+                        }
+                    }
+                }
+            } else if (tid < ord * ord) {
+                // larger orders (or chunks past the flag words): one lane per
+                // stencil point, sites in order
+                const int c2 = tid / ord, c3 = tid % ord;
+                for (int s = 0; s < ns; ++s) add_point(s, c2, c3);
             }
         }
         __syncthreads();
