@@ -2385,7 +2385,9 @@ __device__ __forceinline__ void pairs_block(PairArgs pa0, int blk0, int nslot, c
     const int ncell = span * span * span;
     for (int c = part + tid * pa.nsplit; c < ncell; c += NT * pa.nsplit) {
         const int dx = c % span, dy = (c / span) % span, dz = c / (span * span);
-        const int qx = pmod(cx + lo + dx, pa.nc), qy = pmod(cy + lo + dy, pa.nc), qz = pmod(cz + lo + dz, pa.nc);
+        // cx + lo + dx lies in [-nreach, nc + nreach) with nreach < nc (or lo = 0):
+        // one conditional wrap
+        const int qx = wrap1(cx + lo + dx, pa.nc), qy = wrap1(cy + lo + dy, pa.nc), qz = wrap1(cz + lo + dz, pa.nc);
         const unsigned int key = cell_key(qx, qy, qz);
         const unsigned int p0 = cstart[key], p1 = cstart[key + 1];
         // the cell's molecules four at a time: indices, then centres
