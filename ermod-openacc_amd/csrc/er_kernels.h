@@ -921,20 +921,24 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                 *q = make_float2(q->x + v.x, q->y + v.y);
             };
             const int Q = (ns + 3) >> 2;
-            if (ord * ord <= 16 && (Q + 31) / 32 <= 2 * (NT / 64)) {
+            if (ord * ord <= 16 && (Q + 31) / 32 <= NT / 64) {
                 // one wave, four sites at a time: lane (k, p) adds point p of
                 // site j + k Q (Q = a quarter of the chunk, so that bonded
                 // neighbours fall in different groups).  The ORD^2 points of a
                 // site are distinct; when no two of the group's ORD x ORD
                 // blocks meet (circular distance of their corners below ORD
                 // on both axes) the four sites go in one step, else one after
-                // another in k order.  Every point therefore receives its
-                // additions in the fixed order (group, k), whatever the step.
-                // The groups' meet flags are found first by the whole
+                // another: in two steps, k = 0, 2 then k = 1, 3, when neither
+                // pair of a step meets, else in k order.  Every point
+                // therefore receives its additions in an order fixed by the
+                // sites alone.  The groups' flags are found first by the whole
                 // workgroup, one thread per (group, pair of sites), as bits in
-                // the self-energy reduction array (unused until the end).
-                unsigned int *meet = reinterpret_cast<unsigned int *>(red);  // Q bits in 2 (NT / 64) words
-                if (tid < (Q + 31) / 32) meet[tid] = 0u;
+                // the self-energy reduction array (unused until the end):
+                // meet = some pair meets, far = pair (0, 2) or (1, 3) meets.
+                const int nw = (Q + 31) / 32;                                  // words per bit set
+                unsigned int *meet = reinterpret_cast<unsigned int *>(red);  // [nw], then far [nw]
+                unsigned int *far = meet + nw;
+                if (tid < 2 * nw) meet[tid] = 0u;
                 __syncthreads();
                 for (int t = tid; t < Q * 6; t += NT) {
                     const int jg = t / 6, pr = t % 6;
@@ -948,7 +952,10 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                         dy += (dy < 0) ? K3 : 0;
                         dx = min(dx, K2 - dx);
                         dy = min(dy, K3 - dy);
-                        if (dx < ord && dy < ord) atomicOr(&meet[jg >> 5], 1u << (jg & 31));
+                        if (dx < ord && dy < ord) {
+                            atomicOr(&meet[jg >> 5], 1u << (jg & 31));
+                            if (v == u + 2) atomicOr(&far[jg >> 5], 1u << (jg & 31));
+                        }
                     }
                 }
                 __syncthreads();
@@ -958,15 +965,27 @@ __global__ void __launch_bounds__(NT) k_pme_spec2d(int nslot, const SlotDesc *__
                     const int c2 = on ? pnt / ord : 0, c3 = on ? pnt % ord : 0;
                     for (int jg = 0; jg < Q; ++jg) {
                         const bool clash = (meet[jg >> 5] >> (jg & 31)) & 1u;
+                        const bool clash_far = (far[jg >> 5] >> (jg & 31)) & 1u;
                         const int s = jg + k * Q;
                         const bool mine = on && s < ns;
+                        // lanes of different k may add to one point in
+                        // consecutive steps: a wavefront fence between steps
+                        // keeps the compiler from merging or reordering them
+                        // (the wave executes them in order)
                         if (!clash) {
                             if (mine) add_point(s, c2, c3);
+                        } else if (!clash_far) {
+                            if (mine && (k & 1) == 0) add_point(s, c2, c3);
+                            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                            if (mine && (k & 1) == 1) add_point(s, c2, c3);
                         } else {
 #pragma unroll
-                            for (int u = 0; u < 4; ++u)
+                            for (int u = 0; u < 4; ++u) {
                                 if (k == u && mine) add_point(s, c2, c3);
+                                __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+                            }
                         }
+                        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
                     }
                 }
             } else if (tid < ord * ord) {
