@@ -1525,11 +1525,20 @@ template <class T> static int want(er_ctx *c, DBuf<T> &b, size_t count, const ch
                 c->capturing ? "er_graph_begin" : "er_run_noalloc", name, b.n, count);
 }
 
-// insertion slices of the split-K edens/ecorr sums over nch insertions:
-// slices of kper (a multiple of 4, at least ecorr_kmin) insertions
+// insertion slices of the split-K edens/ecorr sums over a solute's nch
+// insertions: slices of kper (a multiple of 4, at least ecorr_kmin)
+// insertions, as many as give the solute's (64 x 64 tile, slice) blocks
+// about four per CU and at most ER_KSLICES.  Every slice writes a partial
+// tile set of ermax^2 that the reduce reads back, so a solute with many bins
+// takes fewer slices.  The count depends on the solute alone, so a solute's
+// sums are the same whatever else shares its run.
 static void ecorr_slices(const er_ctx *c, int nch, int &kper, int &nks)
 {
-    kper = std::max(c->ecorr_kmin, (((nch + ER_KSLICES - 1) / ER_KSLICES) + 3) / 4 * 4);
+    const long nb = (c->ermax + 63) / 64;
+    const long tiles = std::max(1L, nb * (nb + 1) / 2);
+    const long fill = 4L * std::max(1, c->ncu);
+    const int want = (int)std::max(1L, std::min<long>(ER_KSLICES, (fill + tiles - 1) / tiles));
+    kper = std::max(c->ecorr_kmin, (((nch + want - 1) / want) + 3) / 4 * 4);
     nks = (nch + kper - 1) / kper;
 }
 

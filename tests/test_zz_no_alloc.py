@@ -81,8 +81,9 @@ def test_noalloc_needs_sized_buffers(cfg0):
 
 
 def test_noalloc_rejects_partials_beyond_the_scratch():
-    """1600 bins with ecorr: 500 insertions need 8 slices x 1600^2 x 8 bytes
-    of partial sums, above the 64 MiB scratch."""
+    """1600 bins with ecorr: one solute's 500 insertions take 4 slices (enough
+    64 x 64 tiles, slices to fill the chip) x 1600^2 x 8 bytes of partial
+    sums, above the 64 MiB scratch."""
     cfg3 = synth.config3()
     w = cfg3.workload(0)
     p = U.workload_params(w)
