@@ -1527,17 +1527,19 @@ template <class T> static int want(er_ctx *c, DBuf<T> &b, size_t count, const ch
 
 // insertion slices of the split-K edens/ecorr sums over a solute's nch
 // insertions: slices of kper (a multiple of 4, at least ecorr_kmin)
-// insertions, as many as give the solute's (64 x 64 tile, slice) blocks
-// about four per CU and at most ER_KSLICES.  Every slice writes a partial
-// tile set of ermax^2 that the reduce reads back, so a solute with many bins
-// takes fewer slices.  The count depends on the solute alone, so a solute's
+// insertions.  Every slice writes a partial tile set of ermax^2 that the
+// reduce reads back, while one slice adds straight into ecorr: a solute
+// whose 64 x 64 tiles alone give every CU a block takes one slice, a
+// smaller one as many as give its (tile, slice) blocks about four per CU (at
+// most ER_KSLICES).  The count depends on the solute alone, so a solute's
 // sums are the same whatever else shares its run.
 static void ecorr_slices(const er_ctx *c, int nch, int &kper, int &nks)
 {
     const long nb = (c->ermax + 63) / 64;
     const long tiles = std::max(1L, nb * (nb + 1) / 2);
-    const long fill = 4L * std::max(1, c->ncu);
-    const int want = (int)std::max(1L, std::min<long>(ER_KSLICES, (fill + tiles - 1) / tiles));
+    const long ncu = std::max(1, c->ncu);
+    const int want =
+        tiles >= ncu ? 1 : (int)std::max(1L, std::min<long>(ER_KSLICES, (4 * ncu + tiles - 1) / tiles));
     kper = std::max(c->ecorr_kmin, (((nch + want - 1) / want) + 3) / 4 * 4);
     nks = (nch + kper - 1) / kper;
 }

@@ -81,15 +81,19 @@ def test_noalloc_needs_sized_buffers(cfg0):
 
 
 def test_noalloc_rejects_partials_beyond_the_scratch():
-    """1600 bins with ecorr: one solute's 500 insertions take 4 slices (enough
-    64 x 64 tiles, slices to fill the chip) x 1600^2 x 8 bytes of partial
-    sums, above the 64 MiB scratch."""
+    """1600 bins with ecorr, all 16 solutes in one context: every solute takes
+    at least one slice of 1600^2 x 8 bytes of partial sums, so the run needs
+    at least 16 x 20.5 MB, above the 64 MiB scratch."""
     cfg3 = synth.config3()
-    w = cfg3.workload(0)
-    p = U.workload_params(w)
-    ctx = _ctx(w, p)
-    ctx.set_frame(w.frame(0), w.cell)
-    ctx.set_insertions(w.insertions(0))
+    ws = [cfg3.workload(k) for k in range(16)]
+    p = U.workload_params(ws[0])
+    ctx = capi.Context(ws[0].topo, p)
+    ctx.set_mesh(*cfg3.uvmax_uvsoft_uvcrd())
+    ctx.set_solutes([(w.topo.slt_ljtype, w.topo.slt_charge32) for w in ws])
+    ctx.set_frame(cfg3.frame(0), cfg3.cell)
+    for k, w in enumerate(ws):
+        ctx.select_solute(k)
+        ctx.set_insertions(w.insertions(0)[:24])
     with pytest.raises(capi.ErError) as e:
         ctx.run_noalloc(capi.ER_BATCHED)
     assert e.value.status == capi.ER_E_UNSUPPORTED
