@@ -1876,8 +1876,16 @@ static hipError_t run_accumulate(er_ctx *c, AccRange ar)
         else
             hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256, nz), dim3(256), 0, c->stream, ar, nslot, sd,
                                c->ermax, c->d_C.p, c->d_w.p, ap.epart);
-        hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, mfma ? c->ermax : 1, ns), dim3(256), 0,
-                           c->stream, ar, sd, c->ermax, (int)mfma, ap.part, ap.epart);
+        // k_ecorr adds a one-slice solute straight into ecorr and edens: when
+        // every solute of the run has one slice, the reduce has nothing to add
+        bool reduce = !mfma;
+        if (mfma && all)
+            for (const auto &d : P.desc) reduce = reduce || d.nks > 1;
+        else if (mfma)
+            reduce = nz > 1;
+        if (reduce)
+            hipLaunchKernelGGL(k_ecorr_reduce, dim3((c->ermax + 255) / 256, mfma ? c->ermax : 1, ns), dim3(256), 0,
+                               c->stream, ar, sd, c->ermax, (int)mfma, ap.part, ap.epart);
         if (c->prm.corrcal == 1 && !c->use_mfma) {
             const int nb16 = (c->ermax + 15) / 16;
             hipLaunchKernelGGL(k_ecorr_valu, dim3(nb16, nb16, ns), dim3(16, 16), 0, c->stream, ar, sd, c->ermax,
