@@ -652,7 +652,9 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
     // (R1 - 1) (R2 - 1) < K3, so it needs no reduction modulo the length
     static_assert((R1 - 1) * (R2 - 1) < K3 && (C1 - 1) * (C2 - 1) < K2, "twiddle index range");
     // ---- rows (axis 3), pass A in place: each thread's items are loaded
-    // together, then transformed, then stored (their LDS latencies overlap) ----
+    // together, then transformed, then stored (their LDS latencies overlap).
+    // Items run rows fastest, so a wave's lanes share n2 and read one
+    // twiddle t3[n2 k1] ----
     {
         constexpr int IA = (K2 * R2 + NT - 1) / NT;
         float2 v[IA][R1];
@@ -660,7 +662,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         for (int j = 0; j < IA; ++j) {
             const unsigned int it = (unsigned int)tid + j * NT;
             if (it < K2 * R2) {
-                const int m2 = it / R2, n2 = it % R2;
+                const int m2 = it % K2, n2 = it / K2;
 #pragma unroll
                 for (int a = 0; a < R1; ++a) v[j][a] = plane[m2 * PS + R2 * a + n2];
             }
@@ -669,7 +671,7 @@ __device__ __forceinline__ void plane_dft(float2 *plane, const float2 *t2, const
         for (int j = 0; j < IA; ++j) {
             const unsigned int it = (unsigned int)tid + j * NT;
             if (it < K2 * R2) {
-                const int m2 = it / R2, n2 = it % R2;
+                const int m2 = it % K2, n2 = it / K2;
                 erfft::fft<R1, SIGN>(v[j], tws + tws_off(R1));
 #pragma unroll
                 for (int k1 = 1; k1 < R1; ++k1) v[j][k1] = erfft::twmul<SIGN>(v[j][k1], t3[n2 * k1]);
