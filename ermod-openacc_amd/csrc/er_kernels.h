@@ -3009,8 +3009,8 @@ __global__ void k_mfma_layout(double *__restrict__ out)
 // to part[slice] (no atomics); the diagonal blocks also write the slice's
 // edens partial.  k_ecorr_reduce then adds the slices in a fixed order, so
 // results do not depend on scheduling.  Insertions are taken four per MFMA
-// step (kq = lane / 16), the next step's operands loaded before the current
-// step's MFMAs.
+// step (kq = lane / 16), the operands of the next four steps in flight
+// while the current step's MFMAs run.
 __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const SlotDesc *__restrict__ sd, int ermax,
                                                const uint32_t *__restrict__ Cnt, const double *__restrict__ w,
                                                const int *__restrict__ dmap, double *__restrict__ part,
@@ -3050,20 +3050,30 @@ __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const Slo
             bv[a] = (ok && b2 < ermax) ? (double)row[b2] : 0.0;
         }
     };
-    double av[2], bv[2];
-    load(k_begin, av, bv);
-    for (int k0 = k_begin; k0 < k_end; k0 += 4) {
-        double an[2], bn[2];
-        load(k0 + 4, an, bn);
+    // PF steps of operands in flight: stage p holds step k0 + 4 p and is
+    // refilled with step k0 + 4 (p + PF) once its MFMAs are issued.  The
+    // count rows come from beyond the L2 (every solute's rows are read by
+    // blocks on all XCDs), so one step ahead left the MFMAs waiting; four
+    // (96 registers, five waves per SIMD) measured fastest, six and eight
+    // lost waves
+    constexpr int PF = 4;
+    double av[PF][2], bv[PF][2];
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
-            ed[a] += av[a];
+    for (int p = 0; p < PF; ++p) load(k_begin + 4 * p, av[p], bv[p]);
+    for (int k0 = k_begin; k0 < k_end; k0 += 4 * PF) {
 #pragma unroll
-            for (int b = 0; b < 2; ++b)
-                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        for (int p = 0; p < PF; ++p) {
+            if (k0 + 4 * p < k_end) {
+#pragma unroll
+                for (int a = 0; a < 2; ++a) {
+                    ed[a] += av[p][a];
+#pragma unroll
+                    for (int b = 0; b < 2; ++b)
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[p][a], bv[p][b], acc[a][b], 0, 0, 0);
+                }
+            }
+            load(k0 + 4 * (p + PF), av[p], bv[p]);
         }
-        av[0] = an[0]; av[1] = an[1];
-        bv[0] = bn[0]; bv[1] = bn[1];
     }
     double *P = part + (size_t)z * plane;
     __attribute__((address_space(1))) double *E = gptr(sd[slot].ecorr);
