@@ -1870,9 +1870,12 @@ static hipError_t run_accumulate(er_ctx *c, AccRange ar)
     {
         ProfScope ps(c, ER_K_ECORR);
         const int nb = (c->ermax + 63) / 64;
+        // k_ecorr: (slice, upper-triangle tile) pairs, a whole number per XCD
+        const long ntri = (long)nb * (nb + 1) / 2;
+        const long nwg = 8 * ((ntri * nz + 7) / 8);
         if (mfma)
-            hipLaunchKernelGGL(k_ecorr, dim3(nb, nb, nz), dim3(256), 0, c->stream, ar, nslot, sd, c->ermax, c->d_C.p,
-                               c->d_w.p, c->d_dmap.p, ap.part, ap.epart);
+            hipLaunchKernelGGL(k_ecorr, dim3((unsigned)nwg), dim3(256), 0, c->stream, ar, nslot, sd, c->ermax, nz,
+                               c->d_C.p, c->d_w.p, c->d_dmap.p, ap.part, ap.epart);
         else
             hipLaunchKernelGGL(k_edens, dim3((c->ermax + 255) / 256, nz), dim3(256), 0, c->stream, ar, nslot, sd,
                                c->ermax, c->d_C.p, c->d_w.p, ap.epart);

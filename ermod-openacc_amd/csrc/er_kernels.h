@@ -3012,13 +3012,27 @@ __global__ void k_mfma_layout(double *__restrict__ out)
 // step (kq = lane / 16), the operands of the next four steps in flight
 // while the current step's MFMAs run.
 __global__ void __launch_bounds__(256) k_ecorr(AccRange ar, int nslot, const SlotDesc *__restrict__ sd, int ermax,
-                                               const uint32_t *__restrict__ Cnt, const double *__restrict__ w,
+                                               int nz, const uint32_t *__restrict__ Cnt, const double *__restrict__ w,
                                                const int *__restrict__ dmap, double *__restrict__ part,
                                                double *__restrict__ epart)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int by = blockIdx.y, bx = blockIdx.x, z = blockIdx.z;
-    if (by > bx) return;  // lower block triangle: mirrored by the reduce
+    // one workgroup per (slice z, upper-triangle 64 x 64 tile (by, bx)), the
+    // lower triangle being mirrored.  XCD-aware order: workgroup b runs on
+    // XCD b % 8, and each XCD takes one contiguous eighth of the (slice,
+    // tile) list, slices outermost, so an XCD works on one solute's count
+    // rows at a time and they stay in its L2.
+    const int nb = (ermax + 63) / 64, ntri = nb * (nb + 1) / 2;
+    const int total = ntri * nz, per = (total + 7) / 8;
+    const int v = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    if (v >= total) return;
+    const int z = v / ntri;
+    int by = 0, t = v % ntri;
+    while (t >= nb - by) {  // row by holds the tiles bx = by .. nb - 1
+        t -= nb - by;
+        ++by;
+    }
+    const int bx = by + t;
     int slice, n_off, nchunk, kper, slot, nks;
     slice_span(ar, sd, nslot, z, slice, n_off, nchunk, kper, slot, nks);
     // a solute with one slice adds its tile straight into ecorr and edens
