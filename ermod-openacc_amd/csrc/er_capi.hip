@@ -1374,7 +1374,7 @@ template <int NI> static hipError_t launch_hist(er_ctx *c, int n_off, int nch)
     // reduced by one thread)
     const int NT = NI > 1 ? 512 : 256;
     const size_t lds = (size_t)NT * sizeof(double) + (size_t)c->ermax * sizeof(float) +
-                       (size_t)NI * c->ermax * sizeof(uint32_t) + (size_t)NT * 2 * sizeof(float) +
+                       (size_t)NI * (c->ermax | 1) * sizeof(uint32_t) + (size_t)NT * 2 * sizeof(float) +
                        (size_t)c->numslv * ER_LUT_N * sizeof(unsigned short);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     if (lds > 65536)

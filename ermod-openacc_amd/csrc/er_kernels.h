@@ -2591,15 +2591,18 @@ __global__ void __launch_bounds__(512) k_hist(HistArgs ha, int nslot, const Slot
     extern __shared__ __align__(16) unsigned char smem[];
     double *rsum = reinterpret_cast<double *>(smem);                 // [NT]
     float *mesh = reinterpret_cast<float *>(rsum + blockDim.x);      // [ermax]
-    uint32_t *hist = reinterpret_cast<uint32_t *>(mesh + ha.ermax);  // [NI][ermax]
-    float *rmin = reinterpret_cast<float *>(hist + NI * ha.ermax);   // [NT]
+    // [NI][hs]: an odd row stride puts one bin of the NI rows (the lanes of a
+    // wave hold NI insertions) in NI different banks
+    const int hs = ha.ermax | 1;
+    uint32_t *hist = reinterpret_cast<uint32_t *>(mesh + ha.ermax);
+    float *rmin = reinterpret_cast<float *>(hist + NI * hs);         // [NT]
     float *rmax = rmin + blockDim.x;                                  // [NT]
     unsigned short *blut = reinterpret_cast<unsigned short *>(rmax + blockDim.x);  // [numslv][ER_LUT_N]
 
     const int tid = threadIdx.x, NT = blockDim.x;
     const int tile = blockIdx.x, slice = blockIdx.y;
     for (int b = tid; b < ha.ermax; b += NT) mesh[b] = meshf[b];
-    for (int b = tid; b < NI * ha.ermax; b += NT) hist[b] = 0u;
+    for (int b = tid; b < NI * hs; b += NT) hist[b] = 0u;
     for (int b = tid; b < ha.numslv * ER_LUT_N; b += NT) blut[b] = lut[b];
     __syncthreads();
 
@@ -2656,7 +2659,7 @@ __global__ void __launch_bounds__(512) k_hist(HistArgs ha, int nslot, const Slot
                 // the 'mbin' warning (engproc.F90:1158-1167): an energy below
                 // 1e3 in the open top bin; the earliest one is kept
                 if (idpti == num && v < 1.0e3f) gatomic_min(warnkey, halt_key(seq, nloc, j + 1));
-                atomicAdd(&hist[i * ha.ermax + base + max(idpti, 1) - 1], 1u);
+                atomicAdd(&hist[i * hs + base + max(idpti, 1) - 1], 1u);
                 lmin = fminf(lmin, v);
                 lmax = fmaxf(lmax, v);
                 lsum += (double)v;
@@ -2688,7 +2691,7 @@ __global__ void __launch_bounds__(512) k_hist(HistArgs ha, int nslot, const Slot
     }
     for (int f = tid; f < NI * ha.ermax; f += NT) {
         const int ii = f / ha.ermax, b = f % ha.ermax;
-        const uint32_t h = hist[f];
+        const uint32_t h = hist[ii * hs + b];
         if (h && tile * NI + ii < ha.nchunk && ha.n_off + tile * NI + ii < live)
             atomicAdd(&Cnt[(size_t)(ha.n_off + tile * NI + ii) * ha.ermax + b], h);
     }
