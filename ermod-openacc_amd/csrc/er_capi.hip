@@ -1935,7 +1935,7 @@ static int run_slots(er_ctx *c, const std::vector<int> &todo, int mode)
                            c->d_molbeg.p, soln ? c->d_tag.p : (const int *)nullptr, c->fc, c->K1, c->K2, c->K3,
                            c->ord, c->d_insposn.p, c->d_insposu.p, c->d_insw.p, c->d_insg.p);
         HIPCHK(c, hipGetLastError());
-        hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)((P.ntot + 3) / 4)), dim3(256), 0, c->stream, P.ntot, nslot,
+        hipLaunchKernelGGL(k_prep_ins, dim3((unsigned)std::max(1, P.ntot)), dim3(64 * ER_PI_WAVES), 0, c->stream, P.ntot, nslot,
                            sd, c->fc, c->prm.screen, (int)(c->prm.cltype == 0), c->d_insposn.p, c->d_insref.p,
                            c->d_selfreal.p, c->d_resself.p);
         HIPCHK(c, hipGetLastError());

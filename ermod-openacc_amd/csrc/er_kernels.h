@@ -376,20 +376,25 @@ __global__ void k_prep_sites(int nsite, int nslot, const SlotDesc *__restrict__ 
     insposn[is] = make_float4(x, y, z, q);
 }
 
-// One wave per insertion (after k_prep_sites): search centre and reach
-// radius, real-space self energy and residual self term.
-__global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const SlotDesc *__restrict__ sd,
-                                                  FrameConst fc, float screen, int bare,
-                                                  const float4 *__restrict__ insposn, float4 *__restrict__ insref,
-                                                  float *__restrict__ selfreal, float *__restrict__ resself)
+// One workgroup of four waves per insertion (after k_prep_sites): search
+// centre and reach radius (every wave computes them; wave 0 writes), the
+// real-space self energy with the site pairs shared by the four waves, and
+// the residual self term.
+#define ER_PI_WAVES 4
+__global__ void __launch_bounds__(64 * ER_PI_WAVES) k_prep_ins(int ntot, int nslot, const SlotDesc *__restrict__ sd,
+                                                               FrameConst fc, float screen, int bare,
+                                                               const float4 *__restrict__ insposn,
+                                                               float4 *__restrict__ insref,
+                                                               float *__restrict__ selfreal,
+                                                               float *__restrict__ resself)
 {
-    const int lane = threadIdx.x & 63;
-    const int n = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = blockIdx.x;
     if (n >= ntot) return;
     const int k = ER_SLOT_OF_INS(sd, nslot, n);
     const int local = n - sd[k].ins_off;
     if (local >= sd[k].nins) {  // padding of the solute's insertion range
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             insref[n] = make_float4(0.f, 0.f, 0.f, 0.f);
             selfreal[n] = 0.0f;
             resself[n] = 0.0f;
@@ -429,7 +434,7 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
     }
     for (int off = 32; off > 0; off >>= 1) R = fmaxf(R, __shfl_xor(R, off, 64));
     if (bare) {  // cltype 0: realcal_self and residual_self_ene return at once
-        if (lane == 0) {
+        if (threadIdx.x == 0) {
             insref[n] = make_float4(cx, cy, cz, R);
             selfreal[n] = 0.0f;
             resself[n] = 0.0f;
@@ -440,13 +445,15 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
     // in FP64 from the FP32 positions and charges: a 150-site solute has
     // 11 175 terms of up to tens of kcal/mol that largely cancel, and this
     // self energy enters the insertion's weight; computed once per
-    // insertion, FP64 costs little here.  Lane l takes the pairs (s, t) with
-    // t - s - 1 = l mod 64, then a fixed-order tree.
+    // insertion, FP64 costs little here.  Wave w takes the rows s = w mod 4
+    // and its lane l the pairs (s, t) with t - s - 1 = l mod 64, then a
+    // fixed-order tree in each wave and the four waves added in order.
     double e = 0.0;
     const double rsqpi = 0.564189583547756286948079451560772586;  // 1/sqrt(pi)
     const double scr = screen;
-    for (int s = lane; s < nslt; s += 64) e += -(double)P[s].w * (double)P[s].w * scr * rsqpi;
-    for (int s = 0; s < nslt; ++s) {
+    if (wave == 0)
+        for (int s = lane; s < nslt; s += 64) e += -(double)P[s].w * (double)P[s].w * scr * rsqpi;
+    for (int s = wave; s < nslt; s += ER_PI_WAVES) {
         const float4 a = P[s];
         for (int t = s + 1 + lane; t < nslt; t += 64) {
             const float4 b = P[t];
@@ -457,7 +464,12 @@ __global__ void __launch_bounds__(256) k_prep_ins(int ntot, int nslot, const Slo
         }
     }
     for (int off = 32; off > 0; off >>= 1) e += __shfl_xor(e, off, 64);
-    if (lane == 0) {
+    __shared__ double wsum[ER_PI_WAVES];
+    if (lane == 0) wsum[wave] = e;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        e = wsum[0];
+        for (int w = 1; w < ER_PI_WAVES; ++w) e += wsum[w];
         insref[n] = make_float4(cx, cy, cz, R);
         selfreal[n] = (float)e;
         // residual_self_ene (engproc.F90:967-977): -(pi Q^2 / alpha^2 / V) / 2 with
