@@ -626,12 +626,15 @@ __global__ void k_ins_factors(int nrows, int ord, int K1, int K2, int K3, int H,
     else { axis = 2; m = r - H - K2; K = K3; gd = g.z; tw = tw3; }
     const float *w = insw + site * 3 * ord + axis * ord;
     float2 a = make_float2(0.f, 0.f);
+    // e = m (g - c) mod K: one reduction for c = 0 (m, pmod(.) < K <= 512, so
+    // the product fits 32 bits), then e - m mod K for each next c
+    int e = (int)(((unsigned int)m * (unsigned int)pmod(gd, K)) % (unsigned int)K);
     for (int c = 0; c < ord; ++c) {
-        // m, pmod(.) < K <= 512: the product fits 32 bits
-        const int e = (int)(((unsigned int)m * (unsigned int)pmod(gd - c, K)) % (unsigned int)K);
         const float2 t = tw[e];  // exp(+2 pi i e/K); the forward sum uses the conjugate
         a.x += w[c] * t.x;
         a.y -= w[c] * t.y;
+        e -= m;
+        e += (e < 0) ? K : 0;
     }
     if (axis == 0) a = erfft::cscale(a, insposn[site].w);
     Ftab[f] = a;
