@@ -1337,7 +1337,7 @@ template <int ORD, int NI> static hipError_t launch_gather_brick_k(er_ctx *c, in
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
-    const int mpb = ER_MR_MPT * (256 / NI);  // molecules per block
+    const int mpb = NI == 16 ? ER_MR_MPT4 * 64 : ER_MR_MPT * (256 / NI);  // molecules per block
     hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + mpb - 1) / mpb), ntile), dim3(256),
                        0, c->stream, c->nummol,
                        nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
@@ -1358,7 +1358,7 @@ template <int NI> static hipError_t launch_gather_stream_k(er_ctx *c, int n_off,
     if (e != hipSuccess) return e;
     const bool soln = c->prm.slttype == 1;
     const int ntile = (nch + NI - 1) / NI;
-    const int mpb = ER_MR_MPT * (256 / NI);  // molecules per block
+    const int mpb = NI == 16 ? ER_MR_MPT4 * 64 : ER_MR_MPT * (256 / NI);  // molecules per block
     hipLaunchKernelGGL((k_mol_reduce<NI>), dim3((unsigned)((c->nummol + mpb - 1) / mpb), ntile), dim3(256), 0,
                        c->stream, c->nummol, nch, n_off, c->numatm, c->d_molbeg.p, c->d_molq.p, c->d_Eat.p,
                        soln ? c->d_tag.p : (const int *)nullptr, (int)c->plan.desc.size(), c->d_sd.p, c->prm.screen,

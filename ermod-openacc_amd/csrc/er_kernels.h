@@ -2099,13 +2099,96 @@ __global__ void __launch_bounds__(ER_GS_NT) k_gather_stream(int K1, int K2, int 
 
 // per-molecule sum of the atom energies (atom order as in k_gather) plus the
 // residual background term (engproc.F90:979-1020); writes U
-#define ER_MR_MPT 8  // molecules per thread in k_mol_reduce
+#define ER_MR_MPT 8   // molecules per thread in k_mol_reduce
+#define ER_MR_MPT4 2  // molecules per thread in its float4 form (16 insertions)
 template <int NI>
 __global__ void k_mol_reduce(int nmol, int nchunk, int n_off, int natom, const int *__restrict__ molbeg,
                              const float *__restrict__ molq, const float *__restrict__ Eat,
                              const int *__restrict__ tag, int nslot, const SlotDesc *__restrict__ sd,
                              float screen, float volume, float *__restrict__ U)
 {
+    if constexpr (NI == 16) {
+        // block = ER_MR_MPT4 * 64 molecules x the 16 insertions of one tile
+        // (blockIdx.y, one solute: solutes are padded to whole tiles);
+        // thread (m, q) takes insertions 4q .. 4q + 3 of molecules m, m + 64,
+        // ... as float4 (four lanes read a molecule's 64-byte atom row)
+        const int q = threadIdx.x & 3, ms = threadIdx.x >> 2;
+        const int tile = blockIdx.y;
+        const int ks = ER_SLOT_OF_INS(sd, nslot, n_off + tile * NI);
+        bool ok[4];
+        int skip[4];
+        float qs[4];
+        bool all = true, any = false;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int nl = tile * NI + 4 * q + c, n = n_off + nl;
+            ok[c] = nl < nchunk && n - sd[ks].ins_off < sd[ks].nins;
+            skip[c] = (tag && ok[c]) ? tag[n] : -1;
+            qs[c] = tag ? (ok[c] ? molq[skip[c]] : 0.0f) : sd[ks].qslt;
+            all = all && ok[c];
+            any = any || ok[c];
+        }
+        if (!any) return;
+        const float4 *E4 = reinterpret_cast<const float4 *>(Eat + (size_t)tile * natom * NI) + q;
+        // n_off is a whole number of tiles
+        float *Ut = U + (size_t)(n_off / NI + tile) * nmol * NI + 4 * q;
+        int b0[ER_MR_MPT4], b1[ER_MR_MPT4];
+        float mq[ER_MR_MPT4];
+        float4 e[ER_MR_MPT4][4];
+#pragma unroll
+        for (int r = 0; r < ER_MR_MPT4; ++r) {
+            const int mol = (blockIdx.x * ER_MR_MPT4 + r) * 64 + ms;
+            b0[r] = b1[r] = 0;
+            mq[r] = 0.0f;
+            if (mol < nmol) {
+                b0[r] = molbeg[mol];
+                b1[r] = molbeg[mol + 1];
+                mq[r] = molq[mol];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < ER_MR_MPT4; ++r)
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                e[r][k] = (b0[r] + k < b1[r]) ? E4[(size_t)(b0[r] + k) * 4] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int r = 0; r < ER_MR_MPT4; ++r) {
+            const int mol = (blockIdx.x * ER_MR_MPT4 + r) * 64 + ms;
+            if (mol < nmol) {
+                // per insertion the same atom order and arithmetic as below
+                float acc[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (b0[r] + k < b1[r]) {
+                        acc[0] += e[r][k].x;
+                        acc[1] += e[r][k].y;
+                        acc[2] += e[r][k].z;
+                        acc[3] += e[r][k].w;
+                    }
+                for (int a = b0[r] + 4; a < b1[r]; ++a) {
+                    const float4 v = E4[(size_t)a * 4];
+                    acc[0] += v.x;
+                    acc[1] += v.y;
+                    acc[2] += v.z;
+                    acc[3] += v.w;
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    if (mol == skip[c]) acc[c] = 0.0f;
+                    else acc[c] -= ER_PI_F * qs[c] * mq[r] / screen / screen / volume;
+                }
+                float *u = Ut + (size_t)mol * NI;
+                if (all) {
+                    *reinterpret_cast<float4 *>(u) = make_float4(acc[0], acc[1], acc[2], acc[3]);
+                } else {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c)
+                        if (ok[c]) u[c] = acc[c];
+                }
+            }
+        }
+        return;
+    }
     // block = ER_MR_MPT * 256 / NI molecules x NI insertions of one tile
     // (blockIdx.y); thread (m, i) takes molecules m, m + 256 / NI, ...
     const int i = threadIdx.x % NI;
