@@ -1722,10 +1722,23 @@ static int ensure_run_buffers(er_ctx *c, int chunk)
     ER_WANT(c->d_bstart, c->nbrick + 1);
     ER_WANT(c->d_bdesc, c->nbrick);
     if (NI > 1 && brick_order(c->ord)) ER_WANT(c->d_Eat, ntile * NI * (size_t)c->numatm);
-    // enough histogram workgroups to fill the chip
+    // histogram slices: soln runs keep one built-in rule (their flceng partial
+    // sums follow it) from the chunk's tiles; refs runs take about 2048
+    // (tile, slice) workgroups over the tiles of one histogram launch (the
+    // whole run when batched, one insertion when plain), but no slice with
+    // fewer than NI ermax / 4 molecules (every workgroup zeroes and flushes
+    // NI x ermax bins) unless the CUs would otherwise go short of
+    // workgroups; a tuned count wins
     const int ntile_chunk = (chunk + NI - 1) / NI;
-    c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
-    // soln runs keep the built-in slicing: their flceng partial sums follow it
+    if (c->prm.slttype == 1) {
+        c->nslice = std::max(1, std::min(64, (2048 + ntile_chunk - 1) / ntile_chunk));
+    } else {
+        const long ntile_launch = NI > 1 ? std::max(1L, (long)((ntot + NI - 1) / NI)) : 1L;
+        const long by_fill = (2048 + ntile_launch - 1) / ntile_launch;
+        const long by_bins = std::max(1L, 4L * c->nummol / ((long)NI * std::max(1, c->ermax)));
+        const long by_cus = (std::max(1, c->ncu) + ntile_launch - 1) / ntile_launch;
+        c->nslice = (int)std::max(1L, std::min(64L, std::min(by_fill, std::max(by_bins, by_cus))));
+    }
     if (c->hist_slices > 0 && c->prm.slttype != 1) c->nslice = c->hist_slices;
     if (soln) {
         ER_WANT(c->d_flceng, ntot * c->numslv);
